@@ -1,0 +1,187 @@
+/*
+ * kapsel_hip.h -- C ABI of the MI355X smoothed-profile-method (SPM) step.
+ *
+ * One kh_run holds one simulation's state resident in HBM: the reduced
+ * vorticity zeta (2 spectral fields), the k=0 velocity uk_dc, the particle
+ * state, and the work fields of the step. The host hands over particles and
+ * fluid once, calls kh_step() for as many steps as it wants, and downloads
+ * state only when it needs it (output, restart).
+ *
+ * Every call returns KH_OK (0) or a negative kh_status; no exception crosses
+ * this boundary. kh_last_error() gives a readable message for the last
+ * failure on the calling thread. Plain pointers and sizes only.
+ *
+ * Reference interfaces replaced (KAPSEL file:line):
+ *   kh_step            Time_evolution_hydro (sp_3d_ns.cxx:41-194), the
+ *                      function-pointer seam `Time_evolution` (sp_3d_ns.cxx:10)
+ *   kh_init_fluid      Init_zeta_k + initial constraint block
+ *                      (init_fluid.cxx:10-25, sp_3d_ns.cxx:809-826)
+ *   kh_A2a_k/kh_A_k2a  A2a_k / A_k2a (fft_wrapper.h:641-681)
+ *   kh_apply           the individual functions below Time_evolution_hydro,
+ *                      listed with enum kh_fn
+ *   kh_run_create      Init_fft (fft_wrapper.cxx:153-186), Mem_alloc_var
+ *                      (sp_3d_ns.cxx:588-639), Particle_domain
+ *                      (profile.cxx:22-65), the derived constants of
+ *                      Set_global_parameters (input.cxx:424-600)
+ */
+#ifndef KAPSEL_HIP_H
+#define KAPSEL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "kapsel_particle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kh_status {
+    KH_OK              = 0,
+    KH_ERR_ARG         = -1, /* invalid argument or parameter */
+    KH_ERR_HIP         = -2, /* HIP runtime error */
+    KH_ERR_NOMEM       = -3, /* device allocation failed */
+    KH_ERR_UNSUPPORTED = -4, /* size or option this build does not provide */
+    KH_ERR_STATE       = -5, /* call made in the wrong state */
+    KH_ERR_NONFINITE   = -6  /* non-finite particle state detected */
+} kh_status;
+
+/* enum JP (input.h:49) */
+enum { KH_JP_MOTOR = 0, KH_JP_SLIP = 1, KH_JP_OBSTACLE = 2, KH_JP_NONE = 3 };
+/* enum JAX (input.h:48) */
+enum { KH_JAX_X = 0, KH_JAX_Y = 1, KH_JAX_Z = 2, KH_JAX_NONE = 3 };
+/* precision of the field path */
+enum { KH_FP64 = 0, KH_MIXED = 1 };
+
+/* One particle species: object_type.spherical_particle.Particle_spec[i]
+ * (input.cxx:1400-1470). */
+typedef struct kh_species {
+    double mass_ratio;      /* MASS_RATIOS[i]: rho_particle / rho            */
+    int    propulsion;      /* janus_propulsion[i] (KH_JP_*)                   */
+    int    janus_axis;      /* janus_axis[i] (KH_JAX_*)                        */
+    double janus_force[3];  /* body-frame motor force                          */
+    double janus_torque[3]; /* body-frame motor torque                         */
+    double slip_vel;        /* janus_slip_vel[i] = B1                          */
+    double slip_mode;       /* janus_slip_mode[i] = B2/B1 as given in the input;
+                               halved internally (input.cxx:581-585)           */
+    double rotlet_c1;       /* janus_rotlet_C1[i]                              */
+    double rotlet_c2;       /* janus_rotlet_dipole_C2[i]                       */
+} kh_species;
+
+/* Physical and numerical parameters, in the reference's input units
+ * (input.cxx:685-2557); the derived constants are recomputed as
+ * Set_global_parameters does (input.cxx:424-600). */
+typedef struct kh_params {
+    int    nx, ny, nz;  /* grid points per axis (mesh.NPX.. as 1<<NP); sizes 2^a 3^b, 8..512 */
+    double dx;          /* DX */
+    double rho, eta;    /* RHO, ETA */
+    double kbt;         /* kBT; thermal noise on when > 0 */
+    double alpha_v, alpha_o;
+    double dt;          /* time_increment.manual.delta_t */
+    double a, a_xi;     /* A, A_XI (grid units): RADIUS = A*DX, XI = A_XI*DX */
+    double g;           /* gravity.G */
+    int    g_dir;       /* gravity.G_direction: 0,1,2 for -X,-Y,-Z */
+    int    rotation;    /* switch.ROTATION */
+    int    fix_cell[3]; /* switch.FIX_CELL.{x,y,z} */
+    int    lj_truncate; /* switch.LJ_truncate: 1 ON, 0 OFF, -1 NONE */
+    int    lj_powers;   /* LJ_powers: 0 "12:6", 1 "24:12", 2 "36:18" */
+    double epsilon;     /* EPSILON */
+    int    lj_cell_compat; /* 1: count pairs with the multiplicity of the
+                              reference's 16-cell link list (md_force.cxx:14,
+                              56-101), which double-counts when N/16 == 2;
+                              0: count every pair once */
+    int               nspecies;
+    const kh_species *species; /* nspecies entries, copied at kh_run_create */
+    int    slip_iter_max;      /* switch.SLIP_iter (MAX_SLIP_ITER) */
+    double slip_tol;           /* switch.SLIP_tol (MAX_SLIP_TOL) */
+    int    precision;          /* KH_FP64 or KH_MIXED */
+    uint64_t seed;             /* thermal-noise stream key */
+    uint32_t stream_id;        /* second key word: independent runs, one seed */
+} kh_params;
+
+typedef struct kh_run kh_run;
+
+/* Fields that can be read or written in the reference's real-space layout
+ * (NX*NY*NZ_ doubles, NZ_ = 2*(NZ/2+1), padding columns ignored). */
+enum kh_field {
+    KH_FIELD_U0 = 0, KH_FIELD_U1, KH_FIELD_U2,    /* u   (fft_wrapper.cxx:20) */
+    KH_FIELD_UP0, KH_FIELD_UP1, KH_FIELD_UP2,     /* up  (fft_wrapper.cxx:20) */
+    KH_FIELD_PHI, KH_FIELD_PHI_SUM,               /* phi, phi_sum             */
+    KH_FIELD_F0, KH_FIELD_F1, KH_FIELD_F2,        /* f_particle (f_particle.cxx:10) */
+    KH_FIELD_COUNT
+};
+
+/* Single functions of the step, applied to the run's device state, for
+ * per-function parity with the reference. */
+enum kh_fn {
+    KH_FN_NS_SOLVER = 0,      /* NS_solver_slavedEuler        fluid_solver.cxx:348-374 */
+    KH_FN_ZETA_K2U,           /* Zeta_k2u (zeta,uk_dc -> u)   operate_omega.h:134-137 */
+    KH_FN_U2ZETA_K,           /* U2zeta_k (u -> zeta,uk_dc)   operate_omega.h:208-211 */
+    KH_FN_POSITION_EULER,     /* MD_solver_position_Euler     particle_solver.cxx:39-61 */
+    KH_FN_POSITION_AB2,       /* MD_solver_position_AB2       particle_solver.cxx:63-85 */
+    KH_FN_MAKE_PHI,           /* Reset_phi x2 + Make_phi_particle_sum  make_phi.cxx:214-260 */
+    KH_FN_F_HYDRO,            /* Calc_f_hydro_correct_precision        md_force.cxx:383-506 */
+    KH_FN_VELOCITY_EULER,     /* MD_solver_velocity_Euler     particle_solver.cxx:103-142 */
+    KH_FN_VELOCITY_AB2,       /* MD_solver_velocity_AB2_hydro particle_solver.cxx:239-281 */
+    KH_FN_THERMOSTAT,         /* Add_random_force_thermostat  fluct.cxx:10-100 */
+    KH_FN_MAKE_UP,            /* Reset_u + Make_u_particle_sum         make_phi.cxx:313-354 */
+    KH_FN_F_PARTICLE_SOLE,    /* Make_f_particle_dt_sole      f_particle.h:60-67 */
+    KH_FN_ADD_F_PARTICLE,     /* Add_f_particle (u += f)      f_particle.h:68-84 */
+    KH_FN_COUNT
+};
+
+/* -------- lifecycle -------- */
+int kh_run_create(const kh_params *params, int nparticles, kh_run **out);
+int kh_run_destroy(kh_run *run);
+
+/* -------- state transfer (host <-> HBM) -------- */
+/* Particles in KAPSEL's own array-of-structs (variable.h:27-77). */
+int kh_set_particles(kh_run *run, const Particle *p, int n);
+int kh_get_particles(kh_run *run, Particle *p, int n);
+/* zeta in the reference spectral layout (NX*NY*NZ_ doubles each,
+ * (re,im) of mode (kx,ky,kz) at [i][j][2kz]); uk_dc[3]. */
+int kh_set_fluid(kh_run *run, const double *zeta0, const double *zeta1, const double uk_dc[3]);
+int kh_get_fluid(kh_run *run, double *zeta0, double *zeta1, double uk_dc[3]);
+int kh_set_time(kh_run *run, const CTime *jikan);
+int kh_get_time(kh_run *run, CTime *jikan);
+int kh_get_field(kh_run *run, int field, double *out);
+int kh_set_field(kh_run *run, int field, const double *in);
+
+/* -------- the step -------- */
+/* Fluid at rest plus the initial rigidity constraint (sp_3d_ns.cxx:809-826). */
+int kh_init_fluid(kh_run *run);
+/* nsteps calls of Time_evolution_hydro, each followed by ts += 1 and
+ * time += dt (sp_3d_ns.cxx:857, 954-955). Enqueued on the run's stream;
+ * returns once enqueued unless an error is detected. */
+int kh_step(kh_run *run, int nsteps);
+/* One function of the step on the device state (enum kh_fn). */
+int kh_apply(kh_run *run, int fn);
+/* Wait for the run's stream; reports errors raised by enqueued work. */
+int kh_sync(kh_run *run);
+
+/* -------- standalone transforms (A2a_k / A_k2a semantics) --------
+ * In place on a host array in the reference layout (NX*NY*NZ_ doubles).
+ * Forward: unnormalised sum a(x) exp(-i k.x). Inverse: normalised by
+ * 1/(NX*NY*NZ); the kz=0 and kz=NZ/2 planes are taken as Hermitian. */
+int kh_A2a_k(kh_run *run, double *a);
+int kh_A_k2a(kh_run *run, double *a);
+
+/* -------- measurement -------- */
+/* When enabled, every kernel launch of the run is bracketed by HIP events
+ * on the run's stream; kh_timing_get returns the launch count and the summed
+ * event time of kernel `name` since the last reset. */
+int kh_timing_enable(kh_run *run, int on);
+int kh_timing_reset(kh_run *run);
+int kh_timing_get(kh_run *run, const char *name, int *launches, double *total_ms);
+/* Number of distinct kernels timed and the name of the i-th. */
+int kh_timing_count(kh_run *run);
+const char *kh_timing_name(kh_run *run, int i);
+
+const char *kh_strerror(int status);
+const char *kh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KAPSEL_HIP_H */

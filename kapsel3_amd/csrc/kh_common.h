@@ -1,0 +1,70 @@
+// kh_common.h -- device-side basics shared by the SPM kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kh {
+
+// Complex value in registers/LDS/HBM: (re, im), 8- or 16-byte aligned pair.
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cmk(T a, T b) {
+    cpx<T> r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
+    return cmk(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
+    return cmk(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+    return cmk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cconj(cpx<T> a) {
+    return cmk(a.x, -a.y);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) {
+    return cmk(a.x * s, a.y * s);
+}
+// multiply by -i (FWD) or +i (inverse)
+template <typename T, bool FWD>
+__device__ __forceinline__ cpx<T> cmul_mi(cpx<T> a) {
+    return FWD ? cmk(a.y, -a.x) : cmk(-a.y, a.x);
+}
+
+// Grid geometry of one run, passed by value to kernels.
+struct Grid {
+    int    nx, ny, nz;       // real-space points
+    int    nzh;              // nz/2 + 1 complex modes along z
+    int    hnx, hny;         // nx/2, ny/2 (Calc_KX_Ooura: i > HNX -> i - NX)
+    int    trn_x, trn_y, trn_z;  // (N+2)/3: kept |kx| < trn_x, |ky| < trn_y, kz < trn_z
+    double wave_x, wave_y, wave_z;
+    long long nreal;         // nx*ny*nz
+    long long nspec;         // nx*ny*nzh
+};
+
+__device__ __forceinline__ int kx_of(const Grid &g, int i) { return (i > g.hnx) ? i - g.nx : i; }
+__device__ __forceinline__ int ky_of(const Grid &g, int j) { return (j > g.hny) ? j - g.ny : j; }
+__device__ __forceinline__ bool kept_mode(const Grid &g, int kx, int ky, int kz) {
+    return kz < g.trn_z && (ky < 0 ? -ky : ky) < g.trn_y && (kx < 0 ? -kx : kx) < g.trn_x;
+}
+
+}  // namespace kh

@@ -1,0 +1,44 @@
+// kh_launch.h -- host launchers of the SPM kernels (one stream argument each).
+#pragma once
+
+#include "kh_common.h"
+#include "kh_fft.h"
+#include "kh_particle.h"
+
+namespace kh {
+
+// kh_spectral.hip
+template <typename T>
+hipError_t zeta_to_u(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *u0, cpx<T> *u1, cpx<T> *u2,
+                     double *ukdc, int fixmask, int truncate, hipStream_t s);
+template <typename T>
+hipError_t products(long long n, const T *u0, const T *u1, const T *u2, T *const p[5], hipStream_t s);
+template <typename T>
+hipError_t adv_ns(const Grid &g, cpx<T> *const P[5], cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,
+                  hipStream_t s);
+template <typename T>
+hipError_t solenoidal(const Grid &g, cpx<T> *u0, cpx<T> *u1, cpx<T> *u2, hipStream_t s);
+template <typename T>
+hipError_t u_to_zeta(const Grid &g, const cpx<T> *u0, const cpx<T> *u1, const cpx<T> *u2, cpx<T> *z0, cpx<T> *z1,
+                     double *ukdc, hipStream_t s);
+template <typename T>
+hipError_t f_nonsole(long long n, T *const up[3], const T *phisum, T *const u[3], T *const f[3], hipStream_t s);
+template <typename T>
+hipError_t add3(long long n, T *const u[3], T *const f[3], hipStream_t s);
+template <typename T>
+hipError_t fill(long long n, T *p, T v, hipStream_t s);
+
+// kh_particle.hip
+template <typename T>
+hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, hipStream_t s);
+template <typename T>
+hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, double coef, hipStream_t s);
+template <typename T>
+hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, T *const up[3], hipStream_t s);
+hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s);
+hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s);
+hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s);
+hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, int ts, double niv,
+                      double nio, hipStream_t s);
+
+}  // namespace kh

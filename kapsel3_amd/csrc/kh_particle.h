@@ -1,0 +1,121 @@
+// kh_particle.h -- device particle state and the per-point geometry of the
+// smoothed profile (host + device).
+#pragma once
+
+#include "kh_common.h"
+
+namespace kh {
+
+// Device copy of the members of Particle (variable.h:27-77) the step uses.
+struct DevParticle {
+    double x[3], x_prev[3], x_nopbc[3];
+    double v[3], v_old[3], om[3], om_old[3];
+    double q[4], q_old[4];  // (s, v0, v1, v2)
+    double fr[3], fr_prev[3], tr[3], tr_prev[3];
+    double fh[3], fh_prev[3], th[3], th_prev[3];
+    double fs[3], fs_prev[3], ts[3], ts_prev[3];
+    double v_slip[3], om_slip[3];
+    double mass, mass_center[3], inertia[9];
+    double smass, smc[3], sinertia[9], sdv[3], sdw[3];
+    int    spec, pad_;
+};
+
+struct DevSpecies {
+    double mass_ratio, imass, imoi;
+    double janus_force[3], janus_torque[3];
+    double slip_vel, slip_mode_half, c1, c2;
+    int    propulsion, janus_axis;
+};
+
+// Everything the particle kernels need, passed by value.
+struct PParams {
+    int               nx, ny, nz;
+    double            dx, L[3];
+    double            radius, hxi, xi;
+    int               rotation;
+    int               np;
+    int               np_domain;
+    const int        *sek;  // packed Sekibun offsets
+    const DevSpecies *species;
+    int               lj_on, lj_powers, lj_cell_compat;
+    double            lj_dia, lj_cutoff, epsilon;
+    double            g;
+    int               g_dir;
+    double            gravity_on_fluid;
+    int               motor;  // SW_JANUS_MOTOR
+};
+
+__host__ __device__ inline int sek_pack(int i, int j, int k) { return (i + 128) | ((j + 128) << 8) | ((k + 128) << 16); }
+__device__ __forceinline__ void sek_unpack(int c, int *o) {
+    o[0] = (c & 255) - 128;
+    o[1] = ((c >> 8) & 255) - 128;
+    o[2] = ((c >> 16) & 255) - 128;
+}
+
+// Nint (macro.h:44-52)
+__device__ __forceinline__ int nint(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
+
+// Distance0 (interaction.h:113-123): x12 = PBC(x2 - x1)
+__device__ __forceinline__ double distance0(const PParams &pp, const double *x1, const double *x2, double *x12) {
+    double dmy = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        x12[d] = x2[d] - x1[d];
+        x12[d] -= (double)nint(x12[d] / pp.L[d]) * pp.L[d];
+        dmy += x12[d] * x12[d];
+    }
+    return sqrt(dmy);
+}
+
+// H, Phi (profile.h:30-50)
+__device__ __forceinline__ double prof_H(double x, double dx) { return x > 0 ? exp(-(dx / x) * (dx / x)) : 0.0; }
+__device__ __forceinline__ double prof_phi(const PParams &pp, double x) {
+    const double dmy = prof_H(pp.radius + pp.hxi - x, pp.dx);
+    return dmy / (dmy + prof_H(x - pp.radius + pp.hxi, pp.dx));
+}
+
+// Per-particle frame: Particle_cell (make_phi.h:114-139).
+struct PCell {
+    int    x_int[3];
+    double residue[3];
+};
+__device__ __forceinline__ PCell particle_cell(const PParams &pp, const double *xp) {
+    PCell        c;
+    const double idx = 1. / pp.dx;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double dmy = xp[d] * idx;
+        c.x_int[d]       = (int)dmy;
+        c.residue[d]     = (dmy - (double)c.x_int[d]) * pp.dx;
+    }
+    return c;
+}
+
+// One Sekibun point: Relative_coord (make_phi.h:154-170) plus the distance
+// Distance(r_mesh*DX, xp) that every particle loop of the reference uses.
+struct PPoint {
+    long long idx;  // real-space linear index (x, y, z)
+    double    r[3]; // cell*dx - residue
+    double    dist; // minimum-image |r_mesh*DX - xp|
+};
+__device__ __forceinline__ PPoint sek_point(const PParams &pp, const PCell &c, const double *xp, int code) {
+    int cell[3];
+    sek_unpack(code, cell);
+    const int N[3] = {pp.nx, pp.ny, pp.nz};
+    int       rm[3];
+    double    x[3], x12[3];
+    PPoint    q;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        // (x_int + cell + N) % N for valid positions; the double modulo keeps
+        // the index inside the grid even for a non-finite particle state
+        rm[d]  = ((c.x_int[d] + cell[d]) % N[d] + N[d]) % N[d];
+        q.r[d] = (double)cell[d] * pp.dx - c.residue[d];
+        x[d]   = rm[d] * pp.dx;
+    }
+    q.dist = distance0(pp, x, xp, x12);
+    q.idx  = ((long long)rm[0] * pp.ny + rm[1]) * pp.nz + rm[2];
+    return q;
+}
+
+}  // namespace kh

@@ -1,0 +1,822 @@
+// kh_run.hip -- the C ABI of include/kapsel_hip.h: one simulation's state
+// resident in HBM and the SPM step (Time_evolution_hydro,
+// sp_3d_ns.cxx:41-194) enqueued on the run's own stream.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/kapsel_hip.h"
+#include "kh_launch.h"
+
+using namespace kh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int status, const char *fmt, ...) {
+    char    buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+#define KH_HIP(call)                                                                                          \
+    do {                                                                                                      \
+        hipError_t e_ = (call);                                                                               \
+        if (e_ != hipSuccess) return fail(KH_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define KH_TRY(call)                 \
+    do {                             \
+        int s_ = (call);             \
+        if (s_ != KH_OK) return s_;  \
+    } while (0)
+
+constexpr double KH_PI     = 3.14159265358979323846;
+constexpr double kT_snap_v = 1.439;  // input.cxx:309
+constexpr double kT_snap_o = 1.22;   // input.cxx:311
+
+struct KernelTimes {
+    int    launches = 0;
+    double ms       = 0.0;
+};
+struct PendingEvent {
+    std::string name;
+    hipEvent_t  a, b;
+};
+
+}  // namespace
+
+struct kh_run {
+    kh_params               P;
+    std::vector<kh_species> species;
+    int                     device = 0;
+    hipStream_t             stream = nullptr;
+    Grid                    g;
+    PParams                 pp;
+    int                     mixed = 0;
+    size_t                  esz   = 8;
+    int                     np    = 0;
+    // derived constants (Set_global_parameters, input.cxx:424-600)
+    double radius, xi, hxi, rho, eta, nu, dx3, kbt, lj_cutoff, gravity_on_fluid;
+    std::vector<double> mass, imass, moi, imoi;
+    int    motor = 0, slip = 0, fixmask = 0;
+    // device buffers
+    void       *zeta[2]  = {nullptr, nullptr};
+    void       *spec[5]  = {};
+    void       *ureal[3] = {};
+    void       *up[3]    = {};
+    void       *work[5]  = {};
+    void       *phisum   = nullptr;
+    double     *ukdc     = nullptr;
+    DevParticle *dp      = nullptr;
+    DevSpecies  *ds      = nullptr;
+    int         *sek     = nullptr;
+    int          np_domain = 0;
+    CTime        jikan;
+    // measurement
+    bool                               timing = false;
+    std::map<std::string, KernelTimes> times;
+    std::vector<PendingEvent>          pending;
+    std::vector<hipEvent_t>            event_pool;
+    std::vector<std::string>           names;
+};
+
+namespace {
+
+// ---------------------------------------------------------------- timing
+hipEvent_t take_event(kh_run *r) {
+    if (!r->event_pool.empty()) {
+        hipEvent_t e = r->event_pool.back();
+        r->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int resolve_timing(kh_run *r) {
+    if (r->pending.empty()) return KH_OK;
+    KH_HIP(hipStreamSynchronize(r->stream));
+    for (auto &p : r->pending) {
+        float ms = 0.f;
+        KH_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        KernelTimes &kt = r->times[p.name];
+        kt.launches++;
+        kt.ms += ms;
+        r->event_pool.push_back(p.a);
+        r->event_pool.push_back(p.b);
+    }
+    r->pending.clear();
+    return KH_OK;
+}
+
+// Brackets one launch with events when timing is on.
+struct Timed {
+    kh_run     *r;
+    const char *name;
+    hipEvent_t  a = nullptr;
+    Timed(kh_run *r_, const char *n) : r(r_), name(n) {
+        if (r->timing) {
+            a = take_event(r);
+            if (a) (void)hipEventRecord(a, r->stream);
+        }
+    }
+    ~Timed() {
+        if (r->timing && a) {
+            hipEvent_t b = take_event(r);
+            if (b) {
+                (void)hipEventRecord(b, r->stream);
+                r->pending.push_back({name, a, b});
+            }
+        }
+    }
+};
+#define KH_LAUNCH(name, call)        \
+    do {                             \
+        Timed t_(r, name);           \
+        KH_HIP(call);                \
+    } while (0)
+
+// ------------------------------------------------------------ the step
+template <typename T>
+struct Step {
+    static cpx<T> *S(kh_run *r, int i) { return (cpx<T> *)r->spec[i]; }
+    static cpx<T> *Z(kh_run *r, int i) { return (cpx<T> *)r->zeta[i]; }
+    static T      *U(kh_run *r, int i) { return (T *)r->ureal[i]; }
+    static T      *UP(kh_run *r, int i) { return (T *)r->up[i]; }
+    static T      *W(kh_run *r, int i) { return (T *)r->work[i]; }
+    static T      *PS(kh_run *r) { return (T *)r->phisum; }
+
+    static int r2c(kh_run *r, const T *in, cpx<T> *out) {
+        KH_LAUNCH("fft_z_r2c", (fft_z_r2c<T>(r->g, in, out, r->stream)));
+        KH_LAUNCH("fft_y_fwd", (fft_y<T, true>(r->g, out, r->stream)));
+        KH_LAUNCH("fft_x_fwd", (fft_x<T, true>(r->g, out, r->stream)));
+        return KH_OK;
+    }
+    static int c2r(kh_run *r, cpx<T> *in, T *out) {
+        KH_LAUNCH("fft_x_inv", (fft_x<T, false>(r->g, in, r->stream)));
+        KH_LAUNCH("fft_y_inv", (fft_y<T, false>(r->g, in, r->stream)));
+        KH_LAUNCH("fft_z_c2r", (fft_z_c2r<T>(r->g, in, out, r->stream)));
+        return KH_OK;
+    }
+    static int zero(kh_run *r, T *p) {
+        KH_LAUNCH("fill", (fill<T>(r->g.nreal, p, (T)0, r->stream)));
+        return KH_OK;
+    }
+
+    // Zeta_k2u (operate_omega.h:134-137); in the step it is always preceded
+    // by the 2/3 truncation of zeta (truncate = 1) and, after the NS solver,
+    // by the FIX_CELL zeroing of uk_dc (fixmask)
+    static int zeta_k2u(kh_run *r, int fixmask, int truncate = 1) {
+        KH_LAUNCH("zeta_to_u", (zeta_to_u<T>(r->g, Z(r, 0), Z(r, 1), S(r, 0), S(r, 1), S(r, 2), r->ukdc, fixmask,
+                                             truncate, r->stream)));
+        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, S(r, d), U(r, d)));
+        return KH_OK;
+    }
+
+    // NS_solver_slavedEuler (fluid_solver.cxx:348-374) via Zeta_k2advection_k
+    static int ns_solver(kh_run *r) {
+        KH_TRY(zeta_k2u(r, 0));
+        T *p[5] = {W(r, 0), W(r, 1), W(r, 2), W(r, 3), W(r, 4)};
+        KH_LAUNCH("products", (products<T>(r->g.nreal, U(r, 0), U(r, 1), U(r, 2), p, r->stream)));
+        for (int c = 0; c < 5; c++) KH_TRY(r2c(r, W(r, c), S(r, c)));
+        cpx<T> *P5[5] = {S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)};
+        KH_LAUNCH("adv_ns", (adv_ns<T>(r->g, P5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu,
+                                       r->stream)));
+        return KH_OK;
+    }
+
+    // U2zeta_k (operate_omega.h:208-211)
+    static int u2zeta_k(kh_run *r) {
+        for (int d = 0; d < 3; d++) KH_TRY(r2c(r, U(r, d), S(r, d)));
+        KH_LAUNCH("u_to_zeta", (u_to_zeta<T>(r->g, S(r, 0), S(r, 1), S(r, 2), Z(r, 0), Z(r, 1), r->ukdc, r->stream)));
+        return KH_OK;
+    }
+
+    static int make_phi(kh_run *r) {
+        KH_TRY(zero(r, PS(r)));
+        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), r->stream)));
+        return KH_OK;
+    }
+    static int calc_f_hydro(kh_run *r) {
+        T *u[3] = {U(r, 0), U(r, 1), U(r, 2)};
+        KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, u, PS(r), (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->stream)));
+        return KH_OK;
+    }
+    static int velocity_update(kh_run *r, int ab2) {
+        KH_LAUNCH("forces", forces(r->pp, r->dp, r->stream));
+        KH_LAUNCH("velocity", velocity(r->pp, r->dp, ab2, ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->stream));
+        return KH_OK;
+    }
+    static int thermo(kh_run *r) {
+        // fluct.cxx:11-18
+        const double zeta_drag = 6. * KH_PI * r->eta * r->radius;
+        const double zeta_rot  = 8. * KH_PI * r->eta * r->radius * r->radius * r->radius;
+        const double niv       = kT_snap_v * sqrt(zeta_drag * r->jikan.dt_md * r->kbt * r->P.alpha_v);
+        const double nio       = kT_snap_o * sqrt(zeta_rot * r->jikan.dt_md * r->kbt * r->P.alpha_o);
+        KH_LAUNCH("thermostat", thermostat(r->pp, r->dp, r->P.seed, r->P.stream_id, r->jikan.ts, niv, nio, r->stream));
+        return KH_OK;
+    }
+    static int make_up(kh_run *r) {
+        for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
+        T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), up3, r->stream)));
+        return KH_OK;
+    }
+    // Make_f_particle_dt_sole (f_particle.h:60-67) into W[0..2]
+    static int f_particle_sole(kh_run *r) {
+        T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u[3] = {U(r, 0), U(r, 1), U(r, 2)};
+        T *f[3]   = {W(r, 0), W(r, 1), W(r, 2)};
+        KH_LAUNCH("f_nonsole", (f_nonsole<T>(r->g.nreal, up3, PS(r), u, f, r->stream)));
+        for (int d = 0; d < 3; d++) KH_TRY(r2c(r, W(r, d), S(r, d)));
+        KH_LAUNCH("solenoidal", (solenoidal<T>(r->g, S(r, 0), S(r, 1), S(r, 2), r->stream)));
+        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, S(r, d), W(r, d)));
+        return KH_OK;
+    }
+    static int add_f(kh_run *r) {
+        T *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *f[3] = {W(r, 0), W(r, 1), W(r, 2)};
+        KH_LAUNCH("add3", (add3<T>(r->g.nreal, u, f, r->stream)));
+        return KH_OK;
+    }
+
+    // Time_evolution_hydro (sp_3d_ns.cxx:41-194), Navier_Stokes, non-slip
+    static int step(kh_run *r) {
+        const int ab2 = r->jikan.ts != 0;
+        KH_TRY(ns_solver(r));
+        KH_TRY(zeta_k2u(r, r->fixmask));  // FIX_CELL, truncation, u*
+        KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
+        KH_TRY(make_phi(r));
+        KH_TRY(calc_f_hydro(r));
+        KH_TRY(velocity_update(r, ab2));
+        if (r->kbt > 0.) KH_TRY(thermo(r));
+        KH_TRY(make_up(r));
+        KH_TRY(f_particle_sole(r));
+        KH_TRY(add_f(r));
+        KH_TRY(u2zeta_k(r));
+        return KH_OK;
+    }
+
+    // Init_zeta_k on a fluid at rest + sp_3d_ns.cxx:809-826
+    static int init_fluid(kh_run *r) {
+        KH_HIP(hipMemsetAsync(r->zeta[0], 0, sizeof(cpx<T>) * r->g.nspec, r->stream));
+        KH_HIP(hipMemsetAsync(r->zeta[1], 0, sizeof(cpx<T>) * r->g.nspec, r->stream));
+        KH_HIP(hipMemsetAsync(r->ukdc, 0, sizeof(double) * 3, r->stream));
+        KH_TRY(make_phi(r));
+        KH_TRY(make_up(r));
+        KH_TRY(zeta_k2u(r, 0));
+        KH_TRY(f_particle_sole(r));
+        KH_TRY(add_f(r));
+        KH_TRY(u2zeta_k(r));
+        KH_HIP(hipMemsetAsync(r->ukdc, 0, sizeof(double) * 3, r->stream));
+        return KH_OK;
+    }
+
+    static int apply(kh_run *r, int fn) {
+        switch (fn) {
+            case KH_FN_NS_SOLVER: return ns_solver(r);
+            case KH_FN_ZETA_K2U: return zeta_k2u(r, 0, 0);
+            case KH_FN_U2ZETA_K: return u2zeta_k(r);
+            case KH_FN_POSITION_EULER:
+                KH_LAUNCH("position", position(r->pp, r->dp, 0, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
+                return KH_OK;
+            case KH_FN_POSITION_AB2:
+                KH_LAUNCH("position", position(r->pp, r->dp, 1, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
+                return KH_OK;
+            case KH_FN_MAKE_PHI: return make_phi(r);
+            case KH_FN_F_HYDRO: return calc_f_hydro(r);
+            case KH_FN_VELOCITY_EULER: return velocity_update(r, 0);
+            case KH_FN_VELOCITY_AB2: return velocity_update(r, 1);
+            case KH_FN_THERMOSTAT: return thermo(r);
+            case KH_FN_MAKE_UP: return make_up(r);
+            case KH_FN_F_PARTICLE_SOLE: return f_particle_sole(r);
+            case KH_FN_ADD_F_PARTICLE: return add_f(r);
+            default: return fail(KH_ERR_ARG, "kh_apply: unknown function %d", fn);
+        }
+    }
+};
+
+// ------------------------------------------------------------ transfers
+// host reference layout [NX][NY][NZ_] (real in k < NZ) <-> device [NX][NY][NZ]
+int upload_real(kh_run *r, void *dev, const double *host) {
+    const Grid &g   = r->g;
+    const int   nz_ = 2 * g.nzh;
+    if (!r->mixed) {
+        KH_HIP(hipMemcpy2DAsync(dev, sizeof(double) * g.nz, host, sizeof(double) * nz_, sizeof(double) * g.nz,
+                                (size_t)g.nx * g.ny, hipMemcpyHostToDevice, r->stream));
+        KH_HIP(hipStreamSynchronize(r->stream));
+    } else {
+        std::vector<float> tmp((size_t)g.nreal);
+        for (long long row = 0; row < (long long)g.nx * g.ny; row++)
+            for (int k = 0; k < g.nz; k++) tmp[(size_t)row * g.nz + k] = (float)host[(size_t)row * nz_ + k];
+        KH_HIP(hipMemcpyAsync(dev, tmp.data(), sizeof(float) * tmp.size(), hipMemcpyHostToDevice, r->stream));
+        KH_HIP(hipStreamSynchronize(r->stream));
+    }
+    return KH_OK;
+}
+int download_real(kh_run *r, double *host, const void *dev) {
+    const Grid &g   = r->g;
+    const int   nz_ = 2 * g.nzh;
+    if (!r->mixed) {
+        KH_HIP(hipMemcpy2DAsync(host, sizeof(double) * nz_, dev, sizeof(double) * g.nz, sizeof(double) * g.nz,
+                                (size_t)g.nx * g.ny, hipMemcpyDeviceToHost, r->stream));
+        KH_HIP(hipStreamSynchronize(r->stream));
+    } else {
+        std::vector<float> tmp((size_t)g.nreal);
+        KH_HIP(hipMemcpyAsync(tmp.data(), dev, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost, r->stream));
+        KH_HIP(hipStreamSynchronize(r->stream));
+        for (long long row = 0; row < (long long)g.nx * g.ny; row++)
+            for (int k = 0; k < g.nz; k++) host[(size_t)row * nz_ + k] = tmp[(size_t)row * g.nz + k];
+    }
+    for (long long row = 0; row < (long long)g.nx * g.ny; row++)
+        for (int k = g.nz; k < nz_; k++) host[(size_t)row * nz_ + k] = 0.0;
+    return KH_OK;
+}
+// spectral: identical layouts; FP32 runs convert
+int upload_spec(kh_run *r, void *dev, const double *host) {
+    const size_t n = (size_t)r->g.nspec * 2;
+    if (!r->mixed) {
+        KH_HIP(hipMemcpyAsync(dev, host, sizeof(double) * n, hipMemcpyHostToDevice, r->stream));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; i++) tmp[i] = (float)host[i];
+        KH_HIP(hipMemcpyAsync(dev, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, r->stream));
+    }
+    KH_HIP(hipStreamSynchronize(r->stream));
+    return KH_OK;
+}
+int download_spec(kh_run *r, double *host, const void *dev) {
+    const size_t n = (size_t)r->g.nspec * 2;
+    if (!r->mixed) {
+        KH_HIP(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost, r->stream));
+        KH_HIP(hipStreamSynchronize(r->stream));
+    } else {
+        std::vector<float> tmp(n);
+        KH_HIP(hipMemcpyAsync(tmp.data(), dev, sizeof(float) * n, hipMemcpyDeviceToHost, r->stream));
+        KH_HIP(hipStreamSynchronize(r->stream));
+        for (size_t i = 0; i < n; i++) host[i] = tmp[i];
+    }
+    return KH_OK;
+}
+
+void *field_buffer(kh_run *r, int field) {
+    switch (field) {
+        case KH_FIELD_U0: case KH_FIELD_U1: case KH_FIELD_U2: return r->ureal[field - KH_FIELD_U0];
+        case KH_FIELD_UP0: case KH_FIELD_UP1: case KH_FIELD_UP2: return r->up[field - KH_FIELD_UP0];
+        case KH_FIELD_PHI_SUM: return r->phisum;
+        case KH_FIELD_F0: case KH_FIELD_F1: case KH_FIELD_F2: return r->work[field - KH_FIELD_F0];
+        default: return nullptr;
+    }
+}
+
+int free_run(kh_run *r) {
+    if (!r) return KH_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
+                    r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
+                    r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto &p : r->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : r->event_pool) (void)hipEventDestroy(e);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return KH_OK;
+}
+
+int check_params(const kh_params *P, int np) {
+    if (!P) return fail(KH_ERR_ARG, "params is NULL");
+    const int n[3] = {P->nx, P->ny, P->nz};
+    for (int d = 0; d < 3; d++)
+        if (!fft_length_supported(n[d]))
+            return fail(KH_ERR_UNSUPPORTED, "grid size %d is not a supported FFT length (2^a 3^b, 8..512)", n[d]);
+    if (!(P->dx > 0) || !(P->rho > 0) || !(P->eta > 0) || !(P->dt > 0) || !(P->a > 0) || !(P->a_xi > 0))
+        return fail(KH_ERR_ARG, "dx, rho, eta, dt, a, a_xi must be positive");
+    if (P->g_dir < 0 || P->g_dir > 2) return fail(KH_ERR_ARG, "g_dir must be 0, 1 or 2");
+    if (P->nspecies < 1 || !P->species) return fail(KH_ERR_ARG, "at least one species is required");
+    if (P->precision != KH_FP64 && P->precision != KH_MIXED) return fail(KH_ERR_ARG, "precision must be KH_FP64 or KH_MIXED");
+    if (np < 0) return fail(KH_ERR_ARG, "negative particle count");
+    if (P->lj_powers < 0 || P->lj_powers > 2) return fail(KH_ERR_UNSUPPORTED, "LJ_powers must be 12:6, 24:12 or 36:18");
+    int nh = (int)P->a + (int)P->a_xi + 2;  // Particle_domain half-width (profile.cxx:34)
+    if (nh > 100) return fail(KH_ERR_UNSUPPORTED, "particle radius too large for the Sekibun cell encoding");
+    if (np > 0 && (nh >= P->nx || nh >= P->ny || nh >= P->nz))
+        return fail(KH_ERR_ARG, "particle domain (half-width %d) does not fit the grid", nh);
+    for (int i = 0; i < P->nspecies; i++) {
+        if (P->species[i].propulsion == KH_JP_SLIP)
+            return fail(KH_ERR_UNSUPPORTED, "squirmer slip is not available in this build");
+        if (!(P->species[i].mass_ratio > 0)) return fail(KH_ERR_ARG, "mass_ratio must be positive");
+    }
+    return KH_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" {
+
+const char *kh_last_error(void) { return g_last_error.c_str(); }
+
+const char *kh_strerror(int s) {
+    switch (s) {
+        case KH_OK: return "ok";
+        case KH_ERR_ARG: return "invalid argument";
+        case KH_ERR_HIP: return "HIP runtime error";
+        case KH_ERR_NOMEM: return "device allocation failed";
+        case KH_ERR_UNSUPPORTED: return "unsupported";
+        case KH_ERR_STATE: return "invalid state";
+        case KH_ERR_NONFINITE: return "non-finite particle state";
+        default: return "unknown status";
+    }
+}
+
+int kh_run_create(const kh_params *P, int np, kh_run **out) {
+    if (!out) return fail(KH_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    KH_TRY(check_params(P, np));
+    kh_run *r = new kh_run();
+    r->P      = *P;
+    r->species.assign(P->species, P->species + P->nspecies);
+    r->P.species = r->species.data();
+    r->np        = np;
+    r->mixed     = P->precision == KH_MIXED;
+    r->esz       = r->mixed ? 4 : 8;
+    int st       = KH_OK;
+    auto bail    = [&](int s) {
+        free_run(r);
+        return s;
+    };
+    if (hipGetDevice(&r->device) != hipSuccess) return bail(fail(KH_ERR_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(KH_ERR_HIP, "hipStreamCreate failed"));
+
+    // Set_global_parameters (input.cxx:424-600)
+    Grid &g = r->g;
+    g.nx = P->nx;
+    g.ny = P->ny;
+    g.nz = P->nz;
+    g.nzh = P->nz / 2 + 1;
+    g.hnx = P->nx / 2;
+    g.hny = P->ny / 2;
+    g.trn_x = (P->nx + 2) / 3;
+    g.trn_y = (P->ny + 2) / 3;
+    g.trn_z = (P->nz + 2) / 3;
+    g.wave_x = 2. * KH_PI / (P->nx * P->dx);
+    g.wave_y = 2. * KH_PI / (P->ny * P->dx);
+    g.wave_z = 2. * KH_PI / (P->nz * P->dx);
+    g.nreal  = (long long)g.nx * g.ny * g.nz;
+    g.nspec  = (long long)g.nx * g.ny * g.nzh;
+    r->radius = P->a * P->dx;
+    r->xi     = P->a_xi * P->dx;
+    r->hxi    = 0.5 * r->xi;
+    r->rho    = P->rho;
+    r->eta    = P->eta;
+    r->nu     = P->eta / P->rho;
+    r->dx3    = P->dx * P->dx * P->dx;
+    r->kbt    = P->kbt;
+    const double sigma = 2.0 * r->radius;
+    double       a_r_cutoff;
+    if (P->lj_truncate > 0)
+        a_r_cutoff = (P->lj_powers == 0) ? pow(2., 1. / 6.) : (P->lj_powers == 1) ? pow(2., 1. / 12.) : pow(2., 1. / 18.);
+    else if (P->lj_truncate == 0) {
+        int nmin   = P->nx < P->ny ? P->nx : P->ny;
+        nmin       = nmin < P->nz ? nmin : P->nz;
+        a_r_cutoff = fmin(nmin * P->dx * .5 / sigma, 2.5);
+    } else
+        a_r_cutoff = 0.;
+    r->lj_cutoff        = a_r_cutoff * sigma;
+    r->gravity_on_fluid = P->g * P->rho * 4. / 3. * KH_PI * (r->radius * r->radius) * r->radius;
+    r->fixmask          = (P->fix_cell[0] ? 1 : 0) | (P->fix_cell[1] ? 2 : 0) | (P->fix_cell[2] ? 4 : 0);
+    std::vector<DevSpecies> hs(P->nspecies);
+    for (int i = 0; i < P->nspecies; i++) {
+        const kh_species &s     = P->species[i];
+        const double      m     = s.mass_ratio * P->rho * 4. / 3. * KH_PI * (r->radius * r->radius * r->radius);
+        DevSpecies       &d     = hs[i];
+        d.mass_ratio            = s.mass_ratio;
+        d.imass                 = 1. / m;
+        d.imoi                  = 1. / (2. / 5. * m * (r->radius * r->radius));
+        for (int k = 0; k < 3; k++) {
+            d.janus_force[k]  = s.janus_force[k];
+            d.janus_torque[k] = s.janus_torque[k];
+        }
+        d.slip_vel       = s.slip_vel;
+        d.slip_mode_half = s.slip_mode / 2.0;
+        d.c1             = s.rotlet_c1;
+        d.c2             = s.rotlet_c2;
+        d.propulsion     = s.propulsion;
+        d.janus_axis     = s.janus_axis;
+        if (s.propulsion == KH_JP_MOTOR) r->motor = 1;
+        if (s.propulsion == KH_JP_SLIP) r->slip = 1;
+    }
+    // Particle_domain (profile.cxx:22-65)
+    std::vector<int> sek;
+    {
+        const int    nh         = (int)P->a + (int)P->a_xi + 2;
+        const double max_radius = r->radius + r->hxi + 1.7321 * P->dx;
+        for (int i = -nh; i <= nh; i++)
+            for (int j = -nh; j <= nh; j++)
+                for (int k = -nh; k <= nh; k++) {
+                    double dmy = sqrt((i * P->dx) * (i * P->dx) + (j * P->dx) * (j * P->dx) + (k * P->dx) * (k * P->dx));
+                    if (dmy < max_radius) sek.push_back(sek_pack(i, j, k));
+                }
+        r->np_domain = (int)sek.size();
+    }
+
+    // device memory
+    const size_t rb = r->esz * (size_t)g.nreal, sb = 2 * r->esz * (size_t)g.nspec;
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            return fail(KH_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+        }
+        return KH_OK;
+    };
+    for (int i = 0; i < 2 && st == KH_OK; i++) st = alloc(&r->zeta[i], sb);
+    for (int i = 0; i < 5 && st == KH_OK; i++) st = alloc(&r->spec[i], sb);
+    for (int i = 0; i < 3 && st == KH_OK; i++) st = alloc(&r->ureal[i], rb);
+    for (int i = 0; i < 3 && st == KH_OK; i++) st = alloc(&r->up[i], rb);
+    for (int i = 0; i < 5 && st == KH_OK; i++) st = alloc(&r->work[i], rb);
+    if (st == KH_OK) st = alloc(&r->phisum, rb);
+    if (st == KH_OK) st = alloc((void **)&r->ukdc, 3 * sizeof(double));
+    if (st == KH_OK) st = alloc((void **)&r->dp, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1));
+    if (st == KH_OK) st = alloc((void **)&r->ds, sizeof(DevSpecies) * hs.size());
+    if (st == KH_OK) st = alloc((void **)&r->sek, sizeof(int) * sek.size());
+    if (st != KH_OK) return bail(st);
+    if (hipMemcpy(r->ds, hs.data(), sizeof(DevSpecies) * hs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(r->sek, sek.data(), sizeof(int) * sek.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(r->ukdc, 0, 3 * sizeof(double)) != hipSuccess ||
+        hipMemset(r->dp, 0, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1)) != hipSuccess)
+        return bail(fail(KH_ERR_HIP, "initial upload failed"));
+    for (int i = 0; i < 2; i++)
+        if (hipMemset(r->zeta[i], 0, sb) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+    void *zr[] = {r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2],
+                  r->work[0], r->work[1], r->work[2], r->work[3], r->work[4], r->phisum};
+    for (void *b : zr)
+        if (hipMemset(b, 0, rb) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+
+    PParams &pp        = r->pp;
+    pp.nx              = g.nx;
+    pp.ny              = g.ny;
+    pp.nz              = g.nz;
+    pp.dx              = P->dx;
+    pp.L[0]            = g.nx * P->dx;
+    pp.L[1]            = g.ny * P->dx;
+    pp.L[2]            = g.nz * P->dx;
+    pp.radius          = r->radius;
+    pp.hxi             = r->hxi;
+    pp.xi              = r->xi;
+    pp.rotation        = P->rotation;
+    pp.np              = np;
+    pp.np_domain       = r->np_domain;
+    pp.sek             = r->sek;
+    pp.species         = r->ds;
+    pp.lj_on           = P->lj_truncate >= 0;
+    pp.lj_powers       = P->lj_powers;
+    pp.lj_cell_compat  = P->lj_cell_compat;
+    pp.lj_dia          = sigma;
+    pp.lj_cutoff       = r->lj_cutoff;
+    pp.epsilon         = P->epsilon;
+    pp.g               = P->g;
+    pp.g_dir           = P->g_dir;
+    pp.gravity_on_fluid = r->gravity_on_fluid;
+    pp.motor           = r->motor;
+
+    CTime t0  = {0, 0.0, P->dt, P->dt * 0.5, P->dt, P->dt * 0.5};  // sp_3d_ns.cxx:782
+    r->jikan  = t0;
+    *out      = r;
+    return KH_OK;
+}
+
+int kh_run_destroy(kh_run *r) { return free_run(r); }
+
+int kh_set_particles(kh_run *r, const Particle *p, int n) {
+    if (!r || (!p && n > 0)) return fail(KH_ERR_ARG, "NULL argument");
+    if (n != r->np) return fail(KH_ERR_ARG, "particle count %d != run's %d", n, r->np);
+    std::vector<DevParticle> h((size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) {
+        const Particle &s = p[i];
+        DevParticle    &d = h[i];
+        memset(&d, 0, sizeof d);
+        if (s.spec < 0 || s.spec >= r->P.nspecies) return fail(KH_ERR_ARG, "particle %d: bad species %d", i, s.spec);
+        const double L[3] = {r->pp.L[0], r->pp.L[1], r->pp.L[2]};
+        for (int k = 0; k < 3; k++) {
+            if (!(s.x[k] >= 0 && s.x[k] < L[k])) return fail(KH_ERR_ARG, "particle %d outside the box", i);
+            d.x[k] = s.x[k];
+            d.x_prev[k] = s.x_previous[k];
+            d.x_nopbc[k] = s.x_nopbc[k];
+            d.v[k] = s.v[k];
+            d.v_old[k] = s.v_old[k];
+            d.om[k] = s.omega[k];
+            d.om_old[k] = s.omega_old[k];
+            d.fr[k] = s.fr[k];
+            d.fr_prev[k] = s.fr_previous[k];
+            d.tr[k] = s.torque_r[k];
+            d.tr_prev[k] = s.torque_r_previous[k];
+            d.fh[k] = s.f_hydro[k];
+            d.fh_prev[k] = s.f_hydro_previous[k];
+            d.th[k] = s.torque_hydro[k];
+            d.th_prev[k] = s.torque_hydro_previous[k];
+            d.fs[k] = s.f_slip[k];
+            d.fs_prev[k] = s.f_slip_previous[k];
+            d.ts[k] = s.torque_slip[k];
+            d.ts_prev[k] = s.torque_slip_previous[k];
+            d.v_slip[k] = s.v_slip[k];
+            d.om_slip[k] = s.omega_slip[k];
+        }
+        d.q[0] = s.q.s;
+        d.q_old[0] = s.q_old.s;
+        for (int k = 0; k < 3; k++) {
+            d.q[1 + k] = s.q.v[k];
+            d.q_old[1 + k] = s.q_old.v[k];
+        }
+        d.spec = s.spec;
+    }
+    if (n > 0) KH_HIP(hipMemcpyAsync(r->dp, h.data(), sizeof(DevParticle) * n, hipMemcpyHostToDevice, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    return KH_OK;
+}
+
+int kh_get_particles(kh_run *r, Particle *p, int n) {
+    if (!r || (!p && n > 0)) return fail(KH_ERR_ARG, "NULL argument");
+    if (n != r->np) return fail(KH_ERR_ARG, "particle count %d != run's %d", n, r->np);
+    std::vector<DevParticle> h((size_t)(n > 0 ? n : 1));
+    if (n > 0) KH_HIP(hipMemcpyAsync(h.data(), r->dp, sizeof(DevParticle) * n, hipMemcpyDeviceToHost, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    for (int i = 0; i < n; i++) {
+        Particle          &s = p[i];
+        const DevParticle &d = h[i];
+        for (int k = 0; k < 3; k++) {
+            s.x[k] = d.x[k];
+            s.x_previous[k] = d.x_prev[k];
+            s.x_nopbc[k] = d.x_nopbc[k];
+            s.v[k] = d.v[k];
+            s.v_old[k] = d.v_old[k];
+            s.omega[k] = d.om[k];
+            s.omega_old[k] = d.om_old[k];
+            s.fr[k] = d.fr[k];
+            s.fr_previous[k] = d.fr_prev[k];
+            s.torque_r[k] = d.tr[k];
+            s.torque_r_previous[k] = d.tr_prev[k];
+            s.f_hydro[k] = d.fh[k];
+            s.f_hydro_previous[k] = d.fh_prev[k];
+            s.torque_hydro[k] = d.th[k];
+            s.torque_hydro_previous[k] = d.th_prev[k];
+            s.f_slip[k] = d.fs[k];
+            s.f_slip_previous[k] = d.fs_prev[k];
+            s.torque_slip[k] = d.ts[k];
+            s.torque_slip_previous[k] = d.ts_prev[k];
+            s.v_slip[k] = d.v_slip[k];
+            s.omega_slip[k] = d.om_slip[k];
+        }
+        s.q.s = d.q[0];
+        s.q_old.s = d.q_old[0];
+        for (int k = 0; k < 3; k++) {
+            s.q.v[k] = d.q[1 + k];
+            s.q_old.v[k] = d.q_old[1 + k];
+        }
+        s.spec = d.spec;
+    }
+    return KH_OK;
+}
+
+int kh_set_fluid(kh_run *r, const double *z0, const double *z1, const double uk_dc[3]) {
+    if (!r || !z0 || !z1 || !uk_dc) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(upload_spec(r, r->zeta[0], z0));
+    KH_TRY(upload_spec(r, r->zeta[1], z1));
+    KH_HIP(hipMemcpy(r->ukdc, uk_dc, 3 * sizeof(double), hipMemcpyHostToDevice));
+    return KH_OK;
+}
+
+int kh_get_fluid(kh_run *r, double *z0, double *z1, double uk_dc[3]) {
+    if (!r || !z0 || !z1 || !uk_dc) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(download_spec(r, z0, r->zeta[0]));
+    KH_TRY(download_spec(r, z1, r->zeta[1]));
+    KH_HIP(hipMemcpy(uk_dc, r->ukdc, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return KH_OK;
+}
+
+int kh_set_time(kh_run *r, const CTime *t) {
+    if (!r || !t) return fail(KH_ERR_ARG, "NULL argument");
+    r->jikan = *t;
+    return KH_OK;
+}
+int kh_get_time(kh_run *r, CTime *t) {
+    if (!r || !t) return fail(KH_ERR_ARG, "NULL argument");
+    *t = r->jikan;
+    return KH_OK;
+}
+
+int kh_get_field(kh_run *r, int field, double *out) {
+    if (!r || !out) return fail(KH_ERR_ARG, "NULL argument");
+    if (field == KH_FIELD_PHI) {  // phi = min(phi_sum, 1) (make_phi.cxx:248-259)
+        KH_TRY(download_real(r, out, r->phisum));
+        const size_t n = (size_t)r->g.nx * r->g.ny * 2 * r->g.nzh;
+        for (size_t i = 0; i < n; i++) out[i] = fmin(out[i], 1.0);
+        return KH_OK;
+    }
+    void *b = field_buffer(r, field);
+    if (!b) return fail(KH_ERR_ARG, "unknown field %d", field);
+    return download_real(r, out, b);
+}
+
+int kh_set_field(kh_run *r, int field, const double *in) {
+    if (!r || !in) return fail(KH_ERR_ARG, "NULL argument");
+    void *b = field_buffer(r, field);
+    if (!b) return fail(KH_ERR_ARG, "field %d cannot be written", field);
+    return upload_real(r, b, in);
+}
+
+int kh_init_fluid(kh_run *r) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    return r->mixed ? Step<float>::init_fluid(r) : Step<double>::init_fluid(r);
+}
+
+int kh_step(kh_run *r, int nsteps) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
+    for (int s = 0; s < nsteps; s++) {
+        KH_TRY(r->mixed ? Step<float>::step(r) : Step<double>::step(r));
+        r->jikan.time += r->jikan.dt_fluid;  // sp_3d_ns.cxx:955
+        r->jikan.ts++;
+    }
+    return KH_OK;
+}
+
+int kh_apply(kh_run *r, int fn) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    return r->mixed ? Step<float>::apply(r, fn) : Step<double>::apply(r, fn);
+}
+
+int kh_sync(kh_run *r) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    KH_HIP(hipStreamSynchronize(r->stream));
+    return KH_OK;
+}
+
+int kh_A2a_k(kh_run *r, double *a) {
+    if (!r || !a) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(upload_real(r, r->work[4], a));
+    if (r->mixed) {
+        KH_TRY(Step<float>::r2c(r, (float *)r->work[4], (cpx<float> *)r->spec[4]));
+    } else {
+        KH_TRY(Step<double>::r2c(r, (double *)r->work[4], (cpx<double> *)r->spec[4]));
+    }
+    return download_spec(r, a, r->spec[4]);
+}
+
+int kh_A_k2a(kh_run *r, double *a) {
+    if (!r || !a) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(upload_spec(r, r->spec[4], a));
+    if (r->mixed) {
+        KH_TRY(Step<float>::c2r(r, (cpx<float> *)r->spec[4], (float *)r->work[4]));
+    } else {
+        KH_TRY(Step<double>::c2r(r, (cpx<double> *)r->spec[4], (double *)r->work[4]));
+    }
+    return download_real(r, a, r->work[4]);
+}
+
+int kh_timing_enable(kh_run *r, int on) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    r->timing = on != 0;
+    return KH_OK;
+}
+int kh_timing_reset(kh_run *r) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    KH_TRY(resolve_timing(r));
+    r->times.clear();
+    return KH_OK;
+}
+int kh_timing_get(kh_run *r, const char *name, int *launches, double *total_ms) {
+    if (!r || !name) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(resolve_timing(r));
+    auto it = r->times.find(name);
+    if (launches) *launches = it == r->times.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == r->times.end() ? 0.0 : it->second.ms;
+    return KH_OK;
+}
+int kh_timing_count(kh_run *r) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    KH_TRY(resolve_timing(r));
+    r->names.clear();
+    for (auto &kv : r->times) r->names.push_back(kv.first);
+    return (int)r->names.size();
+}
+const char *kh_timing_name(kh_run *r, int i) {
+    if (!r || i < 0 || i >= (int)r->names.size()) return nullptr;
+    return r->names[i].c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+/*
+ * kref.h -- the oracle: a CPU FP64 restatement of KAPSEL's SPM step.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (kapsel3_amd/) links,
+ * loads or calls this code; only tests/, the smoke check and the CPU-baseline
+ * leg of bench.py use it, as the checker and as the timed CPU baseline.
+ *
+ * It works on the reference's own data layout: every field is NX*NY*NZ_
+ * doubles with NZ_ = 2*(NZ/2+1); real-space values live in k < NZ, spectral
+ * values are (re,im) pairs at [i][j][2kz]. Each function cites the reference
+ * file:line it restates. OpenMP parallelises the loops the reference
+ * parallelises, plus the FFT rows.
+ */
+#ifndef KREF_H
+#define KREF_H
+
+#include <stdint.h>
+
+#include "../include/kapsel_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kr_run kr_run;
+
+kr_run *kr_create(const kh_params *params, int nparticles);
+void    kr_destroy(kr_run *r);
+
+int  kr_set_particles(kr_run *r, const Particle *p, int n);
+int  kr_get_particles(kr_run *r, Particle *p, int n);
+void kr_set_fluid(kr_run *r, const double *zeta0, const double *zeta1, const double uk_dc[3]);
+void kr_get_fluid(kr_run *r, double *zeta0, double *zeta1, double uk_dc[3]);
+void kr_set_time(kr_run *r, const CTime *t);
+void kr_get_time(kr_run *r, CTime *t);
+int  kr_get_field(kr_run *r, int field, double *out);
+int  kr_set_field(kr_run *r, int field, const double *in);
+
+void kr_init_fluid(kr_run *r);
+void kr_step(kr_run *r, int nsteps);
+int  kr_apply(kr_run *r, int fn);
+
+/* A2a_k / A_k2a semantics on a reference-layout array of the run's size. */
+void kr_A2a_k(kr_run *r, double *a);
+void kr_A_k2a(kr_run *r, double *a);
+/* Standalone transforms of any supported size (own temporary plans). */
+int kr_fft_forward(int nx, int ny, int nz, double *a);
+int kr_fft_inverse(int nx, int ny, int nz, double *a);
+
+/* Smooth profile phi(r) (profile.h:30-50) and the Sekibun cell. */
+double kr_phi(double r, double radius, double dx, double xi);
+int    kr_sekibun_count(const kr_run *r);
+int    kr_sekibun_cell(const kr_run *r, int *out3);
+
+/* Philox4x32-10 block and the thermal-noise normals of one particle. */
+void kr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void kr_noise_normals(uint64_t seed, uint32_t stream, int particle, int ts, double out6[6]);
+
+int kr_threads(void);
+void kr_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,176 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker. The product never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from kapsel3_amd._abi import CTime, KhParams, PARTICLE_DTYPE, FIELD, FN, ptr
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.kr_create.restype = vp
+        L.kr_create.argtypes = [C.POINTER(KhParams), C.c_int]
+        L.kr_destroy.argtypes = [vp]
+        for name in ("kr_set_particles", "kr_get_particles"):
+            getattr(L, name).argtypes = [vp, vp, C.c_int]
+        for name in ("kr_set_fluid", "kr_get_fluid"):
+            getattr(L, name).argtypes = [vp, vp, vp, vp]
+        L.kr_set_time.argtypes = [vp, C.POINTER(CTime)]
+        L.kr_get_time.argtypes = [vp, C.POINTER(CTime)]
+        L.kr_get_field.argtypes = [vp, C.c_int, vp]
+        L.kr_set_field.argtypes = [vp, C.c_int, vp]
+        L.kr_init_fluid.argtypes = [vp]
+        L.kr_step.argtypes = [vp, C.c_int]
+        L.kr_apply.argtypes = [vp, C.c_int]
+        L.kr_A2a_k.argtypes = [vp, vp]
+        L.kr_A_k2a.argtypes = [vp, vp]
+        L.kr_fft_forward.argtypes = [C.c_int, C.c_int, C.c_int, vp]
+        L.kr_fft_inverse.argtypes = [C.c_int, C.c_int, C.c_int, vp]
+        L.kr_phi.restype = C.c_double
+        L.kr_phi.argtypes = [C.c_double] * 4
+        L.kr_sekibun_count.argtypes = [vp]
+        L.kr_sekibun_cell.argtypes = [vp, vp]
+        L.kr_philox4x32_10.argtypes = [vp, vp, vp]
+        L.kr_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, vp]
+        L.kr_set_threads.argtypes = [C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def fft_forward(a):
+    """A2a_k on a reference-layout array of shape (nx, ny, nz_)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    nx, ny, nz_ = a.shape
+    nz = nz_ - 2
+    if lib().kr_fft_forward(nx, ny, nz, a.ctypes.data) != 0:
+        raise ValueError("unsupported size")
+    return a
+
+
+def fft_inverse(a):
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    nx, ny, nz_ = a.shape
+    if lib().kr_fft_inverse(nx, ny, nz_ - 2, a.ctypes.data) != 0:
+        raise ValueError("unsupported size")
+    return a
+
+
+def phi(r, radius, dx, xi):
+    return lib().kr_phi(r, radius, dx, xi)
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().kr_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
+
+
+def noise_normals(seed, stream, particle, ts):
+    o = np.zeros(6)
+    lib().kr_noise_normals(seed, stream, particle, ts, o.ctypes.data)
+    return o
+
+
+def set_threads(n):
+    lib().kr_set_threads(int(n))
+
+
+class Run:
+    """One oracle simulation; mirrors kapsel3_amd.Run."""
+
+    def __init__(self, params, nparticles):
+        self.params = params
+        self.np = nparticles
+        self.h = lib().kr_create(C.byref(params.c), nparticles)
+        if not self.h:
+            raise ValueError("kr_create rejected the parameters")
+        self.shape = (params.c.nx, params.c.ny, params.nz_)
+
+    def close(self):
+        if self.h:
+            lib().kr_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_particles(self, p):
+        p = np.ascontiguousarray(p, dtype=PARTICLE_DTYPE)
+        if lib().kr_set_particles(self.h, p.ctypes.data, len(p)) != 0:
+            raise ValueError("bad particles")
+
+    def get_particles(self):
+        p = np.zeros(self.np, dtype=PARTICLE_DTYPE)
+        lib().kr_get_particles(self.h, p.ctypes.data, self.np)
+        return p
+
+    def set_fluid(self, z0, z1, ukdc):
+        z0 = np.ascontiguousarray(z0, dtype=np.float64)
+        z1 = np.ascontiguousarray(z1, dtype=np.float64)
+        uk = np.ascontiguousarray(ukdc, dtype=np.float64)
+        lib().kr_set_fluid(self.h, z0.ctypes.data, z1.ctypes.data, uk.ctypes.data)
+
+    def get_fluid(self):
+        z0 = np.zeros(self.shape)
+        z1 = np.zeros(self.shape)
+        uk = np.zeros(3)
+        lib().kr_get_fluid(self.h, z0.ctypes.data, z1.ctypes.data, uk.ctypes.data)
+        return z0, z1, uk
+
+    def get_time(self):
+        t = CTime()
+        lib().kr_get_time(self.h, C.byref(t))
+        return t
+
+    def set_time(self, t):
+        lib().kr_set_time(self.h, C.byref(t))
+
+    def get_field(self, name):
+        a = np.zeros(self.shape)
+        lib().kr_get_field(self.h, FIELD[name], a.ctypes.data)
+        return a
+
+    def set_field(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        lib().kr_set_field(self.h, FIELD[name], a.ctypes.data)
+
+    def init_fluid(self):
+        lib().kr_init_fluid(self.h)
+
+    def step(self, n=1):
+        lib().kr_step(self.h, n)
+
+    def apply(self, fn):
+        if lib().kr_apply(self.h, FN[fn]) != 0:
+            raise ValueError(fn)
+
+    def A2a_k(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).copy()
+        lib().kr_A2a_k(self.h, a.ctypes.data)
+        return a
+
+    def A_k2a(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).copy()
+        lib().kr_A_k2a(self.h, a.ctypes.data)
+        return a
+
+    def sekibun(self):
+        n = lib().kr_sekibun_count(self.h)
+        out = np.zeros((n, 3), dtype=np.int32)
+        lib().kr_sekibun_cell(self.h, out.ctypes.data)
+        return out
