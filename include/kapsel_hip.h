@@ -154,6 +154,12 @@ int kh_init_fluid(kh_run *run);
  * time += dt (sp_3d_ns.cxx:857, 954-955). Enqueued on the run's stream;
  * returns once enqueued unless an error is detected. */
 int kh_step(kh_run *run, int nsteps);
+/* Kernel sequence kh_step uses. KH_PATH_FUSED (default): the dealiasing-
+ * pruned, fused pipeline. KH_PATH_REFERENCE_ORDER: one kernel sequence per
+ * reference function, in the reference's order (20 full 3D FFTs per step).
+ * Both give the reference's step to round-off. */
+enum { KH_PATH_FUSED = 0, KH_PATH_REFERENCE_ORDER = 1 };
+int kh_set_step_path(kh_run *run, int path);
 /* One function of the step on the device state (enum kh_fn). */
 int kh_apply(kh_run *run, int fn);
 /* Wait for the run's stream; reports errors raised by enqueued work. */

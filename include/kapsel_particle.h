@@ -10,7 +10,9 @@
  * (x@8, v@80, f_hydro@152, omega@368, q@728); the library reads and writes
  * only the members the SPM step touches.
  *
- * Valid C99 and C++11.
+ * Valid C99 and C++11. When KAPSEL's own variable.h / quaternion.h have
+ * been included first (include guards VARIABLE_H / QUATERNION_H), their
+ * definitions are used and only the layout checks below apply.
  */
 #ifndef KAPSEL_PARTICLE_H
 #define KAPSEL_PARTICLE_H
@@ -21,6 +23,7 @@
 #define DIM 3
 #endif
 
+#ifndef VARIABLE_H
 typedef struct CTime {
     int    ts;        /* time step */
     double time;      /* time */
@@ -30,11 +33,16 @@ typedef struct CTime {
     double hdt_md;    /* dt/2 */
 } CTime;
 
+#endif /* VARIABLE_H */
+
+#ifndef QUATERNION_H
 typedef struct quaternion {
     double s;      /* scalar part */
     double v[DIM]; /* vector part */
 } quaternion;
+#endif /* QUATERNION_H */
 
+#ifndef VARIABLE_H
 typedef struct Particle {
     int    spec;
     double x[DIM];
@@ -85,6 +93,7 @@ typedef struct Particle {
     double surface_dv[DIM];
     double surface_dw[DIM];
 } Particle;
+#endif /* VARIABLE_H */
 
 #ifdef __cplusplus
 #define KH_STATIC_ASSERT(c, m) static_assert(c, m)

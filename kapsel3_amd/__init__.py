@@ -22,7 +22,7 @@ _LIB = None
 EXPORTS = [
     "kh_run_create", "kh_run_destroy", "kh_set_particles", "kh_get_particles", "kh_set_fluid", "kh_get_fluid",
     "kh_set_time", "kh_get_time", "kh_get_field", "kh_set_field", "kh_init_fluid", "kh_step", "kh_apply",
-    "kh_sync", "kh_A2a_k", "kh_A_k2a", "kh_timing_enable", "kh_timing_reset", "kh_timing_get",
+    "kh_sync", "kh_A2a_k", "kh_A_k2a", "kh_set_step_path", "kh_timing_enable", "kh_timing_reset", "kh_timing_get",
     "kh_timing_count", "kh_timing_name", "kh_strerror", "kh_last_error",
 ]
 
@@ -53,6 +53,7 @@ def lib():
         L.kh_step.argtypes = [vp, C.c_int]
         L.kh_apply.argtypes = [vp, C.c_int]
         L.kh_sync.argtypes = [vp]
+        L.kh_set_step_path.argtypes = [vp, C.c_int]
         L.kh_A2a_k.argtypes = [vp, vp]
         L.kh_A_k2a.argtypes = [vp, vp]
         L.kh_timing_enable.argtypes = [vp, C.c_int]
@@ -150,6 +151,10 @@ class Run:
 
     def apply(self, fn):
         _check(lib().kh_apply(self.h, FN[fn]))
+
+    def set_step_path(self, path):
+        """'fused' (default) or 'reference' (the reference's operation order)."""
+        _check(lib().kh_set_step_path(self.h, {"fused": 0, "reference": 1}[path]))
 
     def sync(self):
         _check(lib().kh_sync(self.h))

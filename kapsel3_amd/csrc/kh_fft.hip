@@ -71,16 +71,16 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_z_r2c(const T *__restrict__ in, 
 
 template <typename T, int L>
 __global__ __launch_bounds__(FFT_NT) void k_fft_z_c2r(const cpx<T> *__restrict__ in, T *__restrict__ out,
-                                                      long long nrows, T scale) {
+                                                      long long nrows, T scale, int kzvalid) {
     constexpr int P   = (FftTile<T>::ZELEM / L) > 0 ? FftTile<T>::ZELEM / L : 1;
     constexpr int NZH = L / 2 + 1;
     __shared__ cpx<T> s0[P * L], s1[P * (L + 2)], tw[L];
     fill_twiddles<T, L, FFT_NT>(tw);
     const long long row0 = (long long)blockIdx.x * (2 * P);
     const cpx<T>   *src  = in + row0 * NZH;
-    for (int e = threadIdx.x; e < 2 * P * NZH; e += FFT_NT) {
-        const int rl = e / NZH;
-        s1[e]        = (row0 + rl < nrows) ? src[e] : cmk((T)0, (T)0);
+    for (int e = threadIdx.x; e < 2 * P * NZH; e += FFT_NT) {  // modes kz >= kzvalid are zero
+        const int rl = e / NZH, k = e - rl * NZH;
+        s1[e]        = (row0 + rl < nrows && k < kzvalid) ? src[e] : cmk((T)0, (T)0);
     }
     __syncthreads();
     // two Hermitian rows A (even), B (odd) -> Z = A + iB over the full line;
@@ -166,13 +166,14 @@ static hipError_t launch_z_r2c(int L, const T *in, cpx<T> *out, long long nrows,
 }
 
 template <typename T>
-static hipError_t launch_z_c2r(int L, const cpx<T> *in, T *out, long long nrows, T scale, hipStream_t s) {
+static hipError_t launch_z_c2r(int L, const cpx<T> *in, T *out, long long nrows, T scale, int kzvalid,
+                               hipStream_t s) {
     switch (L) {
 #define KH_CASE(n)                                                                       \
     case n: {                                                                            \
         constexpr int P  = (FftTile<T>::ZELEM / n) > 0 ? FftTile<T>::ZELEM / n : 1;      \
         long long     nb = (nrows + 2 * P - 1) / (2 * P);                                \
-        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, nrows, scale); \
+        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, nrows, scale, kzvalid); \
         return hipGetLastError();                                                        \
     }
         KH_FOR_EACH_L(KH_CASE)
@@ -204,9 +205,9 @@ hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s) {
     return launch_z_r2c<T>(g.nz, in, out, (long long)g.nx * g.ny, s);
 }
 template <typename T>
-hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s) {
+hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s, int kzvalid) {
     const T scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
-    return launch_z_c2r<T>(g.nz, in, out, (long long)g.nx * g.ny, scale, s);
+    return launch_z_c2r<T>(g.nz, in, out, (long long)g.nx * g.ny, scale, kzvalid < 0 ? g.nzh : kzvalid, s);
 }
 // y: lines along y at stride nzh, columns kz, one outer block per x
 template <typename T, bool FWD>
@@ -238,7 +239,7 @@ hipError_t fft3d_c2r(const Grid &g, cpx<T> *spec_inout, T *real_out, hipStream_t
 
 #define KH_INST(T)                                                                   \
     template hipError_t fft_z_r2c<T>(const Grid &, const T *, cpx<T> *, hipStream_t); \
-    template hipError_t fft_z_c2r<T>(const Grid &, const cpx<T> *, T *, hipStream_t); \
+    template hipError_t fft_z_c2r<T>(const Grid &, const cpx<T> *, T *, hipStream_t, int); \
     template hipError_t fft_y<T, true>(const Grid &, cpx<T> *, hipStream_t);          \
     template hipError_t fft_y<T, false>(const Grid &, cpx<T> *, hipStream_t);         \
     template hipError_t fft_x<T, true>(const Grid &, cpx<T> *, hipStream_t);          \

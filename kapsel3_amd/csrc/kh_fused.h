@@ -1,0 +1,30 @@
+// kh_fused.h -- host launchers of the dealiasing-pruned fused passes
+// (kh_fused.hip).
+#pragma once
+
+#include "kh_common.h"
+
+namespace kh {
+
+// y passes on columns kz < TRN_Z; x passes on kept (ky,kz) columns
+template <typename T, bool FWD>
+hipError_t cols_y(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s);
+template <typename T, bool FWD>
+hipError_t cols_x(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s);
+// zeta -> u_hat (truncated) -> inverse x pass, 3 components
+template <typename T>
+hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const u[3], double *ukdc,
+                        int fixmask, hipStream_t s);
+// inverse z (3) -> products -> forward z (5)
+template <typename T>
+hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStream_t s);
+// f = up - min(phi_sum,1) u* -> forward z (3)
+template <typename T>
+hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s);
+// forward x (3) -> projection + u* + curl -> zeta, uk_dc
+template <typename T>
+hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, double *ukdc, hipStream_t s);
+template <typename T>
+hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s);
+
+}  // namespace kh

@@ -1,0 +1,594 @@
+// kh_fused.hip -- the dealiasing-pruned, fused FFT passes of the SPM step.
+//
+// Every spectral field the step transforms is 2/3-truncated on one side of
+// the transform (Truncate_two_third_rule, fft_wrapper.h:754-796): inverse
+// transforms start from truncated spectra, forward transforms feed consumers
+// that only read kept modes. So only the kept part of each pass is touched:
+//   y passes : columns kz < TRN_Z only; inverse passes read only kept ky
+//              rows (the rest are zero), forward passes write only kept ky
+//   x passes : columns (ky, kz) both kept only; forward passes write only
+//              kept kx
+//   z passes : read/write only kz < TRN_Z
+// Pointwise work between two passes on the same axis is fused into one
+// kernel: the real-space quadratic products sit between the inverse and the
+// forward z pass (k_z_adv), the construction of u_hat from zeta sits in
+// front of the inverse x pass (k_x_zeta2u_inv), and the projection, the
+// addition of u* and the curl sit behind the forward x pass (k_x_f_zeta).
+#include "kh_fft_engine.h"
+#include "kh_fused.h"
+
+namespace kh {
+
+constexpr int FU_NT = 256;
+
+template <typename T>
+struct ColTile;
+template <>
+struct ColTile<double> {
+    static constexpr int C = 8;  // 8 x 16 B = 128 B per line element
+};
+template <>
+struct ColTile<float> {
+    static constexpr int C = 16;  // 16 x 8 B = 128 B
+};
+
+template <typename T>
+struct ZTile;
+template <>
+struct ZTile<double> {
+    static constexpr int ELEM = 2048;  // complex points per z block
+};
+template <>
+struct ZTile<float> {
+    static constexpr int ELEM = 2048;
+};
+
+__device__ __forceinline__ bool kept1(int k, int trn) { return (k < 0 ? -k : k) < trn; }
+__device__ __forceinline__ int  nkept(int trn) { return 2 * trn - 1; }
+// o-th kept index of an axis of n points: 0..trn-1, then n-trn+1..n-1
+__device__ __forceinline__ int kept_index(int o, int n, int trn) { return o < trn ? o : n - nkept(trn) + o; }
+
+// ------------------------------------------------------------------------
+// Column passes. Y mode: outer block = x, line along y, columns kz < ncols.
+// X mode: outer block = kept ky row, line along x, columns kz < ncols.
+// in_mask: rows whose wavenumber is truncated are taken as zero (not read).
+// out_mask: such rows are not written.
+template <typename T, int L, bool FWD, bool XMODE>
+__global__ __launch_bounds__(FU_NT) void k_cols(cpx<T> *__restrict__ data, Grid g, int ncols, int in_mask,
+                                                int out_mask) {
+    constexpr int C  = ColTile<T>::C;
+    constexpr int LS = L + 1;
+    __shared__ cpx<T> s0[C * LS], s1[C * LS], tw[L];
+    fill_twiddles<T, L, FU_NT>(tw);
+    const int ntiles = (ncols + C - 1) / C;
+    const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
+    long long base, lstride;
+    int       trn, nline;
+    if (XMODE) {
+        const int j = kept_index(o, g.ny, g.trn_y);
+        base        = (long long)j * g.nzh + kz0;
+        lstride     = (long long)g.ny * g.nzh;
+        trn         = g.trn_x;
+        nline       = g.nx;
+    } else {
+        base    = (long long)o * g.ny * g.nzh + kz0;
+        lstride = g.nzh;
+        trn     = g.trn_y;
+        nline   = g.ny;
+    }
+    for (int e = threadIdx.x; e < C * L; e += FU_NT) {
+        const int n = e / C, c = e - n * C;
+        const int k = (n > nline / 2) ? n - nline : n;
+        cpx<T>    v = cmk((T)0, (T)0);
+        if (kz0 + c < ncols && (!in_mask || kept1(k, trn))) v = data[base + n * lstride + c];
+        s0[c * LS + n] = v;
+    }
+    __syncthreads();
+    cpx<T> *res = fft_lines<T, L, C, LS, FU_NT, FWD>(s0, s1, tw);
+    for (int e = threadIdx.x; e < C * L; e += FU_NT) {
+        const int n = e / C, c = e - n * C;
+        const int k = (n > nline / 2) ? n - nline : n;
+        if (kz0 + c < ncols && (!out_mask || kept1(k, trn))) data[base + n * lstride + c] = res[c * LS + n];
+    }
+}
+
+// ------------------------------------------------------------------------
+// u_hat from zeta (Zeta_k2u_k, fft_wrapper.cxx:591-688, on the truncated
+// zeta) for one mode; (0,0,0) gives uk_dc.
+template <typename T>
+__device__ __forceinline__ void zeta_to_uhat(const Grid &g, int ikx, int iky, int kz, cpx<T> d1, cpx<T> d2,
+                                             const double *uk, cpx<T> *u) {
+    if (ikx == 0 && iky == 0 && kz == 0) {
+        u[0] = cmk((T)uk[0], (T)0);
+        u[1] = cmk((T)uk[1], (T)0);
+        u[2] = cmk((T)uk[2], (T)0);
+        return;
+    }
+    const double kx = g.wave_x * ikx, ky = g.wave_y * iky, kz_ = g.wave_z * kz;
+    const double k2 = kx * kx + ky * ky + kz_ * kz_;
+    const double ik2 = (k2 > 0.0) ? 1. / k2 : 0.0;
+    double       o0r, o0i, o1r, o1i, o2r, o2i;
+    if (ikx != 0) {
+        o0r = -(1. / kx) * (ky * (double)d1.x + kz_ * (double)d2.x);
+        o0i = -(1. / kx) * (ky * (double)d1.y + kz_ * (double)d2.y);
+        o1r = d1.x;
+        o1i = d1.y;
+        o2r = d2.x;
+        o2i = d2.y;
+    } else if (iky != 0) {
+        const double dmy = -(kz_ / ky);
+        o0r = d2.x;
+        o0i = d2.y;
+        o1r = dmy * (double)d1.x;
+        o1i = dmy * (double)d1.y;
+        o2r = d1.x;
+        o2i = d1.y;
+    } else {
+        o0r = d1.x;
+        o0i = d1.y;
+        o1r = d2.x;
+        o1i = d2.y;
+        o2r = 0.0;
+        o2i = 0.0;
+    }
+    const double qx = kx * ik2, qy = ky * ik2, qz = kz_ * ik2;
+    u[0] = cmk((T)(-(qy * o2i - qz * o1i)), (T)(qy * o2r - qz * o1r));
+    u[1] = cmk((T)(-(qz * o0i - qx * o2i)), (T)(qz * o0r - qx * o2r));
+    u[2] = cmk((T)(-(qx * o1i - qy * o0i)), (T)(qx * o1r - qy * o0r));
+}
+
+// zeta -> u_hat (3 components) on the kept (ky,kz) columns, then the
+// inverse x pass of each component. fixmask: FIX_CELL zeroing of uk_dc
+// (sp_3d_ns.cxx:46-52), done by the block holding (ky,kz) = (0,0).
+template <typename T, int L>
+__global__ __launch_bounds__(FU_NT) void k_x_zeta2u_inv(Grid g, const cpx<T> *__restrict__ z0,
+                                                        const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ u0,
+                                                        cpx<T> *__restrict__ u1, cpx<T> *__restrict__ u2,
+                                                        double *__restrict__ ukdc, int fixmask) {
+    constexpr int C   = ColTile<T>::C;
+    constexpr int LS  = L + 1;
+    constexpr int NPT = (C * L + FU_NT - 1) / FU_NT;
+    __shared__ cpx<T> s0[C * LS], s1[C * LS], tw[L];
+    __shared__ double uk[3];
+    fill_twiddles<T, L, FU_NT>(tw);
+    const int ntiles = (g.trn_z + C - 1) / C;
+    const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
+    const int j = kept_index(o, g.ny, g.trn_y), iky = (j > g.hny) ? j - g.ny : j;
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; d++) uk[d] = (fixmask & (1 << d)) ? 0.0 : ukdc[d];
+    }
+    __syncthreads();
+    if (fixmask && j == 0 && kz0 == 0 && threadIdx.x == 0)
+        for (int d = 0; d < 3; d++)
+            if (fixmask & (1 << d)) ukdc[d] = 0.0;
+    const long long lstride = (long long)g.ny * g.nzh;
+    const long long base    = (long long)j * g.nzh + kz0;
+    cpx<T>          U[3][NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int p = threadIdx.x + i * FU_NT;
+        const int n = p / C, c = p - n * C, kz = kz0 + c;
+        const int ikx = (n > g.hnx) ? n - g.nx : n;
+        U[0][i] = U[1][i] = U[2][i] = cmk((T)0, (T)0);
+        if (p < C * L && kz < g.trn_z && kept1(ikx, g.trn_x)) {
+            cpx<T> u[3];
+            zeta_to_uhat<T>(g, ikx, iky, kz, z0[base + n * lstride + c], z1[base + n * lstride + c], uk, u);
+            U[0][i] = u[0];
+            U[1][i] = u[1];
+            U[2][i] = u[2];
+        }
+    }
+    cpx<T> *out[3] = {u0, u1, u2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int p = threadIdx.x + i * FU_NT;
+            if (p < C * L) {
+                const int n = p / C, c = p - n * C;
+                s0[c * LS + n] = U[d][i];
+            }
+        }
+        __syncthreads();
+        cpx<T> *res = fft_lines<T, L, C, LS, FU_NT, false>(s0, s1, tw);
+        for (int e = threadIdx.x; e < C * L; e += FU_NT) {
+            const int n = e / C, c = e - n * C;
+            if (kz0 + c < g.trn_z) out[d][base + n * lstride + c] = res[c * LS + n];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------
+// z pass of the advection term: three half-spectrum rows per (x,y) (kz <
+// TRN_Z nonzero) -> inverse z -> u in registers -> the five products of
+// U2advection_k (operate_omega.cxx:31-47) -> forward z -> five half spectra
+// (kz < TRN_Z written). Two real rows share one complex line.
+template <typename T, int L>
+__global__ __launch_bounds__(FU_NT) void k_z_adv(Grid g, const cpx<T> *__restrict__ s0g,
+                                                 const cpx<T> *__restrict__ s1g, const cpx<T> *__restrict__ s2g,
+                                                 cpx<T> *__restrict__ p0, cpx<T> *__restrict__ p1,
+                                                 cpx<T> *__restrict__ p2, cpx<T> *__restrict__ p3,
+                                                 cpx<T> *__restrict__ p4, T scale) {
+    constexpr int P   = (ZTile<T>::ELEM / L) > 0 ? ZTile<T>::ELEM / L : 1;
+    constexpr int NZH = L / 2 + 1;
+    constexpr int NPT = (P * L + FU_NT - 1) / FU_NT;
+    __shared__ cpx<T> a[P * (L + 2)], b[P * (L + 2)], tw[L];
+    fill_twiddles<T, L, FU_NT>(tw);
+    const long long nrows = (long long)g.nx * g.ny;
+    const long long row0  = (long long)blockIdx.x * (2 * P);
+    const int       kzv   = g.trn_z;
+    const cpx<T>   *src[3] = {s0g, s1g, s2g};
+    cpx<T>          U[3][NPT];  // (row 2p, row 2p+1) of component d at z = n
+    for (int d = 0; d < 3; d++) {
+        for (int e = threadIdx.x; e < 2 * P * NZH; e += FU_NT) {
+            const int rl = e / NZH, k = e - rl * NZH;
+            b[e] = (row0 + rl < nrows && k < kzv) ? src[d][row0 * NZH + e] : cmk((T)0, (T)0);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < P * L; e += FU_NT) {
+            const int p = e / L, n = e - p * L;
+            const int k = (n <= L / 2) ? n : L - n;
+            cpx<T>    A = b[(2 * p) * NZH + k], B = b[(2 * p + 1) * NZH + k];
+            if (k == 0 || k == L / 2) {
+                A.y = 0;
+                B.y = 0;
+            }
+            if (n > L / 2) {
+                A = cconj(A);
+                B = cconj(B);
+            }
+            a[e] = cmk(A.x - B.y, A.y + B.x);
+        }
+        __syncthreads();
+        cpx<T> *res = fft_lines<T, L, P, L, FU_NT, false>(a, b, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * FU_NT;
+            U[d][i]     = (e < P * L) ? cscale(res[e], scale) : cmk((T)0, (T)0);
+        }
+        __syncthreads();
+    }
+    cpx<T> *dst[5] = {p0, p1, p2, p3, p4};
+    for (int q = 0; q < 5; q++) {
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * FU_NT;
+            if (e >= P * L) continue;
+            cpx<T> v;
+            // component pair of rows (2p, 2p+1): .x = row 2p, .y = row 2p+1
+            const cpx<T> u1 = U[0][i], u2 = U[1][i], u3 = U[2][i];
+            if (q == 0)
+                v = cmk(u2.x * u3.x, u2.y * u3.y);
+            else if (q == 1)
+                v = cmk(u3.x * u1.x, u3.y * u1.y);
+            else if (q == 2)
+                v = cmk(u1.x * u2.x, u1.y * u2.y);
+            else if (q == 3)
+                v = cmk((u2.x - u3.x) * (u2.x + u3.x), (u2.y - u3.y) * (u2.y + u3.y));
+            else
+                v = cmk((u3.x - u1.x) * (u3.x + u1.x), (u3.y - u1.y) * (u3.y + u1.y));
+            a[e] = v;
+        }
+        __syncthreads();
+        cpx<T> *res = fft_lines<T, L, P, L, FU_NT, true>(a, b, tw);
+        for (int e = threadIdx.x; e < 2 * P * NZH; e += FU_NT) {
+            const int rl = e / NZH, k = e - rl * NZH;
+            if (row0 + rl >= nrows) break;
+            if (k >= kzv) continue;
+            const int    p  = rl >> 1;
+            const cpx<T> Z  = res[p * L + k];
+            const cpx<T> Zm = cconj(res[p * L + ((L - k) % L)]);
+            cpx<T>       v;
+            if (rl & 1) {
+                const cpx<T> d = csub(Z, Zm);
+                v              = cmk(d.y * (T)0.5, -d.x * (T)0.5);
+            } else {
+                v = cscale(cadd(Z, Zm), (T)0.5);
+            }
+            dst[q][row0 * NZH + e] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------
+// z pass of the rigidity force: f = up - min(phi_sum, 1) u*
+// (Make_f_particle_dt_nonsole, f_particle.h:35-54; make_phi.cxx:248-259),
+// forward z per component, kz < TRN_Z written.
+template <typename T, int L>
+__global__ __launch_bounds__(FU_NT) void k_z_force(Grid g, const T *__restrict__ up0, const T *__restrict__ up1,
+                                                   const T *__restrict__ up2, const T *__restrict__ phis,
+                                                   const T *__restrict__ us0, const T *__restrict__ us1,
+                                                   const T *__restrict__ us2, cpx<T> *__restrict__ f0,
+                                                   cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2) {
+    constexpr int P   = (ZTile<T>::ELEM / L) > 0 ? ZTile<T>::ELEM / L : 1;
+    constexpr int NZH = L / 2 + 1;
+    constexpr int NPT = (P * L + FU_NT - 1) / FU_NT;
+    __shared__ cpx<T> a[P * L], b[P * L], tw[L];
+    fill_twiddles<T, L, FU_NT>(tw);
+    const long long nrows = (long long)g.nx * g.ny;
+    const long long row0  = (long long)blockIdx.x * (2 * P);
+    const T        *UP[3] = {up0, up1, up2}, *US[3] = {us0, us1, us2};
+    cpx<T>         *dst[3] = {f0, f1, f2};
+    // element e of a complex line: row 2p (re) and 2p+1 (im) at z = n
+    T ph[NPT][2];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int e = threadIdx.x + i * FU_NT;
+        const int p = e / L, n = e - p * L;
+        for (int h = 0; h < 2; h++) {
+            const long long row = row0 + 2 * p + h;
+            ph[i][h] = (e < P * L && row < nrows) ? fmin(phis[row * L + n], (T)1) : (T)0;
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * FU_NT;
+            if (e >= P * L) continue;
+            const int p = e / L, n = e - p * L;
+            T         v[2];
+            for (int h = 0; h < 2; h++) {
+                const long long row = row0 + 2 * p + h;
+                v[h] = (row < nrows) ? UP[d][row * L + n] - ph[i][h] * US[d][row * L + n] : (T)0;
+            }
+            a[e] = cmk(v[0], v[1]);
+        }
+        __syncthreads();
+        cpx<T> *res = fft_lines<T, L, P, L, FU_NT, true>(a, b, tw);
+        for (int e = threadIdx.x; e < 2 * P * NZH; e += FU_NT) {
+            const int rl = e / NZH, k = e - rl * NZH;
+            if (row0 + rl >= nrows) break;
+            if (k >= g.trn_z) continue;
+            const int    p  = rl >> 1;
+            const cpx<T> Z  = res[p * L + k];
+            const cpx<T> Zm = cconj(res[p * L + ((L - k) % L)]);
+            cpx<T>       v;
+            if (rl & 1) {
+                const cpx<T> dd = csub(Z, Zm);
+                v               = cmk(dd.y * (T)0.5, -dd.x * (T)0.5);
+            } else {
+                v = cscale(cadd(Z, Zm), (T)0.5);
+            }
+            dst[d][row0 * NZH + e] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------
+// Forward x pass of the rigidity force on the kept (ky,kz) columns, then per
+// kept mode: truncation + Solenoidal_uk (operate_omega.cxx:171-195), the sum
+// with u* rebuilt from zeta (the reference's U2zeta_k of u* + f), and
+// U_k2zeta_k (fft_wrapper.cxx:407-476). Writes zeta at the kept modes and
+// uk_dc += f_hat(0).
+template <typename T, int L>
+__global__ __launch_bounds__(FU_NT) void k_x_f_zeta(Grid g, cpx<T> *__restrict__ f0, cpx<T> *__restrict__ f1,
+                                                    cpx<T> *__restrict__ f2, cpx<T> *__restrict__ z0,
+                                                    cpx<T> *__restrict__ z1, double *__restrict__ ukdc) {
+    constexpr int C   = ColTile<T>::C;
+    constexpr int LS  = L + 1;
+    constexpr int NPT = (C * L + FU_NT - 1) / FU_NT;
+    __shared__ cpx<T> s0[C * LS], s1[C * LS], tw[L];
+    __shared__ double uk[3];
+    fill_twiddles<T, L, FU_NT>(tw);
+    const int ntiles = (g.trn_z + C - 1) / C;
+    const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
+    const int j = kept_index(o, g.ny, g.trn_y), iky = (j > g.hny) ? j - g.ny : j;
+    if (threadIdx.x < 3) uk[threadIdx.x] = ukdc[threadIdx.x];
+    const long long lstride = (long long)g.ny * g.nzh;
+    const long long base    = (long long)j * g.nzh + kz0;
+    cpx<T>         *F[3]    = {f0, f1, f2};
+    cpx<T>          Fh[3][NPT];
+    for (int d = 0; d < 3; d++) {
+        for (int e = threadIdx.x; e < C * L; e += FU_NT) {
+            const int n = e / C, c = e - n * C;
+            s0[c * LS + n] = (kz0 + c < g.trn_z) ? F[d][base + n * lstride + c] : cmk((T)0, (T)0);
+        }
+        __syncthreads();
+        cpx<T> *res = fft_lines<T, L, C, LS, FU_NT, true>(s0, s1, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int p = threadIdx.x + i * FU_NT;
+            if (p < C * L) {
+                const int n = p / C, c = p - n * C;
+                Fh[d][i]    = res[c * LS + n];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int p = threadIdx.x + i * FU_NT;
+        if (p >= C * L) continue;
+        const int n = p / C, c = p - n * C, kz = kz0 + c;
+        const int ikx = (n > g.hnx) ? n - g.nx : n;
+        if (kz >= g.trn_z || !kept1(ikx, g.trn_x)) continue;
+        const long long idx = base + n * lstride + c;
+        const double    kx = ikx * g.wave_x, ky = iky * g.wave_y, kzz = kz * g.wave_z;
+        const double    K2  = (g.wave_x * ikx) * (g.wave_x * ikx) + (g.wave_y * iky) * (g.wave_y * iky) +
+                          (g.wave_z * kz) * (g.wave_z * kz);
+        const double IK2 = (K2 > 0.0) ? 1. / K2 : 0.0;
+        // Solenoidal_uk on f_hat
+        const cpx<T> a = Fh[0][i], b = Fh[1][i], cc = Fh[2][i];
+        const double dr = IK2 * ((double)a.x * kx + (double)b.x * ky + (double)cc.x * kzz);
+        const double di = IK2 * ((double)a.y * kx + (double)b.y * ky + (double)cc.y * kzz);
+        double       fr[3] = {a.x - dr * kx, b.x - dr * ky, cc.x - dr * kzz};
+        double       fi[3] = {a.y - di * kx, b.y - di * ky, cc.y - di * kzz};
+        // u* rebuilt from the (post-NS, truncated) zeta
+        cpx<T> us[3];
+        zeta_to_uhat<T>(g, ikx, iky, kz, z0[idx], z1[idx], uk, us);
+        double ur[3], ui[3];
+        for (int d = 0; d < 3; d++) {
+            ur[d] = (double)us[d].x + fr[d];
+            ui[d] = (double)us[d].y + fi[d];
+        }
+        if (ikx == 0 && iky == 0 && kz == 0) {  // k = 0 stays FP64: uk_dc + sum(f)
+            for (int d = 0; d < 3; d++) ukdc[d] = uk[d] + fr[d];
+        }
+        double ore[3], oim[3];
+        ore[0] = -(ky * ui[2] - kzz * ui[1]);
+        oim[0] = (ky * ur[2] - kzz * ur[1]);
+        ore[1] = -(kzz * ui[0] - kx * ui[2]);
+        oim[1] = (kzz * ur[0] - kx * ur[2]);
+        ore[2] = -(kx * ui[1] - ky * ui[0]);
+        oim[2] = (kx * ur[1] - ky * ur[0]);
+        int c0, c1;
+        if (ikx != 0) {
+            c0 = 1;
+            c1 = 2;
+        } else if (iky != 0) {
+            c0 = 2;
+            c1 = 0;
+        } else {
+            c0 = 0;
+            c1 = 1;
+        }
+        z0[idx] = cmk((T)ore[c0], (T)oim[c0]);
+        z1[idx] = cmk((T)ore[c1], (T)oim[c1]);
+    }
+}
+
+// zero zeta outside the kept modes (the reference's state there is
+// round-off after U2zeta_k; here it is exactly zero)
+template <typename T>
+__global__ void k_zeta_clean(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nspec) return;
+    const int kz = (int)(idx % g.nzh);
+    const long long rest = idx / g.nzh;
+    const int j = (int)(rest % g.ny), i = (int)(rest / g.ny);
+    if (!kept_mode(g, kx_of(g, i), ky_of(g, j), kz)) {
+        z0[idx] = cmk((T)0, (T)0);
+        z1[idx] = cmk((T)0, (T)0);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+#define KH_FOR_EACH_L(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+static inline int nky(const Grid &g) { return 2 * g.trn_y - 1; }
+
+template <typename T, bool FWD>
+hipError_t cols_y(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s) {
+    constexpr int C  = ColTile<T>::C;
+    const int     nt = (g.trn_z + C - 1) / C;
+    switch (g.ny) {
+#define KH_CASE(n)                                                                                                 \
+    case n:                                                                                                        \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt), dim3(FU_NT), 0, s, data, g, g.trn_z,     \
+                           in_mask, out_mask);                                                                     \
+        return hipGetLastError();
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, bool FWD>
+hipError_t cols_x(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s) {
+    constexpr int C  = ColTile<T>::C;
+    const int     nt = (g.trn_z + C - 1) / C;
+    switch (g.nx) {
+#define KH_CASE(n)                                                                                                  \
+    case n:                                                                                                         \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt), dim3(FU_NT), 0, s, data, g, g.trn_z,     \
+                           in_mask, out_mask);                                                                      \
+        return hipGetLastError();
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const u[3], double *ukdc,
+                        int fixmask, hipStream_t s) {
+    constexpr int C  = ColTile<T>::C;
+    const int     nt = (g.trn_z + C - 1) / C;
+    switch (g.nx) {
+#define KH_CASE(n)                                                                                                   \
+    case n:                                                                                                          \
+        hipLaunchKernelGGL((k_x_zeta2u_inv<T, n>), dim3(nky(g) * nt), dim3(FU_NT), 0, s, g, z0, z1, u[0], u[1],    \
+                           u[2], ukdc, fixmask);                                                                     \
+        return hipGetLastError();
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStream_t s) {
+    const long long nrows = (long long)g.nx * g.ny;
+    const T         scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
+    switch (g.nz) {
+#define KH_CASE(n)                                                                                                 \
+    case n: {                                                                                                      \
+        constexpr int P = (ZTile<T>::ELEM / n) > 0 ? ZTile<T>::ELEM / n : 1;                                       \
+        hipLaunchKernelGGL((k_z_adv<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(FU_NT), 0, s, g, \
+                           s3[0], s3[1], s3[2], p5[0], p5[1], p5[2], p5[3], p5[4], scale);                         \
+        return hipGetLastError();                                                                                  \
+    }
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s) {
+    const long long nrows = (long long)g.nx * g.ny;
+    switch (g.nz) {
+#define KH_CASE(n)                                                                                                   \
+    case n: {                                                                                                        \
+        constexpr int P = (ZTile<T>::ELEM / n) > 0 ? ZTile<T>::ELEM / n : 1;                                         \
+        hipLaunchKernelGGL((k_z_force<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(FU_NT), 0, s, g, \
+                           up[0], up[1], up[2], phis, us[0], us[1], us[2], f[0], f[1], f[2]);                        \
+        return hipGetLastError();                                                                                    \
+    }
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, double *ukdc, hipStream_t s) {
+    constexpr int C  = ColTile<T>::C;
+    const int     nt = (g.trn_z + C - 1) / C;
+    switch (g.nx) {
+#define KH_CASE(n)                                                                                                 \
+    case n:                                                                                                        \
+        hipLaunchKernelGGL((k_x_f_zeta<T, n>), dim3(nky(g) * nt), dim3(FU_NT), 0, s, g, f[0], f[1], f[2], z0, z1, \
+                           ukdc);                                                                                  \
+        return hipGetLastError();
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s) {
+    hipLaunchKernelGGL(k_zeta_clean<T>, dim3((unsigned)((g.nspec + 255) / 256)), dim3(256), 0, s, g, z0, z1);
+    return hipGetLastError();
+}
+
+#define KH_INST(T)                                                                                                  \
+    template hipError_t cols_y<T, true>(const Grid &, cpx<T> *, int, int, hipStream_t);                             \
+    template hipError_t cols_y<T, false>(const Grid &, cpx<T> *, int, int, hipStream_t);                            \
+    template hipError_t cols_x<T, true>(const Grid &, cpx<T> *, int, int, hipStream_t);                             \
+    template hipError_t cols_x<T, false>(const Grid &, cpx<T> *, int, int, hipStream_t);                            \
+    template hipError_t x_zeta2u_inv<T>(const Grid &, const cpx<T> *, const cpx<T> *, cpx<T> *const[3], double *,   \
+                                        int, hipStream_t);                                                          \
+    template hipError_t z_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], hipStream_t);                   \
+    template hipError_t z_force<T>(const Grid &, T *const[3], const T *, T *const[3], cpx<T> *const[3], hipStream_t); \
+    template hipError_t x_f_zeta<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, hipStream_t);    \
+    template hipError_t zeta_clean<T>(const Grid &, cpx<T> *, cpx<T> *, hipStream_t);
+KH_INST(double)
+KH_INST(float)
+#undef KH_INST
+
+}  // namespace kh

@@ -243,22 +243,22 @@ __device__ __forceinline__ int lj_mult(const PParams &pp, const double *xi, cons
 }
 
 // Force (particle_solver.h:88-111): LJ pairs (md_force.cxx:34-196), gravity
-// (md_force.cxx:306-324). One lane per particle sums its own pair forces:
-// F_i = sum_j f(r_ij) (x_i - x_j), identical to the reference's +/- updates.
-__global__ void k_forces(PParams pp, DevParticle *__restrict__ P) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= pp.np) return;
+// (md_force.cxx:306-324). One workgroup per particle i: its lanes split the
+// partners j, each pair giving F_i += f(r_ij) (x_i - x_j), the same value the
+// reference's +/- updates add; the lane sums are reduced in FP64.
+__global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__restrict__ P) {
+    const int    i      = blockIdx.x;
     const double xi[3]  = {P[i].x[0], P[i].x[1], P[i].x[2]};
     const int    si     = P[i].spec;
     const bool   obst_i = pp.species[si].propulsion == 2;
     double       f[3]   = {0.0, 0.0, 0.0};
     if (pp.lj_on) {
-        for (int j = 0; j < pp.np; j++) {
+        for (int j = threadIdx.x; j < pp.np; j += PT_NT) {
             if (j == i) continue;
             if (obst_i && pp.species[P[j].spec].propulsion == 2) continue;
             const double xj[3] = {P[j].x[0], P[j].x[1], P[j].x[2]};
             double       rv[3];
-            // r_ij_vec from the lower index to the higher, as the link cell does
+            // r_ij_vec from the higher index to the lower, as the link cell takes it
             const double rij = (i > j) ? distance0(pp, xi, xj, rv) : distance0(pp, xj, xi, rv);
             if (rij < pp.lj_cutoff) {
                 const int    mult  = lj_mult(pp, xi, xj);
@@ -269,8 +269,11 @@ __global__ void k_forces(PParams pp, DevParticle *__restrict__ P) {
             }
         }
     }
-    for (int d = 0; d < 3; d++) P[i].fr[d] += f[d];
-    if (pp.g != 0.0) P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
+    block_sum<3>(f);
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; d++) P[i].fr[d] += f[d];
+        if (pp.g != 0.0) P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
+    }
 }
 
 // MD_solver_velocity_Euler (particle_solver.cxx:103-142) / _AB2_hydro
@@ -393,7 +396,7 @@ hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, doubl
     return hipGetLastError();
 }
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s) {
-    if (pp.np > 0) hipLaunchKernelGGL(k_forces, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P);
+    if (pp.np > 0) hipLaunchKernelGGL(k_forces, dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
     return hipGetLastError();
 }
 hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s) {

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/kapsel_hip.h"
+#include "kh_fused.h"
 #include "kh_launch.h"
 
 using namespace kh;
@@ -84,6 +85,8 @@ struct kh_run {
     int         *sek     = nullptr;
     int          np_domain = 0;
     CTime        jikan;
+    int          step_path  = KH_PATH_FUSED;
+    int          zeta_dirty = 1;  // zeta may hold non-kept modes
     // measurement
     bool                               timing = false;
     std::map<std::string, KernelTimes> times;
@@ -251,8 +254,50 @@ struct Step {
         return KH_OK;
     }
 
-    // Time_evolution_hydro (sp_3d_ns.cxx:41-194), Navier_Stokes, non-slip
+    // Time_evolution_hydro (sp_3d_ns.cxx:41-194) as the fused, pruned
+    // pipeline of kh_fused.hip. Differences from step_reference_order are
+    // round-off only: FFT(IFFT(a)) = a for the band-limited fields, so the
+    // final U2zeta_k of u* + f is taken directly from u*_hat + P f_hat.
     static int step(kh_run *r) {
+        if (r->step_path == KH_PATH_REFERENCE_ORDER) return step_reference_order(r);
+        const int   ab2 = r->jikan.ts != 0;
+        const Grid &g   = r->g;
+        hipStream_t s   = r->stream;
+        if (r->zeta_dirty) {
+            KH_LAUNCH("zeta_clean", (zeta_clean<T>(g, Z(r, 0), Z(r, 1), s)));
+            r->zeta_dirty = 0;
+        }
+        cpx<T> *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
+        cpx<T> *s5[5] = {S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)};
+        // NS_solver_slavedEuler: zeta -> u -> products -> -Omega -> slaved Euler
+        KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, 0, s)));
+        for (int d = 0; d < 3; d++) KH_LAUNCH("y_inv", (cols_y<T, false>(g, S(r, d), 1, 0, s)));
+        KH_LAUNCH("z_adv", (z_adv<T>(g, s3, s5, s)));
+        for (int q = 0; q < 5; q++) KH_LAUNCH("y_fwd", (cols_y<T, true>(g, S(r, q), 0, 1, s)));
+        for (int q = 0; q < 5; q++) KH_LAUNCH("x_fwd", (cols_x<T, true>(g, S(r, q), 0, 1, s)));
+        KH_LAUNCH("adv_ns", (adv_ns<T>(g, s5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
+        // FIX_CELL, truncation, Zeta_k2u -> u* in real space
+        KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
+        for (int d = 0; d < 3; d++) KH_LAUNCH("y_inv", (cols_y<T, false>(g, S(r, d), 1, 0, s)));
+        for (int d = 0; d < 3; d++) KH_LAUNCH("z_c2r", (fft_z_c2r<T>(g, S(r, d), U(r, d), s, g.trn_z)));
+        // particles
+        KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, s));
+        KH_TRY(make_phi(r));
+        KH_TRY(calc_f_hydro(r));
+        KH_TRY(velocity_update(r, ab2));
+        if (r->kbt > 0.) KH_TRY(thermo(r));
+        KH_TRY(make_up(r));
+        // f = P(up - phi u*), u = u* + f, zeta = curl u
+        T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
+        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, s)));
+        for (int d = 0; d < 3; d++) KH_LAUNCH("y_fwd", (cols_y<T, true>(g, S(r, d), 0, 1, s)));
+        KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));
+        return KH_OK;
+    }
+
+    // Time_evolution_hydro in the reference's own operation order, one
+    // kernel sequence per reference function.
+    static int step_reference_order(kh_run *r) {
         const int ab2 = r->jikan.ts != 0;
         KH_TRY(ns_solver(r));
         KH_TRY(zeta_k2u(r, r->fixmask));  // FIX_CELL, truncation, u*
@@ -284,6 +329,7 @@ struct Step {
     }
 
     static int apply(kh_run *r, int fn) {
+        r->zeta_dirty = 1;
         switch (fn) {
             case KH_FN_NS_SOLVER: return ns_solver(r);
             case KH_FN_ZETA_K2U: return zeta_k2u(r, 0, 0);
@@ -696,6 +742,7 @@ int kh_set_fluid(kh_run *r, const double *z0, const double *z1, const double uk_
     if (!r || !z0 || !z1 || !uk_dc) return fail(KH_ERR_ARG, "NULL argument");
     KH_TRY(upload_spec(r, r->zeta[0], z0));
     KH_TRY(upload_spec(r, r->zeta[1], z1));
+    r->zeta_dirty = 1;
     KH_HIP(hipMemcpy(r->ukdc, uk_dc, 3 * sizeof(double), hipMemcpyHostToDevice));
     return KH_OK;
 }
@@ -741,6 +788,7 @@ int kh_set_field(kh_run *r, int field, const double *in) {
 
 int kh_init_fluid(kh_run *r) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
+    r->zeta_dirty = 1;
     return r->mixed ? Step<float>::init_fluid(r) : Step<double>::init_fluid(r);
 }
 
@@ -786,6 +834,13 @@ int kh_A_k2a(kh_run *r, double *a) {
         KH_TRY(Step<double>::c2r(r, (cpx<double> *)r->spec[4], (double *)r->work[4]));
     }
     return download_real(r, a, r->work[4]);
+}
+
+int kh_set_step_path(kh_run *r, int path) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    if (path != KH_PATH_FUSED && path != KH_PATH_REFERENCE_ORDER) return fail(KH_ERR_ARG, "unknown step path %d", path);
+    r->step_path = path;
+    return KH_OK;
 }
 
 int kh_timing_enable(kh_run *r, int on) {

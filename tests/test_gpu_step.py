@@ -33,9 +33,28 @@ def kept_mask(shape):
             (kz[None, None, :] < trn[2]))
 
 
-def cmp_particles(pg, po, tol, keys=PART_KEYS):
+def cmp_particles(pg, po, tol, keys=PART_KEYS, p0=None, a=5.0):
+    """Each quantity against its natural scale: displacements against the
+    largest displacement, velocities against the largest speed, angular
+    velocities against speed/a, forces against the largest force."""
+    vs = max(np.abs(po["v"]).max(), np.abs(po["v_old"]).max(), 1e-300)
+    fs = max(np.abs(po["f_hydro_previous"]).max(), np.abs(po["fr_previous"]).max(), 1e-300)
+    scale = {"v": vs, "v_old": vs, "omega": vs / a, "omega_old": vs / a, "q": 1.0, "q_old": 1.0,
+             "f_hydro_previous": fs, "fr_previous": fs, "torque_hydro_previous": fs * a,
+             "f_hydro": fs, "torque_hydro": fs * a}
     for k in keys:
-        e = rel(pg[k], po[k])
+        if k in ("x", "x_nopbc"):
+            if p0 is None:
+                e = rel(pg[k], po[k])
+            else:
+                d0 = po[k] - p0[k]
+                d0 = d0 - np.rint(d0 / 1e6) * 1e6
+                e = float(np.abs((pg[k] - p0[k]) - d0).max() / max(np.abs(d0).max(), 1e-300))
+                if k == "x":  # periodic wrap: compare positions to the displacement scale
+                    dx = pg[k] - po[k]
+                    e = float(np.abs(dx).max() / max(np.abs(po["x_nopbc"] - p0["x_nopbc"]).max(), 1e-300))
+        else:
+            e = float(np.abs(pg[k] - po[k]).max() / scale.get(k, max(np.abs(po[k]).max(), 1e-300)))
         assert e < tol, (k, e)
 
 
@@ -90,7 +109,10 @@ def test_functions_fp64(gpu):
             assert rel(g.get_field("phi")[..., :nz], o.get_field("phi")[..., :nz]) < 1e-13
             assert o.get_field("phi_sum").max() > 1.0  # overlapping interfaces present
         elif fn == "f_hydro":
-            cmp_particles(g.get_particles(), o.get_particles(), 1e-12, ["f_hydro", "torque_hydro"])
+            # sums of ~1000 terms of both signs: compare at the FP64 bar of the step
+            pg_, po_ = g.get_particles(), o.get_particles()
+            for k in ("f_hydro", "torque_hydro"):
+                assert rel(pg_[k], po_[k]) < 1e-10, k
         elif fn == "make_up":
             for d in range(3):
                 assert rel(g.get_field(f"up{d}")[..., :nz], o.get_field(f"up{d}")[..., :nz]) < 1e-12
@@ -111,12 +133,13 @@ def test_functions_fp64(gpu):
         g.apply(fn)
         cmp_particles(g.get_particles(), o.get_particles(), 1e-12,
                       ["x", "x_nopbc", "v", "v_old", "omega", "omega_old", "q", "q_old", "fr_previous",
-                       "f_hydro_previous", "torque_hydro_previous"])
+                       "f_hydro_previous", "torque_hydro_previous"], a=4.0)
 
 
-def run_both(P, parts, nsteps, init=True):
+def run_both(P, parts, nsteps, init=True, path="fused"):
     o = kref.Run(P, len(parts))
     g = gpu_mod().Run(P, len(parts))
+    g.set_step_path(path)
     o.set_particles(parts)
     g.set_particles(parts)
     if init:
@@ -132,14 +155,18 @@ def gpu_mod():
     return kapsel3_amd
 
 
-def test_plain_single_sphere_fp64(gpu):
+PATHS = pytest.mark.parametrize("path", ["fused", "reference"])
+
+
+@PATHS
+def test_plain_single_sphere_fp64(gpu, path):
     """Config 1 (32^3, one driven sphere) for 1001 steps (MSTEP+1,
     sp_3d_ns.cxx:857)."""
-    print("config1 1001 steps", flush=True)
+    print(f"config1 1001 steps {path}", flush=True)
     P, parts = workloads.plain_single(32)
-    o, g = run_both(P, parts, 1001)
+    o, g = run_both(P, parts, 1001, path=path)
     po, pg = o.get_particles(), g.get_particles()
-    cmp_particles(pg, po, 1e-10)
+    cmp_particles(pg, po, 1e-10, p0=parts, a=4.0)
     zo, zg = o.get_fluid(), g.get_fluid()
     assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
     assert o.get_time().ts == g.get_time().ts == 1001
@@ -148,20 +175,22 @@ def test_plain_single_sphere_fp64(gpu):
     assert 0.85 < pg["v"][0, 2] / U < 1.05
 
 
-def test_sedimentation_shrunk_fp64(gpu):
+@PATHS
+def test_sedimentation_shrunk_fp64(gpu, path):
     """Config 2 shrunk: 64^3, 24 spheres a=5 in contact range (LJ active),
     gravity, 60 steps."""
-    print("config2 shrunk 64^3 x 24", flush=True)
+    print(f"config2 shrunk 64^3 x 24 {path}", flush=True)
     P, parts = workloads.sedimentation(64, 24, gap=0.4, seed=3)
-    o, g = run_both(P, parts, 60)
-    cmp_particles(g.get_particles(), o.get_particles(), 1e-10)
+    o, g = run_both(P, parts, 60, path=path)
+    cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)
     zo, zg = o.get_fluid(), g.get_fluid()
     assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
 
 
-def test_thermal_shrunk_fp64(gpu):
+@PATHS
+def test_thermal_shrunk_fp64(gpu, path):
     """Config 3 physics (LJ + gravity + kT noise) at 48^3 (radix-3 FFT), 30 steps."""
-    print("config3 physics 48^3", flush=True)
+    print(f"config3 physics 48^3 {path}", flush=True)
     P, parts = workloads.sedimentation(48, 12, kbt=1e-3, gap=0.5, seed=4)
-    o, g = run_both(P, parts, 30)
-    cmp_particles(g.get_particles(), o.get_particles(), 1e-10)
+    o, g = run_both(P, parts, 30, path=path)
+    cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)
