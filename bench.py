@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (torch) so profiler byte counters can be scaled")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -127,6 +129,14 @@ def main():
         sys.exit(2)
     from kapsel3_amd._abi import FP64, MIXED
     n, P = a.grid, a.particles
+    if a.copy_calibration:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     npts = 1935  # Sekibun points of a = 5, xi = 2 (profile.cxx:22-65)
 
     t_mix, st_mix, ok_mix = time_gpu(MIXED, n, P, a.steps, a.warmup)

@@ -18,7 +18,6 @@
 
 namespace kh {
 
-constexpr int FFT_NT = 256;
 
 template <typename T>
 struct FftTile;
@@ -34,78 +33,96 @@ struct FftTile<float> {
 };
 
 // ---------------------------------------------------------------- z pass
+// Block of 2P consecutive real rows = P complex lines (rows 2p and 2p+1 are
+// the real and imaginary parts of line p).
 template <typename T, int L>
-__global__ __launch_bounds__(FFT_NT) void k_fft_z_r2c(const T *__restrict__ in, cpx<T> *__restrict__ out,
-                                                      long long nrows) {
-    constexpr int P   = (FftTile<T>::ZELEM / L) > 0 ? FftTile<T>::ZELEM / L : 1;  // complex lines per WG
-    constexpr int NZH = L / 2 + 1;
-    __shared__ cpx<T> s0[P * L], s1[P * L], tw[L];
-    fill_twiddles<T, L, FFT_NT>(tw);
+struct ZPass {
+    static constexpr int NT  = PassThreads<T>::value;
+    static constexpr int P   = (FftTile<T>::ZELEM / L) > 0 ? FftTile<T>::ZELEM / L : 1;
+    static constexpr int NZH = L / 2 + 1;
+    static constexpr int BS  = ColStride<T, P>::value;
+    static constexpr int SZ  = (L * BS > P * (L + 2)) ? L * BS : P * (L + 2);  // FFT tile or staged rows
+    static constexpr int NR  = (2 * P * L + NT - 1) / NT;                     // real points per thread
+    static constexpr int NC  = (P * L + NT - 1) / NT;                         // complex points per thread
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_r2c(const T *__restrict__ in,
+                                                                     cpx<T> *__restrict__ out, long long nrows) {
+    using Z = ZPass<T, L>;
+    constexpr int NT = Z::NT, P = Z::P, NZH = Z::NZH, BS = Z::BS;
+    __shared__ cpx<T> s[Z::SZ], tw[L];
+    fill_twiddles<T, L, NT>(tw);
     const long long row0 = (long long)blockIdx.x * (2 * P);
     const T        *src  = in + row0 * L;
-    for (int e = threadIdx.x; e < 2 * P * L; e += FFT_NT) {
-        const int rl = e / L, n = e - rl * L;
-        T         v  = (row0 + rl < nrows) ? src[e] : (T)0;
-        T        *d  = reinterpret_cast<T *>(&s0[(rl >> 1) * L + n]);
-        d[rl & 1]    = v;
+    T               v[Z::NR];
+#pragma unroll
+    for (int i = 0; i < Z::NR; i++) {
+        const int e = threadIdx.x + i * NT;
+        v[i]        = (e < 2 * P * L && row0 + e / L < nrows) ? src[e] : (T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < Z::NR; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < 2 * P * L) {
+            const int rl = e / L, n = e - rl * L;
+            reinterpret_cast<T *>(&s[n * BS + (rl >> 1)])[rl & 1] = v[i];
+        }
     }
     __syncthreads();
-    cpx<T> *res = fft_lines<T, L, P, L, FFT_NT, true>(s0, s1, tw);
+    fft_cols_ip<T, L, P, BS, NT, true>(s, tw);
     cpx<T> *dst = out + row0 * NZH;
-    for (int e = threadIdx.x; e < 2 * P * NZH; e += FFT_NT) {
+    for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {
         const int rl = e / NZH, k = e - rl * NZH;
         if (row0 + rl >= nrows) break;
         const int    p  = rl >> 1;
-        const cpx<T> Z  = res[p * L + k];
-        const cpx<T> Zm = cconj(res[p * L + ((L - k) % L)]);
-        cpx<T>       v;
+        const cpx<T> Zk = s[k * BS + p];
+        const cpx<T> Zm = cconj(s[((L - k) % L) * BS + p]);
+        cpx<T>       o;
         if (rl & 1) {  // -i/2 (Z - Zm)
-            cpx<T> d = csub(Z, Zm);
-            v        = cmk(d.y * (T)0.5, -d.x * (T)0.5);
+            const cpx<T> d = csub(Zk, Zm);
+            o              = cmk(d.y * (T)0.5, -d.x * (T)0.5);
         } else {
-            v = cscale(cadd(Z, Zm), (T)0.5);
+            o = cscale(cadd(Zk, Zm), (T)0.5);
         }
-        dst[e] = v;
+        dst[e] = o;
     }
 }
 
 template <typename T, int L>
-__global__ __launch_bounds__(FFT_NT) void k_fft_z_c2r(const cpx<T> *__restrict__ in, T *__restrict__ out,
-                                                      long long nrows, T scale, int kzvalid) {
-    constexpr int P   = (FftTile<T>::ZELEM / L) > 0 ? FftTile<T>::ZELEM / L : 1;
-    constexpr int NZH = L / 2 + 1;
-    __shared__ cpx<T> s0[P * L], s1[P * (L + 2)], tw[L];
-    fill_twiddles<T, L, FFT_NT>(tw);
+__global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_c2r(const cpx<T> *__restrict__ in,
+                                                                     T *__restrict__ out, long long nrows, T scale,
+                                                                     int kzvalid) {
+    using Z = ZPass<T, L>;
+    constexpr int NT = Z::NT, P = Z::P, NZH = Z::NZH, BS = Z::BS;
+    __shared__ cpx<T> s[Z::SZ], tw[L];
+    fill_twiddles<T, L, NT>(tw);
     const long long row0 = (long long)blockIdx.x * (2 * P);
     const cpx<T>   *src  = in + row0 * NZH;
-    for (int e = threadIdx.x; e < 2 * P * NZH; e += FFT_NT) {  // modes kz >= kzvalid are zero
+    for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {  // modes kz >= kzvalid are zero
         const int rl = e / NZH, k = e - rl * NZH;
-        s1[e]        = (row0 + rl < nrows && k < kzvalid) ? src[e] : cmk((T)0, (T)0);
+        s[e]         = (row0 + rl < nrows && k < kzvalid) ? src[e] : cmk((T)0, (T)0);
     }
     __syncthreads();
-    // two Hermitian rows A (even), B (odd) -> Z = A + iB over the full line;
-    // imaginary parts of kz = 0 and kz = L/2 are dropped, as any c2r does.
-    for (int e = threadIdx.x; e < P * L; e += FFT_NT) {
-        const int p = e / L, n = e - p * L;
-        const int k = (n <= L / 2) ? n : L - n;
-        cpx<T>    A = s1[(2 * p) * NZH + k], B = s1[(2 * p + 1) * NZH + k];
-        if (k == 0 || k == L / 2) {
-            A.y = 0;
-            B.y = 0;
-        }
-        if (n > L / 2) {
-            A = cconj(A);
-            B = cconj(B);
-        }
-        s0[e] = cmk(A.x - B.y, A.y + B.x);
+    cpx<T> zv[Z::NC];
+#pragma unroll
+    for (int i = 0; i < Z::NC; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < P * L) zv[i] = hermitian_pair<T, L>(s, e / L, e % L);
     }
     __syncthreads();
-    cpx<T> *res = fft_lines<T, L, P, L, FFT_NT, false>(s0, s1, tw);
-    T      *dst = out + row0 * L;
-    for (int e = threadIdx.x; e < 2 * P * L; e += FFT_NT) {
+#pragma unroll
+    for (int i = 0; i < Z::NC; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < P * L) s[(e % L) * BS + e / L] = zv[i];
+    }
+    __syncthreads();
+    fft_cols_ip<T, L, P, BS, NT, false>(s, tw);
+    T *dst = out + row0 * L;
+    for (int e = threadIdx.x; e < 2 * P * L; e += NT) {
         const int rl = e / L, n = e - rl * L;
         if (row0 + rl >= nrows) break;
-        const cpx<T> z = res[(rl >> 1) * L + n];
+        const cpx<T> z = s[n * BS + (rl >> 1)];
         dst[e]         = ((rl & 1) ? z.y : z.x) * scale;
     }
 }
@@ -115,24 +132,36 @@ __global__ __launch_bounds__(FFT_NT) void k_fft_z_c2r(const cpx<T> *__restrict__
 // Tile t of outer block o starts at data + o*ostride + t*C; `ncols` columns
 // are valid per outer block.
 template <typename T, int L, bool FWD>
-__global__ __launch_bounds__(FFT_NT) void k_fft_strided(cpx<T> *__restrict__ data, long long lstride, int ncols,
-                                                        long long ostride, int tiles) {
-    constexpr int C  = FftTile<T>::C;
-    constexpr int LS = L + 1;
-    __shared__ cpx<T> s0[C * LS], s1[C * LS], tw[L];
-    fill_twiddles<T, L, FFT_NT>(tw);
+__global__ __launch_bounds__(PassThreads<T>::value) void k_fft_strided(cpx<T> *__restrict__ data, long long lstride,
+                                                                       int ncols, long long ostride, int tiles) {
+    constexpr int NT  = PassThreads<T>::value;
+    constexpr int C   = FftTile<T>::C;
+    constexpr int BS  = ColStride<T, C>::value;
+    constexpr int NPT = (C * L + NT - 1) / NT;
+    __shared__ cpx<T> s[L * BS], tw[L];
+    fill_twiddles<T, L, NT>(tw);
     const int o = blockIdx.x / tiles, t = blockIdx.x - o * tiles;
     const int c0 = t * C;
     cpx<T>   *base = data + (long long)o * ostride + c0;
-    for (int e = threadIdx.x; e < C * L; e += FFT_NT) {
+    cpx<T>    v[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int e = threadIdx.x + i * NT;
         const int n = e / C, c = e - n * C;
-        s0[c * LS + n] = (c0 + c < ncols) ? base[(long long)n * lstride + c] : cmk((T)0, (T)0);
+        v[i] = (e < C * L && c0 + c < ncols) ? base[(long long)n * lstride + c] : cmk((T)0, (T)0);
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < C * L) s[(e / C) * BS + e % C] = v[i];
     }
     __syncthreads();
-    cpx<T> *res = fft_lines<T, L, C, LS, FFT_NT, FWD>(s0, s1, tw);
-    for (int e = threadIdx.x; e < C * L; e += FFT_NT) {
+    fft_cols_ip<T, L, C, BS, NT, FWD>(s, tw);
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int e = threadIdx.x + i * NT;
         const int n = e / C, c = e - n * C;
-        if (c0 + c < ncols) base[(long long)n * lstride + c] = res[c * LS + n];
+        if (e < C * L && c0 + c < ncols) base[(long long)n * lstride + c] = s[n * BS + c];
     }
 }
 
@@ -154,9 +183,9 @@ static hipError_t launch_z_r2c(int L, const T *in, cpx<T> *out, long long nrows,
     switch (L) {
 #define KH_CASE(n)                                                                       \
     case n: {                                                                            \
-        constexpr int P  = (FftTile<T>::ZELEM / n) > 0 ? FftTile<T>::ZELEM / n : 1;      \
+        constexpr int P  = ZPass<T, n>::P;                                               \
         long long     nb = (nrows + 2 * P - 1) / (2 * P);                                \
-        hipLaunchKernelGGL((k_fft_z_r2c<T, n>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, nrows); \
+        hipLaunchKernelGGL((k_fft_z_r2c<T, n>), dim3((unsigned)nb), dim3(ZPass<T, n>::NT), 0, s, in, out, nrows); \
         return hipGetLastError();                                                        \
     }
         KH_FOR_EACH_L(KH_CASE)
@@ -171,9 +200,9 @@ static hipError_t launch_z_c2r(int L, const cpx<T> *in, T *out, long long nrows,
     switch (L) {
 #define KH_CASE(n)                                                                       \
     case n: {                                                                            \
-        constexpr int P  = (FftTile<T>::ZELEM / n) > 0 ? FftTile<T>::ZELEM / n : 1;      \
+        constexpr int P  = ZPass<T, n>::P;                                               \
         long long     nb = (nrows + 2 * P - 1) / (2 * P);                                \
-        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, in, out, nrows, scale, kzvalid); \
+        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb), dim3(ZPass<T, n>::NT), 0, s, in, out, nrows, scale, kzvalid); \
         return hipGetLastError();                                                        \
     }
         KH_FOR_EACH_L(KH_CASE)
@@ -191,7 +220,7 @@ static hipError_t launch_strided(int L, cpx<T> *data, long long lstride, int nco
     switch (L) {
 #define KH_CASE(n)                                                                                              \
     case n:                                                                                                     \
-        hipLaunchKernelGGL((k_fft_strided<T, n, FWD>), dim3((unsigned)nb), dim3(FFT_NT), 0, s, data, lstride, ncols, \
+        hipLaunchKernelGGL((k_fft_strided<T, n, FWD>), dim3((unsigned)nb), dim3(PassThreads<T>::value), 0, s, data, lstride, ncols, \
                            ostride, tiles);                                                                     \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)

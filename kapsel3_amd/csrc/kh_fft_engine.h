@@ -187,4 +187,154 @@ __device__ __forceinline__ cpx<T> *fft_lines(cpx<T> *a, cpx<T> *b, const cpx<T> 
     return fft_stages<T, L, B, LS, NT, FWD, 0>(a, b, tw);
 }
 
+// ------------------------------------------------------------------------
+// Line-fastest variant: point n of line c lives at LDS index n*BS + c
+// (BS >= B, padded). Butterfly id -> (line = id % B, j = id / B), so the
+// lanes of a wavefront work on neighbouring lines at the same butterfly:
+// their reads and writes fall on consecutive LDS words in every stage, and
+// the row padding staggers the rows a radix-R stage writes R rows apart.
+template <typename T, int L, int R, int Ns, int B, int BS, int NT, bool FWD>
+__device__ __forceinline__ void fft_stage_cf(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out,
+                                             const cpx<T> *__restrict__ tw) {
+    constexpr int m       = L / R;
+    constexpr int nbf     = B * m;
+    constexpr int tstride = L / (Ns * R);
+    for (int id = threadIdx.x; id < nbf; id += NT) {
+        const int     j    = id / B;
+        const int     line = id - j * B;
+        const int     k    = j % Ns;
+        cpx<T>        v[R];
+        const cpx<T> *src  = in + j * BS + line;
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = src[r * m * BS];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const cpx<T> w = tw[r * k * tstride];
+                v[r]           = FWD ? cmul(v[r], w) : cmulc(v[r], w);
+            }
+        }
+        Dft<T, R, FWD>::run(v);
+        cpx<T> *dst = out + ((j / Ns) * Ns * R + k) * BS + line;
+#pragma unroll
+        for (int r = 0; r < R; r++) dst[r * Ns * BS] = v[r];
+    }
+}
+
+template <typename T, int L, int B, int BS, int NT, bool FWD, int S>
+__device__ __forceinline__ cpx<T> *fft_stages_cf(cpx<T> *a, cpx<T> *b, const cpx<T> *tw) {
+    constexpr RadixPlan P = make_radix_plan(L);
+    if constexpr (S < P.n) {
+        fft_stage_cf<T, L, P.r[S], P.ns[S], B, BS, NT, FWD>(a, b, tw);
+        __syncthreads();
+        return fft_stages_cf<T, L, B, BS, NT, FWD, S + 1>(b, a, tw);
+    } else {
+        return a;
+    }
+}
+
+// B lines, point n of line c at a[n*BS + c]; b is scratch of the same size.
+// Returns the buffer holding the result; caller syncs before the call.
+template <typename T, int L, int B, int BS, int NT, bool FWD>
+__device__ __forceinline__ cpx<T> *fft_cols(cpx<T> *a, cpx<T> *b, const cpx<T> *tw) {
+    static_assert(fft_size_ok(L), "FFT length must be 2^a 3^b");
+    return fft_stages_cf<T, L, B, BS, NT, FWD, 0>(a, b, tw);
+}
+
+// Row stride for B lines of element type T: one extra element staggers the
+// rows a stage writes R apart across the LDS banks.
+template <typename T, int B>
+struct ColStride {
+    static constexpr int value = B + 1;
+};
+
+// In-place, register-staged variant of fft_cols: each stage reads every
+// butterfly's R points into registers, synchronises, then writes the
+// outputs back into the same buffer. One LDS buffer instead of two doubles
+// the workgroups a CU can hold, which is what hides HBM latency here.
+template <typename T, int L, int R, int Ns, int B, int BS, int NT, bool FWD>
+__device__ __forceinline__ void fft_stage_ip(cpx<T> *__restrict__ buf, const cpx<T> *__restrict__ tw) {
+    constexpr int m       = L / R;
+    constexpr int nbf     = B * m;
+    constexpr int NB      = (nbf + NT - 1) / NT;
+    constexpr int tstride = L / (Ns * R);
+    cpx<T>        v[NB][R];
+#pragma unroll
+    for (int t = 0; t < NB; t++) {
+        const int id = threadIdx.x + t * NT;
+        if (NB * NT == nbf || id < nbf) {
+            const int     j    = id / B;
+            const int     line = id - j * B;
+            const cpx<T> *src  = buf + j * BS + line;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[t][r] = src[r * m * BS];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NB; t++) {
+        const int id = threadIdx.x + t * NT;
+        if (NB * NT == nbf || id < nbf) {
+            const int j    = id / B;
+            const int line = id - j * B;
+            const int k    = j % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    const cpx<T> w = tw[r * k * tstride];
+                    v[t][r]        = FWD ? cmul(v[t][r], w) : cmulc(v[t][r], w);
+                }
+            }
+            Dft<T, R, FWD>::run(v[t]);
+            cpx<T> *dst = buf + ((j / Ns) * Ns * R + k) * BS + line;
+#pragma unroll
+            for (int r = 0; r < R; r++) dst[r * Ns * BS] = v[t][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int L, int B, int BS, int NT, bool FWD, int S>
+__device__ __forceinline__ void fft_stages_ip(cpx<T> *buf, const cpx<T> *tw) {
+    constexpr RadixPlan P = make_radix_plan(L);
+    if constexpr (S < P.n) {
+        fft_stage_ip<T, L, P.r[S], P.ns[S], B, BS, NT, FWD>(buf, tw);
+        fft_stages_ip<T, L, B, BS, NT, FWD, S + 1>(buf, tw);
+    }
+}
+
+// B lines, point n of line c at buf[n*BS + c], transformed in place.
+// Caller syncs before the call; the call ends with a barrier.
+template <typename T, int L, int B, int BS, int NT, bool FWD>
+__device__ __forceinline__ void fft_cols_ip(cpx<T> *buf, const cpx<T> *tw) {
+    static_assert(fft_size_ok(L), "FFT length must be 2^a 3^b");
+    fft_stages_ip<T, L, B, BS, NT, FWD, 0>(buf, tw);
+}
+
+// Two Hermitian half-spectrum rows A (even), B (odd) staged row-major in s
+// -> the complex line Z = A + iB at point n (imaginary parts of kz = 0 and
+// kz = L/2 dropped, as any c2r does).
+template <typename T, int L>
+__device__ __forceinline__ cpx<T> hermitian_pair(const cpx<T> *s, int p, int n) {
+    constexpr int NZH = L / 2 + 1;
+    const int     k   = (n <= L / 2) ? n : L - n;
+    cpx<T>        A = s[(2 * p) * NZH + k], B = s[(2 * p + 1) * NZH + k];
+    if (k == 0 || k == L / 2) {
+        A.y = 0;
+        B.y = 0;
+    }
+    if (n > L / 2) {
+        A = cconj(A);
+        B = cconj(B);
+    }
+    return cmk(A.x - B.y, A.y + B.x);
+}
+
+// Threads per workgroup of the FFT pass kernels: 8 wavefronts for FP32
+// tiles, 4 for FP64 (whose tiles have half as many lines).
+template <typename T>
+struct PassThreads {
+    static constexpr int value = sizeof(T) == 4 ? 512 : 256;
+};
+
 }  // namespace kh

@@ -19,8 +19,6 @@
 
 namespace kh {
 
-constexpr int FU_NT = 256;
-
 template <typename T>
 struct ColTile;
 template <>
@@ -54,12 +52,14 @@ __device__ __forceinline__ int kept_index(int o, int n, int trn) { return o < tr
 // in_mask: rows whose wavenumber is truncated are taken as zero (not read).
 // out_mask: such rows are not written.
 template <typename T, int L, bool FWD, bool XMODE>
-__global__ __launch_bounds__(FU_NT) void k_cols(cpx<T> *__restrict__ data, Grid g, int ncols, int in_mask,
-                                                int out_mask) {
-    constexpr int C  = ColTile<T>::C;
-    constexpr int LS = L + 1;
-    __shared__ cpx<T> s0[C * LS], s1[C * LS], tw[L];
-    fill_twiddles<T, L, FU_NT>(tw);
+__global__ __launch_bounds__(PassThreads<T>::value) void k_cols(cpx<T> *__restrict__ data, Grid g, int ncols,
+                                                                int in_mask, int out_mask) {
+    constexpr int NT  = PassThreads<T>::value;
+    constexpr int C   = ColTile<T>::C;
+    constexpr int BS  = ColStride<T, C>::value;
+    constexpr int NPT = (C * L + NT - 1) / NT;
+    __shared__ cpx<T> s[L * BS], tw[L];
+    fill_twiddles<T, L, NT>(tw);
     const int ntiles = (ncols + C - 1) / C;
     const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
     long long base, lstride;
@@ -76,19 +76,28 @@ __global__ __launch_bounds__(FU_NT) void k_cols(cpx<T> *__restrict__ data, Grid 
         trn     = g.trn_y;
         nline   = g.ny;
     }
-    for (int e = threadIdx.x; e < C * L; e += FU_NT) {
+    cpx<T> v[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int e = threadIdx.x + i * NT;
         const int n = e / C, c = e - n * C;
         const int k = (n > nline / 2) ? n - nline : n;
-        cpx<T>    v = cmk((T)0, (T)0);
-        if (kz0 + c < ncols && (!in_mask || kept1(k, trn))) v = data[base + n * lstride + c];
-        s0[c * LS + n] = v;
+        v[i]        = cmk((T)0, (T)0);
+        if (e < C * L && kz0 + c < ncols && (!in_mask || kept1(k, trn))) v[i] = data[base + n * lstride + c];
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < C * L) s[(e / C) * BS + e % C] = v[i];
     }
     __syncthreads();
-    cpx<T> *res = fft_lines<T, L, C, LS, FU_NT, FWD>(s0, s1, tw);
-    for (int e = threadIdx.x; e < C * L; e += FU_NT) {
+    fft_cols_ip<T, L, C, BS, NT, FWD>(s, tw);
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int e = threadIdx.x + i * NT;
         const int n = e / C, c = e - n * C;
         const int k = (n > nline / 2) ? n - nline : n;
-        if (kz0 + c < ncols && (!out_mask || kept1(k, trn))) data[base + n * lstride + c] = res[c * LS + n];
+        if (e < C * L && kz0 + c < ncols && (!out_mask || kept1(k, trn))) data[base + n * lstride + c] = s[n * BS + c];
     }
 }
 
@@ -141,16 +150,16 @@ __device__ __forceinline__ void zeta_to_uhat(const Grid &g, int ikx, int iky, in
 // inverse x pass of each component. fixmask: FIX_CELL zeroing of uk_dc
 // (sp_3d_ns.cxx:46-52), done by the block holding (ky,kz) = (0,0).
 template <typename T, int L>
-__global__ __launch_bounds__(FU_NT) void k_x_zeta2u_inv(Grid g, const cpx<T> *__restrict__ z0,
-                                                        const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ u0,
-                                                        cpx<T> *__restrict__ u1, cpx<T> *__restrict__ u2,
-                                                        double *__restrict__ ukdc, int fixmask) {
+__global__ __launch_bounds__(PassThreads<T>::value) void k_x_zeta2u_inv(
+    Grid g, const cpx<T> *__restrict__ z0, const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ u0,
+    cpx<T> *__restrict__ u1, cpx<T> *__restrict__ u2, double *__restrict__ ukdc, int fixmask) {
+    constexpr int NT  = PassThreads<T>::value;
     constexpr int C   = ColTile<T>::C;
-    constexpr int LS  = L + 1;
-    constexpr int NPT = (C * L + FU_NT - 1) / FU_NT;
-    __shared__ cpx<T> s0[C * LS], s1[C * LS], tw[L];
+    constexpr int BS  = ColStride<T, C>::value;
+    constexpr int NPT = (C * L + NT - 1) / NT;
+    __shared__ cpx<T> s[L * BS], tw[L];
     __shared__ double uk[3];
-    fill_twiddles<T, L, FU_NT>(tw);
+    fill_twiddles<T, L, NT>(tw);
     const int ntiles = (g.trn_z + C - 1) / C;
     const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
     const int j = kept_index(o, g.ny, g.trn_y), iky = (j > g.hny) ? j - g.ny : j;
@@ -163,16 +172,26 @@ __global__ __launch_bounds__(FU_NT) void k_x_zeta2u_inv(Grid g, const cpx<T> *__
             if (fixmask & (1 << d)) ukdc[d] = 0.0;
     const long long lstride = (long long)g.ny * g.nzh;
     const long long base    = (long long)j * g.nzh + kz0;
-    cpx<T>          U[3][NPT];
+    cpx<T>          D1[NPT], D2[NPT];
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
-        const int p = threadIdx.x + i * FU_NT;
+        const int p = threadIdx.x + i * NT;
+        const int n = p / C, c = p - n * C, kz = kz0 + c;
+        const int ikx = (n > g.hnx) ? n - g.nx : n;
+        const bool ok = p < C * L && kz < g.trn_z && kept1(ikx, g.trn_x);
+        D1[i]         = ok ? z0[base + n * lstride + c] : cmk((T)0, (T)0);
+        D2[i]         = ok ? z1[base + n * lstride + c] : cmk((T)0, (T)0);
+    }
+    cpx<T> U[3][NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int p = threadIdx.x + i * NT;
         const int n = p / C, c = p - n * C, kz = kz0 + c;
         const int ikx = (n > g.hnx) ? n - g.nx : n;
         U[0][i] = U[1][i] = U[2][i] = cmk((T)0, (T)0);
         if (p < C * L && kz < g.trn_z && kept1(ikx, g.trn_x)) {
             cpx<T> u[3];
-            zeta_to_uhat<T>(g, ikx, iky, kz, z0[base + n * lstride + c], z1[base + n * lstride + c], uk, u);
+            zeta_to_uhat<T>(g, ikx, iky, kz, D1[i], D2[i], uk, u);
             U[0][i] = u[0];
             U[1][i] = u[1];
             U[2][i] = u[2];
@@ -183,17 +202,16 @@ __global__ __launch_bounds__(FU_NT) void k_x_zeta2u_inv(Grid g, const cpx<T> *__
     for (int d = 0; d < 3; d++) {
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
-            const int p = threadIdx.x + i * FU_NT;
-            if (p < C * L) {
-                const int n = p / C, c = p - n * C;
-                s0[c * LS + n] = U[d][i];
-            }
+            const int p = threadIdx.x + i * NT;
+            if (p < C * L) s[(p / C) * BS + p % C] = U[d][i];
         }
         __syncthreads();
-        cpx<T> *res = fft_lines<T, L, C, LS, FU_NT, false>(s0, s1, tw);
-        for (int e = threadIdx.x; e < C * L; e += FU_NT) {
+        fft_cols_ip<T, L, C, BS, NT, false>(s, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
             const int n = e / C, c = e - n * C;
-            if (kz0 + c < g.trn_z) out[d][base + n * lstride + c] = res[c * LS + n];
+            if (e < C * L && kz0 + c < g.trn_z) out[d][base + n * lstride + c] = s[n * BS + c];
         }
         __syncthreads();
     }
@@ -205,47 +223,47 @@ __global__ __launch_bounds__(FU_NT) void k_x_zeta2u_inv(Grid g, const cpx<T> *__
 // U2advection_k (operate_omega.cxx:31-47) -> forward z -> five half spectra
 // (kz < TRN_Z written). Two real rows share one complex line.
 template <typename T, int L>
-__global__ __launch_bounds__(FU_NT) void k_z_adv(Grid g, const cpx<T> *__restrict__ s0g,
-                                                 const cpx<T> *__restrict__ s1g, const cpx<T> *__restrict__ s2g,
-                                                 cpx<T> *__restrict__ p0, cpx<T> *__restrict__ p1,
-                                                 cpx<T> *__restrict__ p2, cpx<T> *__restrict__ p3,
-                                                 cpx<T> *__restrict__ p4, T scale) {
+__global__ __launch_bounds__(PassThreads<T>::value) void k_z_adv(
+    Grid g, const cpx<T> *__restrict__ s0g, const cpx<T> *__restrict__ s1g, const cpx<T> *__restrict__ s2g,
+    cpx<T> *__restrict__ p0, cpx<T> *__restrict__ p1, cpx<T> *__restrict__ p2, cpx<T> *__restrict__ p3,
+    cpx<T> *__restrict__ p4, T scale) {
+    constexpr int NT  = PassThreads<T>::value;
     constexpr int P   = (ZTile<T>::ELEM / L) > 0 ? ZTile<T>::ELEM / L : 1;
     constexpr int NZH = L / 2 + 1;
-    constexpr int NPT = (P * L + FU_NT - 1) / FU_NT;
-    __shared__ cpx<T> a[P * (L + 2)], b[P * (L + 2)], tw[L];
-    fill_twiddles<T, L, FU_NT>(tw);
-    const long long nrows = (long long)g.nx * g.ny;
-    const long long row0  = (long long)blockIdx.x * (2 * P);
-    const int       kzv   = g.trn_z;
+    constexpr int NPT = (P * L + NT - 1) / NT;
+    constexpr int BS  = ColStride<T, P>::value;
+    constexpr int SZ  = (L * BS > P * (L + 2)) ? L * BS : P * (L + 2);
+    __shared__ cpx<T> s[SZ], tw[L];
+    fill_twiddles<T, L, NT>(tw);
+    const long long nrows  = (long long)g.nx * g.ny;
+    const long long row0   = (long long)blockIdx.x * (2 * P);
+    const int       kzv    = g.trn_z;
     const cpx<T>   *src[3] = {s0g, s1g, s2g};
-    cpx<T>          U[3][NPT];  // (row 2p, row 2p+1) of component d at z = n
+    cpx<T>          U[3][NPT];  // (row 2p, row 2p+1) of component d at z = n, point e = p*L + n
     for (int d = 0; d < 3; d++) {
-        for (int e = threadIdx.x; e < 2 * P * NZH; e += FU_NT) {
+        for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {
             const int rl = e / NZH, k = e - rl * NZH;
-            b[e] = (row0 + rl < nrows && k < kzv) ? src[d][row0 * NZH + e] : cmk((T)0, (T)0);
+            s[e]         = (row0 + rl < nrows && k < kzv) ? src[d][row0 * NZH + e] : cmk((T)0, (T)0);
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < P * L; e += FU_NT) {
-            const int p = e / L, n = e - p * L;
-            const int k = (n <= L / 2) ? n : L - n;
-            cpx<T>    A = b[(2 * p) * NZH + k], B = b[(2 * p + 1) * NZH + k];
-            if (k == 0 || k == L / 2) {
-                A.y = 0;
-                B.y = 0;
-            }
-            if (n > L / 2) {
-                A = cconj(A);
-                B = cconj(B);
-            }
-            a[e] = cmk(A.x - B.y, A.y + B.x);
-        }
-        __syncthreads();
-        cpx<T> *res = fft_lines<T, L, P, L, FU_NT, false>(a, b, tw);
+        cpx<T> zv[NPT];
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
-            const int e = threadIdx.x + i * FU_NT;
-            U[d][i]     = (e < P * L) ? cscale(res[e], scale) : cmk((T)0, (T)0);
+            const int e = threadIdx.x + i * NT;
+            if (e < P * L) zv[i] = hermitian_pair<T, L>(s, e / L, e % L);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            if (e < P * L) s[(e % L) * BS + e / L] = zv[i];
+        }
+        __syncthreads();
+        fft_cols_ip<T, L, P, BS, NT, false>(s, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            U[d][i]     = (e < P * L) ? cscale(s[(e % L) * BS + e / L], scale) : cmk((T)0, (T)0);
         }
         __syncthreads();
     }
@@ -253,10 +271,10 @@ __global__ __launch_bounds__(FU_NT) void k_z_adv(Grid g, const cpx<T> *__restric
     for (int q = 0; q < 5; q++) {
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
-            const int e = threadIdx.x + i * FU_NT;
+            const int e = threadIdx.x + i * NT;
             if (e >= P * L) continue;
             cpx<T> v;
-            // component pair of rows (2p, 2p+1): .x = row 2p, .y = row 2p+1
+            // .x = row 2p, .y = row 2p+1
             const cpx<T> u1 = U[0][i], u2 = U[1][i], u3 = U[2][i];
             if (q == 0)
                 v = cmk(u2.x * u3.x, u2.y * u3.y);
@@ -268,23 +286,23 @@ __global__ __launch_bounds__(FU_NT) void k_z_adv(Grid g, const cpx<T> *__restric
                 v = cmk((u2.x - u3.x) * (u2.x + u3.x), (u2.y - u3.y) * (u2.y + u3.y));
             else
                 v = cmk((u3.x - u1.x) * (u3.x + u1.x), (u3.y - u1.y) * (u3.y + u1.y));
-            a[e] = v;
+            s[(e % L) * BS + e / L] = v;
         }
         __syncthreads();
-        cpx<T> *res = fft_lines<T, L, P, L, FU_NT, true>(a, b, tw);
-        for (int e = threadIdx.x; e < 2 * P * NZH; e += FU_NT) {
+        fft_cols_ip<T, L, P, BS, NT, true>(s, tw);
+        for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {
             const int rl = e / NZH, k = e - rl * NZH;
             if (row0 + rl >= nrows) break;
             if (k >= kzv) continue;
             const int    p  = rl >> 1;
-            const cpx<T> Z  = res[p * L + k];
-            const cpx<T> Zm = cconj(res[p * L + ((L - k) % L)]);
+            const cpx<T> Zk = s[k * BS + p];
+            const cpx<T> Zm = cconj(s[((L - k) % L) * BS + p]);
             cpx<T>       v;
             if (rl & 1) {
-                const cpx<T> d = csub(Z, Zm);
+                const cpx<T> d = csub(Zk, Zm);
                 v              = cmk(d.y * (T)0.5, -d.x * (T)0.5);
             } else {
-                v = cscale(cadd(Z, Zm), (T)0.5);
+                v = cscale(cadd(Zk, Zm), (T)0.5);
             }
             dst[q][row0 * NZH + e] = v;
         }
@@ -297,59 +315,60 @@ __global__ __launch_bounds__(FU_NT) void k_z_adv(Grid g, const cpx<T> *__restric
 // (Make_f_particle_dt_nonsole, f_particle.h:35-54; make_phi.cxx:248-259),
 // forward z per component, kz < TRN_Z written.
 template <typename T, int L>
-__global__ __launch_bounds__(FU_NT) void k_z_force(Grid g, const T *__restrict__ up0, const T *__restrict__ up1,
-                                                   const T *__restrict__ up2, const T *__restrict__ phis,
-                                                   const T *__restrict__ us0, const T *__restrict__ us1,
-                                                   const T *__restrict__ us2, cpx<T> *__restrict__ f0,
-                                                   cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2) {
+__global__ __launch_bounds__(PassThreads<T>::value) void k_z_force(
+    Grid g, const T *__restrict__ up0, const T *__restrict__ up1, const T *__restrict__ up2,
+    const T *__restrict__ phis, const T *__restrict__ us0, const T *__restrict__ us1, const T *__restrict__ us2,
+    cpx<T> *__restrict__ f0, cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2) {
+    constexpr int NT  = PassThreads<T>::value;
     constexpr int P   = (ZTile<T>::ELEM / L) > 0 ? ZTile<T>::ELEM / L : 1;
     constexpr int NZH = L / 2 + 1;
-    constexpr int NPT = (P * L + FU_NT - 1) / FU_NT;
-    __shared__ cpx<T> a[P * L], b[P * L], tw[L];
-    fill_twiddles<T, L, FU_NT>(tw);
+    constexpr int NR  = (2 * P * L + NT - 1) / NT;  // real points per thread
+    constexpr int BS  = ColStride<T, P>::value;
+    __shared__ cpx<T> s[L * BS], tw[L];
+    fill_twiddles<T, L, NT>(tw);
     const long long nrows = (long long)g.nx * g.ny;
     const long long row0  = (long long)blockIdx.x * (2 * P);
     const T        *UP[3] = {up0, up1, up2}, *US[3] = {us0, us1, us2};
     cpx<T>         *dst[3] = {f0, f1, f2};
-    // element e of a complex line: row 2p (re) and 2p+1 (im) at z = n
-    T ph[NPT][2];
+    // real point e = rl*L + n of the block (contiguous rows)
+    T ph[NR];
 #pragma unroll
-    for (int i = 0; i < NPT; i++) {
-        const int e = threadIdx.x + i * FU_NT;
-        const int p = e / L, n = e - p * L;
-        for (int h = 0; h < 2; h++) {
-            const long long row = row0 + 2 * p + h;
-            ph[i][h] = (e < P * L && row < nrows) ? fmin(phis[row * L + n], (T)1) : (T)0;
-        }
+    for (int i = 0; i < NR; i++) {
+        const int e = threadIdx.x + i * NT;
+        ph[i]       = (e < 2 * P * L && row0 + e / L < nrows) ? fmin(phis[row0 * L + e], (T)1) : (T)0;
     }
     for (int d = 0; d < 3; d++) {
+        T a[NR], b[NR];
 #pragma unroll
-        for (int i = 0; i < NPT; i++) {
-            const int e = threadIdx.x + i * FU_NT;
-            if (e >= P * L) continue;
-            const int p = e / L, n = e - p * L;
-            T         v[2];
-            for (int h = 0; h < 2; h++) {
-                const long long row = row0 + 2 * p + h;
-                v[h] = (row < nrows) ? UP[d][row * L + n] - ph[i][h] * US[d][row * L + n] : (T)0;
+        for (int i = 0; i < NR; i++) {
+            const int  e  = threadIdx.x + i * NT;
+            const bool ok = e < 2 * P * L && row0 + e / L < nrows;
+            a[i]          = ok ? UP[d][row0 * L + e] : (T)0;
+            b[i]          = ok ? US[d][row0 * L + e] : (T)0;
+        }
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int e = threadIdx.x + i * NT;
+            if (e < 2 * P * L) {
+                const int rl = e / L, n = e - rl * L;
+                reinterpret_cast<T *>(&s[n * BS + (rl >> 1)])[rl & 1] = a[i] - ph[i] * b[i];
             }
-            a[e] = cmk(v[0], v[1]);
         }
         __syncthreads();
-        cpx<T> *res = fft_lines<T, L, P, L, FU_NT, true>(a, b, tw);
-        for (int e = threadIdx.x; e < 2 * P * NZH; e += FU_NT) {
+        fft_cols_ip<T, L, P, BS, NT, true>(s, tw);
+        for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {
             const int rl = e / NZH, k = e - rl * NZH;
             if (row0 + rl >= nrows) break;
             if (k >= g.trn_z) continue;
             const int    p  = rl >> 1;
-            const cpx<T> Z  = res[p * L + k];
-            const cpx<T> Zm = cconj(res[p * L + ((L - k) % L)]);
+            const cpx<T> Zk = s[k * BS + p];
+            const cpx<T> Zm = cconj(s[((L - k) % L) * BS + p]);
             cpx<T>       v;
             if (rl & 1) {
-                const cpx<T> dd = csub(Z, Zm);
+                const cpx<T> dd = csub(Zk, Zm);
                 v               = cmk(dd.y * (T)0.5, -dd.x * (T)0.5);
             } else {
-                v = cscale(cadd(Z, Zm), (T)0.5);
+                v = cscale(cadd(Zk, Zm), (T)0.5);
             }
             dst[d][row0 * NZH + e] = v;
         }
@@ -364,15 +383,17 @@ __global__ __launch_bounds__(FU_NT) void k_z_force(Grid g, const T *__restrict__
 // U_k2zeta_k (fft_wrapper.cxx:407-476). Writes zeta at the kept modes and
 // uk_dc += f_hat(0).
 template <typename T, int L>
-__global__ __launch_bounds__(FU_NT) void k_x_f_zeta(Grid g, cpx<T> *__restrict__ f0, cpx<T> *__restrict__ f1,
-                                                    cpx<T> *__restrict__ f2, cpx<T> *__restrict__ z0,
-                                                    cpx<T> *__restrict__ z1, double *__restrict__ ukdc) {
+__global__ __launch_bounds__(PassThreads<T>::value) void k_x_f_zeta(Grid g, cpx<T> *__restrict__ f0,
+                                                                    cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2,
+                                                                    cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
+                                                                    double *__restrict__ ukdc) {
+    constexpr int NT  = PassThreads<T>::value;
     constexpr int C   = ColTile<T>::C;
-    constexpr int LS  = L + 1;
-    constexpr int NPT = (C * L + FU_NT - 1) / FU_NT;
-    __shared__ cpx<T> s0[C * LS], s1[C * LS], tw[L];
+    constexpr int BS  = ColStride<T, C>::value;
+    constexpr int NPT = (C * L + NT - 1) / NT;
+    __shared__ cpx<T> s[L * BS], tw[L];
     __shared__ double uk[3];
-    fill_twiddles<T, L, FU_NT>(tw);
+    fill_twiddles<T, L, NT>(tw);
     const int ntiles = (g.trn_z + C - 1) / C;
     const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
     const int j = kept_index(o, g.ny, g.trn_y), iky = (j > g.hny) ? j - g.ny : j;
@@ -382,25 +403,30 @@ __global__ __launch_bounds__(FU_NT) void k_x_f_zeta(Grid g, cpx<T> *__restrict__
     cpx<T>         *F[3]    = {f0, f1, f2};
     cpx<T>          Fh[3][NPT];
     for (int d = 0; d < 3; d++) {
-        for (int e = threadIdx.x; e < C * L; e += FU_NT) {
-            const int n = e / C, c = e - n * C;
-            s0[c * LS + n] = (kz0 + c < g.trn_z) ? F[d][base + n * lstride + c] : cmk((T)0, (T)0);
-        }
-        __syncthreads();
-        cpx<T> *res = fft_lines<T, L, C, LS, FU_NT, true>(s0, s1, tw);
+        cpx<T> v[NPT];
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
-            const int p = threadIdx.x + i * FU_NT;
-            if (p < C * L) {
-                const int n = p / C, c = p - n * C;
-                Fh[d][i]    = res[c * LS + n];
-            }
+            const int e = threadIdx.x + i * NT;
+            const int n = e / C, c = e - n * C;
+            v[i]        = (e < C * L && kz0 + c < g.trn_z) ? F[d][base + n * lstride + c] : cmk((T)0, (T)0);
+        }
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            if (e < C * L) s[(e / C) * BS + e % C] = v[i];
+        }
+        __syncthreads();
+        fft_cols_ip<T, L, C, BS, NT, true>(s, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int p = threadIdx.x + i * NT;
+            if (p < C * L) Fh[d][i] = s[(p / C) * BS + p % C];
         }
         __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
-        const int p = threadIdx.x + i * FU_NT;
+        const int p = threadIdx.x + i * NT;
         if (p >= C * L) continue;
         const int n = p / C, c = p - n * C, kz = kz0 + c;
         const int ikx = (n > g.hnx) ? n - g.nx : n;
@@ -477,7 +503,7 @@ hipError_t cols_y(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStr
     switch (g.ny) {
 #define KH_CASE(n)                                                                                                 \
     case n:                                                                                                        \
-        hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt), dim3(FU_NT), 0, s, data, g, g.trn_z,     \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt), dim3(PassThreads<T>::value), 0, s, data, g, g.trn_z,     \
                            in_mask, out_mask);                                                                     \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -493,7 +519,7 @@ hipError_t cols_x(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStr
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt), dim3(FU_NT), 0, s, data, g, g.trn_z,     \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, data, g, g.trn_z,     \
                            in_mask, out_mask);                                                                      \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -510,7 +536,7 @@ hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                   \
     case n:                                                                                                          \
-        hipLaunchKernelGGL((k_x_zeta2u_inv<T, n>), dim3(nky(g) * nt), dim3(FU_NT), 0, s, g, z0, z1, u[0], u[1],    \
+        hipLaunchKernelGGL((k_x_zeta2u_inv<T, n>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, g, z0, z1, u[0], u[1],    \
                            u[2], ukdc, fixmask);                                                                     \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -527,7 +553,7 @@ hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStr
 #define KH_CASE(n)                                                                                                 \
     case n: {                                                                                                      \
         constexpr int P = (ZTile<T>::ELEM / n) > 0 ? ZTile<T>::ELEM / n : 1;                                       \
-        hipLaunchKernelGGL((k_z_adv<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(FU_NT), 0, s, g, \
+        hipLaunchKernelGGL((k_z_adv<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(PassThreads<T>::value), 0, s, g, \
                            s3[0], s3[1], s3[2], p5[0], p5[1], p5[2], p5[3], p5[4], scale);                         \
         return hipGetLastError();                                                                                  \
     }
@@ -544,7 +570,7 @@ hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3],
 #define KH_CASE(n)                                                                                                   \
     case n: {                                                                                                        \
         constexpr int P = (ZTile<T>::ELEM / n) > 0 ? ZTile<T>::ELEM / n : 1;                                         \
-        hipLaunchKernelGGL((k_z_force<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(FU_NT), 0, s, g, \
+        hipLaunchKernelGGL((k_z_force<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(PassThreads<T>::value), 0, s, g, \
                            up[0], up[1], up[2], phis, us[0], us[1], us[2], f[0], f[1], f[2]);                        \
         return hipGetLastError();                                                                                    \
     }
@@ -561,7 +587,7 @@ hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, d
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                 \
     case n:                                                                                                        \
-        hipLaunchKernelGGL((k_x_f_zeta<T, n>), dim3(nky(g) * nt), dim3(FU_NT), 0, s, g, f[0], f[1], f[2], z0, z1, \
+        hipLaunchKernelGGL((k_x_f_zeta<T, n>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, g, f[0], f[1], f[2], z0, z1, \
                            ukdc);                                                                                  \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)

@@ -30,6 +30,8 @@ hipError_t fill(long long n, T *p, T v, hipStream_t s);
 
 // kh_particle.hip
 template <typename T>
+hipError_t clear_particles(const PParams &pp, const DevParticle *P, T *const f[4], hipStream_t s);
+template <typename T>
 hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, hipStream_t s);
 template <typename T>
 hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, double coef, hipStream_t s);

@@ -144,6 +144,30 @@ __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParti
     }
 }
 
+// Zero the four particle-coupled fields (u_p and phi_sum) at every Sekibun
+// point of every particle: after a step these are the only points the
+// scatters wrote, so this restores all-zero fields with a small fraction of
+// the stores of a full fill. Overlapping particles store the same zero.
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_clear_particles(PParams pp, const DevParticle *__restrict__ P,
+                                                           T *__restrict__ f0, T *__restrict__ f1,
+                                                           T *__restrict__ f2, T *__restrict__ f3) {
+    const int    n     = blockIdx.x;
+    const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
+    const PCell  c     = particle_cell(pp, xp);
+    const int    N[3]  = {pp.nx, pp.ny, pp.nz};
+    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
+        int cell[3], rm[3];
+        sek_unpack(pp.sek[m], cell);
+        for (int d = 0; d < 3; d++) rm[d] = ((c.x_int[d] + cell[d]) % N[d] + N[d]) % N[d];
+        const long long idx = ((long long)rm[0] * pp.ny + rm[1]) * pp.nz + rm[2];
+        f0[idx] = (T)0;
+        f1[idx] = (T)0;
+        f2[idx] = (T)0;
+        f3[idx] = (T)0;
+    }
+}
+
 // ---------------------------------------------------------- quaternions
 // qtn_prod (quaternion.h:160-168) with alpha; qdot SPACE_FRAME (rigid_body.cxx:157-168)
 __device__ __forceinline__ void qdot_space(double *dq, const double *q, const double *w) {
@@ -410,7 +434,15 @@ hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t clear_particles(const PParams &pp, const DevParticle *P, T *const f[4], hipStream_t s) {
+    if (pp.np > 0)
+        hipLaunchKernelGGL(k_clear_particles<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, f[0], f[1], f[2], f[3]);
+    return hipGetLastError();
+}
+
 #define KH_INST(T)                                                                                              \
+    template hipError_t clear_particles<T>(const PParams &, const DevParticle *, T *const[4], hipStream_t);     \
     template hipError_t phi_scatter<T>(const PParams &, const DevParticle *, T *, hipStream_t);                 \
     template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, double, hipStream_t); \
     template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, T *const[3], hipStream_t);

@@ -87,6 +87,7 @@ struct kh_run {
     CTime        jikan;
     int          step_path  = KH_PATH_FUSED;
     int          zeta_dirty = 1;  // zeta may hold non-kept modes
+    int          pfield_dirty = 1;  // u_p / phi_sum not the scatter of the current positions
     // measurement
     bool                               timing = false;
     std::map<std::string, KernelTimes> times;
@@ -280,13 +281,22 @@ struct Step {
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
         for (int d = 0; d < 3; d++) KH_LAUNCH("y_inv", (cols_y<T, false>(g, S(r, d), 1, 0, s)));
         for (int d = 0; d < 3; d++) KH_LAUNCH("z_c2r", (fft_z_c2r<T>(g, S(r, d), U(r, d), s, g.trn_z)));
-        // particles
+        // particles; u_p and phi_sum hold the last step's scatter at the
+        // current positions, so clear just those points (Reset_phi/Reset_u)
+        T *pf[4] = {UP(r, 0), UP(r, 1), UP(r, 2), PS(r)};
+        if (r->pfield_dirty) {
+            for (int d = 0; d < 4; d++) KH_TRY(zero(r, pf[d]));
+            r->pfield_dirty = 0;
+        } else {
+            KH_LAUNCH("clear_particles", (clear_particles<T>(r->pp, r->dp, pf, s)));
+        }
         KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, s));
-        KH_TRY(make_phi(r));
+        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), s)));
         KH_TRY(calc_f_hydro(r));
         KH_TRY(velocity_update(r, ab2));
         if (r->kbt > 0.) KH_TRY(thermo(r));
-        KH_TRY(make_up(r));
+        T *up3s[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), up3s, s)));
         // f = P(up - phi u*), u = u* + f, zeta = curl u
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
         KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, s)));
@@ -329,7 +339,8 @@ struct Step {
     }
 
     static int apply(kh_run *r, int fn) {
-        r->zeta_dirty = 1;
+        r->zeta_dirty   = 1;
+        r->pfield_dirty = 1;
         switch (fn) {
             case KH_FN_NS_SOLVER: return ns_solver(r);
             case KH_FN_ZETA_K2U: return zeta_k2u(r, 0, 0);
@@ -690,6 +701,7 @@ int kh_set_particles(kh_run *r, const Particle *p, int n) {
         }
         d.spec = s.spec;
     }
+    r->pfield_dirty = 1;
     if (n > 0) KH_HIP(hipMemcpyAsync(r->dp, h.data(), sizeof(DevParticle) * n, hipMemcpyHostToDevice, r->stream));
     KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
@@ -781,6 +793,7 @@ int kh_get_field(kh_run *r, int field, double *out) {
 
 int kh_set_field(kh_run *r, int field, const double *in) {
     if (!r || !in) return fail(KH_ERR_ARG, "NULL argument");
+    r->pfield_dirty = 1;
     void *b = field_buffer(r, field);
     if (!b) return fail(KH_ERR_ARG, "field %d cannot be written", field);
     return upload_real(r, b, in);
@@ -788,7 +801,8 @@ int kh_set_field(kh_run *r, int field, const double *in) {
 
 int kh_init_fluid(kh_run *r) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
-    r->zeta_dirty = 1;
+    r->zeta_dirty   = 1;
+    r->pfield_dirty = 1;
     return r->mixed ? Step<float>::init_fluid(r) : Step<double>::init_fluid(r);
 }
 
