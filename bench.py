@@ -31,52 +31,76 @@ def field_words(n):
     return n * n * (n + 2)
 
 
-# Algorithmic field passes per launch (reads of distinct inputs + writes of
-# outputs, each counted once at full padded size), by kernel name.
-FIELD_PASSES = {
-    "fft_z_r2c": 2, "fft_z_c2r": 2, "fft_y_fwd": 2, "fft_y_inv": 2, "fft_x_fwd": 2, "fft_x_inv": 2,
-    "zeta_to_u": 5, "products": 8, "adv_ns": 9, "solenoidal": 6, "u_to_zeta": 5, "f_nonsole": 10,
-    "add3": 9, "fill": 1,
-}
-# Particle accesses per Sekibun point per launch
-POINT_ACCESSES = {"phi_scatter": 2, "f_hydro": 4, "up_scatter": 7}
-
-
 def step_bytes(n, p, npts, sf, sp):
-    """Algorithmic bytes of one reference step: 113 field passes plus 13
-    accesses per particle point (no squirmer slip)."""
+    """Algorithmic bytes of one reference-order step: 113 full-field passes
+    plus 13 accesses per particle point (no squirmer slip)."""
     return sf * field_words(n) * 113 + sp * p * npts * 13
 
 
 def kernel_bytes(name, n, p, npts, sf):
-    if name in FIELD_PASSES:
-        return FIELD_PASSES[name] * field_words(n) * sf
-    if name in POINT_ACCESSES:
-        return POINT_ACCESSES[name] * p * npts * sf
-    return None
+    """Algorithmic HBM bytes of one launch: every distinct input read once
+    and every output written once, over the modes and points the kernel's
+    function needs (2/3-kept modes |kx|,|ky| < T, kz < T with T = (N+2)/3;
+    Sekibun points of every particle). sf = bytes of one field scalar."""
+    t = (n + 2) // 3
+    kx, ky, kz = 2 * t - 1, 2 * t - 1, t
+    cs = 2 * sf                      # one complex
+    kept = kx * ky * kz              # kept modes
+    xcol = n * ky * kz               # x lines of the kept (ky, kz) columns
+    ycol = n * n * kz                # y/z lines of the kept kz planes
+    real = n ** 3
+    pt = p * npts
+    table = {
+        # fused pipeline
+        "x_zeta2u_inv": cs * (2 * kept + 3 * xcol),
+        "y_inv": cs * (3 * xcol + 3 * ycol),
+        "z_adv": cs * (3 * ycol + 5 * ycol),
+        "y_fwd_adv": cs * (5 * ycol + 5 * xcol),
+        "x_fwd_adv": cs * (5 * xcol + 5 * kept),
+        "adv_ns": cs * (5 + 2 + 2) * kept,
+        "z_c2r": cs * 3 * ycol + sf * 3 * real,
+        "z_force": sf * 7 * real + cs * 3 * ycol,
+        "y_fwd_force": cs * (3 * ycol + 3 * xcol),
+        "x_f_zeta": cs * (3 * xcol + 2 * kept + 2 * kept),
+        "fill": sf * real,
+        # particle kernels, per Sekibun point: phi cache (sf) + index (4 B)
+        "phi_scatter": pt * (sf + 4 + 2 * sf),          # cache write, phi_sum read-modify-write
+        "f_hydro": pt * (sf + 4 + sf + 3 * sf),          # cache, phi_sum, u
+        "up_scatter": pt * (sf + 4 + sf + 6 * sf),       # cache, phi_sum, u_p read-modify-write
+    }
+    return table.get(name)
 
 
-def time_gpu(precision, n, nparticles, steps, warmup, seed=3):
+def time_gpu(precision, n, nparticles, steps, warmup, graph=True, seed=3):
+    """K steps between two device synchronisations (the timed region; steps
+    after the first replay one captured HIP graph unless graph=False), then
+    K more steps uncaptured with a HIP event pair around every launch on the
+    run's stream for the per-kernel table."""
     import kapsel3_amd
     from kapsel3_amd import workloads
     P, parts = workloads.sedimentation(n, nparticles, kbt=1e-3, precision=precision, seed=seed)
     run = kapsel3_amd.Run(P, nparticles)
     run.set_particles(parts)
     run.init_fluid()
+    run.set_graph(graph)
     run.step(warmup)
-    run.sync()
-    run.timing(True)
-    run.timing_reset()
     run.sync()
     t0 = time.perf_counter()
     run.step(steps)
     run.sync()
     t1 = time.perf_counter()
+    run.timing(True)
+    run.timing_reset()
+    t2 = time.perf_counter()
+    run.step(steps)
+    run.sync()
+    t3 = time.perf_counter()
     stats = run.timing_stats()
+    run.timing(False)
     p = run.get_particles()
     finite = bool(np.isfinite(p["x"]).all() and np.isfinite(p["v"]).all())
     run.close()
-    return (t1 - t0), stats, finite
+    return (t1 - t0), stats, finite, (t3 - t2)
 
 
 def cpu_baseline(n, nparticles, steps, seed=3):
@@ -96,17 +120,37 @@ def cpu_baseline(n, nparticles, steps, seed=3):
     return steps / (t1 - t0), cores, t1 - t0
 
 
-def roofline_of(stats, n, nparticles, npts, sf, wall_ms_per_step, steps):
+def roofline_of(stats, n, nparticles, npts, sf, pass_ms_per_step, steps, traffic=None):
+    """Roofline entry of the kernel with the largest total time."""
     best = max(stats.items(), key=lambda kv: kv[1][1])
     name, (launches, ms) = best
     per_launch_ms = ms / launches
     b = kernel_bytes(name, n, nparticles, npts, sf)
     achieved = (b / (per_launch_ms * 1e-3) / 1e9) if b else None
-    share = ms / (wall_ms_per_step * steps)
+    share = ms / (pass_ms_per_step * steps)
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": (traffic or {}).get(name),
             "launches": launches, "avg_us": per_launch_ms * 1e3, "bytes_per_launch": b,
             "share_of_step": share}
+
+
+def kernel_table(stats, n, nparticles, npts, sf):
+    out = {}
+    for k, (c, ms) in sorted(stats.items()):
+        b = kernel_bytes(k, n, nparticles, npts, sf)
+        us = ms / c * 1e3
+        out[k] = {"launches": c, "avg_us": us, "bytes_per_launch": b,
+                  "gbs": (b / (us * 1e-6) / 1e9) if b else None}
+    return out
+
+
+def load_traffic(path):
+    """Measured HBM bytes per launch by kernel (profiler FETCH_SIZE/WRITE_SIZE
+    scaled by a plain copy's reading), as written next to the profiles."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def main():
@@ -119,7 +163,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="time the mixed step only")
+    ap.add_argument("--fp64-only", action="store_true", help="time the FP64 step only (value = FP64 steps/s)")
+    ap.add_argument("--no-graph", action="store_true", help="run the timed steps uncaptured")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_256_2000.json"),
+                    help="measured HBM bytes per launch (JSON by precision and kernel)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (torch) so profiler byte counters can be scaled")
     a = ap.parse_args()
@@ -139,30 +187,37 @@ def main():
         del src, dst
     npts = 1935  # Sekibun points of a = 5, xi = 2 (profile.cxx:22-65)
 
-    t_mix, st_mix, ok_mix = time_gpu(MIXED, n, P, a.steps, a.warmup)
-    ms_mix = t_mix / a.steps * 1e3
-    out = {"metric": METRIC, "value": a.steps / t_mix, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": ms_mix, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32 fields / f64 particles", "data": "synthetic",
-           "config": {"workload": f"SPM step, {n}^3 grid, {P} spheres a=5 xi=2, LJ 12:6 + gravity 1e-3 + kT 1e-3, "
-                                  "dt 0.05, mixed precision", "grid": n, "particles": P, "precision": "mixed"},
-           "grid_point_steps_per_s": n ** 3 * a.steps / t_mix, "finite": ok_mix}
-    sb = step_bytes(n, P, npts, 4, 4)
-    out["step_roofline"] = {"algorithmic_bytes": sb, "achieved": sb / (ms_mix * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": sb / (ms_mix * 1e-3) / 1e9 / HBM_PEAK_GBS}
-    out["roofline"] = roofline_of(st_mix, n, P, npts, 4, ms_mix, a.steps)
-    out["kernels_mixed"] = {k: {"launches": c, "avg_us": ms / c * 1e3} for k, (c, ms) in sorted(st_mix.items())}
+    traffic = load_traffic(a.traffic) if (n, P) == (256, 2000) else None
 
-    if not a.no_fp64:
-        k64 = a.fp64_steps or a.steps
-        t64, st64, ok64 = time_gpu(FP64, n, P, k64, a.warmup)
-        ms64 = t64 / k64 * 1e3
-        sb64 = step_bytes(n, P, npts, 8, 8)
-        out["fp64"] = {"value": k64 / t64, "unit": "steps/s", "ms_per_step": ms64, "steps": k64, "finite": ok64,
-                       "step_roofline": {"algorithmic_bytes": sb64, "achieved": sb64 / (ms64 * 1e-3) / 1e9,
-                                         "frac": sb64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS},
-                       "roofline": roofline_of(st64, n, P, npts, 8, ms64, k64)}
-        out["speedup_mixed_vs_fp64"] = out["value"] / out["fp64"]["value"]
+    def measure(prec, name, sf, k):
+        t, st, ok, tk = time_gpu(prec, n, P, k, a.warmup, graph=not a.no_graph)
+        ms, kms = t / k * 1e3, tk / k * 1e3
+        table = kernel_table(st, n, P, npts, sf)
+        sb = step_bytes(n, P, npts, sf, sf)
+        pruned = sum(v["bytes_per_launch"] * v["launches"] / k for v in table.values() if v["bytes_per_launch"])
+        return {"value": k / t, "unit": "steps/s", "ms_per_step": ms, "steps": k, "finite": ok,
+                "step_roofline": {"reference_step_bytes": sb, "achieved": sb / (ms * 1e-3) / 1e9,
+                                  "frac": sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "pruned_step_bytes": pruned, "pruned_frac": pruned / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+                "roofline": roofline_of(st, n, P, npts, sf, kms, k, (traffic or {}).get(name)),
+                "kernel_pass_ms_per_step": kms, "kernels": table}
+
+    head = measure(FP64, "fp64", 8, a.steps) if a.fp64_only else measure(MIXED, "mixed", 4, a.steps)
+    prec = "fp64" if a.fp64_only else "mixed"
+    out = {"metric": METRIC, "value": head["value"], "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64" if a.fp64_only else "f32 fields / f64 particles", "data": "synthetic",
+           "config": {"workload": f"SPM step, {n}^3 grid, {P} spheres a=5 xi=2, LJ 12:6 + gravity 1e-3 + kT 1e-3, "
+                                  f"dt 0.05, {prec} precision", "grid": n, "particles": P, "precision": prec},
+           "roofline": head["roofline"], "cpu_baseline": None,
+           "grid_point_steps_per_s": n ** 3 * head["value"], "finite": head["finite"], "graph": not a.no_graph,
+           "step_roofline": head["step_roofline"], "kernel_pass_ms_per_step": head["kernel_pass_ms_per_step"],
+           "kernels": head["kernels"]}
+    if not a.no_fp64 and not a.fp64_only:
+        f64 = measure(FP64, "fp64", 8, a.fp64_steps or a.steps)
+        out["fp64"] = f64
+        out["speedup_mixed_vs_fp64"] = out["value"] / f64["value"]
 
     if not a.no_cpu_baseline:
         sps, cores, secs = cpu_baseline(n, P, a.cpu_steps)

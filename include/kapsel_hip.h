@@ -160,6 +160,10 @@ int kh_step(kh_run *run, int nsteps);
  * Both give the reference's step to round-off. */
 enum { KH_PATH_FUSED = 0, KH_PATH_REFERENCE_ORDER = 1 };
 int kh_set_step_path(kh_run *run, int path);
+/* Steady-state steps (ts >= 1) of the fused path run as one captured HIP
+ * graph by default (captured on first use); 0 turns that off. Kernel
+ * timing (kh_timing_enable) also runs steps uncaptured. */
+int kh_set_graph(kh_run *run, int on);
 /* One function of the step on the device state (enum kh_fn). */
 int kh_apply(kh_run *run, int fn);
 /* Wait for the run's stream; reports errors raised by enqueued work. */

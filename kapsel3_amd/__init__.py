@@ -22,7 +22,7 @@ _LIB = None
 EXPORTS = [
     "kh_run_create", "kh_run_destroy", "kh_set_particles", "kh_get_particles", "kh_set_fluid", "kh_get_fluid",
     "kh_set_time", "kh_get_time", "kh_get_field", "kh_set_field", "kh_init_fluid", "kh_step", "kh_apply",
-    "kh_sync", "kh_A2a_k", "kh_A_k2a", "kh_set_step_path", "kh_timing_enable", "kh_timing_reset", "kh_timing_get",
+    "kh_sync", "kh_A2a_k", "kh_A_k2a", "kh_set_step_path", "kh_set_graph", "kh_timing_enable", "kh_timing_reset", "kh_timing_get",
     "kh_timing_count", "kh_timing_name", "kh_strerror", "kh_last_error",
 ]
 
@@ -54,6 +54,7 @@ def lib():
         L.kh_apply.argtypes = [vp, C.c_int]
         L.kh_sync.argtypes = [vp]
         L.kh_set_step_path.argtypes = [vp, C.c_int]
+        L.kh_set_graph.argtypes = [vp, C.c_int]
         L.kh_A2a_k.argtypes = [vp, vp]
         L.kh_A_k2a.argtypes = [vp, vp]
         L.kh_timing_enable.argtypes = [vp, C.c_int]
@@ -155,6 +156,10 @@ class Run:
     def set_step_path(self, path):
         """'fused' (default) or 'reference' (the reference's operation order)."""
         _check(lib().kh_set_step_path(self.h, {"fused": 0, "reference": 1}[path]))
+
+    def set_graph(self, on=True):
+        """Run steady-state steps as one captured HIP graph (default on)."""
+        _check(lib().kh_set_graph(self.h, 1 if on else 0))
 
     def sync(self):
         _check(lib().kh_sync(self.h))

@@ -13,6 +13,14 @@ hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s);
 // kzvalid >= 0: modes kz >= kzvalid are taken as zero (not read)
 template <typename T>
 hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s, int kzvalid = -1);
+// up to five components in one launch
+template <typename T>
+struct ZIO {
+    const cpx<T> *in[5];
+    T            *out[5];
+};
+template <typename T>
+hipError_t fft_z_c2r_multi(const Grid &g, const ZIO<T> &io, int ncomp, hipStream_t s, int kzvalid = -1);
 template <typename T, bool FWD>
 hipError_t fft_y(const Grid &g, cpx<T> *data, hipStream_t s);
 template <typename T, bool FWD>

@@ -90,9 +90,10 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_r2c(const T *__
 }
 
 template <typename T, int L>
-__global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_c2r(const cpx<T> *__restrict__ in,
-                                                                     T *__restrict__ out, long long nrows, T scale,
+__global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_c2r(ZIO<T> io, long long nrows, T scale,
                                                                      int kzvalid) {
+    const cpx<T> *__restrict__ in  = io.in[blockIdx.y];  // one component per grid row
+    T *__restrict__            out = io.out[blockIdx.y];
     using Z = ZPass<T, L>;
     constexpr int NT = Z::NT, P = Z::P, NZH = Z::NZH, BS = Z::BS;
     __shared__ cpx<T> s[Z::SZ], tw[L];
@@ -195,14 +196,14 @@ static hipError_t launch_z_r2c(int L, const T *in, cpx<T> *out, long long nrows,
 }
 
 template <typename T>
-static hipError_t launch_z_c2r(int L, const cpx<T> *in, T *out, long long nrows, T scale, int kzvalid,
+static hipError_t launch_z_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzvalid,
                                hipStream_t s) {
     switch (L) {
 #define KH_CASE(n)                                                                       \
     case n: {                                                                            \
         constexpr int P  = ZPass<T, n>::P;                                               \
         long long     nb = (nrows + 2 * P - 1) / (2 * P);                                \
-        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb), dim3(ZPass<T, n>::NT), 0, s, in, out, nrows, scale, kzvalid); \
+        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb, ncomp), dim3(ZPass<T, n>::NT), 0, s, io, nrows, scale, kzvalid); \
         return hipGetLastError();                                                        \
     }
         KH_FOR_EACH_L(KH_CASE)
@@ -235,8 +236,15 @@ hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s) {
 }
 template <typename T>
 hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s, int kzvalid) {
+    ZIO<T> io{};
+    io.in[0]  = in;
+    io.out[0] = out;
+    return fft_z_c2r_multi<T>(g, io, 1, s, kzvalid);
+}
+template <typename T>
+hipError_t fft_z_c2r_multi(const Grid &g, const ZIO<T> &io, int ncomp, hipStream_t s, int kzvalid) {
     const T scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
-    return launch_z_c2r<T>(g.nz, in, out, (long long)g.nx * g.ny, scale, kzvalid < 0 ? g.nzh : kzvalid, s);
+    return launch_z_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzvalid < 0 ? g.nzh : kzvalid, s);
 }
 // y: lines along y at stride nzh, columns kz, one outer block per x
 template <typename T, bool FWD>
@@ -269,6 +277,7 @@ hipError_t fft3d_c2r(const Grid &g, cpx<T> *spec_inout, T *real_out, hipStream_t
 #define KH_INST(T)                                                                   \
     template hipError_t fft_z_r2c<T>(const Grid &, const T *, cpx<T> *, hipStream_t); \
     template hipError_t fft_z_c2r<T>(const Grid &, const cpx<T> *, T *, hipStream_t, int); \
+    template hipError_t fft_z_c2r_multi<T>(const Grid &, const ZIO<T> &, int, hipStream_t, int); \
     template hipError_t fft_y<T, true>(const Grid &, cpx<T> *, hipStream_t);          \
     template hipError_t fft_y<T, false>(const Grid &, cpx<T> *, hipStream_t);         \
     template hipError_t fft_x<T, true>(const Grid &, cpx<T> *, hipStream_t);          \

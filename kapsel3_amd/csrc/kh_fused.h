@@ -6,11 +6,17 @@
 
 namespace kh {
 
+// up to five spectral components transformed by one launch (grid row = component)
+template <typename T>
+struct Comps {
+    cpx<T> *p[5];
+};
+
 // y passes on columns kz < TRN_Z; x passes on kept (ky,kz) columns
 template <typename T, bool FWD>
-hipError_t cols_y(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s);
+hipError_t cols_y(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, int out_mask, hipStream_t s);
 template <typename T, bool FWD>
-hipError_t cols_x(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s);
+hipError_t cols_x(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, int out_mask, hipStream_t s);
 // zeta -> u_hat (truncated) -> inverse x pass, 3 components
 template <typename T>
 hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const u[3], double *ukdc,

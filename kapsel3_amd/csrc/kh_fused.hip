@@ -52,8 +52,9 @@ __device__ __forceinline__ int kept_index(int o, int n, int trn) { return o < tr
 // in_mask: rows whose wavenumber is truncated are taken as zero (not read).
 // out_mask: such rows are not written.
 template <typename T, int L, bool FWD, bool XMODE>
-__global__ __launch_bounds__(PassThreads<T>::value) void k_cols(cpx<T> *__restrict__ data, Grid g, int ncols,
-                                                                int in_mask, int out_mask) {
+__global__ __launch_bounds__(PassThreads<T>::value) void k_cols(Comps<T> comps, Grid g, int ncols, int in_mask,
+                                                                int out_mask) {
+    cpx<T> *__restrict__ data = comps.p[blockIdx.y];  // one component per grid row
     constexpr int NT  = PassThreads<T>::value;
     constexpr int C   = ColTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
@@ -497,14 +498,14 @@ __global__ void k_zeta_clean(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict
 static inline int nky(const Grid &g) { return 2 * g.trn_y - 1; }
 
 template <typename T, bool FWD>
-hipError_t cols_y(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s) {
+hipError_t cols_y(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, int out_mask, hipStream_t s) {
     constexpr int C  = ColTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
     switch (g.ny) {
 #define KH_CASE(n)                                                                                                 \
     case n:                                                                                                        \
-        hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt), dim3(PassThreads<T>::value), 0, s, data, g, g.trn_z,     \
-                           in_mask, out_mask);                                                                     \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt, ncomp), dim3(PassThreads<T>::value), 0, s, \
+                           data, g, g.trn_z, in_mask, out_mask);                                                   \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
 #undef KH_CASE
@@ -513,14 +514,14 @@ hipError_t cols_y(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStr
 }
 
 template <typename T, bool FWD>
-hipError_t cols_x(const Grid &g, cpx<T> *data, int in_mask, int out_mask, hipStream_t s) {
+hipError_t cols_x(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, int out_mask, hipStream_t s) {
     constexpr int C  = ColTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, data, g, g.trn_z,     \
-                           in_mask, out_mask);                                                                      \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt, ncomp), dim3(PassThreads<T>::value), 0, s, \
+                           data, g, g.trn_z, in_mask, out_mask);                                                    \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
 #undef KH_CASE
@@ -603,10 +604,10 @@ hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s) {
 }
 
 #define KH_INST(T)                                                                                                  \
-    template hipError_t cols_y<T, true>(const Grid &, cpx<T> *, int, int, hipStream_t);                             \
-    template hipError_t cols_y<T, false>(const Grid &, cpx<T> *, int, int, hipStream_t);                            \
-    template hipError_t cols_x<T, true>(const Grid &, cpx<T> *, int, int, hipStream_t);                             \
-    template hipError_t cols_x<T, false>(const Grid &, cpx<T> *, int, int, hipStream_t);                            \
+    template hipError_t cols_y<T, true>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);                \
+    template hipError_t cols_y<T, false>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);               \
+    template hipError_t cols_x<T, true>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);                \
+    template hipError_t cols_x<T, false>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);               \
     template hipError_t x_zeta2u_inv<T>(const Grid &, const cpx<T> *, const cpx<T> *, cpx<T> *const[3], double *,   \
                                         int, hipStream_t);                                                          \
     template hipError_t z_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], hipStream_t);                   \

@@ -28,19 +28,22 @@ hipError_t add3(long long n, T *const u[3], T *const f[3], hipStream_t s);
 template <typename T>
 hipError_t fill(long long n, T *p, T v, hipStream_t s);
 
-// kh_particle.hip
+// kh_particle.hip (cphi/cidx: per-particle cache of phi and grid index at
+// every Sekibun point, np * np_domain entries)
 template <typename T>
-hipError_t clear_particles(const PParams &pp, const DevParticle *P, T *const f[4], hipStream_t s);
+hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter,
+                       hipStream_t s);
 template <typename T>
-hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, hipStream_t s);
+hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi, const int *cidx,
+                   double coef, hipStream_t s);
 template <typename T>
-hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, double coef, hipStream_t s);
-template <typename T>
-hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, T *const up[3], hipStream_t s);
+hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
+                      T *const up[3], hipStream_t s);
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s);
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s);
 hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s);
-hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, int ts, double niv,
+hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s);
+hipError_t tick(int *d_ts, hipStream_t s);
 
 }  // namespace kh

@@ -62,48 +62,67 @@ __device__ __forceinline__ void angular2v(int rotation, const double *om, const 
     }
 }
 
-// Make_phi_particle_sum scatter (make_phi.cxx:220-246)
-template <typename T>
+// Make_phi_particle_sum scatter (make_phi.cxx:220-246). Also stores each
+// point's phi and grid index in the per-particle cache the gather and the
+// u_p scatter of the same step read (positions do not change in between).
+// SCATTER = false only fills the cache.
+template <typename T, bool SCATTER>
 __global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, const DevParticle *__restrict__ P,
-                                                       T *__restrict__ phisum) {
-    const int    n     = blockIdx.x;
-    const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
-    const PCell  c     = particle_cell(pp, xp);
+                                                       T *__restrict__ phisum, T *__restrict__ cphi,
+                                                       int *__restrict__ cidx) {
+    const int       n     = blockIdx.x;
+    const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
+    const PCell     c     = particle_cell(pp, xp);
+    const long long cb    = (long long)n * pp.np_domain;
     for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
         const PPoint q  = sek_point(pp, c, xp, pp.sek[m]);
         const double ph = prof_phi(pp, q.dist);
-        if (ph != 0.0) unsafeAtomicAdd(&phisum[q.idx], (T)ph);
+        cphi[cb + m]    = (T)ph;
+        cidx[cb + m]    = (int)q.idx;
+        if (SCATTER && ph != 0.0) unsafeAtomicAdd(&phisum[q.idx], (T)ph);
     }
 }
 
-// Calc_f_hydro_correct_precision (md_force.cxx:383-506), non-rigid branch
+// r = cell*dx - residue (Relative_coord, make_phi.h:154-170)
+__device__ __forceinline__ void sek_r(const PParams &pp, const PCell &c, int code, double *r) {
+    int cell[3];
+    sek_unpack(code, cell);
+#pragma unroll
+    for (int d = 0; d < 3; d++) r[d] = (double)cell[d] * pp.dx - c.residue[d];
+}
+
+// Calc_f_hydro_correct_precision (md_force.cxx:383-506), non-rigid branch,
+// over the cached phi and grid index of each Sekibun point.
 template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__restrict__ P, const T *__restrict__ u0,
                                                    const T *__restrict__ u1, const T *__restrict__ u2,
-                                                   const T *__restrict__ phisum, double coef) {
-    const int    n     = blockIdx.x;
-    const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
-    const double vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
-    const double om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
-    const PCell  c     = particle_cell(pp, xp);
-    Neu          acc[6];
+                                                   const T *__restrict__ phisum, const T *__restrict__ cphi,
+                                                   const int *__restrict__ cidx, double coef) {
+    const int       n     = blockIdx.x;
+    const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
+    const double    vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
+    const double    om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
+    const PCell     c     = particle_cell(pp, xp);
+    const long long cb    = (long long)n * pp.np_domain;
+    Neu             acc[6];
     for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
-        const PPoint q  = sek_point(pp, c, xp, pp.sek[m]);
-        const double ph = prof_phi(pp, q.dist);
+        const double ph = (double)cphi[cb + m];
         if (ph == 0.0) continue;
-        double vr[3];
-        angular2v(pp.rotation, om, q.r, vr);
-        const double dmy_phi = ph / fmax((double)phisum[q.idx], 1.0);
+        const int idx = cidx[cb + m];
+        double    r[3], vr[3];
+        sek_r(pp, c, pp.sek[m], r);
+        angular2v(pp.rotation, om, r, vr);
+        const double dmy_phi = ph / fmax((double)phisum[idx], 1.0);
         double       fp[3];
-        fp[0] = ((vp[0] + vr[0]) - (double)u0[q.idx]) * dmy_phi;
-        fp[1] = ((vp[1] + vr[1]) - (double)u1[q.idx]) * dmy_phi;
-        fp[2] = ((vp[2] + vr[2]) - (double)u2[q.idx]) * dmy_phi;
+        fp[0] = ((vp[0] + vr[0]) - (double)u0[idx]) * dmy_phi;
+        fp[1] = ((vp[1] + vr[1]) - (double)u1[idx]) * dmy_phi;
+        fp[2] = ((vp[2] + vr[2]) - (double)u2[idx]) * dmy_phi;
         acc[0].add(fp[0]);
         acc[1].add(fp[1]);
         acc[2].add(fp[2]);
-        acc[3].add(q.r[1] * fp[2] - q.r[2] * fp[1]);
-        acc[4].add(q.r[2] * fp[0] - q.r[0] * fp[2]);
-        acc[5].add(q.r[0] * fp[1] - q.r[1] * fp[0]);
+        acc[3].add(r[1] * fp[2] - r[2] * fp[1]);
+        acc[4].add(r[2] * fp[0] - r[0] * fp[2]);
+        acc[5].add(r[0] * fp[1] - r[1] * fp[0]);
     }
     double v[6];
     for (int k = 0; k < 6; k++) v[k] = acc[k].val();
@@ -121,50 +140,29 @@ __global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__re
     }
 }
 
-// Make_u_particle_sum scatter (make_phi.cxx:313-354)
+// Make_u_particle_sum scatter (make_phi.cxx:313-354) over the cache
 template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParticle *__restrict__ P,
-                                                      const T *__restrict__ phisum, T *__restrict__ up0,
+                                                      const T *__restrict__ phisum, const T *__restrict__ cphi,
+                                                      const int *__restrict__ cidx, T *__restrict__ up0,
                                                       T *__restrict__ up1, T *__restrict__ up2) {
-    const int    n     = blockIdx.x;
-    const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
-    const double vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
-    const double om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
-    const PCell  c     = particle_cell(pp, xp);
+    const int       n     = blockIdx.x;
+    const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
+    const double    vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
+    const double    om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
+    const PCell     c     = particle_cell(pp, xp);
+    const long long cb    = (long long)n * pp.np_domain;
     for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
-        const PPoint q  = sek_point(pp, c, xp, pp.sek[m]);
-        const double ph = prof_phi(pp, q.dist);
+        const double ph = (double)cphi[cb + m];
         if (ph == 0.0) continue;
-        const double dmy_phi = ph / fmax((double)phisum[q.idx], 1.0);
-        double       vr[3];
-        angular2v(pp.rotation, om, q.r, vr);
-        unsafeAtomicAdd(&up0[q.idx], (T)((vp[0] + vr[0]) * dmy_phi));
-        unsafeAtomicAdd(&up1[q.idx], (T)((vp[1] + vr[1]) * dmy_phi));
-        unsafeAtomicAdd(&up2[q.idx], (T)((vp[2] + vr[2]) * dmy_phi));
-    }
-}
-
-// Zero the four particle-coupled fields (u_p and phi_sum) at every Sekibun
-// point of every particle: after a step these are the only points the
-// scatters wrote, so this restores all-zero fields with a small fraction of
-// the stores of a full fill. Overlapping particles store the same zero.
-template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_clear_particles(PParams pp, const DevParticle *__restrict__ P,
-                                                           T *__restrict__ f0, T *__restrict__ f1,
-                                                           T *__restrict__ f2, T *__restrict__ f3) {
-    const int    n     = blockIdx.x;
-    const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
-    const PCell  c     = particle_cell(pp, xp);
-    const int    N[3]  = {pp.nx, pp.ny, pp.nz};
-    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
-        int cell[3], rm[3];
-        sek_unpack(pp.sek[m], cell);
-        for (int d = 0; d < 3; d++) rm[d] = ((c.x_int[d] + cell[d]) % N[d] + N[d]) % N[d];
-        const long long idx = ((long long)rm[0] * pp.ny + rm[1]) * pp.nz + rm[2];
-        f0[idx] = (T)0;
-        f1[idx] = (T)0;
-        f2[idx] = (T)0;
-        f3[idx] = (T)0;
+        const int    idx     = cidx[cb + m];
+        const double dmy_phi = ph / fmax((double)phisum[idx], 1.0);
+        double       r[3], vr[3];
+        sek_r(pp, c, pp.sek[m], r);
+        angular2v(pp.rotation, om, r, vr);
+        unsafeAtomicAdd(&up0[idx], (T)((vp[0] + vr[0]) * dmy_phi));
+        unsafeAtomicAdd(&up1[idx], (T)((vp[1] + vr[1]) * dmy_phi));
+        unsafeAtomicAdd(&up2[idx], (T)((vp[2] + vr[2]) * dmy_phi));
     }
 }
 
@@ -372,10 +370,11 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
 
 // Add_random_force_thermostat (fluct.cxx:10-43); normals from Philox keyed
 // by (seed, stream) with counter (particle, step, block, stream).
-__global__ void k_thermostat(PParams pp, DevParticle *__restrict__ P, uint64_t seed, uint32_t stream, int ts,
-                             double niv, double nio) {
+__global__ void k_thermostat(PParams pp, DevParticle *__restrict__ P, uint64_t seed, uint32_t stream,
+                             const int *__restrict__ d_ts, double niv, double nio) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= pp.np) return;
+    const int ts = *d_ts;
     DevParticle      &p = P[n];
     const DevSpecies &s = pp.species[p.spec];
     if (s.propulsion == 2) return;
@@ -399,20 +398,30 @@ __global__ void k_thermostat(PParams pp, DevParticle *__restrict__ P, uint64_t s
 static inline unsigned pblocks(int n) { return (unsigned)((n + 63) / 64); }
 
 template <typename T>
-hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, hipStream_t s) {
-    if (pp.np > 0) hipLaunchKernelGGL(k_phi_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum);
+hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter,
+                       hipStream_t s) {
+    if (pp.np > 0) {
+        if (scatter)
+            hipLaunchKernelGGL((k_phi_scatter<T, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx);
+        else
+            hipLaunchKernelGGL((k_phi_scatter<T, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx);
+    }
     return hipGetLastError();
 }
 template <typename T>
-hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, double coef, hipStream_t s) {
+hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi, const int *cidx,
+                   double coef, hipStream_t s) {
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_f_hydro<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, coef);
+        hipLaunchKernelGGL(k_f_hydro<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, cphi, cidx,
+                           coef);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, T *const up[3], hipStream_t s) {
+hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
+                      T *const up[3], hipStream_t s) {
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_up_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, up[0], up[1], up[2]);
+        hipLaunchKernelGGL(k_up_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx, up[0], up[1],
+                           up[2]);
     return hipGetLastError();
 }
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s) {
@@ -427,25 +436,28 @@ hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStr
     if (pp.np > 0) hipLaunchKernelGGL(k_velocity, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, ab2, h);
     return hipGetLastError();
 }
-hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, int ts, double niv,
+hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s) {
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_thermostat, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, seed, stream, ts, niv, nio);
+        hipLaunchKernelGGL(k_thermostat, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, seed, stream, d_ts, niv, nio);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t clear_particles(const PParams &pp, const DevParticle *P, T *const f[4], hipStream_t s) {
-    if (pp.np > 0)
-        hipLaunchKernelGGL(k_clear_particles<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, f[0], f[1], f[2], f[3]);
+// jikan.ts += 1 on the device (the step counter the noise stream is keyed by)
+__global__ void k_tick(int *__restrict__ d_ts) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *d_ts += 1;
+}
+hipError_t tick(int *d_ts, hipStream_t s) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, d_ts);
     return hipGetLastError();
 }
 
 #define KH_INST(T)                                                                                              \
-    template hipError_t clear_particles<T>(const PParams &, const DevParticle *, T *const[4], hipStream_t);     \
-    template hipError_t phi_scatter<T>(const PParams &, const DevParticle *, T *, hipStream_t);                 \
-    template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, double, hipStream_t); \
-    template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, T *const[3], hipStream_t);
+    template hipError_t phi_scatter<T>(const PParams &, const DevParticle *, T *, T *, int *, int, hipStream_t);   \
+    template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *, const int *, \
+                                   double, hipStream_t);                                                          \
+    template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, const T *, const int *,     \
+                                      T *const[3], hipStream_t);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST

@@ -83,11 +83,19 @@ struct kh_run {
     DevParticle *dp      = nullptr;
     DevSpecies  *ds      = nullptr;
     int         *sek     = nullptr;
+    void        *cphi    = nullptr;  // per-particle phi at each Sekibun point (T)
+    int         *cidx    = nullptr;  // per-particle grid index of each Sekibun point
+    int          pcache_valid = 0;
+    int         *d_ts         = nullptr;  // jikan.ts on the device (noise counter)
+    // the steady-state step (ts >= 1) captured once as a HIP graph
+    int              use_graph  = 1;
+    bool             capturing  = false;
+    hipGraph_t       graph      = nullptr;
+    hipGraphExec_t   graph_exec = nullptr;
     int          np_domain = 0;
     CTime        jikan;
     int          step_path  = KH_PATH_FUSED;
     int          zeta_dirty = 1;  // zeta may hold non-kept modes
-    int          pfield_dirty = 1;  // u_p / phi_sum not the scatter of the current positions
     // measurement
     bool                               timing = false;
     std::map<std::string, KernelTimes> times;
@@ -132,13 +140,13 @@ struct Timed {
     const char *name;
     hipEvent_t  a = nullptr;
     Timed(kh_run *r_, const char *n) : r(r_), name(n) {
-        if (r->timing) {
+        if (r->timing && !r->capturing) {
             a = take_event(r);
             if (a) (void)hipEventRecord(a, r->stream);
         }
     }
     ~Timed() {
-        if (r->timing && a) {
+        if (r->timing && !r->capturing && a) {
             hipEvent_t b = take_event(r);
             if (b) {
                 (void)hipEventRecord(b, r->stream);
@@ -209,14 +217,34 @@ struct Step {
         return KH_OK;
     }
 
-    static int make_phi(kh_run *r) {
-        KH_TRY(zero(r, PS(r)));
-        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), r->stream)));
+    static T *CP(kh_run *r) { return (T *)r->cphi; }
+    static int pos_update(kh_run *r, int ab2) {
+        KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
+        r->pcache_valid = 0;
         return KH_OK;
     }
+    // scatter phi (Make_phi_particle_sum) and fill the per-particle cache
+    static int scatter_phi(kh_run *r) {
+        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, 1, r->stream)));
+        r->pcache_valid = 1;
+        return KH_OK;
+    }
+    static int ensure_cache(kh_run *r) {
+        if (!r->pcache_valid) {
+            KH_LAUNCH("phi_cache", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, 0, r->stream)));
+            r->pcache_valid = 1;
+        }
+        return KH_OK;
+    }
+    static int make_phi(kh_run *r) {
+        KH_TRY(zero(r, PS(r)));
+        return scatter_phi(r);
+    }
     static int calc_f_hydro(kh_run *r) {
+        KH_TRY(ensure_cache(r));
         T *u[3] = {U(r, 0), U(r, 1), U(r, 2)};
-        KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, u, PS(r), (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->stream)));
+        KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, u, PS(r), CP(r), r->cidx,
+                                         (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->stream)));
         return KH_OK;
     }
     static int velocity_update(kh_run *r, int ab2) {
@@ -230,14 +258,18 @@ struct Step {
         const double zeta_rot  = 8. * KH_PI * r->eta * r->radius * r->radius * r->radius;
         const double niv       = kT_snap_v * sqrt(zeta_drag * r->jikan.dt_md * r->kbt * r->P.alpha_v);
         const double nio       = kT_snap_o * sqrt(zeta_rot * r->jikan.dt_md * r->kbt * r->P.alpha_o);
-        KH_LAUNCH("thermostat", thermostat(r->pp, r->dp, r->P.seed, r->P.stream_id, r->jikan.ts, niv, nio, r->stream));
+        KH_LAUNCH("thermostat", thermostat(r->pp, r->dp, r->P.seed, r->P.stream_id, r->d_ts, niv, nio, r->stream));
+        return KH_OK;
+    }
+    static int scatter_up(kh_run *r) {
+        KH_TRY(ensure_cache(r));
+        T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, up3, r->stream)));
         return KH_OK;
     }
     static int make_up(kh_run *r) {
         for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
-        T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
-        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), up3, r->stream)));
-        return KH_OK;
+        return scatter_up(r);
     }
     // Make_f_particle_dt_sole (f_particle.h:60-67) into W[0..2]
     static int f_particle_sole(kh_run *r) {
@@ -259,7 +291,14 @@ struct Step {
     // pipeline of kh_fused.hip. Differences from step_reference_order are
     // round-off only: FFT(IFFT(a)) = a for the band-limited fields, so the
     // final U2zeta_k of u* + f is taken directly from u*_hat + P f_hat.
+    // one step + the device step counter
     static int step(kh_run *r) {
+        KH_TRY(step_body(r));
+        KH_LAUNCH("tick", tick(r->d_ts, r->stream));
+        return KH_OK;
+    }
+
+    static int step_body(kh_run *r) {
         if (r->step_path == KH_PATH_REFERENCE_ORDER) return step_reference_order(r);
         const int   ab2 = r->jikan.ts != 0;
         const Grid &g   = r->g;
@@ -268,39 +307,34 @@ struct Step {
             KH_LAUNCH("zeta_clean", (zeta_clean<T>(g, Z(r, 0), Z(r, 1), s)));
             r->zeta_dirty = 0;
         }
-        cpx<T> *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
-        cpx<T> *s5[5] = {S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)};
+        cpx<T>  *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
+        cpx<T>  *s5[5] = {S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)};
+        Comps<T> c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
         // NS_solver_slavedEuler: zeta -> u -> products -> -Omega -> slaved Euler
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, 0, s)));
-        for (int d = 0; d < 3; d++) KH_LAUNCH("y_inv", (cols_y<T, false>(g, S(r, d), 1, 0, s)));
+        KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
         KH_LAUNCH("z_adv", (z_adv<T>(g, s3, s5, s)));
-        for (int q = 0; q < 5; q++) KH_LAUNCH("y_fwd", (cols_y<T, true>(g, S(r, q), 0, 1, s)));
-        for (int q = 0; q < 5; q++) KH_LAUNCH("x_fwd", (cols_x<T, true>(g, S(r, q), 0, 1, s)));
+        KH_LAUNCH("y_fwd_adv", (cols_y<T, true>(g, c5, 5, 0, 1, s)));
+        KH_LAUNCH("x_fwd_adv", (cols_x<T, true>(g, c5, 5, 0, 1, s)));
         KH_LAUNCH("adv_ns", (adv_ns<T>(g, s5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
         // FIX_CELL, truncation, Zeta_k2u -> u* in real space
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
-        for (int d = 0; d < 3; d++) KH_LAUNCH("y_inv", (cols_y<T, false>(g, S(r, d), 1, 0, s)));
-        for (int d = 0; d < 3; d++) KH_LAUNCH("z_c2r", (fft_z_c2r<T>(g, S(r, d), U(r, d), s, g.trn_z)));
-        // particles; u_p and phi_sum hold the last step's scatter at the
-        // current positions, so clear just those points (Reset_phi/Reset_u)
-        T *pf[4] = {UP(r, 0), UP(r, 1), UP(r, 2), PS(r)};
-        if (r->pfield_dirty) {
-            for (int d = 0; d < 4; d++) KH_TRY(zero(r, pf[d]));
-            r->pfield_dirty = 0;
-        } else {
-            KH_LAUNCH("clear_particles", (clear_particles<T>(r->pp, r->dp, pf, s)));
-        }
-        KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, s));
-        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), s)));
+        KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
+        ZIO<T> zio{{S(r, 0), S(r, 1), S(r, 2)}, {U(r, 0), U(r, 1), U(r, 2)}};
+        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z)));
+        // particles (Reset_phi / Reset_u, scatters, gather, forces, integrators)
+        for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
+        KH_TRY(zero(r, PS(r)));
+        KH_TRY(pos_update(r, ab2));
+        KH_TRY(scatter_phi(r));
         KH_TRY(calc_f_hydro(r));
         KH_TRY(velocity_update(r, ab2));
         if (r->kbt > 0.) KH_TRY(thermo(r));
-        T *up3s[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
-        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), up3s, s)));
+        KH_TRY(scatter_up(r));
         // f = P(up - phi u*), u = u* + f, zeta = curl u
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
         KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, s)));
-        for (int d = 0; d < 3; d++) KH_LAUNCH("y_fwd", (cols_y<T, true>(g, S(r, d), 0, 1, s)));
+        KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
         KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));
         return KH_OK;
     }
@@ -311,7 +345,7 @@ struct Step {
         const int ab2 = r->jikan.ts != 0;
         KH_TRY(ns_solver(r));
         KH_TRY(zeta_k2u(r, r->fixmask));  // FIX_CELL, truncation, u*
-        KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
+        KH_TRY(pos_update(r, ab2));
         KH_TRY(make_phi(r));
         KH_TRY(calc_f_hydro(r));
         KH_TRY(velocity_update(r, ab2));
@@ -339,18 +373,13 @@ struct Step {
     }
 
     static int apply(kh_run *r, int fn) {
-        r->zeta_dirty   = 1;
-        r->pfield_dirty = 1;
+        r->zeta_dirty = 1;
         switch (fn) {
             case KH_FN_NS_SOLVER: return ns_solver(r);
             case KH_FN_ZETA_K2U: return zeta_k2u(r, 0, 0);
             case KH_FN_U2ZETA_K: return u2zeta_k(r);
-            case KH_FN_POSITION_EULER:
-                KH_LAUNCH("position", position(r->pp, r->dp, 0, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
-                return KH_OK;
-            case KH_FN_POSITION_AB2:
-                KH_LAUNCH("position", position(r->pp, r->dp, 1, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
-                return KH_OK;
+            case KH_FN_POSITION_EULER: return pos_update(r, 0);
+            case KH_FN_POSITION_AB2: return pos_update(r, 1);
             case KH_FN_MAKE_PHI: return make_phi(r);
             case KH_FN_F_HYDRO: return calc_f_hydro(r);
             case KH_FN_VELOCITY_EULER: return velocity_update(r, 0);
@@ -440,9 +469,12 @@ void *field_buffer(kh_run *r, int field) {
 int free_run(kh_run *r) {
     if (!r) return KH_OK;
     if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
     void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
                     r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
-                    r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek};
+                    r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek, r->cphi, r->cidx,
+                    r->d_ts};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &p : r->pending) {
@@ -478,6 +510,13 @@ int check_params(const kh_params *P, int np) {
         if (!(P->species[i].mass_ratio > 0)) return fail(KH_ERR_ARG, "mass_ratio must be positive");
     }
     return KH_OK;
+}
+
+void drop_graph(kh_run *r) {
+    if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
+    r->graph_exec = nullptr;
+    r->graph      = nullptr;
 }
 
 }  // namespace
@@ -608,13 +647,17 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     for (int i = 0; i < 5 && st == KH_OK; i++) st = alloc(&r->work[i], rb);
     if (st == KH_OK) st = alloc(&r->phisum, rb);
     if (st == KH_OK) st = alloc((void **)&r->ukdc, 3 * sizeof(double));
+    if (st == KH_OK) st = alloc((void **)&r->d_ts, sizeof(int));
     if (st == KH_OK) st = alloc((void **)&r->dp, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1));
     if (st == KH_OK) st = alloc((void **)&r->ds, sizeof(DevSpecies) * hs.size());
     if (st == KH_OK) st = alloc((void **)&r->sek, sizeof(int) * sek.size());
+    const size_t ncache = (size_t)(np > 0 ? np : 1) * sek.size();
+    if (st == KH_OK) st = alloc(&r->cphi, r->esz * ncache);
+    if (st == KH_OK) st = alloc((void **)&r->cidx, sizeof(int) * ncache);
     if (st != KH_OK) return bail(st);
     if (hipMemcpy(r->ds, hs.data(), sizeof(DevSpecies) * hs.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(r->sek, sek.data(), sizeof(int) * sek.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(r->ukdc, 0, 3 * sizeof(double)) != hipSuccess ||
+        hipMemset(r->ukdc, 0, 3 * sizeof(double)) != hipSuccess || hipMemset(r->d_ts, 0, sizeof(int)) != hipSuccess ||
         hipMemset(r->dp, 0, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1)) != hipSuccess)
         return bail(fail(KH_ERR_HIP, "initial upload failed"));
     for (int i = 0; i < 2; i++)
@@ -701,7 +744,7 @@ int kh_set_particles(kh_run *r, const Particle *p, int n) {
         }
         d.spec = s.spec;
     }
-    r->pfield_dirty = 1;
+    r->pcache_valid = 0;
     if (n > 0) KH_HIP(hipMemcpyAsync(r->dp, h.data(), sizeof(DevParticle) * n, hipMemcpyHostToDevice, r->stream));
     KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
@@ -769,7 +812,12 @@ int kh_get_fluid(kh_run *r, double *z0, double *z1, double uk_dc[3]) {
 
 int kh_set_time(kh_run *r, const CTime *t) {
     if (!r || !t) return fail(KH_ERR_ARG, "NULL argument");
+    if (t->dt_fluid != r->jikan.dt_fluid || t->dt_md != r->jikan.dt_md) {
+        drop_graph(r);  // dt enters the captured kernel arguments
+    }
     r->jikan = *t;
+    KH_HIP(hipMemcpyAsync(r->d_ts, &r->jikan.ts, sizeof(int), hipMemcpyHostToDevice, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
 }
 int kh_get_time(kh_run *r, CTime *t) {
@@ -793,7 +841,6 @@ int kh_get_field(kh_run *r, int field, double *out) {
 
 int kh_set_field(kh_run *r, int field, const double *in) {
     if (!r || !in) return fail(KH_ERR_ARG, "NULL argument");
-    r->pfield_dirty = 1;
     void *b = field_buffer(r, field);
     if (!b) return fail(KH_ERR_ARG, "field %d cannot be written", field);
     return upload_real(r, b, in);
@@ -801,19 +848,51 @@ int kh_set_field(kh_run *r, int field, const double *in) {
 
 int kh_init_fluid(kh_run *r) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
-    r->zeta_dirty   = 1;
-    r->pfield_dirty = 1;
+    r->zeta_dirty = 1;
     return r->mixed ? Step<float>::init_fluid(r) : Step<double>::init_fluid(r);
+}
+
+// Capture one steady-state step (ts >= 1, fused path, zeta clean) as a HIP
+// graph; replays then cost one launch per step.
+static int capture_step(kh_run *r) {
+    KH_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+    r->capturing = true;
+    int st       = r->mixed ? Step<float>::step(r) : Step<double>::step(r);
+    r->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(r->stream, &g);
+    if (st != KH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(KH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    r->graph = g;
+    KH_HIP(hipGraphInstantiate(&r->graph_exec, r->graph, nullptr, nullptr, 0));
+    return KH_OK;
 }
 
 int kh_step(kh_run *r, int nsteps) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
     for (int s = 0; s < nsteps; s++) {
-        KH_TRY(r->mixed ? Step<float>::step(r) : Step<double>::step(r));
+        const bool graph_ok = r->use_graph && !r->timing && r->step_path == KH_PATH_FUSED && r->jikan.ts > 0 &&
+                              !r->zeta_dirty;
+        if (graph_ok) {
+            if (!r->graph_exec) KH_TRY(capture_step(r));
+            KH_HIP(hipGraphLaunch(r->graph_exec, r->stream));
+            r->pcache_valid = 1;
+        } else {
+            KH_TRY(r->mixed ? Step<float>::step(r) : Step<double>::step(r));
+        }
         r->jikan.time += r->jikan.dt_fluid;  // sp_3d_ns.cxx:955
         r->jikan.ts++;
     }
+    return KH_OK;
+}
+
+int kh_set_graph(kh_run *r, int on) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    r->use_graph = on != 0;
     return KH_OK;
 }
 
@@ -854,6 +933,7 @@ int kh_set_step_path(kh_run *r, int path) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (path != KH_PATH_FUSED && path != KH_PATH_REFERENCE_ORDER) return fail(KH_ERR_ARG, "unknown step path %d", path);
     r->step_path = path;
+    drop_graph(r);
     return KH_OK;
 }
 

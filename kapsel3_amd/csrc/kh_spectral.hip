@@ -241,10 +241,19 @@ __global__ __launch_bounds__(PW_NT) void k_add3(long long n, T *__restrict__ u0,
     u2[i] += f2[i];
 }
 
+// 16-byte stores; fields are hipMalloc'd (256-byte aligned)
 template <typename T>
 __global__ __launch_bounds__(PW_NT) void k_fill(long long n, T *__restrict__ p, T v) {
-    const long long i = (long long)blockIdx.x * PW_NT + threadIdx.x;
-    if (i < n) p[i] = v;
+    constexpr int   W  = 16 / sizeof(T);
+    const long long i  = ((long long)blockIdx.x * PW_NT + threadIdx.x) * W;
+    if (i + W <= n) {
+        T tmp[W];
+#pragma unroll
+        for (int k = 0; k < W; k++) tmp[k] = v;
+        *reinterpret_cast<uint4 *>(p + i) = *reinterpret_cast<const uint4 *>(tmp);
+    } else {
+        for (long long k = i; k < n; k++) p[k] = v;
+    }
 }
 
 // ------------------------------------------------------------ launchers
@@ -292,7 +301,8 @@ hipError_t add3(long long n, T *const u[3], T *const f[3], hipStream_t s) {
 }
 template <typename T>
 hipError_t fill(long long n, T *p, T v, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks(n)), dim3(PW_NT), 0, s, n, p, v);
+    constexpr int W = 16 / sizeof(T);
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks((n + W - 1) / W)), dim3(PW_NT), 0, s, n, p, v);
     return hipGetLastError();
 }
 
