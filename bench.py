@@ -161,7 +161,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--particles", type=int, default=2000)
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="time the mixed step only")
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 step only (value = FP64 steps/s)")
