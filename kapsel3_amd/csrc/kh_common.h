@@ -54,11 +54,12 @@ __device__ __forceinline__ cpx<T> cmul_mi(cpx<T> a) {
 struct Grid {
     int    nx, ny, nz;       // real-space points
     int    nzh;              // nz/2 + 1 complex modes along z
+    int    nzp;              // row pitch of spectral arrays (complex elements): nzh rounded up to 128 bytes
     int    hnx, hny;         // nx/2, ny/2 (Calc_KX_Ooura: i > HNX -> i - NX)
     int    trn_x, trn_y, trn_z;  // (N+2)/3: kept |kx| < trn_x, |ky| < trn_y, kz < trn_z
     double wave_x, wave_y, wave_z;
     long long nreal;         // nx*ny*nz
-    long long nspec;         // nx*ny*nzh
+    long long nspec;         // nx*ny*nzp (allocated spectral elements, row padding included)
 };
 
 __device__ __forceinline__ int kx_of(const Grid &g, int i) { return (i > g.hnx) ? i - g.nx : i; }

@@ -15,6 +15,7 @@
 //   y/x    : a tile of C neighbouring columns (C*elem = 128 B per row)
 #include "kh_fft.h"
 #include "kh_fft_engine.h"
+#include "kh_zwave.h"
 
 namespace kh {
 
@@ -48,7 +49,8 @@ struct ZPass {
 
 template <typename T, int L>
 __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_r2c(const T *__restrict__ in,
-                                                                     cpx<T> *__restrict__ out, long long nrows) {
+                                                                     cpx<T> *__restrict__ out, long long nrows,
+                                                                     int pitch) {
     using Z = ZPass<T, L>;
     constexpr int NT = Z::NT, P = Z::P, NZH = Z::NZH, BS = Z::BS;
     __shared__ cpx<T> s[Z::SZ], tw[L];
@@ -71,7 +73,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_r2c(const T *__
     }
     __syncthreads();
     fft_cols_ip<T, L, P, BS, NT, true>(s, tw);
-    cpx<T> *dst = out + row0 * NZH;
+    cpx<T> *dst = out + row0 * pitch;
     for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {
         const int rl = e / NZH, k = e - rl * NZH;
         if (row0 + rl >= nrows) break;
@@ -85,13 +87,13 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_r2c(const T *__
         } else {
             o = cscale(cadd(Zk, Zm), (T)0.5);
         }
-        dst[e] = o;
+        dst[rl * pitch + k] = o;
     }
 }
 
 template <typename T, int L>
 __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_c2r(ZIO<T> io, long long nrows, T scale,
-                                                                     int kzvalid) {
+                                                                     int kzvalid, int pitch) {
     const cpx<T> *__restrict__ in  = io.in[blockIdx.y];  // one component per grid row
     T *__restrict__            out = io.out[blockIdx.y];
     using Z = ZPass<T, L>;
@@ -99,10 +101,10 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_c2r(ZIO<T> io, 
     __shared__ cpx<T> s[Z::SZ], tw[L];
     fill_twiddles<T, L, NT>(tw);
     const long long row0 = (long long)blockIdx.x * (2 * P);
-    const cpx<T>   *src  = in + row0 * NZH;
+    const cpx<T>   *src  = in + row0 * pitch;
     for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {  // modes kz >= kzvalid are zero
         const int rl = e / NZH, k = e - rl * NZH;
-        s[e]         = (row0 + rl < nrows && k < kzvalid) ? src[e] : cmk((T)0, (T)0);
+        s[e]         = (row0 + rl < nrows && k < kzvalid) ? src[rl * pitch + k] : cmk((T)0, (T)0);
     }
     __syncthreads();
     cpx<T> zv[Z::NC];
@@ -180,13 +182,13 @@ bool fft_length_supported(int L) {
 }
 
 template <typename T>
-static hipError_t launch_z_r2c(int L, const T *in, cpx<T> *out, long long nrows, hipStream_t s) {
+static hipError_t launch_z_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s) {
     switch (L) {
 #define KH_CASE(n)                                                                       \
     case n: {                                                                            \
         constexpr int P  = ZPass<T, n>::P;                                               \
         long long     nb = (nrows + 2 * P - 1) / (2 * P);                                \
-        hipLaunchKernelGGL((k_fft_z_r2c<T, n>), dim3((unsigned)nb), dim3(ZPass<T, n>::NT), 0, s, in, out, nrows); \
+        hipLaunchKernelGGL((k_fft_z_r2c<T, n>), dim3((unsigned)nb), dim3(ZPass<T, n>::NT), 0, s, in, out, nrows, pitch); \
         return hipGetLastError();                                                        \
     }
         KH_FOR_EACH_L(KH_CASE)
@@ -197,13 +199,13 @@ static hipError_t launch_z_r2c(int L, const T *in, cpx<T> *out, long long nrows,
 
 template <typename T>
 static hipError_t launch_z_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzvalid,
-                               hipStream_t s) {
+                               int pitch, hipStream_t s) {
     switch (L) {
 #define KH_CASE(n)                                                                       \
     case n: {                                                                            \
         constexpr int P  = ZPass<T, n>::P;                                               \
         long long     nb = (nrows + 2 * P - 1) / (2 * P);                                \
-        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb, ncomp), dim3(ZPass<T, n>::NT), 0, s, io, nrows, scale, kzvalid); \
+        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb, ncomp), dim3(ZPass<T, n>::NT), 0, s, io, nrows, scale, kzvalid, pitch); \
         return hipGetLastError();                                                        \
     }
         KH_FOR_EACH_L(KH_CASE)
@@ -232,7 +234,8 @@ static hipError_t launch_strided(int L, cpx<T> *data, long long lstride, int nco
 
 template <typename T>
 hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s) {
-    return launch_z_r2c<T>(g.nz, in, out, (long long)g.nx * g.ny, s);
+    if (zw_supported(g.nz)) return zw_r2c<T>(g.nz, in, out, (long long)g.nx * g.ny, g.nzp, s);
+    return launch_z_r2c<T>(g.nz, in, out, (long long)g.nx * g.ny, g.nzp, s);
 }
 template <typename T>
 hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s, int kzvalid) {
@@ -243,18 +246,21 @@ hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s, int
 }
 template <typename T>
 hipError_t fft_z_c2r_multi(const Grid &g, const ZIO<T> &io, int ncomp, hipStream_t s, int kzvalid) {
-    const T scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
-    return launch_z_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzvalid < 0 ? g.nzh : kzvalid, s);
+    const T   scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
+    const int kzv   = kzvalid < 0 ? g.nzh : kzvalid;
+    if (zw_supported(g.nz)) return zw_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s);
+    return launch_z_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s);
 }
-// y: lines along y at stride nzh, columns kz, one outer block per x
+// y: lines along y at stride nzp, columns kz < nzh, one outer block per x
 template <typename T, bool FWD>
 hipError_t fft_y(const Grid &g, cpx<T> *data, hipStream_t s) {
-    return launch_strided<T, FWD>(g.ny, data, g.nzh, g.nzh, (long long)g.ny * g.nzh, g.nx, s);
+    return launch_strided<T, FWD>(g.ny, data, g.nzp, g.nzh, (long long)g.ny * g.nzp, g.nx, s);
 }
-// x: lines along x at stride ny*nzh, columns (y,kz) flattened
+// x: lines along x at stride ny*nzp, columns (y,kz) flattened (the row
+// padding columns are transformed too: they hold zeros)
 template <typename T, bool FWD>
 hipError_t fft_x(const Grid &g, cpx<T> *data, hipStream_t s) {
-    const int M = g.ny * g.nzh;
+    const int M = g.ny * g.nzp;
     return launch_strided<T, FWD>(g.nx, data, M, M, 0, 1, s);
 }
 

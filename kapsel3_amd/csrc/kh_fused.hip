@@ -16,6 +16,7 @@
 // addition of u* and the curl sit behind the forward x pass (k_x_f_zeta).
 #include "kh_fft_engine.h"
 #include "kh_fused.h"
+#include "kh_zwave.h"
 
 namespace kh {
 
@@ -67,13 +68,13 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_cols(Comps<T> comps, 
     int       trn, nline;
     if (XMODE) {
         const int j = kept_index(o, g.ny, g.trn_y);
-        base        = (long long)j * g.nzh + kz0;
-        lstride     = (long long)g.ny * g.nzh;
+        base        = (long long)j * g.nzp + kz0;
+        lstride     = (long long)g.ny * g.nzp;
         trn         = g.trn_x;
         nline       = g.nx;
     } else {
-        base    = (long long)o * g.ny * g.nzh + kz0;
-        lstride = g.nzh;
+        base    = (long long)o * g.ny * g.nzp + kz0;
+        lstride = g.nzp;
         trn     = g.trn_y;
         nline   = g.ny;
     }
@@ -171,8 +172,8 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_x_zeta2u_inv(
     if (fixmask && j == 0 && kz0 == 0 && threadIdx.x == 0)
         for (int d = 0; d < 3; d++)
             if (fixmask & (1 << d)) ukdc[d] = 0.0;
-    const long long lstride = (long long)g.ny * g.nzh;
-    const long long base    = (long long)j * g.nzh + kz0;
+    const long long lstride = (long long)g.ny * g.nzp;
+    const long long base    = (long long)j * g.nzp + kz0;
     cpx<T>          D1[NPT], D2[NPT];
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_adv(
     for (int d = 0; d < 3; d++) {
         for (int e = threadIdx.x; e < 2 * P * NZH; e += NT) {
             const int rl = e / NZH, k = e - rl * NZH;
-            s[e]         = (row0 + rl < nrows && k < kzv) ? src[d][row0 * NZH + e] : cmk((T)0, (T)0);
+            s[e]         = (row0 + rl < nrows && k < kzv) ? src[d][(row0 + rl) * g.nzp + k] : cmk((T)0, (T)0);
         }
         __syncthreads();
         cpx<T> zv[NPT];
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_adv(
             } else {
                 v = cscale(cadd(Zk, Zm), (T)0.5);
             }
-            dst[q][row0 * NZH + e] = v;
+            dst[q][(row0 + rl) * g.nzp + k] = v;
         }
         __syncthreads();
     }
@@ -371,7 +372,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_force(
             } else {
                 v = cscale(cadd(Zk, Zm), (T)0.5);
             }
-            dst[d][row0 * NZH + e] = v;
+            dst[d][(row0 + rl) * g.nzp + k] = v;
         }
         __syncthreads();
     }
@@ -399,8 +400,8 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_x_f_zeta(Grid g, cpx<
     const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
     const int j = kept_index(o, g.ny, g.trn_y), iky = (j > g.hny) ? j - g.ny : j;
     if (threadIdx.x < 3) uk[threadIdx.x] = ukdc[threadIdx.x];
-    const long long lstride = (long long)g.ny * g.nzh;
-    const long long base    = (long long)j * g.nzh + kz0;
+    const long long lstride = (long long)g.ny * g.nzp;
+    const long long base    = (long long)j * g.nzp + kz0;
     cpx<T>         *F[3]    = {f0, f1, f2};
     cpx<T>          Fh[3][NPT];
     for (int d = 0; d < 3; d++) {
@@ -483,10 +484,10 @@ template <typename T>
 __global__ void k_zeta_clean(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nspec) return;
-    const int kz = (int)(idx % g.nzh);
-    const long long rest = idx / g.nzh;
+    const int kz = (int)(idx % g.nzp);
+    const long long rest = idx / g.nzp;
     const int j = (int)(rest % g.ny), i = (int)(rest / g.ny);
-    if (!kept_mode(g, kx_of(g, i), ky_of(g, j), kz)) {
+    if (kz < g.nzh && !kept_mode(g, kx_of(g, i), ky_of(g, j), kz)) {
         z0[idx] = cmk((T)0, (T)0);
         z1[idx] = cmk((T)0, (T)0);
     }
@@ -550,6 +551,7 @@ template <typename T>
 hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStream_t s) {
     const long long nrows = (long long)g.nx * g.ny;
     const T         scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
+    if (zw_supported(g.nz)) return zw_adv<T>(g, s3, p5, scale, s);
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                 \
     case n: {                                                                                                      \
@@ -567,6 +569,7 @@ hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStr
 template <typename T>
 hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s) {
     const long long nrows = (long long)g.nx * g.ny;
+    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, s);
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                   \
     case n: {                                                                                                        \

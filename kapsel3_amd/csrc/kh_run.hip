@@ -429,29 +429,38 @@ int download_real(kh_run *r, double *host, const void *dev) {
         for (int k = g.nz; k < nz_; k++) host[(size_t)row * nz_ + k] = 0.0;
     return KH_OK;
 }
-// spectral: identical layouts; FP32 runs convert
+// spectral: host rows of NZ_ = 2 nzh doubles ((re,im) of kz at [2kz]),
+// device rows nzp complex apart (padding left untouched); FP32 runs convert
 int upload_spec(kh_run *r, void *dev, const double *host) {
-    const size_t n = (size_t)r->g.nspec * 2;
+    const Grid     &g    = r->g;
+    const long long rows = (long long)g.nx * g.ny;
     if (!r->mixed) {
-        KH_HIP(hipMemcpyAsync(dev, host, sizeof(double) * n, hipMemcpyHostToDevice, r->stream));
+        KH_HIP(hipMemcpy2DAsync(dev, sizeof(double) * 2 * g.nzp, host, sizeof(double) * 2 * g.nzh,
+                                sizeof(double) * 2 * g.nzh, (size_t)rows, hipMemcpyHostToDevice, r->stream));
     } else {
-        std::vector<float> tmp(n);
-        for (size_t i = 0; i < n; i++) tmp[i] = (float)host[i];
-        KH_HIP(hipMemcpyAsync(dev, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, r->stream));
+        std::vector<float> tmp((size_t)g.nspec * 2, 0.f);
+        for (long long row = 0; row < rows; row++)
+            for (int k = 0; k < 2 * g.nzh; k++)
+                tmp[(size_t)row * 2 * g.nzp + k] = (float)host[(size_t)row * 2 * g.nzh + k];
+        KH_HIP(hipMemcpyAsync(dev, tmp.data(), sizeof(float) * tmp.size(), hipMemcpyHostToDevice, r->stream));
     }
     KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
 }
 int download_spec(kh_run *r, double *host, const void *dev) {
-    const size_t n = (size_t)r->g.nspec * 2;
+    const Grid     &g    = r->g;
+    const long long rows = (long long)g.nx * g.ny;
     if (!r->mixed) {
-        KH_HIP(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost, r->stream));
+        KH_HIP(hipMemcpy2DAsync(host, sizeof(double) * 2 * g.nzh, dev, sizeof(double) * 2 * g.nzp,
+                                sizeof(double) * 2 * g.nzh, (size_t)rows, hipMemcpyDeviceToHost, r->stream));
         KH_HIP(hipStreamSynchronize(r->stream));
     } else {
-        std::vector<float> tmp(n);
-        KH_HIP(hipMemcpyAsync(tmp.data(), dev, sizeof(float) * n, hipMemcpyDeviceToHost, r->stream));
+        std::vector<float> tmp((size_t)g.nspec * 2);
+        KH_HIP(hipMemcpyAsync(tmp.data(), dev, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost, r->stream));
         KH_HIP(hipStreamSynchronize(r->stream));
-        for (size_t i = 0; i < n; i++) host[i] = tmp[i];
+        for (long long row = 0; row < rows; row++)
+            for (int k = 0; k < 2 * g.nzh; k++)
+                host[(size_t)row * 2 * g.nzh + k] = tmp[(size_t)row * 2 * g.nzp + k];
     }
     return KH_OK;
 }
@@ -565,6 +574,10 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     g.ny = P->ny;
     g.nz = P->nz;
     g.nzh = P->nz / 2 + 1;
+    {  // spectral rows start on 128-byte boundaries
+        const int per = 128 / (2 * (P->precision == KH_MIXED ? 4 : 8));
+        g.nzp         = (g.nzh + per - 1) / per * per;
+    }
     g.hnx = P->nx / 2;
     g.hny = P->ny / 2;
     g.trn_x = (P->nx + 2) / 3;
@@ -574,7 +587,7 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     g.wave_y = 2. * KH_PI / (P->ny * P->dx);
     g.wave_z = 2. * KH_PI / (P->nz * P->dx);
     g.nreal  = (long long)g.nx * g.ny * g.nz;
-    g.nspec  = (long long)g.nx * g.ny * g.nzh;
+    g.nspec  = (long long)g.nx * g.ny * g.nzp;
     r->radius = P->a * P->dx;
     r->xi     = P->a_xi * P->dx;
     r->hxi    = 0.5 * r->xi;
@@ -662,6 +675,8 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
         return bail(fail(KH_ERR_HIP, "initial upload failed"));
     for (int i = 0; i < 2; i++)
         if (hipMemset(r->zeta[i], 0, sb) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+    for (int i = 0; i < 5; i++)  // the row padding stays zero for good
+        if (hipMemset(r->spec[i], 0, sb) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
     void *zr[] = {r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2],
                   r->work[0], r->work[1], r->work[2], r->work[3], r->work[4], r->phisum};
     for (void *b : zr)

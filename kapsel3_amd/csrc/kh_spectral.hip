@@ -8,9 +8,10 @@ namespace kh {
 
 constexpr int PW_NT = 256;
 
+// idx over nx*ny*nzp; kz >= nzh are the row padding (never touched)
 __device__ __forceinline__ void spec_index(const Grid &g, long long idx, int &i, int &j, int &kz) {
-    kz             = (int)(idx % g.nzh);
-    long long rest = idx / g.nzh;
+    kz             = (int)(idx % g.nzp);
+    long long rest = idx / g.nzp;
     j              = (int)(rest % g.ny);
     i              = (int)(rest / g.ny);
 }
@@ -31,6 +32,7 @@ __global__ __launch_bounds__(PW_NT) void k_zeta_to_u(Grid g, const cpx<T> *__res
     if (idx >= g.nspec) return;
     int i, j, kz;
     spec_index(g, idx, i, j, kz);
+    if (kz >= g.nzh) return;
     const int ikx = kx_of(g, i), iky = ky_of(g, j);
     cpx<T>    r0 = cmk((T)0, (T)0), r1 = r0, r2 = r0;
     if (idx == 0) {
@@ -109,6 +111,7 @@ __global__ __launch_bounds__(PW_NT) void k_adv_ns(Grid g, const cpx<T> *__restri
     if (idx >= g.nspec) return;
     int i, j, kz;
     spec_index(g, idx, i, j, kz);
+    if (kz >= g.nzh) return;
     const int ikx = kx_of(g, i), iky = ky_of(g, j);
     if (!kept_mode(g, ikx, iky, kz)) return;
     const double k1 = ikx * g.wave_x, k2 = iky * g.wave_y, k3 = kz * g.wave_z;
@@ -156,6 +159,7 @@ __global__ __launch_bounds__(PW_NT) void k_solenoidal(Grid g, cpx<T> *__restrict
     if (idx >= g.nspec) return;
     int i, j, kz;
     spec_index(g, idx, i, j, kz);
+    if (kz >= g.nzh) return;
     const int ikx = kx_of(g, i), iky = ky_of(g, j);
     if (!kept_mode(g, ikx, iky, kz)) {
         u0[idx] = u1[idx] = u2[idx] = cmk((T)0, (T)0);
@@ -183,6 +187,7 @@ __global__ __launch_bounds__(PW_NT) void k_u_to_zeta(Grid g, const cpx<T> *__res
     if (idx >= g.nspec) return;
     int i, j, kz;
     spec_index(g, idx, i, j, kz);
+    if (kz >= g.nzh) return;
     const int    ikx = kx_of(g, i), iky = ky_of(g, j);
     const double ks0 = ikx * g.wave_x, ks1 = iky * g.wave_y, ks2 = kz * g.wave_z;
     const cpx<T> a = u0[idx], b = u1[idx], c = u2[idx];

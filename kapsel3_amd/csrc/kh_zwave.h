@@ -1,0 +1,25 @@
+// kh_zwave.h -- z passes on the wavefront-scope FFT engine (kh_zwave.hip),
+// used for power-of-two NZ from 8 to 512.
+#pragma once
+
+#include "kh_common.h"
+#include "kh_fft.h"
+
+namespace kh {
+
+bool zw_supported(int L);
+// rows of L reals -> half spectra (all L/2+1 modes); spectral rows `pitch` complex apart
+template <typename T>
+hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s);
+// half spectra (modes kz < kzv, rest zero) -> rows of L reals * scale; grid y = component
+template <typename T>
+hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
+                  hipStream_t s);
+// inverse z (3) -> advection products -> forward z (5), modes kz < TRN_Z
+template <typename T>
+hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T scale, hipStream_t s);
+// f = up - min(phi_sum,1) u* -> forward z (3), modes kz < TRN_Z
+template <typename T>
+hipError_t zw_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s);
+
+}  // namespace kh

@@ -1,0 +1,313 @@
+// kh_zwave.hip -- z passes on the wavefront-scope FFT engine (power-of-two
+// NZ from 8 to 512).
+//
+// Rows along z are contiguous in both layouts (real R[x][y][z], spectral
+// S[x][y][kz]), so every lane reads its points straight from HBM in the
+// engine's input layout and no staging is needed on the way in. Two real
+// rows share one complex line (row 2p in the real part, row 2p+1 in the
+// imaginary part); the forward passes separate them again through the
+// line's LDS strip, where Z[k] and Z[L-k] meet:
+//   A[k] = (Z[k] + conj Z[L-k]) / 2,  B[k] = (Z[k] - conj Z[L-k]) / 2i.
+// Only modes kz < kzv are read (the rest are zero) or written.
+#include "kh_fused.h"
+#include "kh_wave_fft.h"
+#include "kh_zwave.h"
+
+namespace kh {
+
+// Lane / line bookkeeping shared by the kernels below.
+template <typename T, int L, int E>
+struct WaveLane {
+    using Geo = WaveGeom<T, L, E>;
+    int       g;     // lane within the line
+    long long line;  // complex line = real rows 2*line, 2*line+1
+    bool      okA, okB;
+    int       pitch;  // spectral row pitch (complex elements)
+    cpx<T>   *buf;    // this line's LDS strip
+    __device__ __forceinline__ WaveLane(cpx<T> *strips, long long nrows, int pitch_) : pitch(pitch_) {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, w = lane / Geo::G;
+        g              = lane - w * Geo::G;
+        line           = ((long long)blockIdx.x * (Geo::NT / 64) + wv) * Geo::W + w;
+        okA            = 2 * line < nrows;
+        okB            = 2 * line + 1 < nrows;
+        buf            = strips + (wv * Geo::W + w) * Geo::LP;
+    }
+};
+
+// Two half-spectrum rows -> the complex line Z = A + iB in the input layout
+// of the inverse engine F (imaginary parts of kz = 0 and L/2 dropped).
+template <typename T, int L, typename F>
+__device__ __forceinline__ void load_pair(const cpx<T> *__restrict__ src, const WaveLane<T, L, F::E> &wl, int kzv,
+                                          cpx<T> *v) {
+    const cpx<T> *ra = src + (2 * wl.line) * wl.pitch;
+    const cpx<T> *rb = ra + wl.pitch;
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int n = F::in_point(wl.g, i);
+        const int k = n <= L / 2 ? n : L - n;
+        cpx<T>    a = cmk((T)0, (T)0), b = a;
+        if (k < kzv) {
+            if (wl.okA) a = ra[k];
+            if (wl.okB) b = rb[k];
+        }
+        if (k == 0 || k == L / 2) {
+            a.y = 0;
+            b.y = 0;
+        }
+        if (n > L / 2) {
+            a = cconj(a);
+            b = cconj(b);
+        }
+        v[i] = cmk(a.x - b.y, a.y + b.x);
+    }
+}
+
+// Forward engine output (layout of F) -> half spectra of the two rows,
+// modes k < kzv written.
+template <typename T, int L, typename F>
+__device__ __forceinline__ void store_pair(const cpx<T> *v, const WaveLane<T, L, F::E> &wl, int kzv,
+                                           cpx<T> *__restrict__ dst) {
+    constexpr int NZH = L / 2 + 1, G = F::G;
+#pragma unroll
+    for (int i = 0; i < F::E; i++) wl.buf[WavePad<T>::at(F::out_point(wl.g, i))] = v[i];
+    wave_sync();
+    cpx<T> *ra = dst + (2 * wl.line) * wl.pitch;
+    cpx<T> *rb = ra + wl.pitch;
+#pragma unroll
+    for (int t = 0; t < (NZH + G - 1) / G; t++) {
+        const int k = wl.g + G * t;
+        if (k < kzv) {
+            const cpx<T> Zk = wl.buf[WavePad<T>::at(k)];
+            const cpx<T> Zm = cconj(wl.buf[WavePad<T>::at((L - k) & (L - 1))]);
+            const cpx<T> d  = csub(Zk, Zm);
+            if (wl.okA) ra[k] = cscale(cadd(Zk, Zm), (T)0.5);
+            if (wl.okB) rb[k] = cmk(d.y * (T)0.5, -d.x * (T)0.5);
+        }
+    }
+    wave_sync();
+}
+
+// ------------------------------------------------------------- kernels
+template <typename T, int L, int E>
+__global__ __launch_bounds__(256) void k_zw_r2c(const T *__restrict__ in, cpx<T> *__restrict__ out, long long nrows,
+                                                int pitch) {
+    using Geo = WaveGeom<T, L, E>;
+    using F   = WaveFft<T, L, E, true, true>;
+    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    fill_twiddles<T, L, Geo::NT>(tw);
+    __syncthreads();
+    const WaveLane<T, L, E> wl(strips, nrows, pitch);
+    const T             *ra = in + (2 * wl.line) * L;
+    const T             *rb = ra + L;
+    cpx<T>               v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int n = F::in_point(wl.g, i);
+        v[i]        = cmk(wl.okA ? ra[n] : (T)0, wl.okB ? rb[n] : (T)0);
+    }
+    F::run(v, wl.buf, tw, wl.g);
+    store_pair<T, L, F>(v, wl, L / 2 + 1, out);
+}
+
+template <typename T, int L, int E>
+__global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T scale, int kzv, int pitch) {
+    using Geo = WaveGeom<T, L, E>;
+    using F   = WaveFft<T, L, E, false, false>;
+    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    fill_twiddles<T, L, Geo::NT>(tw);
+    __syncthreads();
+    const WaveLane<T, L, E> wl(strips, nrows, pitch);
+    cpx<T>               v[F::E];
+    load_pair<T, L, F>(io.in[blockIdx.y], wl, kzv, v);
+    F::run(v, wl.buf, tw, wl.g);
+    T *ra = io.out[blockIdx.y] + (2 * wl.line) * L;
+    T *rb = ra + L;
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int n = F::out_point(wl.g, i);
+        if (wl.okA) ra[n] = v[i].x * scale;
+        if (wl.okB) rb[n] = v[i].y * scale;
+    }
+}
+
+// Advection products (U2advection_k, operate_omega.cxx:31-47): three
+// inverse z transforms, the five products in registers, five forward z
+// transforms.
+template <typename T, int L, int E>
+__global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict__ s0, const cpx<T> *__restrict__ s1,
+                                                const cpx<T> *__restrict__ s2, cpx<T> *__restrict__ p0,
+                                                cpx<T> *__restrict__ p1, cpx<T> *__restrict__ p2,
+                                                cpx<T> *__restrict__ p3, cpx<T> *__restrict__ p4, T scale) {
+    using Geo = WaveGeom<T, L, E>;
+    using Fi  = WaveFft<T, L, E, false, false>;
+    using Ff  = WaveFft<T, L, E, true, true>;
+    static_assert(Fi::RN == Ff::R0, "inverse output layout must be the forward input layout");
+    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    fill_twiddles<T, L, Geo::NT>(tw);
+    __syncthreads();
+    const WaveLane<T, L, E> wl(strips, (long long)g.nx * g.ny, g.nzp);
+    const int            kzv = g.trn_z;
+    cpx<T>               u1[E], u2[E], u3[E];
+    load_pair<T, L, Fi>(s0, wl, kzv, u1);
+    Fi::run(u1, wl.buf, tw, wl.g);
+    load_pair<T, L, Fi>(s1, wl, kzv, u2);
+    Fi::run(u2, wl.buf, tw, wl.g);
+    load_pair<T, L, Fi>(s2, wl, kzv, u3);
+    Fi::run(u3, wl.buf, tw, wl.g);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        u1[i] = cscale(u1[i], scale);
+        u2[i] = cscale(u2[i], scale);
+        u3[i] = cscale(u3[i], scale);
+    }
+    cpx<T> *dst[5] = {p0, p1, p2, p3, p4};
+#pragma unroll 1
+    for (int q = 0; q < 5; q++) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) {  // .x = row 2p, .y = row 2p+1
+            const cpx<T> a = u1[i], b = u2[i], c = u3[i];
+            if (q == 0)
+                v[i] = cmk(b.x * c.x, b.y * c.y);
+            else if (q == 1)
+                v[i] = cmk(c.x * a.x, c.y * a.y);
+            else if (q == 2)
+                v[i] = cmk(a.x * b.x, a.y * b.y);
+            else if (q == 3)
+                v[i] = cmk((b.x - c.x) * (b.x + c.x), (b.y - c.y) * (b.y + c.y));
+            else
+                v[i] = cmk((c.x - a.x) * (c.x + a.x), (c.y - a.y) * (c.y + a.y));
+        }
+        Ff::run(v, wl.buf, tw, wl.g);
+        store_pair<T, L, Ff>(v, wl, kzv, dst[q]);
+    }
+}
+
+// Rigidity force f = up - min(phi_sum, 1) u* (Make_f_particle_dt_nonsole,
+// f_particle.h:35-54; make_phi.cxx:248-259), forward z per component.
+template <typename T, int L, int E>
+__global__ __launch_bounds__(256) void k_zw_force(Grid g, const T *__restrict__ up0, const T *__restrict__ up1,
+                                                  const T *__restrict__ up2, const T *__restrict__ phis,
+                                                  const T *__restrict__ us0, const T *__restrict__ us1,
+                                                  const T *__restrict__ us2, cpx<T> *__restrict__ f0,
+                                                  cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2) {
+    using Geo = WaveGeom<T, L, E>;
+    using Ff  = WaveFft<T, L, E, true, true>;
+    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    fill_twiddles<T, L, Geo::NT>(tw);
+    __syncthreads();
+    const long long      nrows = (long long)g.nx * g.ny;
+    const WaveLane<T, L, E> wl(strips, nrows, g.nzp);
+    const long long      oa = (2 * wl.line) * L, ob = oa + L;
+    cpx<T>               ph[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int n = Ff::in_point(wl.g, i);
+        ph[i]       = cmk(wl.okA ? fmin(phis[oa + n], (T)1) : (T)0, wl.okB ? fmin(phis[ob + n], (T)1) : (T)0);
+    }
+    const T *UP[3] = {up0, up1, up2}, *US[3] = {us0, us1, us2};
+    cpx<T>  *dst[3] = {f0, f1, f2};
+#pragma unroll 1
+    for (int d = 0; d < 3; d++) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int n = Ff::in_point(wl.g, i);
+            const T   a = wl.okA ? UP[d][oa + n] - ph[i].x * US[d][oa + n] : (T)0;
+            const T   b = wl.okB ? UP[d][ob + n] - ph[i].y * US[d][ob + n] : (T)0;
+            v[i]        = cmk(a, b);
+        }
+        Ff::run(v, wl.buf, tw, wl.g);
+        store_pair<T, L, Ff>(v, wl, g.trn_z, dst[d]);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+#define KH_FOR_EACH_WL(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512)
+
+bool zw_supported(int L) { return wave_fft_ok(L); }
+
+// points per lane of each pass (the advection pass holds three components
+// in registers, so it takes fewer points per lane)
+#ifndef KH_ZW_ADV_E32
+#define KH_ZW_ADV_E32 8
+#endif
+template <typename T, int L>
+struct ZwE {
+    static constexpr int pass = WaveE<T, L>::value;
+    static constexpr int adv  = WaveE<T, L, sizeof(T) == 4 ? KH_ZW_ADV_E32 : 8>::value;
+};
+
+template <typename T, int L, int E>
+static inline unsigned zw_blocks(long long nrows) {
+    const long long lines = (nrows + 1) / 2;
+    return (unsigned)((lines + WaveGeom<T, L, E>::LINES - 1) / WaveGeom<T, L, E>::LINES);
+}
+
+template <typename T>
+hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s) {
+    switch (L) {
+#define KH_CASE(n)                                                                                   \
+    case n:                                                                                          \
+        hipLaunchKernelGGL((k_zw_r2c<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, in, out, nrows, pitch); \
+        return hipGetLastError();
+        KH_FOR_EACH_WL(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
+                  hipStream_t s) {
+    switch (L) {
+#define KH_CASE(n)                                                                                              \
+    case n:                                                                                                     \
+        hipLaunchKernelGGL((k_zw_c2r<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows), ncomp), dim3(256), 0, s, io, nrows, scale, kzv, pitch); \
+        return hipGetLastError();
+        KH_FOR_EACH_WL(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T scale, hipStream_t s) {
+    const long long nrows = (long long)g.nx * g.ny;
+    switch (g.nz) {
+#define KH_CASE(n)                                                                                                \
+    case n:                                                                                                       \
+        hipLaunchKernelGGL((k_zw_adv<T, n, ZwE<T, n>::adv>), dim3(zw_blocks<T, n, ZwE<T, n>::adv>(nrows)), dim3(256), 0, s, g, s3[0], s3[1], s3[2], \
+                           p5[0], p5[1], p5[2], p5[3], p5[4], scale);                                             \
+        return hipGetLastError();
+        KH_FOR_EACH_WL(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t zw_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s) {
+    const long long nrows = (long long)g.nx * g.ny;
+    switch (g.nz) {
+#define KH_CASE(n)                                                                                                  \
+    case n:                                                                                                         \
+        hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, g, up[0], up[1], up[2], \
+                           phis, us[0], us[1], us[2], f[0], f[1], f[2]);                                            \
+        return hipGetLastError();
+        KH_FOR_EACH_WL(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+#define KH_INST(T)                                                                                           \
+    template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t);                    \
+    template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t);            \
+    template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
+    template hipError_t zw_force<T>(const Grid &, T *const[3], const T *, T *const[3], cpx<T> *const[3], hipStream_t);
+KH_INST(double)
+KH_INST(float)
+#undef KH_INST
+
+}  // namespace kh

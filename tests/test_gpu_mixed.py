@@ -21,7 +21,7 @@ def _run(gpu, P, parts, nsteps):
     return p, z
 
 
-@pytest.mark.parametrize("n,np_,kbt,steps", [(64, 24, 1e-3, 200), (48, 10, 0.0, 200)])
+@pytest.mark.parametrize("n,np_,kbt,steps", [(64, 24, 1e-3, 200), (48, 10, 0.0, 200), (256, 300, 1e-3, 40)])
 def test_mixed_vs_fp64(gpu, n, np_, kbt, steps):
     print(f"mixed vs fp64 {n}^3 x {np_} kT={kbt}", flush=True)
     P64, parts = workloads.sedimentation(n, np_, kbt=kbt, gap=0.5, seed=6, precision=FP64)

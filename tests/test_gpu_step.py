@@ -194,3 +194,15 @@ def test_thermal_shrunk_fp64(gpu, path):
     P, parts = workloads.sedimentation(48, 12, kbt=1e-3, gap=0.5, seed=4)
     o, g = run_both(P, parts, 30, path=path)
     cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)
+
+
+@pytest.mark.parametrize("n,np_,steps", [(128, 60, 6), (256, 40, 3)])
+def test_sedimentation_production_grid_fp64(gpu, n, np_, steps):
+    """Config 2 geometry at 128^3 and the full 256^3 grid: the z passes of
+    the production size (wavefront-scope engine) against the oracle."""
+    print(f"config2 {n}^3 x {np_}, {steps} steps", flush=True)
+    P, parts = workloads.sedimentation(n, np_, gap=0.4, seed=5, kbt=1e-3)
+    o, g = run_both(P, parts, steps)
+    cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)
+    zo, zg = o.get_fluid(), g.get_fluid()
+    assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
