@@ -58,6 +58,7 @@ def kernel_bytes(name, n, p, npts, sf):
         "y_fwd_adv": cs * (5 * ycol + 5 * xcol),
         "x_fwd_adv": cs * (5 * xcol + 5 * kept),
         "adv_ns": cs * (5 + 2 + 2) * kept,
+        "x_adv_ns": cs * (5 * xcol + 2 * kept + 2 * kept),
         "z_c2r": cs * 3 * ycol + sf * 3 * real,
         "z_force": sf * 7 * real + cs * 3 * ycol,
         "y_fwd_force": cs * (3 * ycol + 3 * xcol),
@@ -120,6 +121,29 @@ def cpu_baseline(n, nparticles, steps, seed=3):
     return steps / (t1 - t0), cores, t1 - t0
 
 
+def serve_bench(count, rate, nsteps, max_active, seed=2024):
+    """Config 4: `count` independent runs (grid uniform in {32,48,64,96}^3,
+    1-64 colloids a=4, gravity 1e-3, mixed precision, `nsteps` steps each,
+    per-step trajectories out) arriving as a Poisson process at `rate`/s,
+    served by one continuous-batching Server."""
+    import kapsel3_amd
+    from kapsel3_amd import serving
+    reqs = serving.requests(count, seed=seed)
+    times = serving.arrivals(count, rate, seed=seed + 1)
+    with kapsel3_amd.Server(max_active=max_active, chunk_steps=8) as srv:
+        t0 = time.perf_counter()
+        serving.serve(srv, reqs, times, nsteps)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    return {"runs": count, "steps_per_run": nsteps, "arrival_rate_per_s": rate, "max_active": max_active,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"], "max_ms": st["max_ms"],
+            "completed": st["completed"], "failed": st["failed"], "peak_active": st["peak_active"],
+            "peak_bytes": st["peak_bytes"], "wall_s": wall, "runs_per_s": st["runs_per_s"],
+            "steps_per_s": st["steps_per_s"], "grid_point_steps_per_s": st["grid_point_steps_per_s"],
+            "workload": "grid uniform {32,48,64,96}^3, colloids uniform 1-64 (capped at 20% volume), a=4 xi=2, "
+                        "gravity 1e-3, mixed precision, Poisson arrivals"}
+
+
 def roofline_of(stats, n, nparticles, npts, sf, pass_ms_per_step, steps, traffic=None):
     """Roofline entry of the kernel with the largest total time."""
     best = max(stats.items(), key=lambda kv: kv[1][1])
@@ -166,6 +190,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="time the mixed step only")
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 step only (value = FP64 steps/s)")
     ap.add_argument("--no-graph", action="store_true", help="run the timed steps uncaptured")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-runs", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-steps", type=int, default=200)
+    ap.add_argument("--serve-active", type=int, default=16)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_256_2000.json"),
                     help="measured HBM bytes per launch (JSON by precision and kernel)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -218,6 +247,9 @@ def main():
         f64 = measure(FP64, "fp64", 8, a.fp64_steps or a.steps)
         out["fp64"] = f64
         out["speedup_mixed_vs_fp64"] = out["value"] / f64["value"]
+
+    if not a.no_serve:
+        out["serving"] = serve_bench(a.serve_runs, a.serve_rate, a.serve_steps, a.serve_active)
 
     if not a.no_cpu_baseline:
         sps, cores, secs = cpu_baseline(n, P, a.cpu_steps)

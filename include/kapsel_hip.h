@@ -127,6 +127,9 @@ enum kh_fn {
     KH_FN_MAKE_UP,            /* Reset_u + Make_u_particle_sum         make_phi.cxx:313-354 */
     KH_FN_F_PARTICLE_SOLE,    /* Make_f_particle_dt_sole      f_particle.h:60-67 */
     KH_FN_ADD_F_PARTICLE,     /* Add_f_particle (u += f)      f_particle.h:68-84 */
+    KH_FN_SLIP_MOMENTUM,      /* Make_particle_momentum_factor (u)     operate_surface.cxx:12-153 */
+    KH_FN_SLIP_SCATTER,       /* Reset_u + Make_force_u_slip_particle  operate_surface.cxx:200-363 */
+    KH_FN_SLIP_ITERATION,     /* the slip loop + Swap_mem(u, up)       sp_3d_ns.cxx:121-161 */
     KH_FN_COUNT
 };
 
@@ -160,6 +163,19 @@ int kh_step(kh_run *run, int nsteps);
  * Both give the reference's step to round-off. */
 enum { KH_PATH_FUSED = 0, KH_PATH_REFERENCE_ORDER = 1 };
 int kh_set_step_path(kh_run *run, int path);
+/* Per-step particle record kept on the device: from the next step on, the
+ * (x[3], v[3]) of every particle after each of the next nsteps steps
+ * (nsteps x np x 6 doubles; 0 stops recording). No counterpart in the
+ * reference step; it feeds trajectory output (Output_udf's per-frame
+ * particle records, output.cxx:268-375) without a host copy per step. */
+int kh_set_trajectory(kh_run *run, int nsteps);
+int kh_get_trajectory(kh_run *run, double *out, int nsteps);
+/* 1 when all work enqueued on the run (kh_step is asynchronous) has
+ * completed, 0 while some is pending, a negative status on a device error. */
+int kh_query(kh_run *run);
+/* Device bytes kh_run_create would allocate for these parameters. */
+long long kh_run_bytes(const kh_params *params, int nparticles);
+
 /* Steady-state steps (ts >= 1) of the fused path run as one captured HIP
  * graph by default (captured on first use); 0 turns that off. Kernel
  * timing (kh_timing_enable) also runs steps uncaptured. */
@@ -189,6 +205,45 @@ const char *kh_timing_name(kh_run *run, int i);
 
 const char *kh_strerror(int status);
 const char *kh_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * Serving: continuous batching of independent runs on one GPU. No
+ * counterpart in the reference (one simulation per process there); it
+ * runs many SPM simulations (Time_evolution_hydro steps, as kh_step) side by
+ * side. Runs are admitted in arrival order while fewer than max_active are
+ * resident and their kh_run_bytes() fit the budget (hbm_budget bytes, or
+ * hbm_fraction of the device memory free at creation); each stays on the
+ * device until its nsteps are done, then its trajectory (as
+ * kh_get_trajectory) is copied to traj_out (nsteps x np x 6 doubles, may be
+ * NULL) and its memory is released. Latency = submission to that copy.
+ * Submit/poll/wait/stats may be called from any thread. */
+typedef struct kh_server kh_server;
+typedef struct {
+    int       max_active;    /* runs resident at once (>= 1)                       */
+    int       chunk_steps;   /* steps enqueued per run per scheduler visit (>= 1)  */
+    double    hbm_fraction;  /* share of free device memory for runs, (0, 1]      */
+    long long hbm_budget;    /* bytes; > 0 overrides hbm_fraction                  */
+} kh_server_config;
+typedef struct {
+    long long submitted, completed, failed, queued, active;
+    int       peak_active;
+    long long peak_bytes, budget;
+    double    p50_ms, p99_ms, mean_ms, max_ms;  /* submission -> trajectory copied  */
+    double    wall_s;                          /* first submission -> last completion */
+    double    runs_per_s, steps_per_s, grid_point_steps_per_s;
+} kh_server_stats;
+int kh_server_create(const kh_server_config *cfg, kh_server **out);
+int kh_server_submit(kh_server *srv, const kh_params *params, const Particle *particles, int nparticles,
+                     int nsteps, double *traj_out, long long *id);
+/* 1 done, 0 pending, < 0 the run's failure status (kh_last_error has it) */
+int kh_server_poll(kh_server *srv, long long id);
+/* as poll, blocking up to timeout_s seconds (0 on timeout) */
+int kh_server_wait(kh_server *srv, long long id, double timeout_s);
+/* seconds queued and seconds resident of a completed run */
+int kh_server_run_times(kh_server *srv, long long id, double *queued_s, double *run_s);
+int kh_server_get_stats(kh_server *srv, kh_server_stats *out);
+/* finishes every submitted run, then frees the server */
+int kh_server_destroy(kh_server *srv);
 
 #ifdef __cplusplus
 }

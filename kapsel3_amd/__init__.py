@@ -23,8 +23,24 @@ EXPORTS = [
     "kh_run_create", "kh_run_destroy", "kh_set_particles", "kh_get_particles", "kh_set_fluid", "kh_get_fluid",
     "kh_set_time", "kh_get_time", "kh_get_field", "kh_set_field", "kh_init_fluid", "kh_step", "kh_apply",
     "kh_sync", "kh_A2a_k", "kh_A_k2a", "kh_set_step_path", "kh_set_graph", "kh_timing_enable", "kh_timing_reset", "kh_timing_get",
-    "kh_timing_count", "kh_timing_name", "kh_strerror", "kh_last_error",
+    "kh_timing_count", "kh_timing_name", "kh_strerror", "kh_last_error", "kh_set_trajectory", "kh_get_trajectory",
+    "kh_run_bytes", "kh_query", "kh_server_create", "kh_server_submit", "kh_server_poll", "kh_server_wait",
+    "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy",
 ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_active", C.c_int), ("chunk_steps", C.c_int), ("hbm_fraction", C.c_double),
+                ("hbm_budget", C.c_longlong)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("failed", C.c_longlong),
+                ("queued", C.c_longlong), ("active", C.c_longlong), ("peak_active", C.c_int),
+                ("peak_bytes", C.c_longlong), ("budget", C.c_longlong),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("wall_s", C.c_double), ("runs_per_s", C.c_double), ("steps_per_s", C.c_double),
+                ("grid_point_steps_per_s", C.c_double)]
 
 
 class KhError(RuntimeError):
@@ -66,6 +82,18 @@ def lib():
         L.kh_strerror.restype = C.c_char_p
         L.kh_strerror.argtypes = [C.c_int]
         L.kh_last_error.restype = C.c_char_p
+        L.kh_set_trajectory.argtypes = [vp, C.c_int]
+        L.kh_get_trajectory.argtypes = [vp, vp, C.c_int]
+        L.kh_run_bytes.argtypes = [C.POINTER(KhParams), C.c_int]
+        L.kh_run_bytes.restype = C.c_longlong
+        L.kh_query.argtypes = [vp]
+        L.kh_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.kh_server_submit.argtypes = [vp, C.POINTER(KhParams), vp, C.c_int, C.c_int, vp, C.POINTER(C.c_longlong)]
+        L.kh_server_poll.argtypes = [vp, C.c_longlong]
+        L.kh_server_wait.argtypes = [vp, C.c_longlong, C.c_double]
+        L.kh_server_run_times.argtypes = [vp, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.kh_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.kh_server_destroy.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -87,6 +115,7 @@ class Run:
         _check(lib().kh_run_create(C.byref(params.c), self.np, C.byref(h)))
         self.h = h
         self.shape = (params.c.nx, params.c.ny, params.nz_)
+        self._traj_steps = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -161,6 +190,22 @@ class Run:
         """Run steady-state steps as one captured HIP graph (default on)."""
         _check(lib().kh_set_graph(self.h, 1 if on else 0))
 
+    def set_trajectory(self, nsteps):
+        """Record (x, v) of every particle after each of the next nsteps steps on the device."""
+        _check(lib().kh_set_trajectory(self.h, nsteps))
+        self._traj_steps = nsteps
+
+    def get_trajectory(self, nsteps=None):
+        """(nsteps, np, 6) array: x[3], v[3] per particle per recorded step."""
+        n = self._traj_steps if nsteps is None else nsteps
+        out = np.zeros((n, self.np, 6))
+        _check(lib().kh_get_trajectory(self.h, out.ctypes.data, n))
+        return out
+
+    def idle(self):
+        """True when every enqueued step has completed."""
+        return _check(lib().kh_query(self.h)) == 1
+
     def sync(self):
         _check(lib().kh_sync(self.h))
 
@@ -191,3 +236,61 @@ class Run:
             _check(L.kh_timing_get(self.h, name, C.byref(cnt), C.byref(ms)))
             out[name.decode()] = (cnt.value, ms.value)
         return out
+
+
+def run_bytes(P, nparticles):
+    """Device bytes one run of these parameters allocates."""
+    return lib().kh_run_bytes(C.byref(P.c), nparticles)
+
+
+class Server:
+    """Continuous batching of independent runs on one GPU (kh_server_*).
+
+    submit() returns an id at once; the run is admitted when a slot and HBM
+    budget are free, stays on the device until its steps are done, and its
+    per-step trajectory (nsteps, np, 6) lands in the array submit() returned.
+    """
+
+    def __init__(self, max_active=16, chunk_steps=8, hbm_fraction=0.8, hbm_budget=0):
+        cfg = ServerConfig(max_active, chunk_steps, hbm_fraction, hbm_budget)
+        h = C.c_void_p()
+        _check(lib().kh_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, P, particles, nsteps):
+        parts = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        traj = np.zeros((nsteps, len(parts), 6))
+        rid = C.c_longlong()
+        _check(lib().kh_server_submit(self.h, C.byref(P.c), parts.ctypes.data, len(parts), nsteps,
+                                      traj.ctypes.data, C.byref(rid)))
+        self._keep[rid.value] = (P, parts, traj)  # alive until the run completes
+        return rid.value, traj
+
+    def poll(self, rid):
+        return _check(lib().kh_server_poll(self.h, rid)) == 1
+
+    def wait(self, rid, timeout=600.0):
+        if _check(lib().kh_server_wait(self.h, rid, timeout)) != 1:
+            raise KhError(f"run {rid} not done after {timeout} s")
+
+    def run_times(self, rid):
+        q, r = C.c_double(), C.c_double()
+        _check(lib().kh_server_run_times(self.h, rid, C.byref(q), C.byref(r)))
+        return q.value, r.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().kh_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            _check(lib().kh_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -27,9 +27,16 @@ hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStr
 // f = up - min(phi_sum,1) u* -> forward z (3)
 template <typename T>
 hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s);
+// forward x (5 products) -> Omega + slaved-Euler update of zeta at the kept modes
+template <typename T>
+hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,
+                    hipStream_t s);
 // forward x (3) -> projection + u* + curl -> zeta, uk_dc
 template <typename T>
 hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, double *ukdc, hipStream_t s);
+template <typename T>
+hipError_t zeta_add(const Grid &g, cpx<T> *z0, cpx<T> *z1, const cpx<T> *a0, const cpx<T> *a1, double *ukdc,
+                    const double *dc, hipStream_t s);
 template <typename T>
 hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s);
 

@@ -16,6 +16,7 @@
 // addition of u* and the curl sit behind the forward x pass (k_x_f_zeta).
 #include "kh_fft_engine.h"
 #include "kh_fused.h"
+#include "kh_modes.h"
 #include "kh_zwave.h"
 
 namespace kh {
@@ -379,6 +380,67 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_force(
 }
 
 // ------------------------------------------------------------------------
+// Forward x pass of the five advection products on the kept (ky,kz)
+// columns, then per kept mode Omega_k2zeta_k + the slaved-Euler update of
+// zeta (NS_solver_slavedEuler, fluid_solver.cxx:348-374): the products are
+// never written back in spectral form.
+template <typename T, int L>
+__global__ __launch_bounds__(PassThreads<T>::value) void k_x_adv_ns(Grid g, Comps<T> prod, cpx<T> *__restrict__ z0,
+                                                                    cpx<T> *__restrict__ z1, double dmy0,
+                                                                    double dmy1) {
+    constexpr int NT  = PassThreads<T>::value;
+    constexpr int C   = ColTile<T>::C;
+    constexpr int BS  = ColStride<T, C>::value;
+    constexpr int NPT = (C * L + NT - 1) / NT;
+    __shared__ cpx<T> s[L * BS], tw[L];
+    fill_twiddles<T, L, NT>(tw);
+    const int ntiles = (g.trn_z + C - 1) / C;
+    const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
+    const int j = kept_index(o, g.ny, g.trn_y), iky = (j > g.hny) ? j - g.ny : j;
+    const long long lstride = (long long)g.ny * g.nzp;
+    const long long base    = (long long)j * g.nzp + kz0;
+    cpx<T>          Ph[5][NPT];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const cpx<T> *src = prod.p[q];
+        cpx<T>        v[NPT];
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            const int n = e / C, c = e - n * C;
+            v[i]        = (e < C * L && kz0 + c < g.trn_z) ? src[base + n * lstride + c] : cmk((T)0, (T)0);
+        }
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            if (e < C * L) s[(e / C) * BS + e % C] = v[i];
+        }
+        __syncthreads();
+        fft_cols_ip<T, L, C, BS, NT, true>(s, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int p = threadIdx.x + i * NT;
+            if (p < C * L) Ph[q][i] = s[(p / C) * BS + p % C];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int p = threadIdx.x + i * NT;
+        if (p >= C * L) continue;
+        const int n = p / C, c = p - n * C, kz = kz0 + c;
+        const int ikx = (n > g.hnx) ? n - g.nx : n;
+        if (kz >= g.trn_z || !kept1(ikx, g.trn_x)) continue;
+        const long long idx  = base + n * lstride + c;
+        const cpx<T>    a[5] = {Ph[0][i], Ph[1][i], Ph[2][i], Ph[3][i], Ph[4][i]};
+        cpx<T>          x0 = z0[idx], x1 = z1[idx];
+        slaved_euler_mode<T>(g, ikx, iky, kz, a, dmy0, dmy1, x0, x1);
+        z0[idx] = x0;
+        z1[idx] = x1;
+    }
+}
+
+// ------------------------------------------------------------------------
 // Forward x pass of the rigidity force on the kept (ky,kz) columns, then per
 // kept mode: truncation + Solenoidal_uk (operate_omega.cxx:171-195), the sum
 // with u* rebuilt from zeta (the reference's U2zeta_k of u* + f), and
@@ -493,6 +555,19 @@ __global__ void k_zeta_clean(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict
     }
 }
 
+// zeta += (a0, a1) and uk_dc += dc: the projected slip field's vorticity
+// and mean velocity, added after the slip iteration.
+template <typename T>
+__global__ void k_zeta_add(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1, const cpx<T> *__restrict__ a0,
+                           const cpx<T> *__restrict__ a1, double *__restrict__ ukdc, const double *__restrict__ dc) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx == 0)
+        for (int d = 0; d < 3; d++) ukdc[d] += dc[d];
+    if (idx >= g.nspec || (int)(idx % g.nzp) >= g.nzh) return;
+    z0[idx] = cadd(z0[idx], a0[idx]);
+    z1[idx] = cadd(z1[idx], a1[idx]);
+}
+
 // ------------------------------------------------------------ launchers
 #define KH_FOR_EACH_L(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
@@ -585,6 +660,23 @@ hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3],
 }
 
 template <typename T>
+hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,
+                    hipStream_t s) {
+    constexpr int C  = ColTile<T>::C;
+    const int     nt = (g.trn_z + C - 1) / C;
+    switch (g.nx) {
+#define KH_CASE(n)                                                                                                   \
+    case n:                                                                                                          \
+        hipLaunchKernelGGL((k_x_adv_ns<T, n>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, g, prod, z0, z1, \
+                           dmy0, dmy1);                                                                              \
+        return hipGetLastError();
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
 hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, double *ukdc, hipStream_t s) {
     constexpr int C  = ColTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
@@ -598,6 +690,14 @@ hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, d
 #undef KH_CASE
         default: return hipErrorInvalidValue;
     }
+}
+
+template <typename T>
+hipError_t zeta_add(const Grid &g, cpx<T> *z0, cpx<T> *z1, const cpx<T> *a0, const cpx<T> *a1, double *ukdc,
+                    const double *dc, hipStream_t s) {
+    hipLaunchKernelGGL(k_zeta_add<T>, dim3((unsigned)((g.nspec + 255) / 256)), dim3(256), 0, s, g, z0, z1, a0, a1, ukdc,
+                       dc);
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -616,7 +716,10 @@ hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s) {
     template hipError_t z_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], hipStream_t);                   \
     template hipError_t z_force<T>(const Grid &, T *const[3], const T *, T *const[3], cpx<T> *const[3], hipStream_t); \
     template hipError_t x_f_zeta<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, hipStream_t);    \
-    template hipError_t zeta_clean<T>(const Grid &, cpx<T> *, cpx<T> *, hipStream_t);
+    template hipError_t x_adv_ns<T>(const Grid &, const Comps<T> &, cpx<T> *, cpx<T> *, double, double, hipStream_t); \
+    template hipError_t zeta_clean<T>(const Grid &, cpx<T> *, cpx<T> *, hipStream_t);                               \
+    template hipError_t zeta_add<T>(const Grid &, cpx<T> *, cpx<T> *, const cpx<T> *, const cpx<T> *, double *,      \
+                                    const double *, hipStream_t);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST

@@ -45,5 +45,18 @@ hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStr
 hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s);
 hipError_t tick(int *d_ts, hipStream_t s);
+hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int base, int nslots, double *out,
+                  hipStream_t s);
+// squirmer slip iteration (sp_3d_ns.cxx:121-161)
+template <typename T>
+hipError_t slip_moments(const PParams &pp, DevParticle *P, T *const u[3], hipStream_t s);
+template <typename T>
+hipError_t slip_scatter(const PParams &pp, DevParticle *P, T *const u[3], T *const up[3], hipStream_t s);
+template <typename T>
+hipError_t f_slip(const PParams &pp, DevParticle *P, T *const sf[3], double coef, hipStream_t s);
+hipError_t velocity_slip(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s);
+hipError_t reset_forces(const PParams &pp, DevParticle *P, hipStream_t s);
+hipError_t slip_update(const PParams &pp, DevParticle *P, int iter, int restore, hipStream_t s);
+hipError_t slip_conv(const PParams &pp, const DevParticle *P, double *eps, hipStream_t s);
 
 }  // namespace kh

@@ -6,6 +6,9 @@
 
 namespace kh {
 
+// janus_propulsion values (enum JP, input.h; KH_JP_* of the C ABI)
+constexpr int JP_MOTOR = 0, JP_SLIP = 1, JP_OBSTACLE = 2;
+
 // Device copy of the members of Particle (variable.h:27-77) the step uses.
 struct DevParticle {
     double x[3], x_prev[3], x_nopbc[3];

@@ -394,6 +394,385 @@ __global__ void k_thermostat(PParams pp, DevParticle *__restrict__ P, uint64_t s
     }
 }
 
+// ------------------------------------------------------------ squirmers
+// The slip iteration of Time_evolution_hydro (sp_3d_ns.cxx:121-161) for
+// SW_JANUS_SLIP species: surface slip u_s = B1 (sin th + (B2/2B1) sin 2th) e_th
+// + (C1 sin th + 1.5 C2 sin 2th) e_ph in the thin shell |r - a| < xi/2,
+// balanced by a droplet counter-flow so the particle's momentum is unchanged.
+
+__device__ __forceinline__ bool less_than_mp(double a, double b) {  // macro.h:164-167
+    const double eps = (fmax(fabs(a), fabs(b)) + 10.0) * 2.220446049250313080847e-16;
+    return (b - a) > eps;
+}
+// DPhi_compact_sin_norm (profile.h:175-184)
+__device__ __forceinline__ double dphi_sin(const PParams &pp, double x) {
+    const double dmy = pp.radius - x;
+    return (fabs(dmy) < pp.hxi) ? cos(3.14159265358979323846 * dmy / pp.xi) : 0.0;
+}
+// SPACE2BODY rotation: qvq with the conjugate quaternion
+__device__ __forceinline__ void space2body(double *o, const double *q, const double *v) {
+    const double qc[4] = {q[0], -q[1], -q[2], -q[3]};
+    body2space(o, qc, v);
+}
+__device__ __forceinline__ void vcross(double *c, const double *a, const double *b) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Squirmer_coord (make_phi.h:282-376): unit vectors e_r, e_th, e_ph (space
+// frame) and the polar angle th of the body-frame point x about the
+// species' Janus axis.
+__device__ __forceinline__ void squirmer_coord(const double *x, const double *q, int axis, double *nr, double *nth,
+                                               double *nph, double *theta) {
+    double e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0}, e3[3] = {0, 0, 0};
+    if (axis == 2) {  // z
+        e1[0] = 1;
+        e2[1] = 1;
+        e3[2] = 1;
+    } else if (axis == 1) {  // y
+        e1[2] = 1;
+        e2[0] = 1;
+        e3[1] = 1;
+    } else {  // x
+        e1[1] = 1;
+        e2[2] = 1;
+        e3[0] = 1;
+    }
+    double b[3];
+    space2body(b, q, x);
+    double       nrm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    double       dn  = 1.0 / nrm;
+    const double r_[3] = {b[0] * dn, b[1] * dn, b[2] * dn};
+    double       ph[3];
+    vcross(ph, e3, r_);
+    const double dot = e3[0] * r_[0] + e3[1] * r_[1] + e3[2] * r_[2];
+    dn               = sqrt(ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2]);
+    const double tol = 10.0 * 2.220446049250313080847e-16;
+    if (((dn > tol) || (-tol > dn)) && less_than_mp(fabs(dot), 1.0)) {
+        dn = 1.0 / dn;
+        for (int d = 0; d < 3; d++) ph[d] *= dn;
+        *theta = acos(dot);
+        double th[3];
+        vcross(th, ph, r_);
+        dn = 1.0 / sqrt(th[0] * th[0] + th[1] * th[1] + th[2] * th[2]);
+        for (int d = 0; d < 3; d++) th[d] *= dn;
+        body2space(nr, q, r_);
+        body2space(nth, q, th);
+        body2space(nph, q, ph);
+    } else {
+        nrm = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        dn  = 1.0 / nrm;
+        for (int d = 0; d < 3; d++) {
+            nr[d]  = x[d] * dn;
+            nth[d] = nph[d] = 0.0;
+        }
+        *theta = (dot > 0.0) ? 0.0 : 3.14159265358979323846;
+    }
+}
+
+// Make_particle_momentum_factor (operate_surface.cxx:12-153): mass moments
+// of phi and of the surface weight phi_s = (1 - phi) DPhi_compact_sin_norm,
+// and the slip momentum the shell carries against the fluid u.
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_slip_moments(PParams pp, DevParticle *__restrict__ P,
+                                                        const T *__restrict__ u0, const T *__restrict__ u1,
+                                                        const T *__restrict__ u2) {
+    const int         n    = blockIdx.x;
+    DevParticle      &p    = P[n];
+    const DevSpecies &s    = pp.species[p.spec];
+    const bool        slip = s.propulsion == JP_SLIP;  // mass moments for every particle, slip sums for squirmers
+    const double xp[3] = {p.x[0], p.x[1], p.x[2]};
+    const double q[4]  = {p.q[0], p.q[1], p.q[2], p.q[3]};
+    const PCell  c     = particle_cell(pp, xp);
+    const T     *U[3]  = {u0, u1, u2};
+    Neu          acc[32];  // M0, M1[3], M2[9], SM0, SM1[3], SM2[9], dv[3], dw[3]
+    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
+        const PPoint pt  = sek_point(pp, c, xp, pp.sek[m]);
+        const double r   = pt.dist;
+        const double ph  = prof_phi(pp, r);
+        const double *rv = pt.r;
+        acc[0].add(ph);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            acc[1 + d].add(ph * rv[d]);
+#pragma unroll
+            for (int l = 0; l < 3; l++) acc[4 + 3 * d + l].add((d == l ? ph * r * r : 0.0) - ph * rv[d] * rv[l]);
+        }
+        if (slip && less_than_mp(fabs(r - pp.radius), pp.hxi)) {
+            const double phs = (1.0 - ph) * dphi_sin(pp, r);
+            double       nr[3], nth[3], nph[3], th;
+            squirmer_coord(rv, q, s.janus_axis, nr, nth, nph, &th);
+            const double mag = s.slip_vel * (sin(th) + s.slip_mode_half * sin(2.0 * th));
+            const double az  = s.c1 * sin(th) + s.c2 * 1.5 * sin(2.0 * th);
+            double       us[3], rxus[3];
+            for (int d = 0; d < 3; d++) us[d] = nth[d] * mag + nph[d] * az - (double)U[d][pt.idx];
+            vcross(rxus, rv, us);
+            acc[13].add(phs);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                acc[14 + d].add(phs * rv[d]);
+#pragma unroll
+                for (int l = 0; l < 3; l++) acc[17 + 3 * d + l].add((d == l ? phs * r * r : 0.0) - phs * rv[d] * rv[l]);
+                acc[26 + d].add(phs * us[d]);
+                acc[29 + d].add(phs * rxus[d]);
+            }
+        }
+    }
+    double v[32];
+    for (int k = 0; k < 32; k++) v[k] = acc[k].val();
+    block_sum<32>(v);
+    if (threadIdx.x == 0) {
+        p.mass  = v[0];
+        p.smass = v[13];
+        for (int d = 0; d < 3; d++) {
+            p.mass_center[d] = v[1 + d];
+            p.smc[d]         = v[14 + d];
+            p.sdv[d]         = v[26 + d];
+            p.sdw[d]         = v[29 + d];
+        }
+        for (int k = 0; k < 9; k++) {
+            p.inertia[k]  = v[4 + k];
+            p.sinertia[k] = v[17 + k];
+        }
+    }
+}
+
+// slip_droplet (operate_surface.cxx:155-198): the droplet counter-flow
+// (delta_v, delta_w) that cancels the shell's linear and angular momentum.
+__device__ __forceinline__ void slip_droplet(const DevParticle &p, double *vp, double *wp, double *dv, double *dw) {
+    double LL[3][3], dP[3], dL[3], L0[3], mc[3];
+    for (int d = 0; d < 3; d++) {
+        vp[d] = p.v_slip[d];
+        wp[d] = p.om_slip[d];
+        mc[d] = p.mass_center[d];
+    }
+    const double imass = 1.0 / p.mass;
+    double       wxr[3], rxv[3];
+    vcross(wxr, wp, p.smc);
+    vcross(rxv, p.smc, vp);
+    for (int d = 0; d < 3; d++) {
+        dP[d] = -(p.smass * vp[d] + wxr[d] + p.sdv[d]);
+        dL[d] = -(rxv[d] + p.sinertia[3 * d] * wp[0] + p.sinertia[3 * d + 1] * wp[1] + p.sinertia[3 * d + 2] * wp[2] +
+                  p.sdw[d]);
+    }
+    double t[3];
+    vcross(t, mc, dP);
+    const double mc2 = -(mc[0] * mc[0] + mc[1] * mc[1] + mc[2] * mc[2]) * imass;
+    for (int d = 0; d < 3; d++) {
+        L0[d] = dL[d] - imass * t[d];
+        for (int l = 0; l < 3; l++) LL[d][l] = (d == l ? mc2 : 0.0) + (p.inertia[3 * d + l] + imass * mc[d] * mc[l]);
+    }
+    // M_inv (lad3.h:202-233)
+    const double(&A)[3][3] = LL;
+    double idet = -A[0][2] * A[1][1] * A[2][0] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] -
+                  A[0][0] * A[1][2] * A[2][1] - A[0][1] * A[1][0] * A[2][2] + A[0][0] * A[1][1] * A[2][2];
+    idet = 1.0 / idet;
+    double B[3][3];
+    B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) * idet;
+    B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * idet;
+    B[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * idet;
+    B[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) * idet;
+    B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * idet;
+    B[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * idet;
+    B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) * idet;
+    B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * idet;
+    B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * idet;
+    for (int i = 0; i < 3; i++) dw[i] = B[i][0] * L0[0] + B[i][1] * L0[1] + B[i][2] * L0[2];
+    vcross(t, dw, mc);
+    for (int d = 0; d < 3; d++) dv[d] = imass * (dP[d] - t[d]);
+}
+
+// Make_force_u_slip_particle (operate_surface.cxx:200-363): scatter the
+// droplet counter-flow phi (dv + dw x r) and the shell slip force
+// phi_s (v + w x r + u_s - u) into up; f_slip, torque_slip reset.
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_slip_scatter(PParams pp, DevParticle *__restrict__ P,
+                                                        const T *__restrict__ u0, const T *__restrict__ u1,
+                                                        const T *__restrict__ u2, T *__restrict__ up0,
+                                                        T *__restrict__ up1, T *__restrict__ up2) {
+    const int         n = blockIdx.x;
+    DevParticle      &p = P[n];
+    const DevSpecies &s = pp.species[p.spec];
+    __shared__ double sh[12];
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; d++) {
+            p.fs[d] = 0.0;
+            p.ts[d] = 0.0;
+        }
+        if (s.propulsion == JP_SLIP) slip_droplet(p, sh, sh + 3, sh + 6, sh + 9);
+    }
+    if (s.propulsion != JP_SLIP) return;
+    __syncthreads();
+    const double vp[3] = {sh[0], sh[1], sh[2]}, wp[3] = {sh[3], sh[4], sh[5]};
+    const double dv[3] = {sh[6], sh[7], sh[8]}, dw[3] = {sh[9], sh[10], sh[11]};
+    const double xp[3] = {p.x[0], p.x[1], p.x[2]};
+    const double q[4]  = {p.q[0], p.q[1], p.q[2], p.q[3]};
+    const PCell  c     = particle_cell(pp, xp);
+    const T     *U[3]  = {u0, u1, u2};
+    T           *UP[3] = {up0, up1, up2};
+    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
+        const PPoint  pt = sek_point(pp, c, xp, pp.sek[m]);
+        const double  r  = pt.dist;
+        const double  ph = prof_phi(pp, r);
+        const double *rv = pt.r;
+        double        vr[3], f[3];
+        angular2v(pp.rotation, dw, rv, vr);
+        for (int d = 0; d < 3; d++) f[d] = ph * (dv[d] + vr[d]);
+        const double phs = (1.0 - ph) * dphi_sin(pp, r);
+        if (less_than_mp(fabs(r - pp.radius), pp.hxi)) {
+            double vrot[3], nr[3], nth[3], nph[3], th;
+            angular2v(pp.rotation, wp, rv, vrot);
+            squirmer_coord(rv, q, s.janus_axis, nr, nth, nph, &th);
+            const double vs  = s.slip_vel * (sin(th) + s.slip_mode_half * sin(2.0 * th));
+            const double vs2 = s.c1 * sin(th) + s.c2 * 1.5 * sin(2.0 * th);
+            for (int d = 0; d < 3; d++)
+                f[d] += phs * (vp[d] + vrot[d] + nth[d] * vs + nph[d] * vs2 - (double)U[d][pt.idx]);
+        }
+        for (int d = 0; d < 3; d++)
+            if (f[d] != 0.0) unsafeAtomicAdd(&UP[d][pt.idx], (T)f[d]);
+    }
+}
+
+// Calc_f_slip_correct_precision (md_force.cxx:325-382): momentum the
+// projected slip field carries over each particle's profile.
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_f_slip(PParams pp, DevParticle *__restrict__ P, const T *__restrict__ s0,
+                                                  const T *__restrict__ s1, const T *__restrict__ s2, double coef) {
+    const int     n     = blockIdx.x;
+    const double  xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
+    const PCell   c     = particle_cell(pp, xp);
+    Neu           acc[6];
+    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
+        const PPoint pt = sek_point(pp, c, xp, pp.sek[m]);
+        const double ph = prof_phi(pp, pt.dist);
+        if (ph == 0.0) continue;
+        const double f[3] = {(double)s0[pt.idx] * ph, (double)s1[pt.idx] * ph, (double)s2[pt.idx] * ph};
+        acc[0].add(f[0]);
+        acc[1].add(f[1]);
+        acc[2].add(f[2]);
+        acc[3].add(pt.r[1] * f[2] - pt.r[2] * f[1]);
+        acc[4].add(pt.r[2] * f[0] - pt.r[0] * f[2]);
+        acc[5].add(pt.r[0] * f[1] - pt.r[1] * f[0]);
+    }
+    double v[6];
+    for (int k = 0; k < 6; k++) v[k] = acc[k].val();
+    block_sum<6>(v);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; d++) {
+            P[n].fs[d] = coef * v[d];
+            P[n].ts[d] = coef * v[3 + d];
+        }
+}
+
+// MD_solver_velocity_slip_Euler / _AB2 (particle_solver.cxx:144-207):
+// the velocity update with the slip force, forces kept for the iteration.
+__global__ void k_velocity_slip(PParams pp, DevParticle *__restrict__ P, int ab2, double h) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= pp.np) return;
+    DevParticle      &p = P[n];
+    const DevSpecies &s = pp.species[p.spec];
+    const double      dmy = h * s.imass, dmy_rot = h * s.imoi;
+    if (s.propulsion != 2) {
+        double sf[3] = {0, 0, 0}, st[3] = {0, 0, 0};
+        if (pp.motor && s.propulsion == 0) {
+            body2space(sf, p.q, s.janus_force);
+            body2space(st, p.q, s.janus_torque);
+        }
+        for (int d = 0; d < 3; d++) {
+            p.v_old[d]  = p.v[d];
+            p.om_old[d] = p.om[d];
+            if (!ab2) {
+                p.v[d] += dmy * (p.fh[d] + p.fs[d] + p.fr[d] + sf[d]);
+                p.om[d] += dmy_rot * (p.th[d] + p.ts[d] + p.tr[d] + st[d]);
+            } else {
+                p.v[d] += dmy * (2.0 * (p.fh[d] + p.fs[d] + sf[d]) + p.fr[d] + p.fr_prev[d]);
+                p.om[d] += dmy_rot * (2.0 * (p.th[d] + p.ts[d] + st[d]) + p.tr[d] + p.tr_prev[d]);
+            }
+        }
+    } else {
+        for (int d = 0; d < 3; d++) {
+            p.v_old[d]  = p.v[d];
+            p.v[d]      = 0.0;
+            p.om_old[d] = p.om[d];
+            p.om[d]     = 0.0;
+        }
+    }
+}
+
+// reset_Forces (particle_solver.cxx:11-37)
+__global__ void k_reset_forces(PParams pp, DevParticle *__restrict__ P) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= pp.np) return;
+    DevParticle &p = P[n];
+    for (int d = 0; d < 3; d++) {
+        p.fr_prev[d] = p.fr[d];
+        p.fr[d]      = 0.0;
+        p.fh_prev[d] = p.fh[d];
+        p.fh[d]      = 0.0;
+        p.fs_prev[d] = p.fs[d];
+        p.fs[d]      = 0.0;
+        p.tr_prev[d] = p.tr[d];
+        p.tr[d]      = 0.0;
+        p.th_prev[d] = p.th[d];
+        p.th[d]      = 0.0;
+        p.ts_prev[d] = p.ts[d];
+        p.ts[d]      = 0.0;
+    }
+}
+
+// Update_slip_particle_velocity (operate_surface.h:101-121); with `restore`
+// the velocities go back to the values before the iteration (reset_iter).
+__global__ void k_slip_update(PParams pp, DevParticle *__restrict__ P, int iter, int restore) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= pp.np) return;
+    DevParticle &p = P[n];
+    for (int d = 0; d < 3; d++) {
+        if (iter == 0) {
+            p.v_slip[d]  = p.v[d];
+            p.om_slip[d] = p.om[d];
+        } else {
+            p.v_slip[d]  = 0.3 * p.v_slip[d] + 0.7 * p.v[d];
+            p.om_slip[d] = 0.3 * p.om_slip[d] + 0.7 * p.om[d];
+        }
+        if (restore) {
+            p.v[d]  = p.v_old[d];
+            p.om[d] = p.om_old[d];
+        }
+    }
+}
+
+// Slip_particle_convergence (operate_surface.h:72-100): max over slip
+// particles of |v_slip - v|^2/|v|^2 + |w_slip - w|^2/|w|^2, square-rooted.
+__global__ __launch_bounds__(PT_NT) void k_slip_conv(PParams pp, const DevParticle *__restrict__ P,
+                                                     double *__restrict__ eps_out) {
+    __shared__ double red[PT_NT / 64];
+    double            e = 0.0;
+    for (int n = threadIdx.x; n < pp.np; n += PT_NT) {
+        const DevParticle &p = P[n];
+        if (pp.species[p.spec].propulsion != JP_SLIP) continue;
+        double dv = 0, dw = 0, nv = 0, nw = 0;
+        for (int d = 0; d < 3; d++) {
+            nv += p.v[d] * p.v[d];
+            nw += p.om[d] * p.om[d];
+            dv += (p.v_slip[d] - p.v[d]) * (p.v_slip[d] - p.v[d]);
+            dw += (p.om_slip[d] - p.om[d]) * (p.om_slip[d] - p.om[d]);
+        }
+        const double tol = 10.0 * 2.220446049250313080847e-16;
+        if (nv > tol) dv /= nv;
+        if (nw > tol) dw /= nw;
+        e = fmax(e, dv + dw);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e = fmax(e, __shfl_down(e, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < PT_NT / 64; w++) e = fmax(e, red[w]);
+        eps_out[0] = sqrt(e);
+    }
+}
+
 // ------------------------------------------------------------ launchers
 static inline unsigned pblocks(int n) { return (unsigned)((n + 63) / 64); }
 
@@ -452,12 +831,69 @@ hipError_t tick(int *d_ts, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t slip_moments(const PParams &pp, DevParticle *P, T *const u[3], hipStream_t s) {
+    if (pp.np > 0) hipLaunchKernelGGL(k_slip_moments<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2]);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t slip_scatter(const PParams &pp, DevParticle *P, T *const u[3], T *const up[3], hipStream_t s) {
+    if (pp.np > 0)
+        hipLaunchKernelGGL(k_slip_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], up[0], up[1],
+                           up[2]);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t f_slip(const PParams &pp, DevParticle *P, T *const sf[3], double coef, hipStream_t s) {
+    if (pp.np > 0) hipLaunchKernelGGL(k_f_slip<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, sf[0], sf[1], sf[2], coef);
+    return hipGetLastError();
+}
+hipError_t velocity_slip(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s) {
+    if (pp.np > 0) hipLaunchKernelGGL(k_velocity_slip, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, ab2, h);
+    return hipGetLastError();
+}
+hipError_t reset_forces(const PParams &pp, DevParticle *P, hipStream_t s) {
+    if (pp.np > 0) hipLaunchKernelGGL(k_reset_forces, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P);
+    return hipGetLastError();
+}
+hipError_t slip_update(const PParams &pp, DevParticle *P, int iter, int restore, hipStream_t s) {
+    if (pp.np > 0) hipLaunchKernelGGL(k_slip_update, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, iter, restore);
+    return hipGetLastError();
+}
+hipError_t slip_conv(const PParams &pp, const DevParticle *P, double *eps, hipStream_t s) {
+    hipLaunchKernelGGL(k_slip_conv, dim3(1), dim3(PT_NT), 0, s, pp, P, eps);
+    return hipGetLastError();
+}
+
+// Per-step particle record: after the step that made ts = *d_ts, slot
+// ts - base - 1 receives (x, v) of every particle; slots past nslots are
+// not written.
+__global__ void k_record(PParams pp, const DevParticle *__restrict__ P, const int *__restrict__ d_ts, int base,
+                         int nslots, double *__restrict__ out) {
+    const int n    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = *d_ts - base - 1;
+    if (n >= pp.np || slot < 0 || slot >= nslots) return;
+    double *o = out + ((size_t)slot * pp.np + n) * 6;
+    for (int d = 0; d < 3; d++) {
+        o[d]     = P[n].x[d];
+        o[3 + d] = P[n].v[d];
+    }
+}
+hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int base, int nslots, double *out,
+                  hipStream_t s) {
+    if (pp.np > 0) hipLaunchKernelGGL(k_record, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, d_ts, base, nslots, out);
+    return hipGetLastError();
+}
+
 #define KH_INST(T)                                                                                              \
     template hipError_t phi_scatter<T>(const PParams &, const DevParticle *, T *, T *, int *, int, hipStream_t);   \
     template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *, const int *, \
                                    double, hipStream_t);                                                          \
     template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, const T *, const int *,     \
-                                      T *const[3], hipStream_t);
+                                      T *const[3], hipStream_t);                                                  \
+    template hipError_t slip_moments<T>(const PParams &, DevParticle *, T *const[3], hipStream_t);                 \
+    template hipError_t slip_scatter<T>(const PParams &, DevParticle *, T *const[3], T *const[3], hipStream_t);    \
+    template hipError_t f_slip<T>(const PParams &, DevParticle *, T *const[3], double, hipStream_t);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST

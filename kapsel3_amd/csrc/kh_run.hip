@@ -8,6 +8,7 @@
 
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/kapsel_hip.h"
@@ -58,6 +59,12 @@ struct PendingEvent {
 
 }  // namespace
 
+// Sets the calling thread's kh_last_error() text (used by kh_server.hip).
+int kh_set_error(int status, const char *msg) {
+    g_last_error = msg;
+    return status;
+}
+
 struct kh_run {
     kh_params               P;
     std::vector<kh_species> species;
@@ -87,6 +94,12 @@ struct kh_run {
     int         *cidx    = nullptr;  // per-particle grid index of each Sekibun point
     int          pcache_valid = 0;
     int         *d_ts         = nullptr;  // jikan.ts on the device (noise counter)
+    double      *slipeps      = nullptr;  // slip convergence measure (1 double)
+    double      *slipdc       = nullptr;  // k = 0 mode of the projected slip field (3 doubles)
+    int          slip_iters   = 0;        // iterations of the last slip loop
+    double      *traj         = nullptr;  // per-step particle record (x, v) for traj_steps steps
+    int          traj_steps   = 0;
+    int          traj_base    = 0;        // ts when recording started
     // the steady-state step (ts >= 1) captured once as a HIP graph
     int              use_graph  = 1;
     bool             capturing  = false;
@@ -281,6 +294,51 @@ struct Step {
         for (int d = 0; d < 3; d++) KH_TRY(c2r(r, S(r, d), W(r, d)));
         return KH_OK;
     }
+    // The slip iteration of Time_evolution_hydro (sp_3d_ns.cxx:121-161):
+    // Make_particle_momentum_factor, then until converged (or slip_iter_max)
+    // slip scatter -> Solenoidal_u -> Calc_f_slip -> up += u -> (first pass:
+    // LJ + gravity) -> velocity with the slip force; finally u := up. In the
+    // fused step zeta also takes the curl of the projected slip field (its
+    // u_hat is rebuilt from zeta later), and uk_dc its k = 0 mode.
+    static int slip_iteration(kh_run *r, int ab2, bool fused) {
+        T *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *up[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+        KH_LAUNCH("slip_moments", slip_moments<T>(r->pp, r->dp, u, r->stream));
+        KH_LAUNCH("slip_update", slip_update(r->pp, r->dp, 0, 0, r->stream));
+        const double coef = (r->dx3 * r->rho) / r->jikan.dt_fluid;
+        const double h    = ab2 ? r->jikan.hdt_md : r->jikan.dt_md;
+        int          iter = 0;
+        for (;;) {
+            for (int d = 0; d < 3; d++) KH_TRY(zero(r, up[d]));
+            KH_LAUNCH("slip_scatter", slip_scatter<T>(r->pp, r->dp, u, up, r->stream));
+            for (int d = 0; d < 3; d++) KH_TRY(r2c(r, up[d], S(r, d)));
+            KH_LAUNCH("solenoidal", (solenoidal<T>(r->g, S(r, 0), S(r, 1), S(r, 2), r->stream)));
+            if (fused)
+                KH_LAUNCH("slip_zeta", (u_to_zeta<T>(r->g, S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4), r->slipdc,
+                                                     r->stream)));
+            for (int d = 0; d < 3; d++) KH_TRY(c2r(r, S(r, d), up[d]));
+            KH_LAUNCH("f_slip", f_slip<T>(r->pp, r->dp, up, coef, r->stream));
+            KH_LAUNCH("add3", (add3<T>(r->g.nreal, up, u, r->stream)));
+            if (iter == 0) KH_LAUNCH("forces", forces(r->pp, r->dp, r->stream));
+            KH_LAUNCH("velocity_slip", velocity_slip(r->pp, r->dp, ab2, h, r->stream));
+            iter++;
+            KH_LAUNCH("slip_conv", slip_conv(r->pp, r->dp, r->slipeps, r->stream));
+            double eps = 0.0;
+            KH_HIP(hipMemcpyAsync(&eps, r->slipeps, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+            KH_HIP(hipStreamSynchronize(r->stream));
+            if (eps < r->P.slip_tol || iter == r->P.slip_iter_max) {
+                KH_LAUNCH("reset_forces", reset_forces(r->pp, r->dp, r->stream));
+                break;
+            }
+            KH_LAUNCH("slip_update", slip_update(r->pp, r->dp, iter, 1, r->stream));
+        }
+        r->slip_iters = iter;
+        for (int d = 0; d < 3; d++) std::swap(r->ureal[d], r->up[d]);  // Swap_mem(u, up)
+        if (fused)
+            KH_LAUNCH("zeta_add", (zeta_add<T>(r->g, Z(r, 0), Z(r, 1), S(r, 3), S(r, 4), r->ukdc, r->slipdc,
+                                               r->stream)));
+        return KH_OK;
+    }
+
     static int add_f(kh_run *r) {
         T *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *f[3] = {W(r, 0), W(r, 1), W(r, 2)};
         KH_LAUNCH("add3", (add3<T>(r->g.nreal, u, f, r->stream)));
@@ -295,6 +353,8 @@ struct Step {
     static int step(kh_run *r) {
         KH_TRY(step_body(r));
         KH_LAUNCH("tick", tick(r->d_ts, r->stream));
+        if (r->traj)
+            KH_LAUNCH("record", record(r->pp, r->dp, r->d_ts, r->traj_base, r->traj_steps, r->traj, r->stream));
         return KH_OK;
     }
 
@@ -315,8 +375,12 @@ struct Step {
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
         KH_LAUNCH("z_adv", (z_adv<T>(g, s3, s5, s)));
         KH_LAUNCH("y_fwd_adv", (cols_y<T, true>(g, c5, 5, 0, 1, s)));
-        KH_LAUNCH("x_fwd_adv", (cols_x<T, true>(g, c5, 5, 0, 1, s)));
-        KH_LAUNCH("adv_ns", (adv_ns<T>(g, s5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
+        if constexpr (sizeof(T) == 8) {  // forward x + slaved Euler in one pass
+            KH_LAUNCH("x_adv_ns", (x_adv_ns<T>(g, c5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
+        } else {  // FP32: the split passes keep more workgroups resident
+            KH_LAUNCH("x_fwd_adv", (cols_x<T, true>(g, c5, 5, 0, 1, s)));
+            KH_LAUNCH("adv_ns", (adv_ns<T>(g, s5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
+        }
         // FIX_CELL, truncation, Zeta_k2u -> u* in real space
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
@@ -328,7 +392,12 @@ struct Step {
         KH_TRY(pos_update(r, ab2));
         KH_TRY(scatter_phi(r));
         KH_TRY(calc_f_hydro(r));
-        KH_TRY(velocity_update(r, ab2));
+        if (r->slip) {
+            KH_TRY(slip_iteration(r, ab2, true));
+            for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
+        } else {
+            KH_TRY(velocity_update(r, ab2));
+        }
         if (r->kbt > 0.) KH_TRY(thermo(r));
         KH_TRY(scatter_up(r));
         // f = P(up - phi u*), u = u* + f, zeta = curl u
@@ -348,7 +417,10 @@ struct Step {
         KH_TRY(pos_update(r, ab2));
         KH_TRY(make_phi(r));
         KH_TRY(calc_f_hydro(r));
-        KH_TRY(velocity_update(r, ab2));
+        if (r->slip)
+            KH_TRY(slip_iteration(r, ab2, false));
+        else
+            KH_TRY(velocity_update(r, ab2));
         if (r->kbt > 0.) KH_TRY(thermo(r));
         KH_TRY(make_up(r));
         KH_TRY(f_particle_sole(r));
@@ -388,6 +460,20 @@ struct Step {
             case KH_FN_MAKE_UP: return make_up(r);
             case KH_FN_F_PARTICLE_SOLE: return f_particle_sole(r);
             case KH_FN_ADD_F_PARTICLE: return add_f(r);
+            case KH_FN_SLIP_MOMENTUM: {
+                T *u[3] = {U(r, 0), U(r, 1), U(r, 2)};
+                KH_LAUNCH("slip_moments", slip_moments<T>(r->pp, r->dp, u, r->stream));
+                return KH_OK;
+            }
+            case KH_FN_SLIP_SCATTER: {
+                T *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *up[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+                for (int d = 0; d < 3; d++) KH_TRY(zero(r, up[d]));
+                KH_LAUNCH("slip_scatter", slip_scatter<T>(r->pp, r->dp, u, up, r->stream));
+                return KH_OK;
+            }
+            case KH_FN_SLIP_ITERATION:
+                if (!r->slip) return fail(KH_ERR_STATE, "no slip species in this run");
+                return slip_iteration(r, r->jikan.ts != 0, false);
             default: return fail(KH_ERR_ARG, "kh_apply: unknown function %d", fn);
         }
     }
@@ -483,9 +569,11 @@ int free_run(kh_run *r) {
     void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
                     r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
                     r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek, r->cphi, r->cidx,
-                    r->d_ts};
+                    r->d_ts, r->slipeps, r->slipdc};
     for (void *b : bufs)
-        if (b) (void)hipFree(b);
+        if (b) (void)hipFreeAsync(b, r->stream);
+    if (r->traj) (void)hipFreeAsync(r->traj, r->stream);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto &p : r->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -514,11 +602,28 @@ int check_params(const kh_params *P, int np) {
     if (np > 0 && (nh >= P->nx || nh >= P->ny || nh >= P->nz))
         return fail(KH_ERR_ARG, "particle domain (half-width %d) does not fit the grid", nh);
     for (int i = 0; i < P->nspecies; i++) {
-        if (P->species[i].propulsion == KH_JP_SLIP)
-            return fail(KH_ERR_UNSUPPORTED, "squirmer slip is not available in this build");
+        if (P->species[i].propulsion == KH_JP_SLIP && P->slip_iter_max < 1)
+            return fail(KH_ERR_ARG, "slip_iter_max must be at least 1 with slip species");
         if (!(P->species[i].mass_ratio > 0)) return fail(KH_ERR_ARG, "mass_ratio must be positive");
     }
     return KH_OK;
+}
+
+// Device bytes of one run (the allocations of kh_run_create).
+size_t run_footprint(const kh_params &P, int np) {
+    const size_t esz = P.precision == KH_MIXED ? 4 : 8;
+    const size_t nzh = (size_t)P.nz / 2 + 1, per = 128 / (2 * esz), nzp = (nzh + per - 1) / per * per;
+    const size_t nreal = (size_t)P.nx * P.ny * P.nz, nspec = (size_t)P.nx * P.ny * nzp;
+    const int    nh = (int)P.a + (int)P.a_xi + 2;
+    const double rmax = P.a * P.dx + 0.5 * P.a_xi * P.dx + 1.7321 * P.dx;
+    size_t       npts = 0;
+    for (int i = -nh; i <= nh; i++)
+        for (int j = -nh; j <= nh; j++)
+            for (int k = -nh; k <= nh; k++)
+                if (sqrt((i * P.dx) * (i * P.dx) + (j * P.dx) * (j * P.dx) + (k * P.dx) * (k * P.dx)) < rmax) npts++;
+    const size_t npp = (size_t)(np > 0 ? np : 1);
+    return 7 * 2 * esz * nspec + 12 * esz * nreal + npp * sizeof(DevParticle) + npp * npts * (esz + sizeof(int)) +
+           npts * sizeof(int) + (size_t)P.nspecies * sizeof(DevSpecies) + 1024;
 }
 
 void drop_graph(kh_run *r) {
@@ -646,10 +751,12 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
 
     // device memory
     const size_t rb = r->esz * (size_t)g.nreal, sb = 2 * r->esz * (size_t)g.nspec;
+    // stream-ordered allocations: creating or destroying a run does not
+    // synchronise the device, so runs on other streams keep going
     auto alloc = [&](void **p, size_t bytes) -> int {
-        if (hipMalloc(p, bytes) != hipSuccess) {
+        if (hipMallocAsync(p, bytes, r->stream) != hipSuccess) {
             *p = nullptr;
-            return fail(KH_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+            return fail(KH_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
         }
         return KH_OK;
     };
@@ -661,6 +768,8 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     if (st == KH_OK) st = alloc(&r->phisum, rb);
     if (st == KH_OK) st = alloc((void **)&r->ukdc, 3 * sizeof(double));
     if (st == KH_OK) st = alloc((void **)&r->d_ts, sizeof(int));
+    if (st == KH_OK) st = alloc((void **)&r->slipeps, sizeof(double));
+    if (st == KH_OK) st = alloc((void **)&r->slipdc, 3 * sizeof(double));
     if (st == KH_OK) st = alloc((void **)&r->dp, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1));
     if (st == KH_OK) st = alloc((void **)&r->ds, sizeof(DevSpecies) * hs.size());
     if (st == KH_OK) st = alloc((void **)&r->sek, sizeof(int) * sek.size());
@@ -668,19 +777,22 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     if (st == KH_OK) st = alloc(&r->cphi, r->esz * ncache);
     if (st == KH_OK) st = alloc((void **)&r->cidx, sizeof(int) * ncache);
     if (st != KH_OK) return bail(st);
-    if (hipMemcpy(r->ds, hs.data(), sizeof(DevSpecies) * hs.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(r->sek, sek.data(), sizeof(int) * sek.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(r->ukdc, 0, 3 * sizeof(double)) != hipSuccess || hipMemset(r->d_ts, 0, sizeof(int)) != hipSuccess ||
-        hipMemset(r->dp, 0, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1)) != hipSuccess)
+    hipStream_t  cs  = r->stream;
+    const size_t psz = sizeof(DevParticle) * (size_t)(np > 0 ? np : 1);
+    if (hipMemcpyAsync(r->ds, hs.data(), sizeof(DevSpecies) * hs.size(), hipMemcpyHostToDevice, cs) != hipSuccess ||
+        hipMemcpyAsync(r->sek, sek.data(), sizeof(int) * sek.size(), hipMemcpyHostToDevice, cs) != hipSuccess ||
+        hipMemsetAsync(r->ukdc, 0, 3 * sizeof(double), cs) != hipSuccess ||
+        hipMemsetAsync(r->d_ts, 0, sizeof(int), cs) != hipSuccess || hipMemsetAsync(r->dp, 0, psz, cs) != hipSuccess)
         return bail(fail(KH_ERR_HIP, "initial upload failed"));
     for (int i = 0; i < 2; i++)
-        if (hipMemset(r->zeta[i], 0, sb) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+        if (hipMemsetAsync(r->zeta[i], 0, sb, cs) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
     for (int i = 0; i < 5; i++)  // the row padding stays zero for good
-        if (hipMemset(r->spec[i], 0, sb) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+        if (hipMemsetAsync(r->spec[i], 0, sb, cs) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
     void *zr[] = {r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2],
                   r->work[0], r->work[1], r->work[2], r->work[3], r->work[4], r->phisum};
     for (void *b : zr)
-        if (hipMemset(b, 0, rb) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+        if (hipMemsetAsync(b, 0, rb, cs) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+    if (hipStreamSynchronize(cs) != hipSuccess) return bail(fail(KH_ERR_HIP, "initial upload failed"));
 
     PParams &pp        = r->pp;
     pp.nx              = g.nx;
@@ -750,7 +862,17 @@ int kh_set_particles(kh_run *r, const Particle *p, int n) {
             d.ts_prev[k] = s.torque_slip_previous[k];
             d.v_slip[k] = s.v_slip[k];
             d.om_slip[k] = s.omega_slip[k];
+            d.mass_center[k] = s.mass_center[k];
+            d.smc[k] = s.surface_mass_center[k];
+            d.sdv[k] = s.surface_dv[k];
+            d.sdw[k] = s.surface_dw[k];
+            for (int l = 0; l < 3; l++) {
+                d.inertia[3 * k + l]  = s.inertia[k][l];
+                d.sinertia[3 * k + l] = s.surface_inertia[k][l];
+            }
         }
+        d.mass  = s.mass;
+        d.smass = s.surface_mass;
         d.q[0] = s.q.s;
         d.q_old[0] = s.q_old.s;
         for (int k = 0; k < 3; k++) {
@@ -796,7 +918,17 @@ int kh_get_particles(kh_run *r, Particle *p, int n) {
             s.torque_slip_previous[k] = d.ts_prev[k];
             s.v_slip[k] = d.v_slip[k];
             s.omega_slip[k] = d.om_slip[k];
+            s.mass_center[k] = d.mass_center[k];
+            s.surface_mass_center[k] = d.smc[k];
+            s.surface_dv[k] = d.sdv[k];
+            s.surface_dw[k] = d.sdw[k];
+            for (int l = 0; l < 3; l++) {
+                s.inertia[k][l]         = d.inertia[3 * k + l];
+                s.surface_inertia[k][l] = d.sinertia[3 * k + l];
+            }
         }
+        s.mass         = d.mass;
+        s.surface_mass = d.smass;
         s.q.s = d.q[0];
         s.q_old.s = d.q_old[0];
         for (int k = 0; k < 3; k++) {
@@ -890,8 +1022,9 @@ int kh_step(kh_run *r, int nsteps) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
     for (int s = 0; s < nsteps; s++) {
+        // (the slip loop decides its length on the host: not captured)
         const bool graph_ok = r->use_graph && !r->timing && r->step_path == KH_PATH_FUSED && r->jikan.ts > 0 &&
-                              !r->zeta_dirty;
+                              !r->zeta_dirty && !r->slip;
         if (graph_ok) {
             if (!r->graph_exec) KH_TRY(capture_step(r));
             KH_HIP(hipGraphLaunch(r->graph_exec, r->stream));
@@ -903,6 +1036,49 @@ int kh_step(kh_run *r, int nsteps) {
         r->jikan.ts++;
     }
     return KH_OK;
+}
+
+int kh_set_trajectory(kh_run *r, int nsteps) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
+    drop_graph(r);  // the record kernel joins (or leaves) the captured step
+    if (r->traj) {
+        KH_HIP(hipFreeAsync(r->traj, r->stream));
+        r->traj = nullptr;
+    }
+    r->traj_steps = 0;
+    if (nsteps == 0 || r->np == 0) return KH_OK;
+    const size_t bytes = sizeof(double) * 6 * (size_t)nsteps * (size_t)r->np;
+    if (hipMallocAsync((void **)&r->traj, bytes, r->stream) != hipSuccess) {
+        r->traj = nullptr;
+        return fail(KH_ERR_NOMEM, "trajectory buffer of %zu bytes", bytes);
+    }
+    r->traj_steps = nsteps;
+    r->traj_base  = r->jikan.ts;
+    return KH_OK;
+}
+
+int kh_get_trajectory(kh_run *r, double *out, int nsteps) {
+    if (!r || (!out && nsteps > 0)) return fail(KH_ERR_ARG, "NULL argument");
+    if (nsteps < 0 || nsteps > r->traj_steps) return fail(KH_ERR_ARG, "%d steps requested, %d recorded", nsteps, r->traj_steps);
+    if (nsteps == 0 || r->np == 0) return KH_OK;
+    KH_HIP(hipMemcpyAsync(out, r->traj, sizeof(double) * 6 * (size_t)nsteps * (size_t)r->np, hipMemcpyDeviceToHost,
+                          r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    return KH_OK;
+}
+
+long long kh_run_bytes(const kh_params *P, int np) {
+    if (!P || np < 0) return -1;
+    return (long long)run_footprint(*P, np);
+}
+
+int kh_query(kh_run *r) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    const hipError_t e = hipStreamQuery(r->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(KH_ERR_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
 }
 
 int kh_set_graph(kh_run *r, int on) {

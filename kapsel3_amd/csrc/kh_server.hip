@@ -1,0 +1,325 @@
+// kh_server.hip -- continuous batching of independent SPM runs on one GPU.
+//
+// Clients submit runs (parameters, particles, step count, a host buffer for
+// the per-step trajectory) from any thread; one scheduler thread owns the
+// device side. It admits queued runs while a slot is free and their device
+// footprint fits the HBM budget, keeps every admitted run resident on its
+// own non-blocking stream until it completes, and feeds each run a few steps
+// at a time (after the first step each step is one captured-graph launch),
+// so the work of runs of different sizes overlaps on the GPU. A run's
+// trajectory is recorded on the device every step and copied to the
+// client's buffer once, when the run completes. Latency is measured from
+// submission to that copy.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/kapsel_hip.h"
+
+int kh_set_error(int status, const char *msg);
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+struct Request {
+    long long                  id = 0;
+    kh_params                  P{};
+    std::vector<kh_species>    species;
+    std::vector<Particle>      parts;
+    int                        nsteps = 0;
+    double                    *traj_out = nullptr;
+    long long                  bytes    = 0;
+    long long                  grid     = 0;
+    clk::time_point            t_submit, t_admit, t_done;
+    int                        status = 1;  // 1 queued, 2 active, 0 done, < 0 failed (kh_status)
+    std::string                error;
+    // device side (scheduler thread only)
+    kh_run *run      = nullptr;
+    int     enqueued = 0;
+};
+
+}  // namespace
+
+struct kh_server {
+    kh_server_config                       cfg{};
+    int                                    device = 0;
+    long long                              budget = 0;
+    std::mutex                             mu;
+    std::condition_variable                cv;
+    std::deque<Request *>                  queue;
+    std::map<long long, std::unique_ptr<Request>> all;
+    std::vector<Request *>                 active;  // scheduler thread only
+    long long                              used = 0, peak_bytes = 0, next_id = 1;
+    int                                    peak_active = 0, nactive = 0;
+    bool                                   stop = false, started = false;
+    clk::time_point                        t_first, t_last;
+    std::vector<double>                    latency;  // seconds, completed runs
+    long long                              done_steps = 0, failed = 0;
+    double                                 done_points = 0;
+    std::thread                            worker;
+};
+
+namespace {
+
+void finish(kh_server *S, Request *q, int status, const char *err) {
+    if (q->run) {
+        if (status == 0 && q->traj_out && kh_get_trajectory(q->run, q->traj_out, q->nsteps) != KH_OK) {
+            status = KH_ERR_HIP;
+            err    = kh_last_error();
+        }
+        kh_run_destroy(q->run);
+        q->run = nullptr;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    q->t_done = clk::now();
+    q->status = status;
+    if (err) q->error = err;
+    S->used -= q->bytes;
+    S->nactive--;
+    if (status == 0) {
+        S->latency.push_back(secs(q->t_submit, q->t_done));
+        S->done_steps += q->nsteps;
+        S->done_points += (double)q->grid * q->nsteps;
+        S->t_last = q->t_done;
+    } else {
+        S->failed++;
+    }
+    S->cv.notify_all();
+}
+
+// Device side of admission: create the run, upload, initial constraint block.
+int start(Request *q) {
+    int st = kh_run_create(&q->P, (int)q->parts.size(), &q->run);
+    if (st == KH_OK) st = kh_set_particles(q->run, q->parts.data(), (int)q->parts.size());
+    if (st == KH_OK) st = kh_init_fluid(q->run);
+    if (st == KH_OK) st = kh_set_trajectory(q->run, q->nsteps);
+    return st;
+}
+
+void loop(kh_server *S) {
+    (void)hipSetDevice(S->device);
+    for (;;) {
+        bool worked = false;
+        // admission: a free slot and room in the HBM budget, in arrival order
+        for (;;) {
+            Request *q = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                if (S->queue.empty() || (int)S->active.size() >= S->cfg.max_active) break;
+                Request *h = S->queue.front();
+                if (S->used + h->bytes > S->budget && !S->active.empty()) break;
+                S->queue.pop_front();
+                S->used += h->bytes;
+                S->nactive++;
+                S->peak_bytes  = std::max(S->peak_bytes, S->used);
+                S->peak_active = std::max(S->peak_active, S->nactive);
+                h->status      = 2;
+                h->t_admit     = clk::now();
+                q              = h;
+            }
+            const int st = start(q);
+            if (st != KH_OK) {
+                const std::string e = kh_last_error();
+                finish(S, q, st, e.c_str());
+            } else {
+                S->active.push_back(q);
+            }
+            worked = true;
+        }
+        // progress: every run whose last chunk has drained gets the next one
+        for (size_t i = 0; i < S->active.size();) {
+            Request  *q    = S->active[i];
+            const int idle = q->enqueued ? kh_query(q->run) : 1;
+            if (idle == 0) {
+                i++;
+                continue;
+            }
+            if (idle < 0 || q->enqueued == q->nsteps) {
+                const std::string msg = idle < 0 ? kh_last_error() : "";
+                finish(S, q, idle < 0 ? idle : 0, idle < 0 ? msg.c_str() : nullptr);
+                S->active.erase(S->active.begin() + i);
+                worked = true;
+                continue;
+            }
+            const int k  = std::min(S->cfg.chunk_steps, q->nsteps - q->enqueued);
+            const int st = kh_step(q->run, k);
+            if (st != KH_OK) {
+                const std::string msg = kh_last_error();
+                finish(S, q, st, msg.c_str());
+                S->active.erase(S->active.begin() + i);
+                worked = true;
+                continue;
+            }
+            q->enqueued += k;
+            worked = true;
+            i++;
+        }
+        if (!worked) {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->stop && S->queue.empty() && S->active.empty()) break;
+            if (S->active.empty())
+                S->cv.wait_for(lk, std::chrono::milliseconds(5));
+            else {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int kh_server_create(const kh_server_config *cfg, kh_server **out) {
+    if (!cfg || !out) return kh_set_error(KH_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (cfg->max_active < 1 || cfg->chunk_steps < 1) return kh_set_error(KH_ERR_ARG, "max_active and chunk_steps must be >= 1");
+    if (cfg->hbm_budget <= 0 && !(cfg->hbm_fraction > 0 && cfg->hbm_fraction <= 1))
+        return kh_set_error(KH_ERR_ARG, "hbm_fraction must be in (0, 1] without an explicit budget");
+    auto S = new kh_server();
+    S->cfg = *cfg;
+    if (hipGetDevice(&S->device) != hipSuccess) {
+        delete S;
+        return kh_set_error(KH_ERR_HIP, "no HIP device");
+    }
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        delete S;
+        return kh_set_error(KH_ERR_HIP, "hipMemGetInfo failed");
+    }
+    S->budget = cfg->hbm_budget > 0 ? cfg->hbm_budget : (long long)(cfg->hbm_fraction * (double)fr);
+    S->worker = std::thread(loop, S);
+    *out      = S;
+    return KH_OK;
+}
+
+int kh_server_submit(kh_server *S, const kh_params *P, const Particle *parts, int np, int nsteps, double *traj_out,
+                     long long *id) {
+    if (!S || !P || (!parts && np > 0) || !id) return kh_set_error(KH_ERR_ARG, "NULL argument");
+    if (nsteps < 1) return kh_set_error(KH_ERR_ARG, "nsteps must be >= 1");
+    const long long bytes = kh_run_bytes(P, np) + (long long)sizeof(double) * 6 * nsteps * np;
+    if (bytes < 0) return kh_set_error(KH_ERR_ARG, "bad run parameters");
+    if (bytes > S->budget) return kh_set_error(KH_ERR_NOMEM, "run needs more device memory than the server's budget");
+    auto q     = std::make_unique<Request>();
+    q->P       = *P;
+    q->species.assign(P->species, P->species + P->nspecies);
+    q->P.species = q->species.data();
+    q->parts.assign(parts, parts + np);
+    q->nsteps   = nsteps;
+    q->traj_out = traj_out;
+    q->bytes    = bytes;
+    q->grid     = (long long)P->nx * P->ny * P->nz;
+    q->t_submit = clk::now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) return kh_set_error(KH_ERR_STATE, "server is shutting down");
+    q->id = S->next_id++;
+    *id   = q->id;
+    if (!S->started) {
+        S->started = true;
+        S->t_first = q->t_submit;
+    }
+    S->queue.push_back(q.get());
+    S->all[q->id] = std::move(q);
+    S->cv.notify_all();
+    return KH_OK;
+}
+
+int kh_server_poll(kh_server *S, long long id) {
+    if (!S) return kh_set_error(KH_ERR_ARG, "NULL server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return kh_set_error(KH_ERR_ARG, "unknown run id");
+    const Request *q = it->second.get();
+    if (q->status < 0) return kh_set_error(q->status, q->error.c_str());
+    return q->status == 0 ? 1 : 0;
+}
+
+int kh_server_wait(kh_server *S, long long id, double timeout_s) {
+    if (!S) return kh_set_error(KH_ERR_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return kh_set_error(KH_ERR_ARG, "unknown run id");
+    const Request *q  = it->second.get();
+    const auto     dl = clk::now() + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(timeout_s));
+    while (q->status > 0)
+        if (S->cv.wait_until(lk, dl) == std::cv_status::timeout && q->status > 0) return 0;
+    if (q->status < 0) return kh_set_error(q->status, q->error.c_str());
+    return 1;
+}
+
+int kh_server_run_times(kh_server *S, long long id, double *queued_s, double *run_s) {
+    if (!S || !queued_s || !run_s) return kh_set_error(KH_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return kh_set_error(KH_ERR_ARG, "unknown run id");
+    const Request *q = it->second.get();
+    if (q->status != 0) return kh_set_error(KH_ERR_STATE, "run not complete");
+    *queued_s = secs(q->t_submit, q->t_admit);
+    *run_s    = secs(q->t_admit, q->t_done);
+    return KH_OK;
+}
+
+int kh_server_get_stats(kh_server *S, kh_server_stats *o) {
+    if (!S || !o) return kh_set_error(KH_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    memset(o, 0, sizeof *o);
+    o->submitted   = (long long)S->all.size();
+    o->completed   = (long long)S->latency.size();
+    o->failed      = S->failed;
+    o->queued      = (long long)S->queue.size();
+    o->active      = S->nactive;
+    o->peak_active = S->peak_active;
+    o->peak_bytes  = S->peak_bytes;
+    o->budget      = S->budget;
+    if (!S->latency.empty()) {
+        std::vector<double> l = S->latency;
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double p) {  // nearest rank
+            size_t k = (size_t)std::max(1.0, std::ceil(p * (double)l.size())) - 1;
+            return 1e3 * l[std::min(k, l.size() - 1)];
+        };
+        double sum = 0;
+        for (double x : l) sum += x;
+        o->p50_ms  = pct(0.50);
+        o->p99_ms  = pct(0.99);
+        o->mean_ms = 1e3 * sum / (double)l.size();
+        o->max_ms  = 1e3 * l.back();
+        o->wall_s  = secs(S->t_first, S->t_last);
+        if (o->wall_s > 0) {
+            o->runs_per_s             = (double)l.size() / o->wall_s;
+            o->steps_per_s            = (double)S->done_steps / o->wall_s;
+            o->grid_point_steps_per_s = S->done_points / o->wall_s;
+        }
+    }
+    return KH_OK;
+}
+
+int kh_server_destroy(kh_server *S) {
+    if (!S) return KH_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cv.notify_all();
+    }
+    if (S->worker.joinable()) S->worker.join();
+    delete S;
+    return KH_OK;
+}
+
+}  // extern "C"

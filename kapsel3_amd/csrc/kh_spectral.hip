@@ -3,6 +3,7 @@
 // computed from the index on the fly (no K tables in HBM); the formulas and
 // their operation order follow the reference lines cited per kernel.
 #include "kh_launch.h"
+#include "kh_modes.h"
 
 namespace kh {
 
@@ -114,41 +115,8 @@ __global__ __launch_bounds__(PW_NT) void k_adv_ns(Grid g, const cpx<T> *__restri
     if (kz >= g.nzh) return;
     const int ikx = kx_of(g, i), iky = ky_of(g, j);
     if (!kept_mode(g, ikx, iky, kz)) return;
-    const double k1 = ikx * g.wave_x, k2 = iky * g.wave_y, k3 = kz * g.wave_z;
-    const double k1k2 = k1 * k2, k2k3 = k2 * k3, k3k1 = k3 * k1;
-    const double K2   = (g.wave_x * ikx) * (g.wave_x * ikx) + (g.wave_y * iky) * (g.wave_y * iky) +
-                      (g.wave_z * kz) * (g.wave_z * kz);
-    const double IK2 = (K2 > 0.0) ? 1. / K2 : 0.0;
-    const cpx<T> a0 = P0[idx], a1 = P1[idx], a2 = P2[idx], a3 = P3[idx], a4 = P4[idx];
-    double       om[3][2];
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-        const double u2u3 = c ? a0.y : a0.x, u3u1 = c ? a1.y : a1.x, u1u2 = c ? a2.y : a2.x;
-        const double u22_u32 = c ? a3.y : a3.x, u32_u12 = c ? a4.y : a4.x;
-        const double u12_u22 = -u32_u12 - u22_u32;
-        om[0][c] = -(k2k3 * u22_u32 + (k3 - k2) * (k3 + k2) * u2u3 + k3k1 * u1u2 - k1k2 * u3u1);
-        om[1][c] = -(k3k1 * u32_u12 + (k1 - k3) * (k1 + k3) * u3u1 + k1k2 * u2u3 - k2k3 * u1u2);
-        om[2][c] = -(k1k2 * u12_u22 + (k2 - k1) * (k2 + k1) * u1u2 + k2k3 * u3u1 - k3k1 * u2u3);
-    }
-    int c0, c1;
-    if (ikx != 0) {
-        c0 = 1;
-        c1 = 2;
-    } else if (iky != 0) {
-        c0 = 2;
-        c1 = 0;
-    } else {
-        c0 = 0;
-        c1 = 1;
-    }
-    const double dmy = exp(dmy0 * K2) - 1.;
-    cpx<T>       x0 = z0[idx], x1 = z1[idx];
-    const double nx0 = (double)x0.x + dmy * ((double)x0.x - dmy1 * IK2 * om[c0][0]);
-    const double ny0 = (double)x0.y + dmy * ((double)x0.y - dmy1 * IK2 * om[c0][1]);
-    const double nx1 = (double)x1.x + dmy * ((double)x1.x - dmy1 * IK2 * om[c1][0]);
-    const double ny1 = (double)x1.y + dmy * ((double)x1.y - dmy1 * IK2 * om[c1][1]);
-    z0[idx]          = cmk((T)nx0, (T)ny0);
-    z1[idx]          = cmk((T)nx1, (T)ny1);
+    const cpx<T> a[5] = {P0[idx], P1[idx], P2[idx], P3[idx], P4[idx]};
+    slaved_euler_mode<T>(g, ikx, iky, kz, a, dmy0, dmy1, z0[idx], z1[idx]);
 }
 
 // Truncation + Solenoidal_uk (operate_omega.h:105-112, operate_omega.cxx:171-195).

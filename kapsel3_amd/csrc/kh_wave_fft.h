@@ -107,10 +107,11 @@ __host__ __device__ constexpr WavePlan make_wave_plan(int L, int E, bool rev) {
     return p;
 }
 
-// Points per lane: `pref` (16 for FP32, 8 for FP64 by default), at most L.
+// Points per lane: `pref` (16 for FP32, 8 for FP64 by default), at most L
+// and at least L/64 (a line fits one wavefront).
 template <typename T, int L, int PREF = (sizeof(T) == 4 ? 16 : 8)>
 struct WaveE {
-    static constexpr int value = L < PREF ? L : PREF;
+    static constexpr int value = L < PREF ? L : (L / PREF > 64 ? L / 64 : PREF);
 };
 
 __host__ __device__ constexpr bool wave_fft_ok(int L) { return L >= 8 && L <= 512 && (L & (L - 1)) == 0; }

@@ -9,6 +9,8 @@
 // line's LDS strip, where Z[k] and Z[L-k] meet:
 //   A[k] = (Z[k] + conj Z[L-k]) / 2,  B[k] = (Z[k] - conj Z[L-k]) / 2i.
 // Only modes kz < kzv are read (the rest are zero) or written.
+#include <type_traits>
+
 #include "kh_fused.h"
 #include "kh_wave_fft.h"
 #include "kh_zwave.h"
@@ -109,25 +111,53 @@ __global__ __launch_bounds__(256) void k_zw_r2c(const T *__restrict__ in, cpx<T>
     store_pair<T, L, F>(v, wl, L / 2 + 1, out);
 }
 
+// Real rows of a line <-> its LDS strip as (row 2p, row 2p+1) pairs, four
+// consecutive points per lane per access: each lane moves 16-byte vectors
+// of both rows, so the HBM side of the z passes is fully coalesced.
+template <typename T, int L>
+struct RowStage {
+    static constexpr int V  = 4;      // points per vector
+    static constexpr int NV = L / V;  // vectors per row
+    using vec = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+    __device__ __forceinline__ static vec ld(const T *p) { return *reinterpret_cast<const vec *>(p); }
+    __device__ __forceinline__ static void st(T *p, const vec &v) { *reinterpret_cast<vec *>(p) = v; }
+};
+
 template <typename T, int L, int E>
 __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T scale, int kzv, int pitch) {
     using Geo = WaveGeom<T, L, E>;
     using F   = WaveFft<T, L, E, false, false>;
+    using RS  = RowStage<T, L>;
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, nrows, pitch);
-    cpx<T>               v[F::E];
+    cpx<T>                  v[F::E];
     load_pair<T, L, F>(io.in[blockIdx.y], wl, kzv, v);
     F::run(v, wl.buf, tw, wl.g);
+#pragma unroll
+    for (int i = 0; i < F::E; i++) wl.buf[WavePad<T>::at(F::out_point(wl.g, i))] = v[i];
+    wave_sync();
     T *ra = io.out[blockIdx.y] + (2 * wl.line) * L;
     T *rb = ra + L;
+    if constexpr (L >= 4 * Geo::G) {
 #pragma unroll
-    for (int i = 0; i < F::E; i++) {
-        const int n = F::out_point(wl.g, i);
-        if (wl.okA) ra[n] = v[i].x * scale;
-        if (wl.okB) rb[n] = v[i].y * scale;
+        for (int t = 0; t < RS::NV / Geo::G; t++) {
+            const int    z = 4 * (wl.g + Geo::G * t);
+            const cpx<T> a = wl.buf[WavePad<T>::at(z)], b = wl.buf[WavePad<T>::at(z + 1)],
+                         c = wl.buf[WavePad<T>::at(z + 2)], d = wl.buf[WavePad<T>::at(z + 3)];
+            if (wl.okA) RS::st(ra + z, {a.x * scale, b.x * scale, c.x * scale, d.x * scale});
+            if (wl.okB) RS::st(rb + z, {a.y * scale, b.y * scale, c.y * scale, d.y * scale});
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < F::E; i++) {
+            const int n = F::out_point(wl.g, i);
+            if (wl.okA) ra[n] = v[i].x * scale;
+            if (wl.okB) rb[n] = v[i].y * scale;
+        }
     }
+    wave_sync();
 }
 
 // Advection products (U2advection_k, operate_omega.cxx:31-47): three
@@ -193,30 +223,44 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, const T *__restrict__ 
                                                   cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2) {
     using Geo = WaveGeom<T, L, E>;
     using Ff  = WaveFft<T, L, E, true, true>;
+    using RS  = RowStage<T, L>;
+    static_assert(L >= 4 * Geo::G, "four points per lane per row");
+    constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
-    const long long      nrows = (long long)g.nx * g.ny;
+    const long long         nrows = (long long)g.nx * g.ny;
     const WaveLane<T, L, E> wl(strips, nrows, g.nzp);
-    const long long      oa = (2 * wl.line) * L, ob = oa + L;
-    cpx<T>               ph[E];
+    const long long         oa = (2 * wl.line) * L, ob = oa + L;
+    using vec                  = typename RS::vec;
+    const vec zero             = {(T)0, (T)0, (T)0, (T)0};
+    vec       pa[NT4], pb[NT4];  // min(phi_sum, 1) of both rows
 #pragma unroll
-    for (int i = 0; i < E; i++) {
-        const int n = Ff::in_point(wl.g, i);
-        ph[i]       = cmk(wl.okA ? fmin(phis[oa + n], (T)1) : (T)0, wl.okB ? fmin(phis[ob + n], (T)1) : (T)0);
+    for (int t = 0; t < NT4; t++) {
+        const int z = 4 * (wl.g + Geo::G * t);
+        vec       a = wl.okA ? RS::ld(phis + oa + z) : zero, b = wl.okB ? RS::ld(phis + ob + z) : zero;
+        pa[t]       = {fmin(a.x, (T)1), fmin(a.y, (T)1), fmin(a.z, (T)1), fmin(a.w, (T)1)};
+        pb[t]       = {fmin(b.x, (T)1), fmin(b.y, (T)1), fmin(b.z, (T)1), fmin(b.w, (T)1)};
     }
     const T *UP[3] = {up0, up1, up2}, *US[3] = {us0, us1, us2};
     cpx<T>  *dst[3] = {f0, f1, f2};
 #pragma unroll 1
     for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int t = 0; t < NT4; t++) {
+            const int z  = 4 * (wl.g + Geo::G * t);
+            const vec ua = wl.okA ? RS::ld(UP[d] + oa + z) : zero, ub = wl.okB ? RS::ld(UP[d] + ob + z) : zero;
+            const vec sa = wl.okA ? RS::ld(US[d] + oa + z) : zero, sb = wl.okB ? RS::ld(US[d] + ob + z) : zero;
+            wl.buf[WavePad<T>::at(z)]     = cmk(ua.x - pa[t].x * sa.x, ub.x - pb[t].x * sb.x);
+            wl.buf[WavePad<T>::at(z + 1)] = cmk(ua.y - pa[t].y * sa.y, ub.y - pb[t].y * sb.y);
+            wl.buf[WavePad<T>::at(z + 2)] = cmk(ua.z - pa[t].z * sa.z, ub.z - pb[t].z * sb.z);
+            wl.buf[WavePad<T>::at(z + 3)] = cmk(ua.w - pa[t].w * sa.w, ub.w - pb[t].w * sb.w);
+        }
+        wave_sync();
         cpx<T> v[E];
 #pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int n = Ff::in_point(wl.g, i);
-            const T   a = wl.okA ? UP[d][oa + n] - ph[i].x * US[d][oa + n] : (T)0;
-            const T   b = wl.okB ? UP[d][ob + n] - ph[i].y * US[d][ob + n] : (T)0;
-            v[i]        = cmk(a, b);
-        }
+        for (int i = 0; i < E; i++) v[i] = wl.buf[WavePad<T>::at(Ff::in_point(wl.g, i))];
+        wave_sync();
         Ff::run(v, wl.buf, tw, wl.g);
         store_pair<T, L, Ff>(v, wl, g.trn_z, dst[d]);
     }
@@ -232,10 +276,13 @@ bool zw_supported(int L) { return wave_fft_ok(L); }
 #ifndef KH_ZW_ADV_E32
 #define KH_ZW_ADV_E32 8
 #endif
+#ifndef KH_ZW_ADV_E64
+#define KH_ZW_ADV_E64 8
+#endif
 template <typename T, int L>
 struct ZwE {
     static constexpr int pass = WaveE<T, L>::value;
-    static constexpr int adv  = WaveE<T, L, sizeof(T) == 4 ? KH_ZW_ADV_E32 : 8>::value;
+    static constexpr int adv  = WaveE<T, L, sizeof(T) == 4 ? KH_ZW_ADV_E32 : KH_ZW_ADV_E64>::value;
 };
 
 template <typename T, int L, int E>

@@ -843,11 +843,10 @@ static void thermostat(kr_run *r) {
 /* squirmer slip: operate_surface.cxx, make_phi.h:282-376, md_force.cxx:325-382 */
 /* ------------------------------------------------------------------ */
 /* Squirmer_coord (make_phi.h:282-376) */
-static void squirmer_coord(const kr_run *r, const double *x, double *nr, double *th, double *ph, double *norm_r,
-                           double *theta_angle, double *phi_angle, const Particle *p) {
+static void squirmer_coord_q(int ax, const quaternion *qp, const double *x, double *nr, double *th, double *ph,
+                             double *norm_r, double *theta_angle, double *phi_angle) {
     static const double ex[3] = {1, 0, 0}, ey[3] = {0, 1, 0}, ez[3] = {0, 0, 1};
     const double       *e1, *e2, *e3;
-    int                 ax = r->spec[p->spec].janus_axis;
     if (ax == KH_JAX_Z) {
         e1 = ex;
         e2 = ey;
@@ -861,7 +860,7 @@ static void squirmer_coord(const kr_run *r, const double *x, double *nr, double 
         e2 = ez;
         e3 = ex;
     }
-    space2body(nr, x, &p->q);
+    space2body(nr, x, qp);
     *norm_r      = sqrt(SQ(nr[0]) + SQ(nr[1]) + SQ(nr[2]));
     double dnorm = 1.0 / *norm_r;
     for (int d = 0; d < 3; d++) nr[d] *= dnorm;
@@ -881,11 +880,11 @@ static void squirmer_coord(const kr_run *r, const double *x, double *nr, double 
         double d2  = e2[0] * r12[0] + e2[1] * r12[1] + e2[2] * r12[2];
         *phi_angle = atan2(d2, d1);
         double tmp[3];
-        body2space(tmp, nr, &p->q);
+        body2space(tmp, nr, qp);
         memcpy(nr, tmp, sizeof tmp);
-        body2space(tmp, th, &p->q);
+        body2space(tmp, th, qp);
         memcpy(th, tmp, sizeof tmp);
-        body2space(tmp, ph, &p->q);
+        body2space(tmp, ph, qp);
         memcpy(ph, tmp, sizeof tmp);
     } else {
         *norm_r = sqrt(SQ(x[0]) + SQ(x[1]) + SQ(x[2]));
@@ -897,6 +896,20 @@ static void squirmer_coord(const kr_run *r, const double *x, double *nr, double 
         *theta_angle = (dot_e3_r > 0.0) ? 0.0 : KR_PI;
         *phi_angle   = 0.0;
     }
+}
+
+static void squirmer_coord(const kr_run *r, const double *x, double *nr, double *th, double *ph, double *norm_r,
+                           double *theta_angle, double *phi_angle, const Particle *p) {
+    squirmer_coord_q(r->spec[p->spec].janus_axis, &p->q, x, nr, th, ph, norm_r, theta_angle, phi_angle);
+}
+
+void kr_squirmer_coord(const double q4[4], int axis, const double x[3], double nr[3], double nth[3], double nph[3],
+                       double *theta, double *phi) {
+    quaternion q;
+    q.s = q4[0];
+    for (int d = 0; d < 3; d++) q.v[d] = q4[1 + d];
+    double norm_r;
+    squirmer_coord_q(axis, &q, x, nr, nth, nph, &norm_r, theta, phi);
 }
 
 /* Make_particle_momentum_factor (operate_surface.cxx:12-153) */
@@ -1292,6 +1305,12 @@ int kr_apply(kr_run *r, int fn) {
             solenoidal_u(r, r->f);
             break;
         case KH_FN_ADD_F_PARTICLE: add_f_particle(r, r->u, r->f); break;
+        case KH_FN_SLIP_MOMENTUM: make_particle_momentum_factor(r, r->u); break;
+        case KH_FN_SLIP_SCATTER:
+            for (int d = 0; d < 3; d++) reset_field(r, r->up[d]);
+            make_force_u_slip_particle(r, r->up, r->u);
+            break;
+        case KH_FN_SLIP_ITERATION: slip_iteration(r); break;
         default: return -1;
     }
     return 0;

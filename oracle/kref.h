@@ -49,6 +49,10 @@ int kr_fft_inverse(int nx, int ny, int nz, double *a);
 
 /* Smooth profile phi(r) (profile.h:30-50) and the Sekibun cell. */
 double kr_phi(double r, double radius, double dx, double xi);
+/* Squirmer_coord (make_phi.h:282-376) for quaternion (s, v0, v1, v2) and
+ * Janus axis KH_JAX_*: unit vectors and polar/azimuthal angles of x */
+void kr_squirmer_coord(const double q4[4], int axis, const double x[3], double nr[3], double nth[3], double nph[3],
+                       double *theta, double *phi);
 int    kr_sekibun_count(const kr_run *r);
 int    kr_sekibun_cell(const kr_run *r, int *out3);
 

@@ -42,6 +42,8 @@ def lib():
         L.kr_fft_inverse.argtypes = [C.c_int, C.c_int, C.c_int, vp]
         L.kr_phi.restype = C.c_double
         L.kr_phi.argtypes = [C.c_double] * 4
+        L.kr_squirmer_coord.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.kr_squirmer_coord.restype = None
         L.kr_sekibun_count.argtypes = [vp]
         L.kr_sekibun_cell.argtypes = [vp, vp]
         L.kr_philox4x32_10.argtypes = [vp, vp, vp]
