@@ -21,12 +21,16 @@ hipError_t cols_x(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, i
 template <typename T>
 hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const u[3], double *ukdc,
                         int fixmask, hipStream_t s);
-// inverse z (3) -> products -> forward z (5)
+// inverse z (3) -> products -> forward z (5); for power-of-two NZ, p5[2] may
+// be s3[0], p5[1] s3[2] and p5[0] s3[1] (and no other aliasing)
 template <typename T>
 hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStream_t s);
-// f = up - min(phi_sum,1) u* -> forward z (3)
+// f = up - min(phi_sum,1) u* -> forward z (3); with `clear` (possible when
+// z_force_clears(g)) up and phi_sum are left zero for the next step
+bool z_force_clears(const Grid &g);
 template <typename T>
-hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s);
+hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
+                   hipStream_t s);
 // forward x (5 products) -> Omega + slaved-Euler update of zeta at the kept modes
 template <typename T>
 hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,

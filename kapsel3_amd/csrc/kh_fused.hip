@@ -641,10 +641,14 @@ hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStr
     }
 }
 
+bool z_force_clears(const Grid &g) { return zw_supported(g.nz); }
+
 template <typename T>
-hipError_t z_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s) {
+hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
+                   hipStream_t s) {
     const long long nrows = (long long)g.nx * g.ny;
-    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, s);
+    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, s);
+    if (clear) return hipErrorInvalidValue;
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                   \
     case n: {                                                                                                        \
@@ -714,7 +718,8 @@ hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s) {
     template hipError_t x_zeta2u_inv<T>(const Grid &, const cpx<T> *, const cpx<T> *, cpx<T> *const[3], double *,   \
                                         int, hipStream_t);                                                          \
     template hipError_t z_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], hipStream_t);                   \
-    template hipError_t z_force<T>(const Grid &, T *const[3], const T *, T *const[3], cpx<T> *const[3], hipStream_t); \
+    template hipError_t z_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int,                 \
+                                   hipStream_t);                                                                   \
     template hipError_t x_f_zeta<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, hipStream_t);    \
     template hipError_t x_adv_ns<T>(const Grid &, const Comps<T> &, cpx<T> *, cpx<T> *, double, double, hipStream_t); \
     template hipError_t zeta_clean<T>(const Grid &, cpx<T> *, cpx<T> *, hipStream_t);                               \

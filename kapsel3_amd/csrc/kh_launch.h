@@ -30,9 +30,10 @@ hipError_t fill(long long n, T *p, T v, hipStream_t s);
 
 // kh_particle.hip (cphi/cidx: per-particle cache of phi and grid index at
 // every Sekibun point, np * np_domain entries)
+// move = 1: the particle is first advanced by the position integrator
 template <typename T>
-hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter,
-                       hipStream_t s);
+hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter, hipStream_t s,
+                       int move = 0, int ab2 = 0, double dt = 0.0, double hdt = 0.0);
 template <typename T>
 hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi, const int *cidx,
                    double coef, hipStream_t s);
@@ -41,7 +42,10 @@ hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, 
                       T *const up[3], hipStream_t s);
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s);
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s);
-hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s);
+// noise = 1: the thermostat's kick follows the update in the same launch
+hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s, int noise = 0,
+                    uint64_t seed = 0, uint32_t stream = 0, const int *d_ts = nullptr, double niv = 0.0,
+                    double nio = 0.0);
 hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s);
 hipError_t tick(int *d_ts, hipStream_t s);

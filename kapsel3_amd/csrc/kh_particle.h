@@ -46,6 +46,7 @@ struct PParams {
     int               g_dir;
     double            gravity_on_fluid;
     int               motor;  // SW_JANUS_MOTOR
+    double           *xs;     // np x 4: particle position, 1.0 for obstacles (compact copy for pair loops)
 };
 
 __host__ __device__ inline int sek_pack(int i, int j, int k) { return (i + 128) | ((j + 128) << 8) | ((k + 128) << 16); }
@@ -110,9 +111,12 @@ __device__ __forceinline__ PPoint sek_point(const PParams &pp, const PCell &c, c
     PPoint    q;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        // (x_int + cell + N) % N for valid positions; the double modulo keeps
-        // the index inside the grid even for a non-finite particle state
-        rm[d]  = ((c.x_int[d] + cell[d]) % N[d] + N[d]) % N[d];
+        // (x_int + cell + N) % N: x_int is in [0, N) and |cell| < N, so one
+        // wrap suffices; the clamp keeps the index inside the grid even for
+        // a non-finite particle state
+        int m = c.x_int[d] + cell[d];
+        m     = m < 0 ? m + N[d] : (m >= N[d] ? m - N[d] : m);
+        rm[d] = min(max(m, 0), N[d] - 1);
         q.r[d] = (double)cell[d] * pp.dx - c.residue[d];
         x[d]   = rm[d] * pp.dx;
     }

@@ -65,13 +65,27 @@ __device__ __forceinline__ void angular2v(int rotation, const double *om, const 
 // Make_phi_particle_sum scatter (make_phi.cxx:220-246). Also stores each
 // point's phi and grid index in the per-particle cache the gather and the
 // u_p scatter of the same step read (positions do not change in between).
-// SCATTER = false only fills the cache.
-template <typename T, bool SCATTER>
-__global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, const DevParticle *__restrict__ P,
+// SCATTER = false only fills the cache. MOVE = true first advances the
+// particle (the position integrator of the same step, one lane).
+__device__ void position_one(const PParams &pp, DevParticle *P, int n, int ab2, double dt, double hdt);
+
+template <typename T, bool SCATTER, bool MOVE>
+__global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *__restrict__ P,
                                                        T *__restrict__ phisum, T *__restrict__ cphi,
-                                                       int *__restrict__ cidx) {
-    const int       n     = blockIdx.x;
-    const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
+                                                       int *__restrict__ cidx, int ab2, double dt, double hdt) {
+    const int n = blockIdx.x;
+    double    xp[3];
+    if constexpr (MOVE) {
+        __shared__ double xs[3];
+        if (threadIdx.x == 0) {
+            position_one(pp, P, n, ab2, dt, hdt);
+            for (int d = 0; d < 3; d++) xs[d] = P[n].x[d];
+        }
+        __syncthreads();
+        for (int d = 0; d < 3; d++) xp[d] = xs[d];
+    } else {
+        for (int d = 0; d < 3; d++) xp[d] = P[n].x[d];
+    }
     const PCell     c     = particle_cell(pp, xp);
     const long long cb    = (long long)n * pp.np_domain;
     for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
@@ -156,13 +170,26 @@ __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParti
         const double ph = (double)cphi[cb + m];
         if (ph == 0.0) continue;
         const int    idx     = cidx[cb + m];
-        const double dmy_phi = ph / fmax((double)phisum[idx], 1.0);
+        const T      ps      = phisum[idx];
+        const double dmy_phi = ph / fmax((double)ps, 1.0);
         double       r[3], vr[3];
         sek_r(pp, c, pp.sek[m], r);
         angular2v(pp.rotation, om, r, vr);
-        unsafeAtomicAdd(&up0[idx], (T)((vp[0] + vr[0]) * dmy_phi));
-        unsafeAtomicAdd(&up1[idx], (T)((vp[1] + vr[1]) * dmy_phi));
-        unsafeAtomicAdd(&up2[idx], (T)((vp[2] + vr[2]) * dmy_phi));
+        const T a = (T)((vp[0] + vr[0]) * dmy_phi), b = (T)((vp[1] + vr[1]) * dmy_phi),
+                e = (T)((vp[2] + vr[2]) * dmy_phi);
+        if (ps == cphi[cb + m]) {
+            // phi_sum is this particle's phi alone: no other particle adds
+            // here (a contribution below half an ulp of phi would be the only
+            // exception, and it is below the rounding of the sum), so the
+            // sum onto the zeroed field is a plain store
+            up0[idx] = a;
+            up1[idx] = b;
+            up2[idx] = e;
+        } else {
+            unsafeAtomicAdd(&up0[idx], a);
+            unsafeAtomicAdd(&up1[idx], b);
+            unsafeAtomicAdd(&up2[idx], e);
+        }
     }
 }
 
@@ -193,7 +220,9 @@ __device__ __forceinline__ void body2space(double *o, const double *q, const dou
 // orientation updates (particle_rotation_solver.h:23-29, 72-81).
 __global__ void k_position(PParams pp, DevParticle *__restrict__ P, int ab2, double dt, double hdt) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= pp.np) return;
+    if (n < pp.np) position_one(pp, P, n, ab2, dt, hdt);
+}
+__device__ void position_one(const PParams &pp, DevParticle *P, int n, int ab2, double dt, double hdt) {
     DevParticle &p = P[n];
     if (pp.species[p.spec].propulsion == 2 /* obstacle */) return;
     for (int d = 0; d < 3; d++) {
@@ -203,6 +232,7 @@ __global__ void k_position(PParams pp, DevParticle *__restrict__ P, int ab2, dou
         p.x[d] += delta_x;
     }
     for (int d = 0; d < 3; d++) p.x[d] = fmod(p.x[d] + pp.L[d], pp.L[d]);
+    for (int d = 0; d < 3; d++) pp.xs[4 * n + d] = p.x[d];
     if (pp.rotation) {
         double dq[4];
         if (!ab2) {
@@ -277,8 +307,8 @@ __global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__res
     if (pp.lj_on) {
         for (int j = threadIdx.x; j < pp.np; j += PT_NT) {
             if (j == i) continue;
-            if (obst_i && pp.species[P[j].spec].propulsion == 2) continue;
-            const double xj[3] = {P[j].x[0], P[j].x[1], P[j].x[2]};
+            if (obst_i && pp.xs[4 * j + 3] != 0.0) continue;
+            const double xj[3] = {pp.xs[4 * j], pp.xs[4 * j + 1], pp.xs[4 * j + 2]};
             double       rv[3];
             // r_ij_vec from the higher index to the lower, as the link cell takes it
             const double rij = (i > j) ? distance0(pp, xi, xj, rv) : distance0(pp, xj, xi, rv);
@@ -300,7 +330,16 @@ __global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__res
 
 // MD_solver_velocity_Euler (particle_solver.cxx:103-142) / _AB2_hydro
 // (239-281), self_propulsion (88-101) and reset_Forces (11-37).
-__global__ void k_velocity(PParams pp, DevParticle *__restrict__ P, int ab2, double h) {
+struct Noise {  // Add_random_force_thermostat arguments (fluct.cxx:10-100)
+    int        on;
+    uint64_t   seed;
+    uint32_t   stream;
+    const int *d_ts;
+    double     niv, nio;
+};
+__device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise &nz);
+
+__global__ void k_velocity(PParams pp, DevParticle *__restrict__ P, int ab2, double h, Noise nz) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= pp.np) return;
     DevParticle      &p = P[n];
@@ -345,6 +384,7 @@ __global__ void k_velocity(PParams pp, DevParticle *__restrict__ P, int ab2, dou
         p.ts_prev[d] = p.ts[d];
         p.ts[d]      = 0.0;
     }
+    if (nz.on) add_noise(pp, P, n, nz);  // the thermostat follows the update (sp_3d_ns.cxx)
 }
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
@@ -370,11 +410,15 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
 
 // Add_random_force_thermostat (fluct.cxx:10-43); normals from Philox keyed
 // by (seed, stream) with counter (particle, step, block, stream).
-__global__ void k_thermostat(PParams pp, DevParticle *__restrict__ P, uint64_t seed, uint32_t stream,
-                             const int *__restrict__ d_ts, double niv, double nio) {
+__global__ void k_thermostat(PParams pp, DevParticle *__restrict__ P, Noise nz) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= pp.np) return;
-    const int ts = *d_ts;
+    if (n < pp.np) add_noise(pp, P, n, nz);
+}
+__device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise &nz) {
+    const uint64_t seed = nz.seed;
+    const uint32_t stream = nz.stream;
+    const double   niv = nz.niv, nio = nz.nio;
+    const int      ts = *nz.d_ts;
     DevParticle      &p = P[n];
     const DevSpecies &s = pp.species[p.spec];
     if (s.propulsion == 2) return;
@@ -777,13 +821,18 @@ __global__ __launch_bounds__(PT_NT) void k_slip_conv(PParams pp, const DevPartic
 static inline unsigned pblocks(int n) { return (unsigned)((n + 63) / 64); }
 
 template <typename T>
-hipError_t phi_scatter(const PParams &pp, const DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter,
-                       hipStream_t s) {
+hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter, hipStream_t s,
+                       int move, int ab2, double dt, double hdt) {
     if (pp.np > 0) {
-        if (scatter)
-            hipLaunchKernelGGL((k_phi_scatter<T, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx);
+        if (move)
+            hipLaunchKernelGGL((k_phi_scatter<T, true, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx,
+                               ab2, dt, hdt);
+        else if (scatter)
+            hipLaunchKernelGGL((k_phi_scatter<T, true, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi,
+                               cidx, ab2, dt, hdt);
         else
-            hipLaunchKernelGGL((k_phi_scatter<T, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx);
+            hipLaunchKernelGGL((k_phi_scatter<T, false, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi,
+                               cidx, ab2, dt, hdt);
     }
     return hipGetLastError();
 }
@@ -811,14 +860,16 @@ hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s) {
     if (pp.np > 0) hipLaunchKernelGGL(k_forces, dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
     return hipGetLastError();
 }
-hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s) {
-    if (pp.np > 0) hipLaunchKernelGGL(k_velocity, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, ab2, h);
+hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s, int noise, uint64_t seed,
+                    uint32_t stream, const int *d_ts, double niv, double nio) {
+    const Noise nz{noise, seed, stream, d_ts, niv, nio};
+    if (pp.np > 0) hipLaunchKernelGGL(k_velocity, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, ab2, h, nz);
     return hipGetLastError();
 }
 hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s) {
-    if (pp.np > 0)
-        hipLaunchKernelGGL(k_thermostat, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, seed, stream, d_ts, niv, nio);
+    const Noise nz{1, seed, stream, d_ts, niv, nio};
+    if (pp.np > 0) hipLaunchKernelGGL(k_thermostat, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, nz);
     return hipGetLastError();
 }
 
@@ -886,7 +937,8 @@ hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int 
 }
 
 #define KH_INST(T)                                                                                              \
-    template hipError_t phi_scatter<T>(const PParams &, const DevParticle *, T *, T *, int *, int, hipStream_t);   \
+    template hipError_t phi_scatter<T>(const PParams &, DevParticle *, T *, T *, int *, int, hipStream_t, int, int,  \
+                                       double, double);                                                           \
     template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *, const int *, \
                                    double, hipStream_t);                                                          \
     template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, const T *, const int *,     \

@@ -95,6 +95,9 @@ struct kh_run {
     int          pcache_valid = 0;
     int         *d_ts         = nullptr;  // jikan.ts on the device (noise counter)
     double      *slipeps      = nullptr;  // slip convergence measure (1 double)
+    double      *xs           = nullptr;  // compact particle positions (np x 4)
+    bool         zclear       = false;    // the fused step's z_force leaves u_p and phi_sum zero
+    bool         pclean       = false;    // u_p and phi_sum are zero now
     double      *slipdc       = nullptr;  // k = 0 mode of the projected slip field (3 doubles)
     int          slip_iters   = 0;        // iterations of the last slip loop
     double      *traj         = nullptr;  // per-step particle record (x, v) for traj_steps steps
@@ -242,6 +245,13 @@ struct Step {
         r->pcache_valid = 1;
         return KH_OK;
     }
+    // the position integrator and the phi scatter in one launch
+    static int move_scatter_phi(kh_run *r, int ab2) {
+        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, 1, r->stream, 1, ab2,
+                                                 r->jikan.dt_md, r->jikan.hdt_md)));
+        r->pcache_valid = 1;
+        return KH_OK;
+    }
     static int ensure_cache(kh_run *r) {
         if (!r->pcache_valid) {
             KH_LAUNCH("phi_cache", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, 0, r->stream)));
@@ -260,17 +270,26 @@ struct Step {
                                          (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->stream)));
         return KH_OK;
     }
-    static int velocity_update(kh_run *r, int ab2) {
+    // noise amplitudes of Add_random_force_thermostat (fluct.cxx:11-18)
+    static void noise_amp(const kh_run *r, double &niv, double &nio) {
+        const double zeta_drag = 6. * KH_PI * r->eta * r->radius;
+        const double zeta_rot  = 8. * KH_PI * r->eta * r->radius * r->radius * r->radius;
+        niv                    = kT_snap_v * sqrt(zeta_drag * r->jikan.dt_md * r->kbt * r->P.alpha_v);
+        nio                    = kT_snap_o * sqrt(zeta_rot * r->jikan.dt_md * r->kbt * r->P.alpha_o);
+    }
+    // LJ + gravity, then the velocity update; with `noise` the thermostat's
+    // kick follows in the same launch
+    static int velocity_update(kh_run *r, int ab2, bool noise = false) {
         KH_LAUNCH("forces", forces(r->pp, r->dp, r->stream));
-        KH_LAUNCH("velocity", velocity(r->pp, r->dp, ab2, ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->stream));
+        double niv = 0.0, nio = 0.0;
+        if (noise) noise_amp(r, niv, nio);
+        KH_LAUNCH("velocity", velocity(r->pp, r->dp, ab2, ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->stream,
+                                       noise ? 1 : 0, r->P.seed, r->P.stream_id, r->d_ts, niv, nio));
         return KH_OK;
     }
     static int thermo(kh_run *r) {
-        // fluct.cxx:11-18
-        const double zeta_drag = 6. * KH_PI * r->eta * r->radius;
-        const double zeta_rot  = 8. * KH_PI * r->eta * r->radius * r->radius * r->radius;
-        const double niv       = kT_snap_v * sqrt(zeta_drag * r->jikan.dt_md * r->kbt * r->P.alpha_v);
-        const double nio       = kT_snap_o * sqrt(zeta_rot * r->jikan.dt_md * r->kbt * r->P.alpha_o);
+        double niv, nio;
+        noise_amp(r, niv, nio);
         KH_LAUNCH("thermostat", thermostat(r->pp, r->dp, r->P.seed, r->P.stream_id, r->d_ts, niv, nio, r->stream));
         return KH_OK;
     }
@@ -368,41 +387,49 @@ struct Step {
             r->zeta_dirty = 0;
         }
         cpx<T>  *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
-        cpx<T>  *s5[5] = {S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)};
         Comps<T> c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
+        // buffer of each advection product (u2u3, u3u1, u1u2, u2^2-u3^2,
+        // u3^2-u1^2): z_adv reads u1, u2, u3 from S0, S1, S2 (u2 twice) and
+        // writes each product only into a buffer it has finished reading
+        cpx<T>  *q5[5] = {S(r, 1), S(r, 2), S(r, 0), S(r, 4), S(r, 3)};
+        Comps<T> cq5{{q5[0], q5[1], q5[2], q5[3], q5[4]}};
         // NS_solver_slavedEuler: zeta -> u -> products -> -Omega -> slaved Euler
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, 0, s)));
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
-        KH_LAUNCH("z_adv", (z_adv<T>(g, s3, s5, s)));
-        KH_LAUNCH("y_fwd_adv", (cols_y<T, true>(g, c5, 5, 0, 1, s)));
+        KH_LAUNCH("z_adv", (z_adv<T>(g, s3, q5, s)));
+        KH_LAUNCH("y_fwd_adv", (cols_y<T, true>(g, cq5, 5, 0, 1, s)));
         if constexpr (sizeof(T) == 8) {  // forward x + slaved Euler in one pass
-            KH_LAUNCH("x_adv_ns", (x_adv_ns<T>(g, c5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
+            KH_LAUNCH("x_adv_ns", (x_adv_ns<T>(g, cq5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
         } else {  // FP32: the split passes keep more workgroups resident
-            KH_LAUNCH("x_fwd_adv", (cols_x<T, true>(g, c5, 5, 0, 1, s)));
-            KH_LAUNCH("adv_ns", (adv_ns<T>(g, s5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
+            KH_LAUNCH("x_fwd_adv", (cols_x<T, true>(g, cq5, 5, 0, 1, s)));
+            KH_LAUNCH("adv_ns", (adv_ns<T>(g, q5, Z(r, 0), Z(r, 1), -r->nu * r->jikan.dt_fluid, 1. / r->nu, s)));
         }
         // FIX_CELL, truncation, Zeta_k2u -> u* in real space
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
         ZIO<T> zio{{S(r, 0), S(r, 1), S(r, 2)}, {U(r, 0), U(r, 1), U(r, 2)}};
         KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z)));
-        // particles (Reset_phi / Reset_u, scatters, gather, forces, integrators)
-        for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
-        KH_TRY(zero(r, PS(r)));
-        KH_TRY(pos_update(r, ab2));
-        KH_TRY(scatter_phi(r));
+        // particles (Reset_phi / Reset_u, scatters, gather, forces, integrators);
+        // the previous fused step's z_force already left u_p and phi_sum zero
+        if (!(r->zclear && r->pclean)) {
+            for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
+            KH_TRY(zero(r, PS(r)));
+        }
+        r->pclean = false;
+        KH_TRY(move_scatter_phi(r, ab2));
         KH_TRY(calc_f_hydro(r));
         if (r->slip) {
             KH_TRY(slip_iteration(r, ab2, true));
             for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
+            if (r->kbt > 0.) KH_TRY(thermo(r));
         } else {
-            KH_TRY(velocity_update(r, ab2));
+            KH_TRY(velocity_update(r, ab2, r->kbt > 0.));
         }
-        if (r->kbt > 0.) KH_TRY(thermo(r));
         KH_TRY(scatter_up(r));
         // f = P(up - phi u*), u = u* + f, zeta = curl u
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
-        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, s)));
+        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear, s)));
+        r->pclean = r->zclear;
         KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
         KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));
         return KH_OK;
@@ -412,6 +439,7 @@ struct Step {
     // kernel sequence per reference function.
     static int step_reference_order(kh_run *r) {
         const int ab2 = r->jikan.ts != 0;
+        r->pclean     = false;
         KH_TRY(ns_solver(r));
         KH_TRY(zeta_k2u(r, r->fixmask));  // FIX_CELL, truncation, u*
         KH_TRY(pos_update(r, ab2));
@@ -431,6 +459,7 @@ struct Step {
 
     // Init_zeta_k on a fluid at rest + sp_3d_ns.cxx:809-826
     static int init_fluid(kh_run *r) {
+        r->pclean = false;
         KH_HIP(hipMemsetAsync(r->zeta[0], 0, sizeof(cpx<T>) * r->g.nspec, r->stream));
         KH_HIP(hipMemsetAsync(r->zeta[1], 0, sizeof(cpx<T>) * r->g.nspec, r->stream));
         KH_HIP(hipMemsetAsync(r->ukdc, 0, sizeof(double) * 3, r->stream));
@@ -446,6 +475,7 @@ struct Step {
 
     static int apply(kh_run *r, int fn) {
         r->zeta_dirty = 1;
+        r->pclean     = false;
         switch (fn) {
             case KH_FN_NS_SOLVER: return ns_solver(r);
             case KH_FN_ZETA_K2U: return zeta_k2u(r, 0, 0);
@@ -569,7 +599,7 @@ int free_run(kh_run *r) {
     void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
                     r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
                     r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek, r->cphi, r->cidx,
-                    r->d_ts, r->slipeps, r->slipdc};
+                    r->d_ts, r->slipeps, r->slipdc, r->xs};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, r->stream);
     if (r->traj) (void)hipFreeAsync(r->traj, r->stream);
@@ -622,7 +652,8 @@ size_t run_footprint(const kh_params &P, int np) {
             for (int k = -nh; k <= nh; k++)
                 if (sqrt((i * P.dx) * (i * P.dx) + (j * P.dx) * (j * P.dx) + (k * P.dx) * (k * P.dx)) < rmax) npts++;
     const size_t npp = (size_t)(np > 0 ? np : 1);
-    return 7 * 2 * esz * nspec + 12 * esz * nreal + npp * sizeof(DevParticle) + npp * npts * (esz + sizeof(int)) +
+    return 7 * 2 * esz * nspec + 12 * esz * nreal + npp * (sizeof(DevParticle) + 4 * sizeof(double)) +
+           npp * npts * (esz + sizeof(int)) +
            npts * sizeof(int) + (size_t)P.nspecies * sizeof(DevSpecies) + 1024;
 }
 
@@ -769,6 +800,7 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     if (st == KH_OK) st = alloc((void **)&r->ukdc, 3 * sizeof(double));
     if (st == KH_OK) st = alloc((void **)&r->d_ts, sizeof(int));
     if (st == KH_OK) st = alloc((void **)&r->slipeps, sizeof(double));
+    if (st == KH_OK) st = alloc((void **)&r->xs, 4 * sizeof(double) * (size_t)(np > 0 ? np : 1));
     if (st == KH_OK) st = alloc((void **)&r->slipdc, 3 * sizeof(double));
     if (st == KH_OK) st = alloc((void **)&r->dp, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1));
     if (st == KH_OK) st = alloc((void **)&r->ds, sizeof(DevSpecies) * hs.size());
@@ -820,6 +852,8 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     pp.g_dir           = P->g_dir;
     pp.gravity_on_fluid = r->gravity_on_fluid;
     pp.motor           = r->motor;
+    pp.xs              = r->xs;
+    r->zclear          = !r->slip && z_force_clears(g);
 
     CTime t0  = {0, 0.0, P->dt, P->dt * 0.5, P->dt, P->dt * 0.5};  // sp_3d_ns.cxx:782
     r->jikan  = t0;
@@ -882,7 +916,13 @@ int kh_set_particles(kh_run *r, const Particle *p, int n) {
         d.spec = s.spec;
     }
     r->pcache_valid = 0;
+    std::vector<double> xs(4 * (size_t)(n > 0 ? n : 1), 0.0);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) xs[4 * i + k] = h[i].x[k];
+        xs[4 * i + 3] = r->species[h[i].spec].propulsion == KH_JP_OBSTACLE ? 1.0 : 0.0;
+    }
     if (n > 0) KH_HIP(hipMemcpyAsync(r->dp, h.data(), sizeof(DevParticle) * n, hipMemcpyHostToDevice, r->stream));
+    if (n > 0) KH_HIP(hipMemcpyAsync(r->xs, xs.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, r->stream));
     KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
 }
@@ -988,6 +1028,7 @@ int kh_get_field(kh_run *r, int field, double *out) {
 
 int kh_set_field(kh_run *r, int field, const double *in) {
     if (!r || !in) return fail(KH_ERR_ARG, "NULL argument");
+    r->pclean = false;
     void *b = field_buffer(r, field);
     if (!b) return fail(KH_ERR_ARG, "field %d cannot be written", field);
     return upload_real(r, b, in);
@@ -1024,7 +1065,7 @@ int kh_step(kh_run *r, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
         // (the slip loop decides its length on the host: not captured)
         const bool graph_ok = r->use_graph && !r->timing && r->step_path == KH_PATH_FUSED && r->jikan.ts > 0 &&
-                              !r->zeta_dirty && !r->slip;
+                              !r->zeta_dirty && !r->slip && (!r->zclear || r->pclean);
         if (graph_ok) {
             if (!r->graph_exec) KH_TRY(capture_step(r));
             KH_HIP(hipGraphLaunch(r->graph_exec, r->stream));

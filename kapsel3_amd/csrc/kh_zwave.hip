@@ -122,6 +122,10 @@ struct RowStage {
     __device__ __forceinline__ static vec ld(const T *p) { return *reinterpret_cast<const vec *>(p); }
     __device__ __forceinline__ static void st(T *p, const vec &v) { *reinterpret_cast<vec *>(p) = v; }
 };
+template <typename V>
+__device__ __forceinline__ bool nonzero(const V &v) {
+    return !(v.x == 0 && v.y == 0 && v.z == 0 && v.w == 0);
+}
 
 template <typename T, int L, int E>
 __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T scale, int kzv, int pitch) {
@@ -160,9 +164,12 @@ __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T sc
     wave_sync();
 }
 
-// Advection products (U2advection_k, operate_omega.cxx:31-47): three
-// inverse z transforms, the five products in registers, five forward z
-// transforms.
+// Advection products (U2advection_k, operate_omega.cxx:31-47): inverse z
+// transforms of u1 (s0), u2 (s1), u3 (s2), the five products in registers,
+// forward z transforms into p0..p4. Two components are live at a time, so u2
+// is read and transformed twice; the order is u1 u2 -> p2, then u3 -> p1, p4,
+// then u2 again -> p0, p3. A product buffer may therefore be an input buffer
+// that is no longer read: p2 = s0, p1 = s2, p0 = s1 are allowed.
 template <typename T, int L, int E>
 __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict__ s0, const cpx<T> *__restrict__ s1,
                                                 const cpx<T> *__restrict__ s2, cpx<T> *__restrict__ p0,
@@ -177,50 +184,57 @@ __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, (long long)g.nx * g.ny, g.nzp);
     const int            kzv = g.trn_z;
-    cpx<T>               u1[E], u2[E], u3[E];
-    load_pair<T, L, Fi>(s0, wl, kzv, u1);
-    Fi::run(u1, wl.buf, tw, wl.g);
-    load_pair<T, L, Fi>(s1, wl, kzv, u2);
-    Fi::run(u2, wl.buf, tw, wl.g);
-    load_pair<T, L, Fi>(s2, wl, kzv, u3);
-    Fi::run(u3, wl.buf, tw, wl.g);
+    cpx<T> ua[E], ub[E], v[E];
+    load_pair<T, L, Fi>(s0, wl, kzv, ua);
+    Fi::run(ua, wl.buf, tw, wl.g);
+    load_pair<T, L, Fi>(s1, wl, kzv, ub);
+    Fi::run(ub, wl.buf, tw, wl.g);
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        u1[i] = cscale(u1[i], scale);
-        u2[i] = cscale(u2[i], scale);
-        u3[i] = cscale(u3[i], scale);
+        ua[i] = cscale(ua[i], scale);
+        ub[i] = cscale(ub[i], scale);
+        v[i]  = cmk(ua[i].x * ub[i].x, ua[i].y * ub[i].y);  // u1 u2
     }
-    cpx<T> *dst[5] = {p0, p1, p2, p3, p4};
-#pragma unroll 1
-    for (int q = 0; q < 5; q++) {
-        cpx<T> v[E];
+    Ff::run(v, wl.buf, tw, wl.g);
+    store_pair<T, L, Ff>(v, wl, kzv, p2);
+    load_pair<T, L, Fi>(s2, wl, kzv, ub);  // ub := u3
+    Fi::run(ub, wl.buf, tw, wl.g);
 #pragma unroll
-        for (int i = 0; i < E; i++) {  // .x = row 2p, .y = row 2p+1
-            const cpx<T> a = u1[i], b = u2[i], c = u3[i];
-            if (q == 0)
-                v[i] = cmk(b.x * c.x, b.y * c.y);
-            else if (q == 1)
-                v[i] = cmk(c.x * a.x, c.y * a.y);
-            else if (q == 2)
-                v[i] = cmk(a.x * b.x, a.y * b.y);
-            else if (q == 3)
-                v[i] = cmk((b.x - c.x) * (b.x + c.x), (b.y - c.y) * (b.y + c.y));
-            else
-                v[i] = cmk((c.x - a.x) * (c.x + a.x), (c.y - a.y) * (c.y + a.y));
-        }
-        Ff::run(v, wl.buf, tw, wl.g);
-        store_pair<T, L, Ff>(v, wl, kzv, dst[q]);
+    for (int i = 0; i < E; i++) {
+        ub[i] = cscale(ub[i], scale);
+        v[i]  = cmk(ub[i].x * ua[i].x, ub[i].y * ua[i].y);  // u3 u1
     }
+    Ff::run(v, wl.buf, tw, wl.g);
+    store_pair<T, L, Ff>(v, wl, kzv, p1);
+#pragma unroll
+    for (int i = 0; i < E; i++)  // u3^2 - u1^2
+        v[i] = cmk((ub[i].x - ua[i].x) * (ub[i].x + ua[i].x), (ub[i].y - ua[i].y) * (ub[i].y + ua[i].y));
+    Ff::run(v, wl.buf, tw, wl.g);
+    store_pair<T, L, Ff>(v, wl, kzv, p4);
+    load_pair<T, L, Fi>(s1, wl, kzv, ua);  // ua := u2 again
+    Fi::run(ua, wl.buf, tw, wl.g);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        ua[i] = cscale(ua[i], scale);
+        v[i]  = cmk(ua[i].x * ub[i].x, ua[i].y * ub[i].y);  // u2 u3
+    }
+    Ff::run(v, wl.buf, tw, wl.g);
+    store_pair<T, L, Ff>(v, wl, kzv, p0);
+#pragma unroll
+    for (int i = 0; i < E; i++)  // u2^2 - u3^2
+        v[i] = cmk((ua[i].x - ub[i].x) * (ua[i].x + ub[i].x), (ua[i].y - ub[i].y) * (ua[i].y + ub[i].y));
+    Ff::run(v, wl.buf, tw, wl.g);
+    store_pair<T, L, Ff>(v, wl, kzv, p3);
 }
 
 // Rigidity force f = up - min(phi_sum, 1) u* (Make_f_particle_dt_nonsole,
 // f_particle.h:35-54; make_phi.cxx:248-259), forward z per component.
 template <typename T, int L, int E>
-__global__ __launch_bounds__(256) void k_zw_force(Grid g, const T *__restrict__ up0, const T *__restrict__ up1,
-                                                  const T *__restrict__ up2, const T *__restrict__ phis,
+__global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T *__restrict__ up1,
+                                                  T *__restrict__ up2, T *__restrict__ phis,
                                                   const T *__restrict__ us0, const T *__restrict__ us1,
                                                   const T *__restrict__ us2, cpx<T> *__restrict__ f0,
-                                                  cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2) {
+                                                  cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2, int clear) {
     using Geo = WaveGeom<T, L, E>;
     using Ff  = WaveFft<T, L, E, true, true>;
     using RS  = RowStage<T, L>;
@@ -239,10 +253,15 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, const T *__restrict__ 
     for (int t = 0; t < NT4; t++) {
         const int z = 4 * (wl.g + Geo::G * t);
         vec       a = wl.okA ? RS::ld(phis + oa + z) : zero, b = wl.okB ? RS::ld(phis + ob + z) : zero;
+        if (clear) {  // leave phi_sum zero for the next step's scatter
+            if (nonzero(a)) RS::st(phis + oa + z, zero);
+            if (nonzero(b)) RS::st(phis + ob + z, zero);
+        }
         pa[t]       = {fmin(a.x, (T)1), fmin(a.y, (T)1), fmin(a.z, (T)1), fmin(a.w, (T)1)};
         pb[t]       = {fmin(b.x, (T)1), fmin(b.y, (T)1), fmin(b.z, (T)1), fmin(b.w, (T)1)};
     }
-    const T *UP[3] = {up0, up1, up2}, *US[3] = {us0, us1, us2};
+    T       *UP[3] = {up0, up1, up2};
+    const T *US[3] = {us0, us1, us2};
     cpx<T>  *dst[3] = {f0, f1, f2};
 #pragma unroll 1
     for (int d = 0; d < 3; d++) {
@@ -251,6 +270,10 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, const T *__restrict__ 
             const int z  = 4 * (wl.g + Geo::G * t);
             const vec ua = wl.okA ? RS::ld(UP[d] + oa + z) : zero, ub = wl.okB ? RS::ld(UP[d] + ob + z) : zero;
             const vec sa = wl.okA ? RS::ld(US[d] + oa + z) : zero, sb = wl.okB ? RS::ld(US[d] + ob + z) : zero;
+            if (clear) {  // and u_p zero
+                if (nonzero(ua)) RS::st(UP[d] + oa + z, zero);
+                if (nonzero(ub)) RS::st(UP[d] + ob + z, zero);
+            }
             wl.buf[WavePad<T>::at(z)]     = cmk(ua.x - pa[t].x * sa.x, ub.x - pb[t].x * sb.x);
             wl.buf[WavePad<T>::at(z + 1)] = cmk(ua.y - pa[t].y * sa.y, ub.y - pb[t].y * sb.y);
             wl.buf[WavePad<T>::at(z + 2)] = cmk(ua.z - pa[t].z * sa.z, ub.z - pb[t].z * sb.z);
@@ -273,16 +296,10 @@ bool zw_supported(int L) { return wave_fft_ok(L); }
 
 // points per lane of each pass (the advection pass holds three components
 // in registers, so it takes fewer points per lane)
-#ifndef KH_ZW_ADV_E32
-#define KH_ZW_ADV_E32 8
-#endif
-#ifndef KH_ZW_ADV_E64
-#define KH_ZW_ADV_E64 8
-#endif
 template <typename T, int L>
 struct ZwE {
     static constexpr int pass = WaveE<T, L>::value;
-    static constexpr int adv  = WaveE<T, L, sizeof(T) == 4 ? KH_ZW_ADV_E32 : KH_ZW_ADV_E64>::value;
+    static constexpr int adv  = WaveE<T, L, 8>::value;
 };
 
 template <typename T, int L, int E>
@@ -334,13 +351,14 @@ hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T sca
 }
 
 template <typename T>
-hipError_t zw_force(const Grid &g, T *const up[3], const T *phis, T *const us[3], cpx<T> *const f[3], hipStream_t s) {
+hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
+                    hipStream_t s) {
     const long long nrows = (long long)g.nx * g.ny;
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
         hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, g, up[0], up[1], up[2], \
-                           phis, us[0], us[1], us[2], f[0], f[1], f[2]);                                            \
+                           phis, us[0], us[1], us[2], f[0], f[1], f[2], clear);                                     \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)
 #undef KH_CASE
@@ -352,7 +370,8 @@ hipError_t zw_force(const Grid &g, T *const up[3], const T *phis, T *const us[3]
     template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t);                    \
     template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t);            \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
-    template hipError_t zw_force<T>(const Grid &, T *const[3], const T *, T *const[3], cpx<T> *const[3], hipStream_t);
+    template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int,               \
+                                    hipStream_t);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST
