@@ -101,7 +101,7 @@ def time_gpu(precision, n, nparticles, steps, warmup, graph=True, seed=3):
     p = run.get_particles()
     finite = bool(np.isfinite(p["x"]).all() and np.isfinite(p["v"]).all())
     run.close()
-    return (t1 - t0), stats, finite, (t3 - t2)
+    return (t1 - t0), stats, finite, (t3 - t2), (parts, p)
 
 
 def cpu_baseline(n, nparticles, steps, seed=3):
@@ -119,6 +119,46 @@ def cpu_baseline(n, nparticles, steps, seed=3):
     cores = kref.lib().kr_threads()
     r.close()
     return steps / (t1 - t0), cores, t1 - t0
+
+
+def accuracy(mixed, fp64, steps, a):
+    """Mixed against FP64 after the same steps from the same start: max
+    errors relative to the FP64 velocity scale, displacement and omega scale."""
+    p0, pm = mixed
+    _, pd = fp64
+    vs = max(np.abs(pd["v"]).max(), 1e-300)
+    disp = max(np.abs(pd["x_nopbc"] - p0["x_nopbc"]).max(), 1e-300)
+    return {"steps": steps, "velocity_rel_err": float(np.abs(pm["v"] - pd["v"]).max() / vs),
+            "position_rel_err": float(np.abs(pm["x_nopbc"] - pd["x_nopbc"]).max() / disp),
+            "omega_rel_err": float(np.abs(pm["omega"] - pd["omega"]).max() / (vs / a))}
+
+
+def squirmer_bench(steps, n=256, count=1000):
+    """Config 5: 1000 squirmers (B1 = 0.01, B2/B1 uniform in [-2, 2], random
+    orientations) at 256^3, steps/s of the whole step with the slip
+    iteration, mixed and FP64 (after 2 warm-up steps)."""
+    import kapsel3_amd
+    from kapsel3_amd import workloads
+    from kapsel3_amd._abi import FP64, MIXED
+    out = {"workload": f"{count} squirmers a=5 xi=2 at {n}^3, B1 0.01, slip tol 0.005, at most 4 iterations",
+           "steps": steps}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+        P, parts = workloads.squirmers(n, count, precision=prec)
+        run = kapsel3_amd.Run(P, count)
+        run.set_particles(parts)
+        run.init_fluid()
+        run.step(2)
+        run.sync()
+        t0 = time.perf_counter()
+        run.step(steps)
+        run.sync()
+        t = time.perf_counter() - t0
+        p = run.get_particles()
+        run.close()
+        out[name] = {"value": steps / t, "unit": "steps/s", "ms_per_step": t / steps * 1e3,
+                     "finite": bool(np.isfinite(p["v"]).all())}
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
 
 
 def serve_bench(count, rate, nsteps, max_active, seed=2024):
@@ -191,6 +231,8 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 step only (value = FP64 steps/s)")
     ap.add_argument("--no-graph", action="store_true", help="run the timed steps uncaptured")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-squirm", action="store_true", help="skip the squirmer suspension")
+    ap.add_argument("--squirm-steps", type=int, default=10)
     ap.add_argument("--serve-runs", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-steps", type=int, default=200)
@@ -219,7 +261,7 @@ def main():
     traffic = load_traffic(a.traffic) if (n, P) == (256, 2000) else None
 
     def measure(prec, name, sf, k):
-        t, st, ok, tk = time_gpu(prec, n, P, k, a.warmup, graph=not a.no_graph)
+        t, st, ok, tk, state = time_gpu(prec, n, P, k, a.warmup, graph=not a.no_graph)
         ms, kms = t / k * 1e3, tk / k * 1e3
         table = kernel_table(st, n, P, npts, sf)
         sb = step_bytes(n, P, npts, sf, sf)
@@ -230,7 +272,7 @@ def main():
                                   "pruned_step_bytes": pruned, "pruned_frac": pruned / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s"},
                 "roofline": roofline_of(st, n, P, npts, sf, kms, k, (traffic or {}).get(name)),
-                "kernel_pass_ms_per_step": kms, "kernels": table}
+                "kernel_pass_ms_per_step": kms, "kernels": table, "_state": state}
 
     head = measure(FP64, "fp64", 8, a.steps) if a.fp64_only else measure(MIXED, "mixed", 4, a.steps)
     prec = "fp64" if a.fp64_only else "mixed"
@@ -243,10 +285,16 @@ def main():
            "grid_point_steps_per_s": n ** 3 * head["value"], "finite": head["finite"], "graph": not a.no_graph,
            "step_roofline": head["step_roofline"], "kernel_pass_ms_per_step": head["kernel_pass_ms_per_step"],
            "kernels": head["kernels"]}
+    head_state = head.pop("_state")
     if not a.no_fp64 and not a.fp64_only:
         f64 = measure(FP64, "fp64", 8, a.fp64_steps or a.steps)
+        f64_state = f64.pop("_state")
         out["fp64"] = f64
         out["speedup_mixed_vs_fp64"] = out["value"] / f64["value"]
+        if (a.fp64_steps or a.steps) == a.steps:  # same trajectory length: mixed against FP64
+            out["accuracy_mixed_vs_fp64"] = accuracy(head_state, f64_state, a.warmup + 2 * a.steps, 5.0)
+    if not a.no_squirm:
+        out["squirmers"] = squirmer_bench(a.squirm_steps)
 
     if not a.no_serve:
         out["serving"] = serve_bench(a.serve_runs, a.serve_rate, a.serve_steps, a.serve_active)

@@ -64,6 +64,10 @@ struct Grid {
 
 __device__ __forceinline__ int kx_of(const Grid &g, int i) { return (i > g.hnx) ? i - g.nx : i; }
 __device__ __forceinline__ int ky_of(const Grid &g, int j) { return (j > g.hny) ? j - g.ny : j; }
+__device__ __forceinline__ bool kept1(int k, int trn) { return (k < 0 ? -k : k) < trn; }
+__device__ __forceinline__ int  nkept(int trn) { return 2 * trn - 1; }
+// o-th kept index of an axis of n points: 0..trn-1, then n-trn+1..n-1
+__device__ __forceinline__ int kept_index(int o, int n, int trn) { return o < trn ? o : n - nkept(trn) + o; }
 __device__ __forceinline__ bool kept_mode(const Grid &g, int kx, int ky, int kz) {
     return kz < g.trn_z && (ky < 0 ? -ky : ky) < g.trn_y && (kx < 0 ? -kx : kx) < g.trn_x;
 }

@@ -32,6 +32,18 @@ struct ColTile<float> {
     static constexpr int C = 16;  // 16 x 8 B = 128 B
 };
 
+// Column tile of the fused x passes (all components in one workgroup)
+#ifndef KH_XTILE32
+#define KH_XTILE32 16
+#endif
+#ifndef KH_XTILE64
+#define KH_XTILE64 8
+#endif
+template <typename T>
+struct XTile {
+    static constexpr int C = sizeof(T) == 4 ? KH_XTILE32 : KH_XTILE64;
+};
+
 template <typename T>
 struct ZTile;
 template <>
@@ -43,10 +55,6 @@ struct ZTile<float> {
     static constexpr int ELEM = 2048;
 };
 
-__device__ __forceinline__ bool kept1(int k, int trn) { return (k < 0 ? -k : k) < trn; }
-__device__ __forceinline__ int  nkept(int trn) { return 2 * trn - 1; }
-// o-th kept index of an axis of n points: 0..trn-1, then n-trn+1..n-1
-__device__ __forceinline__ int kept_index(int o, int n, int trn) { return o < trn ? o : n - nkept(trn) + o; }
 
 // ------------------------------------------------------------------------
 // Column passes. Y mode: outer block = x, line along y, columns kz < ncols.
@@ -157,7 +165,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_x_zeta2u_inv(
     Grid g, const cpx<T> *__restrict__ z0, const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ u0,
     cpx<T> *__restrict__ u1, cpx<T> *__restrict__ u2, double *__restrict__ ukdc, int fixmask) {
     constexpr int NT  = PassThreads<T>::value;
-    constexpr int C   = ColTile<T>::C;
+    constexpr int C   = XTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
     constexpr int NPT = (C * L + NT - 1) / NT;
     __shared__ cpx<T> s[L * BS], tw[L];
@@ -389,7 +397,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_x_adv_ns(Grid g, Comp
                                                                     cpx<T> *__restrict__ z1, double dmy0,
                                                                     double dmy1) {
     constexpr int NT  = PassThreads<T>::value;
-    constexpr int C   = ColTile<T>::C;
+    constexpr int C   = XTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
     constexpr int NPT = (C * L + NT - 1) / NT;
     __shared__ cpx<T> s[L * BS], tw[L];
@@ -452,7 +460,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_x_f_zeta(Grid g, cpx<
                                                                     cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
                                                                     double *__restrict__ ukdc) {
     constexpr int NT  = PassThreads<T>::value;
-    constexpr int C   = ColTile<T>::C;
+    constexpr int C   = XTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
     constexpr int NPT = (C * L + NT - 1) / NT;
     __shared__ cpx<T> s[L * BS], tw[L];
@@ -608,7 +616,7 @@ hipError_t cols_x(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, i
 template <typename T>
 hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const u[3], double *ukdc,
                         int fixmask, hipStream_t s) {
-    constexpr int C  = ColTile<T>::C;
+    constexpr int C  = XTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                   \
@@ -666,7 +674,7 @@ hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T
 template <typename T>
 hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,
                     hipStream_t s) {
-    constexpr int C  = ColTile<T>::C;
+    constexpr int C  = XTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                   \
@@ -682,7 +690,7 @@ hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1,
 
 template <typename T>
 hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, double *ukdc, hipStream_t s) {
-    constexpr int C  = ColTile<T>::C;
+    constexpr int C  = XTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                 \

@@ -74,8 +74,14 @@ __device__ __forceinline__ double distance0(const PParams &pp, const double *x1,
 // H, Phi (profile.h:30-50)
 __device__ __forceinline__ double prof_H(double x, double dx) { return x > 0 ? exp(-(dx / x) * (dx / x)) : 0.0; }
 __device__ __forceinline__ double prof_phi(const PParams &pp, double x) {
-    const double dmy = prof_H(pp.radius + pp.hxi - x, pp.dx);
-    return dmy / (dmy + prof_H(x - pp.radius + pp.hxi, pp.dx));
+    // inside the core H(x - a + xi/2) = 0, so phi = H/H = 1; outside the
+    // interface H(a + xi/2 - x) = 0, so phi = 0: the same values the full
+    // expression gives, without its two exp
+    const double a1 = pp.radius + pp.hxi - x, a2 = x - pp.radius + pp.hxi;
+    if (a2 <= 0.0) return 1.0;
+    if (a1 <= 0.0) return 0.0;
+    const double dmy = prof_H(a1, pp.dx);
+    return dmy / (dmy + prof_H(a2, pp.dx));
 }
 
 // Per-particle frame: Particle_cell (make_phi.h:114-139).

@@ -108,14 +108,16 @@ __global__ __launch_bounds__(PW_NT) void k_adv_ns(Grid g, const cpx<T> *__restri
                                                   const cpx<T> *__restrict__ P2, const cpx<T> *__restrict__ P3,
                                                   const cpx<T> *__restrict__ P4, cpx<T> *__restrict__ z0,
                                                   cpx<T> *__restrict__ z1, double dmy0, double dmy1) {
-    const long long idx = (long long)blockIdx.x * PW_NT + threadIdx.x;
-    if (idx >= g.nspec) return;
-    int i, j, kz;
-    spec_index(g, idx, i, j, kz);
-    if (kz >= g.nzh) return;
-    const int ikx = kx_of(g, i), iky = ky_of(g, j);
-    if (!kept_mode(g, ikx, iky, kz)) return;
-    const cpx<T> a[5] = {P0[idx], P1[idx], P2[idx], P3[idx], P4[idx]};
+    // one thread per kept mode: (kx, ky) kept rows, kz < TRN_Z
+    const long long t  = (long long)blockIdx.x * PW_NT + threadIdx.x;
+    const int       ky = nkept(g.trn_y);
+    if (t >= (long long)nkept(g.trn_x) * ky * g.trn_z) return;
+    const int       kz   = (int)(t % g.trn_z);
+    const long long rest = t / g.trn_z;
+    const int       j    = kept_index((int)(rest % ky), g.ny, g.trn_y), i = kept_index((int)(rest / ky), g.nx, g.trn_x);
+    const int       ikx = kx_of(g, i), iky = ky_of(g, j);
+    const long long idx = ((long long)i * g.ny + j) * g.nzp + kz;
+    const cpx<T>    a[5] = {P0[idx], P1[idx], P2[idx], P3[idx], P4[idx]};
     slaved_euler_mode<T>(g, ikx, iky, kz, a, dmy0, dmy1, z0[idx], z1[idx]);
 }
 
@@ -246,7 +248,8 @@ hipError_t products(long long n, const T *u0, const T *u1, const T *u2, T *const
 template <typename T>
 hipError_t adv_ns(const Grid &g, cpx<T> *const P[5], cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,
                   hipStream_t s) {
-    hipLaunchKernelGGL(k_adv_ns<T>, dim3(nblocks(g.nspec)), dim3(PW_NT), 0, s, g, P[0], P[1], P[2], P[3], P[4], z0, z1,
+    const long long nk = (long long)(2 * g.trn_x - 1) * (2 * g.trn_y - 1) * g.trn_z;
+    hipLaunchKernelGGL(k_adv_ns<T>, dim3(nblocks(nk)), dim3(PW_NT), 0, s, g, P[0], P[1], P[2], P[3], P[4], z0, z1,
                        dmy0, dmy1);
     return hipGetLastError();
 }
