@@ -60,7 +60,9 @@ def kernel_bytes(name, n, p, npts, sf):
         "adv_ns": cs * (5 + 2 + 2) * kept,
         "x_adv_ns": cs * (5 * xcol + 2 * kept + 2 * kept),
         "z_c2r": cs * 3 * ycol + sf * 3 * real,
-        "z_force": sf * 7 * real + cs * 3 * ycol,
+        # phi_sum everywhere; u_p and u* only where some particle's phi > 0
+        # (at most the P NP Sekibun points): elsewhere f = 0
+        "z_force": sf * real + 6 * sf * min(real, pt) + cs * 3 * ycol,
         "y_fwd_force": cs * (3 * ycol + 3 * xcol),
         "x_f_zeta": cs * (3 * xcol + 2 * kept + 2 * kept),
         "fill": sf * real,

@@ -248,14 +248,20 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
     const long long         oa = (2 * wl.line) * L, ob = oa + L;
     using vec                  = typename RS::vec;
     const vec zero             = {(T)0, (T)0, (T)0, (T)0};
-    vec       pa[NT4], pb[NT4];  // min(phi_sum, 1) of both rows
+    // Outside every particle phi_sum = 0, and so is u_p (it is scattered on
+    // the same points with the same phi): f = 0 there whatever u* is, so
+    // u_p and u* are read only where some phi_sum of the vector is nonzero.
+    vec  pa[NT4], pb[NT4];  // min(phi_sum, 1) of both rows
+    bool na[NT4], nb[NT4];
 #pragma unroll
     for (int t = 0; t < NT4; t++) {
         const int z = 4 * (wl.g + Geo::G * t);
         vec       a = wl.okA ? RS::ld(phis + oa + z) : zero, b = wl.okB ? RS::ld(phis + ob + z) : zero;
+        na[t]       = nonzero(a);
+        nb[t]       = nonzero(b);
         if (clear) {  // leave phi_sum zero for the next step's scatter
-            if (nonzero(a)) RS::st(phis + oa + z, zero);
-            if (nonzero(b)) RS::st(phis + ob + z, zero);
+            if (na[t]) RS::st(phis + oa + z, zero);
+            if (nb[t]) RS::st(phis + ob + z, zero);
         }
         pa[t]       = {fmin(a.x, (T)1), fmin(a.y, (T)1), fmin(a.z, (T)1), fmin(a.w, (T)1)};
         pb[t]       = {fmin(b.x, (T)1), fmin(b.y, (T)1), fmin(b.z, (T)1), fmin(b.w, (T)1)};
@@ -268,11 +274,11 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
 #pragma unroll
         for (int t = 0; t < NT4; t++) {
             const int z  = 4 * (wl.g + Geo::G * t);
-            const vec ua = wl.okA ? RS::ld(UP[d] + oa + z) : zero, ub = wl.okB ? RS::ld(UP[d] + ob + z) : zero;
-            const vec sa = wl.okA ? RS::ld(US[d] + oa + z) : zero, sb = wl.okB ? RS::ld(US[d] + ob + z) : zero;
+            const vec ua = na[t] ? RS::ld(UP[d] + oa + z) : zero, ub = nb[t] ? RS::ld(UP[d] + ob + z) : zero;
+            const vec sa = na[t] ? RS::ld(US[d] + oa + z) : zero, sb = nb[t] ? RS::ld(US[d] + ob + z) : zero;
             if (clear) {  // and u_p zero
-                if (nonzero(ua)) RS::st(UP[d] + oa + z, zero);
-                if (nonzero(ub)) RS::st(UP[d] + ob + z, zero);
+                if (na[t] && nonzero(ua)) RS::st(UP[d] + oa + z, zero);
+                if (nb[t] && nonzero(ub)) RS::st(UP[d] + ob + z, zero);
             }
             wl.buf[WavePad<T>::at(z)]     = cmk(ua.x - pa[t].x * sa.x, ub.x - pb[t].x * sb.x);
             wl.buf[WavePad<T>::at(z + 1)] = cmk(ua.y - pa[t].y * sa.y, ub.y - pb[t].y * sb.y);
@@ -296,10 +302,16 @@ bool zw_supported(int L) { return wave_fft_ok(L); }
 
 // points per lane of each pass (the advection pass holds three components
 // in registers, so it takes fewer points per lane)
+#ifndef KH_ADV_E32
+#define KH_ADV_E32 8
+#endif
+#ifndef KH_PASS_E32
+#define KH_PASS_E32 16
+#endif
 template <typename T, int L>
 struct ZwE {
-    static constexpr int pass = WaveE<T, L>::value;
-    static constexpr int adv  = WaveE<T, L, 8>::value;
+    static constexpr int pass = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 8>::value;
+    static constexpr int adv  = WaveE<T, L, sizeof(T) == 4 ? KH_ADV_E32 : 8>::value;
 };
 
 template <typename T, int L, int E>
