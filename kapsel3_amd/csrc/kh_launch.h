@@ -39,7 +39,7 @@ hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *ph
                    double coef, hipStream_t s);
 template <typename T>
 hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
-                      T *const up[3], hipStream_t s);
+                      T *const up[3], hipStream_t s, int *tick = nullptr);
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s);
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s);
 // noise = 1: the thermostat's kick follows the update in the same launch

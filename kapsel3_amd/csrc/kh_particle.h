@@ -47,6 +47,7 @@ struct PParams {
     double            gravity_on_fluid;
     int               motor;  // SW_JANUS_MOTOR
     double           *xs;     // np x 4: particle position, 1.0 for obstacles (compact copy for pair loops)
+    int               direct_dist;  // every Sekibun offset is within half a box: |r| is the minimum-image distance
 };
 
 __host__ __device__ inline int sek_pack(int i, int j, int k) { return (i + 128) | ((j + 128) << 8) | ((k + 128) << 16); }
@@ -126,7 +127,12 @@ __device__ __forceinline__ PPoint sek_point(const PParams &pp, const PCell &c, c
         q.r[d] = (double)cell[d] * pp.dx - c.residue[d];
         x[d]   = rm[d] * pp.dx;
     }
-    q.dist = distance0(pp, x, xp, x12);
+    // Distance(r_mesh*DX, xp): when no offset reaches half a box the minimum
+    // image is the offset itself, |cell*dx - residue| (equal up to rounding)
+    if (pp.direct_dist)
+        q.dist = sqrt(q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2]);
+    else
+        q.dist = distance0(pp, x, xp, x12);
     q.idx  = ((long long)rm[0] * pp.ny + rm[1]) * pp.nz + rm[2];
     return q;
 }

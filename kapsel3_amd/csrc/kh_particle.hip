@@ -159,8 +159,11 @@ template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParticle *__restrict__ P,
                                                       const T *__restrict__ phisum, const T *__restrict__ cphi,
                                                       const int *__restrict__ cidx, T *__restrict__ up0,
-                                                      T *__restrict__ up1, T *__restrict__ up2) {
+                                                      T *__restrict__ up1, T *__restrict__ up2, int *__restrict__ tick) {
     const int       n     = blockIdx.x;
+    // the step counter advances here when asked: nothing after the
+    // thermostat and before the next step's reads it
+    if (tick && n == 0 && threadIdx.x == 0) *tick += 1;
     const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
     const double    vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
     const double    om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
@@ -844,12 +847,15 @@ hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *ph
                            coef);
     return hipGetLastError();
 }
+__global__ void k_tick(int *__restrict__ d_ts);
 template <typename T>
 hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
-                      T *const up[3], hipStream_t s) {
+                      T *const up[3], hipStream_t s, int *tick) {
     if (pp.np > 0)
         hipLaunchKernelGGL(k_up_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx, up[0], up[1],
-                           up[2]);
+                           up[2], tick);
+    else if (tick)
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, tick);
     return hipGetLastError();
 }
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s) {
@@ -942,7 +948,7 @@ hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int 
     template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *, const int *, \
                                    double, hipStream_t);                                                          \
     template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, const T *, const int *,     \
-                                      T *const[3], hipStream_t);                                                  \
+                                      T *const[3], hipStream_t, int *);                                                  \
     template hipError_t slip_moments<T>(const PParams &, DevParticle *, T *const[3], hipStream_t);                 \
     template hipError_t slip_scatter<T>(const PParams &, DevParticle *, T *const[3], T *const[3], hipStream_t);    \
     template hipError_t f_slip<T>(const PParams &, DevParticle *, T *const[3], double, hipStream_t);

@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <utility>
@@ -293,10 +294,13 @@ struct Step {
         KH_LAUNCH("thermostat", thermostat(r->pp, r->dp, r->P.seed, r->P.stream_id, r->d_ts, niv, nio, r->stream));
         return KH_OK;
     }
-    static int scatter_up(kh_run *r) {
+    // tick: also advance the device step counter (the fused step's last
+    // reader of it, the thermostat, has run)
+    static int scatter_up(kh_run *r, bool tick = false) {
         KH_TRY(ensure_cache(r));
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
-        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, up3, r->stream)));
+        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, up3, r->stream,
+                                               tick ? r->d_ts : nullptr)));
         return KH_OK;
     }
     static int make_up(kh_run *r) {
@@ -371,7 +375,8 @@ struct Step {
     // one step + the device step counter
     static int step(kh_run *r) {
         KH_TRY(step_body(r));
-        KH_LAUNCH("tick", tick(r->d_ts, r->stream));
+        if (r->step_path == KH_PATH_REFERENCE_ORDER)  // the fused body ticks inside its u_p scatter
+            KH_LAUNCH("tick", tick(r->d_ts, r->stream));
         if (r->traj)
             KH_LAUNCH("record", record(r->pp, r->dp, r->d_ts, r->traj_base, r->traj_steps, r->traj, r->stream));
         return KH_OK;
@@ -425,7 +430,7 @@ struct Step {
         } else {
             KH_TRY(velocity_update(r, ab2, r->kbt > 0.));
         }
-        KH_TRY(scatter_up(r));
+        KH_TRY(scatter_up(r, true));
         // f = P(up - phi u*), u = u* + f, zeta = curl u
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
         KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear, s)));
@@ -853,6 +858,10 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     pp.gravity_on_fluid = r->gravity_on_fluid;
     pp.motor           = r->motor;
     pp.xs              = r->xs;
+    {
+        const int nh    = (int)P->a + (int)P->a_xi + 2, nmin = std::min(P->nx, std::min(P->ny, P->nz));
+        pp.direct_dist = 2 * (nh + 2) <= nmin;
+    }
     r->zclear          = !r->slip && z_force_clears(g);
 
     CTime t0  = {0, 0.0, P->dt, P->dt * 0.5, P->dt, P->dt * 0.5};  // sp_3d_ns.cxx:782

@@ -306,7 +306,7 @@ bool zw_supported(int L) { return wave_fft_ok(L); }
 #define KH_ADV_E32 8
 #endif
 #ifndef KH_PASS_E32
-#define KH_PASS_E32 16
+#define KH_PASS_E32 8
 #endif
 template <typename T, int L>
 struct ZwE {
