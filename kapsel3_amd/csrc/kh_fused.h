@@ -26,11 +26,12 @@ hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T
 template <typename T>
 hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStream_t s);
 // f = up - min(phi_sum,1) u* -> forward z (3); with `clear` (possible when
-// z_force_clears(g)) up and phi_sum are left zero for the next step
+// z_force_clears(g)) up and phi_sum are left zero for the next step, and a
+// non-null `tick` (same condition) is incremented once
 bool z_force_clears(const Grid &g);
 template <typename T>
 hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                   hipStream_t s);
+                   int *tick, hipStream_t s);
 // forward x (5 products) -> Omega + slaved-Euler update of zeta at the kept modes
 template <typename T>
 hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,

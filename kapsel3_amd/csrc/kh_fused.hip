@@ -32,16 +32,19 @@ struct ColTile<float> {
     static constexpr int C = 16;  // 16 x 8 B = 128 B
 };
 
-// Column tile of the fused x passes (all components in one workgroup)
-#ifndef KH_XTILE32
-#define KH_XTILE32 16
-#endif
-#ifndef KH_XTILE64
-#define KH_XTILE64 8
-#endif
+// Column tile and workgroup size of the fused x passes (all components in
+// one workgroup)
 template <typename T>
 struct XTile {
-    static constexpr int C = sizeof(T) == 4 ? KH_XTILE32 : KH_XTILE64;
+    static constexpr int C  = ColTile<T>::C;
+    static constexpr int NT = PassThreads<T>::value;
+};
+// zeta -> u_hat -> inverse x: FP32 runs narrower tiles on 4-wavefront
+// workgroups (more workgroups resident, a shorter last round)
+template <typename T>
+struct XZTile {
+    static constexpr int C  = sizeof(T) == 4 ? 8 : XTile<T>::C;
+    static constexpr int NT = sizeof(T) == 4 ? 256 : XTile<T>::NT;
 };
 
 template <typename T>
@@ -161,11 +164,11 @@ __device__ __forceinline__ void zeta_to_uhat(const Grid &g, int ikx, int iky, in
 // inverse x pass of each component. fixmask: FIX_CELL zeroing of uk_dc
 // (sp_3d_ns.cxx:46-52), done by the block holding (ky,kz) = (0,0).
 template <typename T, int L>
-__global__ __launch_bounds__(PassThreads<T>::value) void k_x_zeta2u_inv(
+__global__ __launch_bounds__(XZTile<T>::NT) void k_x_zeta2u_inv(
     Grid g, const cpx<T> *__restrict__ z0, const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ u0,
     cpx<T> *__restrict__ u1, cpx<T> *__restrict__ u2, double *__restrict__ ukdc, int fixmask) {
-    constexpr int NT  = PassThreads<T>::value;
-    constexpr int C   = XTile<T>::C;
+    constexpr int NT  = XZTile<T>::NT;
+    constexpr int C   = XZTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
     constexpr int NPT = (C * L + NT - 1) / NT;
     __shared__ cpx<T> s[L * BS], tw[L];
@@ -393,10 +396,10 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_force(
 // zeta (NS_solver_slavedEuler, fluid_solver.cxx:348-374): the products are
 // never written back in spectral form.
 template <typename T, int L>
-__global__ __launch_bounds__(PassThreads<T>::value) void k_x_adv_ns(Grid g, Comps<T> prod, cpx<T> *__restrict__ z0,
+__global__ __launch_bounds__(XTile<T>::NT) void k_x_adv_ns(Grid g, Comps<T> prod, cpx<T> *__restrict__ z0,
                                                                     cpx<T> *__restrict__ z1, double dmy0,
                                                                     double dmy1) {
-    constexpr int NT  = PassThreads<T>::value;
+    constexpr int NT  = XTile<T>::NT;
     constexpr int C   = XTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
     constexpr int NPT = (C * L + NT - 1) / NT;
@@ -455,11 +458,11 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_x_adv_ns(Grid g, Comp
 // U_k2zeta_k (fft_wrapper.cxx:407-476). Writes zeta at the kept modes and
 // uk_dc += f_hat(0).
 template <typename T, int L>
-__global__ __launch_bounds__(PassThreads<T>::value) void k_x_f_zeta(Grid g, cpx<T> *__restrict__ f0,
+__global__ __launch_bounds__(XTile<T>::NT) void k_x_f_zeta(Grid g, cpx<T> *__restrict__ f0,
                                                                     cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2,
                                                                     cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
                                                                     double *__restrict__ ukdc) {
-    constexpr int NT  = PassThreads<T>::value;
+    constexpr int NT  = XTile<T>::NT;
     constexpr int C   = XTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
     constexpr int NPT = (C * L + NT - 1) / NT;
@@ -616,12 +619,12 @@ hipError_t cols_x(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, i
 template <typename T>
 hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const u[3], double *ukdc,
                         int fixmask, hipStream_t s) {
-    constexpr int C  = XTile<T>::C;
+    constexpr int C  = XZTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                   \
     case n:                                                                                                          \
-        hipLaunchKernelGGL((k_x_zeta2u_inv<T, n>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, g, z0, z1, u[0], u[1],    \
+        hipLaunchKernelGGL((k_x_zeta2u_inv<T, n>), dim3(nky(g) * nt), dim3(XZTile<T>::NT), 0, s, g, z0, z1, u[0], u[1],    \
                            u[2], ukdc, fixmask);                                                                     \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -653,10 +656,10 @@ bool z_force_clears(const Grid &g) { return zw_supported(g.nz); }
 
 template <typename T>
 hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                   hipStream_t s) {
+                   int *tick, hipStream_t s) {
     const long long nrows = (long long)g.nx * g.ny;
-    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, s);
-    if (clear) return hipErrorInvalidValue;
+    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, tick, s);
+    if (clear || tick) return hipErrorInvalidValue;
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                   \
     case n: {                                                                                                        \
@@ -679,7 +682,7 @@ hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1,
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                   \
     case n:                                                                                                          \
-        hipLaunchKernelGGL((k_x_adv_ns<T, n>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, g, prod, z0, z1, \
+        hipLaunchKernelGGL((k_x_adv_ns<T, n>), dim3(nky(g) * nt), dim3(XTile<T>::NT), 0, s, g, prod, z0, z1, \
                            dmy0, dmy1);                                                                              \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -695,7 +698,7 @@ hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, d
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                 \
     case n:                                                                                                        \
-        hipLaunchKernelGGL((k_x_f_zeta<T, n>), dim3(nky(g) * nt), dim3(PassThreads<T>::value), 0, s, g, f[0], f[1], f[2], z0, z1, \
+        hipLaunchKernelGGL((k_x_f_zeta<T, n>), dim3(nky(g) * nt), dim3(XTile<T>::NT), 0, s, g, f[0], f[1], f[2], z0, z1, \
                            ukdc);                                                                                  \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -726,7 +729,7 @@ hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s) {
     template hipError_t x_zeta2u_inv<T>(const Grid &, const cpx<T> *, const cpx<T> *, cpx<T> *const[3], double *,   \
                                         int, hipStream_t);                                                          \
     template hipError_t z_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], hipStream_t);                   \
-    template hipError_t z_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int,                 \
+    template hipError_t z_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,          \
                                    hipStream_t);                                                                   \
     template hipError_t x_f_zeta<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, hipStream_t);    \
     template hipError_t x_adv_ns<T>(const Grid &, const Comps<T> &, cpx<T> *, cpx<T> *, double, double, hipStream_t); \

@@ -39,7 +39,7 @@ hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *ph
                    double coef, hipStream_t s);
 template <typename T>
 hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
-                      T *const up[3], hipStream_t s, int *tick = nullptr);
+                      T *const up[3], hipStream_t s);
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s);
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s);
 // noise = 1: the thermostat's kick follows the update in the same launch
@@ -51,6 +51,12 @@ hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t
 hipError_t tick(int *d_ts, hipStream_t s);
 hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int base, int nslots, double *out,
                   hipStream_t s);
+// F_H gather, LJ + gravity, velocity (+ thermostat kick), u_p scatter in one
+// launch (the fused step after the phi scatter)
+template <typename T>
+hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi,
+                         const int *cidx, double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream,
+                         const int *d_ts, double niv, double nio, T *const up[3], hipStream_t s);
 // squirmer slip iteration (sp_3d_ns.cxx:121-161)
 template <typename T>
 hipError_t slip_moments(const PParams &pp, DevParticle *P, T *const u[3], hipStream_t s);

@@ -108,11 +108,10 @@ __device__ __forceinline__ void sek_r(const PParams &pp, const PCell &c, int cod
 // Calc_f_hydro_correct_precision (md_force.cxx:383-506), non-rigid branch,
 // over the cached phi and grid index of each Sekibun point.
 template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__restrict__ P, const T *__restrict__ u0,
-                                                   const T *__restrict__ u1, const T *__restrict__ u2,
-                                                   const T *__restrict__ phisum, const T *__restrict__ cphi,
-                                                   const int *__restrict__ cidx, double coef) {
-    const int       n     = blockIdx.x;
+__device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__restrict__ P, int n,
+                                              const T *__restrict__ u0, const T *__restrict__ u1,
+                                              const T *__restrict__ u2, const T *__restrict__ phisum,
+                                              const T *__restrict__ cphi, const int *__restrict__ cidx, double coef) {
     const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
     const double    vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
     const double    om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
@@ -153,20 +152,20 @@ __global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__re
             }
     }
 }
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__restrict__ P, const T *__restrict__ u0,
+                                                   const T *__restrict__ u1, const T *__restrict__ u2,
+                                                   const T *__restrict__ phisum, const T *__restrict__ cphi,
+                                                   const int *__restrict__ cidx, double coef) {
+    f_hydro_block<T>(pp, P, blockIdx.x, u0, u1, u2, phisum, cphi, cidx, coef);
+}
 
 // Make_u_particle_sum scatter (make_phi.cxx:313-354) over the cache
 template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParticle *__restrict__ P,
-                                                      const T *__restrict__ phisum, const T *__restrict__ cphi,
-                                                      const int *__restrict__ cidx, T *__restrict__ up0,
-                                                      T *__restrict__ up1, T *__restrict__ up2, int *__restrict__ tick) {
-    const int       n     = blockIdx.x;
-    // the step counter advances here when asked: nothing after the
-    // thermostat and before the next step's reads it
-    if (tick && n == 0 && threadIdx.x == 0) *tick += 1;
-    const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
-    const double    vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
-    const double    om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
+__device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const double *xp, const double *vp,
+                                                 const double *om, const T *__restrict__ phisum,
+                                                 const T *__restrict__ cphi, const int *__restrict__ cidx,
+                                                 T *__restrict__ up0, T *__restrict__ up1, T *__restrict__ up2) {
     const PCell     c     = particle_cell(pp, xp);
     const long long cb    = (long long)n * pp.np_domain;
     for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
@@ -194,6 +193,17 @@ __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParti
             unsafeAtomicAdd(&up2[idx], e);
         }
     }
+}
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParticle *__restrict__ P,
+                                                      const T *__restrict__ phisum, const T *__restrict__ cphi,
+                                                      const int *__restrict__ cidx, T *__restrict__ up0,
+                                                      T *__restrict__ up1, T *__restrict__ up2) {
+    const int    n     = blockIdx.x;
+    const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
+    const double vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
+    const double om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
+    up_scatter_block<T>(pp, n, xp, vp, om, phisum, cphi, cidx, up0, up1, up2);
 }
 
 // ---------------------------------------------------------- quaternions
@@ -301,8 +311,7 @@ __device__ __forceinline__ int lj_mult(const PParams &pp, const double *xi, cons
 // (md_force.cxx:306-324). One workgroup per particle i: its lanes split the
 // partners j, each pair giving F_i += f(r_ij) (x_i - x_j), the same value the
 // reference's +/- updates add; the lane sums are reduced in FP64.
-__global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__restrict__ P) {
-    const int    i      = blockIdx.x;
+__device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__restrict__ P, int i) {
     const double xi[3]  = {P[i].x[0], P[i].x[1], P[i].x[2]};
     const int    si     = P[i].spec;
     const bool   obst_i = pp.species[si].propulsion == 2;
@@ -330,6 +339,9 @@ __global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__res
         if (pp.g != 0.0) P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
     }
 }
+__global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__restrict__ P) {
+    forces_block(pp, P, blockIdx.x);
+}
 
 // MD_solver_velocity_Euler (particle_solver.cxx:103-142) / _AB2_hydro
 // (239-281), self_propulsion (88-101) and reset_Forces (11-37).
@@ -342,9 +354,14 @@ struct Noise {  // Add_random_force_thermostat arguments (fluct.cxx:10-100)
 };
 __device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise &nz);
 
+__device__ void velocity_one(const PParams &pp, DevParticle *__restrict__ P, int n, int ab2, double h,
+                             const Noise &nz);
 __global__ void k_velocity(PParams pp, DevParticle *__restrict__ P, int ab2, double h, Noise nz) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= pp.np) return;
+    if (n < pp.np) velocity_one(pp, P, n, ab2, h, nz);
+}
+__device__ void velocity_one(const PParams &pp, DevParticle *__restrict__ P, int n, int ab2, double h,
+                             const Noise &nz) {
     DevParticle      &p = P[n];
     const DevSpecies &s = pp.species[p.spec];
     const double      dmy = h * s.imass, dmy_rot = h * s.imoi;
@@ -439,6 +456,36 @@ __device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise 
         p.v[d] += g[d] * niv * s.imass;
         if (pp.rotation) p.om[d] += g[d + 3] * nio * s.imoi;
     }
+}
+
+// The particle part of the fused step after the phi scatter, one workgroup
+// per particle: Calc_f_hydro_correct_precision, Force (LJ + gravity), the
+// velocity update with the thermostat's kick, Make_u_particle_sum. Each
+// stage reads only this particle's state and inputs already complete when
+// the launch starts (u*, phi_sum, every particle's new position), so the
+// four reference loops need no grid-wide barrier between them.
+template <typename T>
+__global__ __launch_bounds__(PT_NT) void k_particle_step(PParams pp, DevParticle *__restrict__ P,
+                                                         const T *__restrict__ u0, const T *__restrict__ u1,
+                                                         const T *__restrict__ u2, const T *__restrict__ phisum,
+                                                         const T *__restrict__ cphi, const int *__restrict__ cidx,
+                                                         double coef, int ab2, double h, Noise nz,
+                                                         T *__restrict__ up0, T *__restrict__ up1,
+                                                         T *__restrict__ up2) {
+    const int n = blockIdx.x;
+    f_hydro_block<T>(pp, P, n, u0, u1, u2, phisum, cphi, cidx, coef);
+    forces_block(pp, P, n);
+    __shared__ double st[9];
+    if (threadIdx.x == 0) {
+        velocity_one(pp, P, n, ab2, h, nz);
+        for (int d = 0; d < 3; d++) {
+            st[d]     = P[n].x[d];
+            st[3 + d] = P[n].v[d];
+            st[6 + d] = P[n].om[d];
+        }
+    }
+    __syncthreads();
+    up_scatter_block<T>(pp, n, st, st + 3, st + 6, phisum, cphi, cidx, up0, up1, up2);
 }
 
 // ------------------------------------------------------------ squirmers
@@ -847,15 +894,12 @@ hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *ph
                            coef);
     return hipGetLastError();
 }
-__global__ void k_tick(int *__restrict__ d_ts);
 template <typename T>
 hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
-                      T *const up[3], hipStream_t s, int *tick) {
+                      T *const up[3], hipStream_t s) {
     if (pp.np > 0)
         hipLaunchKernelGGL(k_up_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx, up[0], up[1],
-                           up[2], tick);
-    else if (tick)
-        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, tick);
+                           up[2]);
     return hipGetLastError();
 }
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s) {
@@ -888,6 +932,16 @@ hipError_t tick(int *d_ts, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi,
+                         const int *cidx, double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream,
+                         const int *d_ts, double niv, double nio, T *const up[3], hipStream_t s) {
+    const Noise nz{noise, seed, stream, d_ts, niv, nio};
+    if (pp.np > 0)
+        hipLaunchKernelGGL(k_particle_step<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, cphi,
+                           cidx, coef, ab2, h, nz, up[0], up[1], up[2]);
+    return hipGetLastError();
+}
 template <typename T>
 hipError_t slip_moments(const PParams &pp, DevParticle *P, T *const u[3], hipStream_t s) {
     if (pp.np > 0) hipLaunchKernelGGL(k_slip_moments<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2]);
@@ -948,8 +1002,11 @@ hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int 
     template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *, const int *, \
                                    double, hipStream_t);                                                          \
     template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, const T *, const int *,     \
-                                      T *const[3], hipStream_t, int *);                                                  \
+                                      T *const[3], hipStream_t);                                                  \
     template hipError_t slip_moments<T>(const PParams &, DevParticle *, T *const[3], hipStream_t);                 \
+    template hipError_t particle_step<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *,         \
+                                         const int *, double, int, double, int, uint64_t, uint32_t, const int *,   \
+                                         double, double, T *const[3], hipStream_t);                               \
     template hipError_t slip_scatter<T>(const PParams &, DevParticle *, T *const[3], T *const[3], hipStream_t);    \
     template hipError_t f_slip<T>(const PParams &, DevParticle *, T *const[3], double, hipStream_t);
 KH_INST(double)

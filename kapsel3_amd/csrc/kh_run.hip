@@ -294,13 +294,10 @@ struct Step {
         KH_LAUNCH("thermostat", thermostat(r->pp, r->dp, r->P.seed, r->P.stream_id, r->d_ts, niv, nio, r->stream));
         return KH_OK;
     }
-    // tick: also advance the device step counter (the fused step's last
-    // reader of it, the thermostat, has run)
-    static int scatter_up(kh_run *r, bool tick = false) {
+    static int scatter_up(kh_run *r) {
         KH_TRY(ensure_cache(r));
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
-        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, up3, r->stream,
-                                               tick ? r->d_ts : nullptr)));
+        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, up3, r->stream)));
         return KH_OK;
     }
     static int make_up(kh_run *r) {
@@ -375,7 +372,7 @@ struct Step {
     // one step + the device step counter
     static int step(kh_run *r) {
         KH_TRY(step_body(r));
-        if (r->step_path == KH_PATH_REFERENCE_ORDER)  // the fused body ticks inside its u_p scatter
+        if (r->step_path == KH_PATH_REFERENCE_ORDER)  // the fused body ticks inside z_force (or right after it)
             KH_LAUNCH("tick", tick(r->d_ts, r->stream));
         if (r->traj)
             KH_LAUNCH("record", record(r->pp, r->dp, r->d_ts, r->traj_base, r->traj_steps, r->traj, r->stream));
@@ -422,18 +419,25 @@ struct Step {
         }
         r->pclean = false;
         KH_TRY(move_scatter_phi(r, ab2));
-        KH_TRY(calc_f_hydro(r));
         if (r->slip) {
+            KH_TRY(calc_f_hydro(r));
             KH_TRY(slip_iteration(r, ab2, true));
             for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
             if (r->kbt > 0.) KH_TRY(thermo(r));
-        } else {
-            KH_TRY(velocity_update(r, ab2, r->kbt > 0.));
+            KH_TRY(scatter_up(r));
+        } else {  // F_H, LJ + gravity, velocity (+ noise), u_p scatter: one launch
+            T     *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *up[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+            double niv = 0.0, nio = 0.0;
+            if (r->kbt > 0.) noise_amp(r, niv, nio);
+            KH_LAUNCH("particle_step",
+                      (particle_step<T>(r->pp, r->dp, u, PS(r), CP(r), r->cidx, (-r->dx3 * r->rho) / r->jikan.dt_fluid,
+                                        ab2, ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->kbt > 0. ? 1 : 0, r->P.seed,
+                                        r->P.stream_id, r->d_ts, niv, nio, up, r->stream)));
         }
-        KH_TRY(scatter_up(r, true));
         // f = P(up - phi u*), u = u* + f, zeta = curl u
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
-        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear, s)));
+        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear, r->zclear ? r->d_ts : nullptr, s)));
+        if (!r->zclear) KH_LAUNCH("tick", tick(r->d_ts, s));
         r->pclean = r->zclear;
         KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
         KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));

@@ -19,9 +19,9 @@ hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, 
 template <typename T>
 hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T scale, hipStream_t s);
 // f = up - min(phi_sum,1) u* -> forward z (3), modes kz < TRN_Z; with
-// `clear` up and phi_sum are left zero
+// `clear` up and phi_sum are left zero; a non-null `tick` is incremented once
 template <typename T>
 hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                    hipStream_t s);
+                    int *tick, hipStream_t s);
 
 }  // namespace kh

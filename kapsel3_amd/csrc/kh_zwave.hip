@@ -234,7 +234,8 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
                                                   T *__restrict__ up2, T *__restrict__ phis,
                                                   const T *__restrict__ us0, const T *__restrict__ us1,
                                                   const T *__restrict__ us2, cpx<T> *__restrict__ f0,
-                                                  cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2, int clear) {
+                                                  cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2, int clear,
+                                                  int *__restrict__ tick) {
     using Geo = WaveGeom<T, L, E>;
     using Ff  = WaveFft<T, L, E, true, true>;
     using RS  = RowStage<T, L>;
@@ -243,6 +244,7 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
+    if (tick && blockIdx.x == 0 && threadIdx.x == 0) *tick += 1;  // the step counter, after every reader
     const long long         nrows = (long long)g.nx * g.ny;
     const WaveLane<T, L, E> wl(strips, nrows, g.nzp);
     const long long         oa = (2 * wl.line) * L, ob = oa + L;
@@ -364,13 +366,13 @@ hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T sca
 
 template <typename T>
 hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                    hipStream_t s) {
+                    int *tick, hipStream_t s) {
     const long long nrows = (long long)g.nx * g.ny;
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
         hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, g, up[0], up[1], up[2], \
-                           phis, us[0], us[1], us[2], f[0], f[1], f[2], clear);                                     \
+                           phis, us[0], us[1], us[2], f[0], f[1], f[2], clear, tick);                               \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)
 #undef KH_CASE
@@ -382,7 +384,7 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
     template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t);                    \
     template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t);            \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
-    template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int,               \
+    template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,        \
                                     hipStream_t);
 KH_INST(double)
 KH_INST(float)
