@@ -66,10 +66,14 @@ def kernel_bytes(name, n, p, npts, sf):
         "y_fwd_force": cs * (3 * ycol + 3 * xcol),
         "x_f_zeta": cs * (3 * xcol + 2 * kept + 2 * kept),
         "fill": sf * real,
-        # particle kernels, per Sekibun point: phi cache (sf) + index (4 B)
-        "phi_scatter": pt * (sf + 4 + 2 * sf),          # cache write, phi_sum read-modify-write
-        "f_hydro": pt * (sf + 4 + sf + 3 * sf),          # cache, phi_sum, u
-        "up_scatter": pt * (sf + 4 + sf + 6 * sf),       # cache, phi_sum, u_p read-modify-write
+        # particle kernels, per Sekibun point of every particle, the accesses
+        # of the reference's loops (the kernels' own point cache is not
+        # counted): phi_sum read-modify-write (2); force gather, u x 3 and
+        # phi_sum (4); u_p scatter, phi_sum and 3 read-modify-writes (7)
+        "phi_scatter": 2 * sf * pt,
+        "f_hydro": 4 * sf * pt,
+        "up_scatter": 7 * sf * pt,
+        "particle_step": 11 * sf * pt,
     }
     return table.get(name)
 

@@ -172,6 +172,36 @@ class Run:
         a = np.ascontiguousarray(a, dtype=np.float64)
         _check(lib().kh_set_field(self.h, FIELD[name], a.ctypes.data))
 
+    # ---- restart
+    def save_state(self, path):
+        """Write the restart state to an .npz file: the data Save_Restart_udf
+        saves (resume.cxx:10-74: zeta, uk_dc, every Particle member including
+        the AB2 history, jikan) plus the grid and precision it belongs to.
+        Thermal noise is keyed on (seed, stream_id, ts, particle), so a run
+        resumed from this state draws the same stream."""
+        z0, z1, uk = self.get_fluid()
+        t = self.get_time()
+        c = self.params.c
+        np.savez(path, zeta0=z0, zeta1=z1, uk_dc=uk, particles=self.get_particles(),
+                 ts=np.int64(t.ts), time=np.array([t.time, t.dt_fluid, t.hdt_fluid, t.dt_md, t.hdt_md]),
+                 grid=np.array([c.nx, c.ny, c.nz]), precision=np.int64(c.precision))
+
+    def load_state(self, path):
+        """Restore a state written by save_state (Force_restore_parameters +
+        the particle/fluid reads, resume.cxx:507-587, 758-923). The grid,
+        particle count and precision must match this run."""
+        with np.load(path, allow_pickle=False) as f:
+            c = self.params.c
+            if tuple(f["grid"]) != (c.nx, c.ny, c.nz) or len(f["particles"]) != self.np:
+                raise KhError(f"restart state is for grid {tuple(f['grid'])} with {len(f['particles'])} "
+                              f"particles, this run has {(c.nx, c.ny, c.nz)} with {self.np}")
+            if int(f["precision"]) != c.precision:
+                raise KhError("restart state was written by a run of another precision")
+            self.set_particles(f["particles"])
+            self.set_fluid(f["zeta0"], f["zeta1"], f["uk_dc"])
+            tm = f["time"]
+            self.set_time(CTime(int(f["ts"]), *(float(v) for v in tm)))
+
     # ---- the step
     def init_fluid(self):
         _check(lib().kh_init_fluid(self.h))

@@ -28,17 +28,17 @@ hipError_t add3(long long n, T *const u[3], T *const f[3], hipStream_t s);
 template <typename T>
 hipError_t fill(long long n, T *p, T v, hipStream_t s);
 
-// kh_particle.hip (cphi/cidx: per-particle cache of phi and grid index at
-// every Sekibun point, np * np_domain entries)
+// kh_particle.hip (PCache: per-particle cache of the Sekibun points with
+// nonzero phi, filled by phi_scatter)
 // move = 1: the particle is first advanced by the position integrator
 template <typename T>
-hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter, hipStream_t s,
+hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, const PCache<T> &pc, int scatter, hipStream_t s,
                        int move = 0, int ab2 = 0, double dt = 0.0, double hdt = 0.0);
 template <typename T>
-hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi, const int *cidx,
+hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
                    double coef, hipStream_t s);
 template <typename T>
-hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
+hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const PCache<T> &pc,
                       T *const up[3], hipStream_t s);
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s);
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s);
@@ -54,9 +54,9 @@ hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int 
 // F_H gather, LJ + gravity, velocity (+ thermostat kick), u_p scatter in one
 // launch (the fused step after the phi scatter)
 template <typename T>
-hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi,
-                         const int *cidx, double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream,
-                         const int *d_ts, double niv, double nio, T *const up[3], hipStream_t s);
+hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
+                         double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream, const int *d_ts,
+                         double niv, double nio, T *const up[3], hipStream_t s);
 // squirmer slip iteration (sp_3d_ns.cxx:121-161)
 template <typename T>
 hipError_t slip_moments(const PParams &pp, DevParticle *P, T *const u[3], hipStream_t s);

@@ -50,6 +50,18 @@ struct PParams {
     int               direct_dist;  // every Sekibun offset is within half a box: |r| is the minimum-image distance
 };
 
+// Per-particle cache of the Sekibun points where phi is nonzero, filled by
+// the phi scatter: entries [n*np_domain, n*np_domain + cnt[n]) hold, for
+// particle n in Sekibun order, phi (field precision T), the grid index and the
+// packed offset. The force gather and the u_p scatter then visit only these.
+template <typename T>
+struct PCache {
+    T   *phi;
+    int *idx;
+    int *code;
+    int *cnt;  // np entries
+};
+
 __host__ __device__ inline int sek_pack(int i, int j, int k) { return (i + 128) | ((j + 128) << 8) | ((k + 128) << 16); }
 __device__ __forceinline__ void sek_unpack(int c, int *o) {
     o[0] = (c & 255) - 128;
@@ -85,6 +97,22 @@ __device__ __forceinline__ double prof_phi(const PParams &pp, double x) {
     return dmy / (dmy + prof_H(a2, pp.dx));
 }
 
+// Phi in the field precision T: FP64 runs evaluate it in FP64; mixed runs in
+// FP32 (the value is stored, scattered and weighted in FP32 anyway).
+template <typename T>
+__device__ __forceinline__ T prof_phi_t(const PParams &pp, T x) {
+    if constexpr (sizeof(T) == 8) {
+        return prof_phi(pp, x);
+    } else {
+        const float a1 = (float)(pp.radius + pp.hxi) - x, a2 = x - (float)(pp.radius - pp.hxi);
+        if (a2 <= 0.0f) return 1.0f;
+        if (a1 <= 0.0f) return 0.0f;
+        const float dx = (float)pp.dx, t1 = dx / a1, t2 = dx / a2;
+        const float h1 = __expf(-t1 * t1), h2 = __expf(-t2 * t2);
+        return h1 / (h1 + h2);
+    }
+}
+
 // Per-particle frame: Particle_cell (make_phi.h:114-139).
 struct PCell {
     int    x_int[3];
@@ -109,6 +137,8 @@ struct PPoint {
     double    r[3]; // cell*dx - residue
     double    dist; // minimum-image |r_mesh*DX - xp|
 };
+// DIST = false leaves `dist` unset (the caller measures |r| itself)
+template <bool DIST = true>
 __device__ __forceinline__ PPoint sek_point(const PParams &pp, const PCell &c, const double *xp, int code) {
     int cell[3];
     sek_unpack(code, cell);
@@ -129,10 +159,12 @@ __device__ __forceinline__ PPoint sek_point(const PParams &pp, const PCell &c, c
     }
     // Distance(r_mesh*DX, xp): when no offset reaches half a box the minimum
     // image is the offset itself, |cell*dx - residue| (equal up to rounding)
-    if (pp.direct_dist)
-        q.dist = sqrt(q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2]);
-    else
-        q.dist = distance0(pp, x, xp, x12);
+    if constexpr (DIST) {
+        if (pp.direct_dist)
+            q.dist = sqrt(q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2]);
+        else
+            q.dist = distance0(pp, x, xp, x12);
+    }
     q.idx  = ((long long)rm[0] * pp.ny + rm[1]) * pp.nz + rm[2];
     return q;
 }

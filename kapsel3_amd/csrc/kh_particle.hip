@@ -15,6 +15,18 @@ namespace kh {
 
 constexpr int PT_NT = 256;  // lanes per particle workgroup (4 wavefronts)
 
+// The fused particle step keeps, for the first PT_CAP cached points of its
+// particle, what the force gather computed (phi/max(phi_sum,1), negated
+// where phi_sum is this particle's phi alone, the grid index, the offset) in
+// LDS for the u_p scatter, which then reads no global memory for them.
+constexpr int PT_CAP = 1024;
+template <typename T>
+struct PStash {
+    T   dmy[PT_CAP];
+    int idx[PT_CAP];
+    int code[PT_CAP];
+};
+
 struct Neu {  // Neumaier-compensated sum
     double s = 0.0, c = 0.0;
     __device__ __forceinline__ void add(double x) {
@@ -52,6 +64,16 @@ __device__ __forceinline__ void block_sum(double *v) {
     }
 }
 
+template <typename T>
+__device__ __forceinline__ void angular2v_t(int rotation, const T *om, const T *r, T *v) {
+    if (rotation) {
+        v[0] = om[1] * r[2] - om[2] * r[1];
+        v[1] = om[2] * r[0] - om[0] * r[2];
+        v[2] = om[0] * r[1] - om[1] * r[0];
+    } else {
+        v[0] = v[1] = v[2] = (T)0;
+    }
+}
 __device__ __forceinline__ void angular2v(int rotation, const double *om, const double *r, double *v) {
     if (rotation) {  // Angular2v_rot_on (make_phi.h:401-405)
         v[0] = om[1] * r[2] - om[2] * r[1];
@@ -71,8 +93,8 @@ __device__ void position_one(const PParams &pp, DevParticle *P, int n, int ab2, 
 
 template <typename T, bool SCATTER, bool MOVE>
 __global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *__restrict__ P,
-                                                       T *__restrict__ phisum, T *__restrict__ cphi,
-                                                       int *__restrict__ cidx, int ab2, double dt, double hdt) {
+                                                       T *__restrict__ phisum, PCache<T> pc, int ab2, double dt,
+                                                       double hdt) {
     const int n = blockIdx.x;
     double    xp[3];
     if constexpr (MOVE) {
@@ -86,15 +108,54 @@ __global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *
     } else {
         for (int d = 0; d < 3; d++) xp[d] = P[n].x[d];
     }
-    const PCell     c     = particle_cell(pp, xp);
-    const long long cb    = (long long)n * pp.np_domain;
-    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
-        const PPoint q  = sek_point(pp, c, xp, pp.sek[m]);
-        const double ph = prof_phi(pp, q.dist);
-        cphi[cb + m]    = (T)ph;
-        cidx[cb + m]    = (int)q.idx;
-        if (SCATTER && ph != 0.0) unsafeAtomicAdd(&phisum[q.idx], (T)ph);
+    const PCell     c  = particle_cell(pp, xp);
+    const long long cb = (long long)n * pp.np_domain;
+    // the nonzero points are packed in Sekibun order: per round of PT_NT
+    // points, a wave's ballot gives each lane its rank among the wave's
+    // nonzero points and the per-wave counts (double-buffered in LDS) the
+    // wave's base
+    __shared__ int wcnt[2][PT_NT / 64];
+    const int      lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int            off  = 0;
+    for (int m0 = 0, par = 0; m0 < pp.np_domain; m0 += PT_NT, par ^= 1) {
+        const int m    = m0 + threadIdx.x;
+        T         ph   = (T)0;
+        long long idx  = 0;
+        int       code = 0;
+        if (m < pp.np_domain) {
+            code = pp.sek[m];
+            PPoint q;
+            if (sizeof(T) == 4 && pp.direct_dist) {  // FP32 distance of the offset
+                q              = sek_point<false>(pp, c, xp, code);
+                const float r0 = (float)q.r[0], r1 = (float)q.r[1], r2 = (float)q.r[2];
+                ph             = prof_phi_t<T>(pp, (T)sqrtf(r0 * r0 + r1 * r1 + r2 * r2));
+            } else {
+                q  = sek_point(pp, c, xp, code);
+                ph = prof_phi_t<T>(pp, (T)q.dist);
+            }
+            idx = q.idx;
+            if (SCATTER && ph != (T)0) unsafeAtomicAdd(&phisum[idx], ph);
+        }
+        const bool               keep = ph != (T)0;
+        const unsigned long long b    = __ballot(keep);
+        if (lane == 0) wcnt[par][wave] = __popcll(b);
+        __syncthreads();
+        int base = off, tot = 0;
+#pragma unroll
+        for (int w = 0; w < PT_NT / 64; w++) {
+            const int k = wcnt[par][w];
+            base += w < wave ? k : 0;
+            tot += k;
+        }
+        if (keep) {
+            const long long j = cb + base + __popcll(b & __lanemask_lt());
+            pc.phi[j]         = ph;
+            pc.idx[j]         = (int)idx;
+            pc.code[j]        = code;
+        }
+        off += tot;
     }
+    if (threadIdx.x == 0) pc.cnt[n] = off;
 }
 
 // r = cell*dx - residue (Relative_coord, make_phi.h:154-170)
@@ -111,31 +172,42 @@ template <typename T>
 __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__restrict__ P, int n,
                                               const T *__restrict__ u0, const T *__restrict__ u1,
                                               const T *__restrict__ u2, const T *__restrict__ phisum,
-                                              const T *__restrict__ cphi, const int *__restrict__ cidx, double coef) {
+                                              const PCache<T> &pc, double coef, PStash<T> *stash = nullptr) {
     const double    xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
     const double    vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
     const double    om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
     const PCell     c     = particle_cell(pp, xp);
     const long long cb    = (long long)n * pp.np_domain;
     Neu             acc[6];
-    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
-        const double ph = (double)cphi[cb + m];
-        if (ph == 0.0) continue;
-        const int idx = cidx[cb + m];
-        double    r[3], vr[3];
-        sek_r(pp, c, pp.sek[m], r);
-        angular2v(pp.rotation, om, r, vr);
-        const double dmy_phi = ph / fmax((double)phisum[idx], 1.0);
-        double       fp[3];
-        fp[0] = ((vp[0] + vr[0]) - (double)u0[idx]) * dmy_phi;
-        fp[1] = ((vp[1] + vr[1]) - (double)u1[idx]) * dmy_phi;
-        fp[2] = ((vp[2] + vr[2]) - (double)u2[idx]) * dmy_phi;
-        acc[0].add(fp[0]);
-        acc[1].add(fp[1]);
-        acc[2].add(fp[2]);
-        acc[3].add(r[1] * fp[2] - r[2] * fp[1]);
-        acc[4].add(r[2] * fp[0] - r[0] * fp[2]);
-        acc[5].add(r[0] * fp[1] - r[1] * fp[0]);
+    // per-point terms in the field precision T, sums in FP64
+    const T vt[3] = {(T)vp[0], (T)vp[1], (T)vp[2]}, wt[3] = {(T)om[0], (T)om[1], (T)om[2]};
+    const int cnt = pc.cnt[n];
+    for (int m = threadIdx.x; m < cnt; m += PT_NT) {
+        const T   ph   = pc.phi[cb + m];
+        const int idx  = pc.idx[cb + m];
+        const int code = pc.code[cb + m];
+        double    rd[3];
+        sek_r(pp, c, code, rd);
+        const T r[3] = {(T)rd[0], (T)rd[1], (T)rd[2]};
+        T       vr[3];
+        angular2v_t<T>(pp.rotation, wt, r, vr);
+        const T ps      = phisum[idx];
+        const T dmy_phi = ph / (ps > (T)1 ? ps : (T)1);
+        if (stash && m < PT_CAP) {
+            stash->dmy[m]  = ps == ph ? -dmy_phi : dmy_phi;
+            stash->idx[m]  = idx;
+            stash->code[m] = code;
+        }
+        T       fp[3];
+        fp[0] = ((vt[0] + vr[0]) - u0[idx]) * dmy_phi;
+        fp[1] = ((vt[1] + vr[1]) - u1[idx]) * dmy_phi;
+        fp[2] = ((vt[2] + vr[2]) - u2[idx]) * dmy_phi;
+        acc[0].add((double)fp[0]);
+        acc[1].add((double)fp[1]);
+        acc[2].add((double)fp[2]);
+        acc[3].add((double)(r[1] * fp[2] - r[2] * fp[1]));
+        acc[4].add((double)(r[2] * fp[0] - r[0] * fp[2]));
+        acc[5].add((double)(r[0] * fp[1] - r[1] * fp[0]));
     }
     double v[6];
     for (int k = 0; k < 6; k++) v[k] = acc[k].val();
@@ -155,31 +227,45 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
 template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__restrict__ P, const T *__restrict__ u0,
                                                    const T *__restrict__ u1, const T *__restrict__ u2,
-                                                   const T *__restrict__ phisum, const T *__restrict__ cphi,
-                                                   const int *__restrict__ cidx, double coef) {
-    f_hydro_block<T>(pp, P, blockIdx.x, u0, u1, u2, phisum, cphi, cidx, coef);
+                                                   const T *__restrict__ phisum, PCache<T> pc, double coef) {
+    f_hydro_block<T>(pp, P, blockIdx.x, u0, u1, u2, phisum, pc, coef);
 }
 
 // Make_u_particle_sum scatter (make_phi.cxx:313-354) over the cache
 template <typename T>
 __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const double *xp, const double *vp,
                                                  const double *om, const T *__restrict__ phisum,
-                                                 const T *__restrict__ cphi, const int *__restrict__ cidx,
-                                                 T *__restrict__ up0, T *__restrict__ up1, T *__restrict__ up2) {
+                                                 const PCache<T> &pc, T *__restrict__ up0, T *__restrict__ up1,
+                                                 T *__restrict__ up2, const PStash<T> *stash = nullptr) {
     const PCell     c     = particle_cell(pp, xp);
     const long long cb    = (long long)n * pp.np_domain;
-    for (int m = threadIdx.x; m < pp.np_domain; m += PT_NT) {
-        const double ph = (double)cphi[cb + m];
-        if (ph == 0.0) continue;
-        const int    idx     = cidx[cb + m];
-        const T      ps      = phisum[idx];
-        const double dmy_phi = ph / fmax((double)ps, 1.0);
-        double       r[3], vr[3];
-        sek_r(pp, c, pp.sek[m], r);
-        angular2v(pp.rotation, om, r, vr);
-        const T a = (T)((vp[0] + vr[0]) * dmy_phi), b = (T)((vp[1] + vr[1]) * dmy_phi),
-                e = (T)((vp[2] + vr[2]) * dmy_phi);
-        if (ps == cphi[cb + m]) {
+    const T vt[3] = {(T)vp[0], (T)vp[1], (T)vp[2]}, wt[3] = {(T)om[0], (T)om[1], (T)om[2]};
+    const int cnt = pc.cnt[n];
+    for (int m = threadIdx.x; m < cnt; m += PT_NT) {
+        int  idx, code;
+        T    dmy_phi;
+        bool alone;  // phi_sum is this particle's phi alone
+        if (stash && m < PT_CAP) {
+            const T d = stash->dmy[m];
+            idx       = stash->idx[m];
+            code      = stash->code[m];
+            alone     = d < (T)0;
+            dmy_phi   = alone ? -d : d;
+        } else {
+            const T ph = pc.phi[cb + m];
+            idx        = pc.idx[cb + m];
+            code       = pc.code[cb + m];
+            const T ps = phisum[idx];
+            dmy_phi    = ph / (ps > (T)1 ? ps : (T)1);
+            alone      = ps == ph;
+        }
+        double rd[3];
+        sek_r(pp, c, code, rd);
+        const T r[3] = {(T)rd[0], (T)rd[1], (T)rd[2]};
+        T       vr[3];
+        angular2v_t<T>(pp.rotation, wt, r, vr);
+        const T a = (vt[0] + vr[0]) * dmy_phi, b = (vt[1] + vr[1]) * dmy_phi, e = (vt[2] + vr[2]) * dmy_phi;
+        if (alone) {
             // phi_sum is this particle's phi alone: no other particle adds
             // here (a contribution below half an ulp of phi would be the only
             // exception, and it is below the rounding of the sum), so the
@@ -196,14 +282,14 @@ __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const
 }
 template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParticle *__restrict__ P,
-                                                      const T *__restrict__ phisum, const T *__restrict__ cphi,
-                                                      const int *__restrict__ cidx, T *__restrict__ up0,
-                                                      T *__restrict__ up1, T *__restrict__ up2) {
+                                                      const T *__restrict__ phisum, PCache<T> pc,
+                                                      T *__restrict__ up0, T *__restrict__ up1,
+                                                      T *__restrict__ up2) {
     const int    n     = blockIdx.x;
     const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
     const double vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
     const double om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
-    up_scatter_block<T>(pp, n, xp, vp, om, phisum, cphi, cidx, up0, up1, up2);
+    up_scatter_block<T>(pp, n, xp, vp, om, phisum, pc, up0, up1, up2);
 }
 
 // ---------------------------------------------------------- quaternions
@@ -317,11 +403,27 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
     const bool   obst_i = pp.species[si].propulsion == 2;
     double       f[3]   = {0.0, 0.0, 0.0};
     if (pp.lj_on) {
+        // pairs clearly beyond the cutoff are dropped by a division-free
+        // minimum image (exact for positions inside the box, up to the
+        // rounding of ties at half a box, hence the margin on the cutoff);
+        // the rest take the reference's Distance0
+        const double cut2 = pp.lj_cutoff * pp.lj_cutoff * (1.0 + 1e-9);
         for (int j = threadIdx.x; j < pp.np; j += PT_NT) {
             if (j == i) continue;
             if (obst_i && pp.xs[4 * j + 3] != 0.0) continue;
             const double xj[3] = {pp.xs[4 * j], pp.xs[4 * j + 1], pp.xs[4 * j + 2]};
-            double       rv[3];
+            double       d2    = 0.0;
+            bool         odd   = false;  // a position outside the box
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double hl = 0.5 * pp.L[d];
+                double       e  = xj[d] - xi[d];
+                e               = e > hl ? e - pp.L[d] : (e < -hl ? e + pp.L[d] : e);
+                odd |= fabs(e) > hl;
+                d2 += e * e;
+            }
+            if (!odd && d2 >= cut2) continue;
+            double rv[3];
             // r_ij_vec from the higher index to the lower, as the link cell takes it
             const double rij = (i > j) ? distance0(pp, xi, xj, rv) : distance0(pp, xj, xi, rv);
             if (rij < pp.lj_cutoff) {
@@ -464,16 +566,18 @@ __device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise 
 // stage reads only this particle's state and inputs already complete when
 // the launch starts (u*, phi_sum, every particle's new position), so the
 // four reference loops need no grid-wide barrier between them.
+// 8 waves per SIMD (64 VGPRs): two thousand particle workgroups are then
+// all resident at once on 256 CUs
 template <typename T>
-__global__ __launch_bounds__(PT_NT) void k_particle_step(PParams pp, DevParticle *__restrict__ P,
+__global__ __launch_bounds__(PT_NT, 8) void k_particle_step(PParams pp, DevParticle *__restrict__ P,
                                                          const T *__restrict__ u0, const T *__restrict__ u1,
                                                          const T *__restrict__ u2, const T *__restrict__ phisum,
-                                                         const T *__restrict__ cphi, const int *__restrict__ cidx,
-                                                         double coef, int ab2, double h, Noise nz,
+                                                         PCache<T> pc, double coef, int ab2, double h, Noise nz,
                                                          T *__restrict__ up0, T *__restrict__ up1,
                                                          T *__restrict__ up2) {
-    const int n = blockIdx.x;
-    f_hydro_block<T>(pp, P, n, u0, u1, u2, phisum, cphi, cidx, coef);
+    const int               n = blockIdx.x;
+    __shared__ PStash<T>    stash;
+    f_hydro_block<T>(pp, P, n, u0, u1, u2, phisum, pc, coef, &stash);
     forces_block(pp, P, n);
     __shared__ double st[9];
     if (threadIdx.x == 0) {
@@ -485,7 +589,8 @@ __global__ __launch_bounds__(PT_NT) void k_particle_step(PParams pp, DevParticle
         }
     }
     __syncthreads();
-    up_scatter_block<T>(pp, n, st, st + 3, st + 6, phisum, cphi, cidx, up0, up1, up2);
+    // each lane reads back only the stash entries it wrote
+    up_scatter_block<T>(pp, n, st, st + 3, st + 6, phisum, pc, up0, up1, up2, &stash);
 }
 
 // ------------------------------------------------------------ squirmers
@@ -871,35 +976,33 @@ __global__ __launch_bounds__(PT_NT) void k_slip_conv(PParams pp, const DevPartic
 static inline unsigned pblocks(int n) { return (unsigned)((n + 63) / 64); }
 
 template <typename T>
-hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, T *cphi, int *cidx, int scatter, hipStream_t s,
+hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, const PCache<T> &pc, int scatter, hipStream_t s,
                        int move, int ab2, double dt, double hdt) {
     if (pp.np > 0) {
         if (move)
-            hipLaunchKernelGGL((k_phi_scatter<T, true, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx,
-                               ab2, dt, hdt);
+            hipLaunchKernelGGL((k_phi_scatter<T, true, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, ab2,
+                               dt, hdt);
         else if (scatter)
-            hipLaunchKernelGGL((k_phi_scatter<T, true, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi,
-                               cidx, ab2, dt, hdt);
+            hipLaunchKernelGGL((k_phi_scatter<T, true, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, ab2,
+                               dt, hdt);
         else
-            hipLaunchKernelGGL((k_phi_scatter<T, false, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi,
-                               cidx, ab2, dt, hdt);
+            hipLaunchKernelGGL((k_phi_scatter<T, false, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc,
+                               ab2, dt, hdt);
     }
     return hipGetLastError();
 }
 template <typename T>
-hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi, const int *cidx,
+hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
                    double coef, hipStream_t s) {
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_f_hydro<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, cphi, cidx,
-                           coef);
+        hipLaunchKernelGGL(k_f_hydro<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, pc, coef);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const T *cphi, const int *cidx,
+hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const PCache<T> &pc,
                       T *const up[3], hipStream_t s) {
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_up_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, cphi, cidx, up[0], up[1],
-                           up[2]);
+        hipLaunchKernelGGL(k_up_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, up[0], up[1], up[2]);
     return hipGetLastError();
 }
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s) {
@@ -933,13 +1036,13 @@ hipError_t tick(int *d_ts, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const T *cphi,
-                         const int *cidx, double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream,
-                         const int *d_ts, double niv, double nio, T *const up[3], hipStream_t s) {
+hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
+                         double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream, const int *d_ts,
+                         double niv, double nio, T *const up[3], hipStream_t s) {
     const Noise nz{noise, seed, stream, d_ts, niv, nio};
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_particle_step<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, cphi,
-                           cidx, coef, ab2, h, nz, up[0], up[1], up[2]);
+        hipLaunchKernelGGL(k_particle_step<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, pc,
+                           coef, ab2, h, nz, up[0], up[1], up[2]);
     return hipGetLastError();
 }
 template <typename T>
@@ -996,18 +1099,18 @@ hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int 
     return hipGetLastError();
 }
 
-#define KH_INST(T)                                                                                              \
-    template hipError_t phi_scatter<T>(const PParams &, DevParticle *, T *, T *, int *, int, hipStream_t, int, int,  \
-                                       double, double);                                                           \
-    template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *, const int *, \
-                                   double, hipStream_t);                                                          \
-    template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, const T *, const int *,     \
-                                      T *const[3], hipStream_t);                                                  \
-    template hipError_t slip_moments<T>(const PParams &, DevParticle *, T *const[3], hipStream_t);                 \
-    template hipError_t particle_step<T>(const PParams &, DevParticle *, T *const[3], const T *, const T *,         \
-                                         const int *, double, int, double, int, uint64_t, uint32_t, const int *,   \
-                                         double, double, T *const[3], hipStream_t);                               \
-    template hipError_t slip_scatter<T>(const PParams &, DevParticle *, T *const[3], T *const[3], hipStream_t);    \
+#define KH_INST(T)                                                                                               \
+    template hipError_t phi_scatter<T>(const PParams &, DevParticle *, T *, const PCache<T> &, int, hipStream_t, int, \
+                                       int, double, double);                                                       \
+    template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const PCache<T> &,       \
+                                   double, hipStream_t);                                                           \
+    template hipError_t up_scatter<T>(const PParams &, const DevParticle *, const T *, const PCache<T> &,           \
+                                      T *const[3], hipStream_t);                                                   \
+    template hipError_t slip_moments<T>(const PParams &, DevParticle *, T *const[3], hipStream_t);                  \
+    template hipError_t particle_step<T>(const PParams &, DevParticle *, T *const[3], const T *, const PCache<T> &, \
+                                         double, int, double, int, uint64_t, uint32_t, const int *, double, double, \
+                                         T *const[3], hipStream_t);                                                \
+    template hipError_t slip_scatter<T>(const PParams &, DevParticle *, T *const[3], T *const[3], hipStream_t);     \
     template hipError_t f_slip<T>(const PParams &, DevParticle *, T *const[3], double, hipStream_t);
 KH_INST(double)
 KH_INST(float)

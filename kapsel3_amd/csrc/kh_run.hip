@@ -93,6 +93,8 @@ struct kh_run {
     int         *sek     = nullptr;
     void        *cphi    = nullptr;  // per-particle phi at each Sekibun point (T)
     int         *cidx    = nullptr;  // per-particle grid index of each Sekibun point
+    int         *ccode   = nullptr;  // per-particle packed Sekibun offset of each cached point
+    int         *ccnt    = nullptr;  // per-particle count of cached (nonzero-phi) points
     int          pcache_valid = 0;
     int         *d_ts         = nullptr;  // jikan.ts on the device (noise counter)
     double      *slipeps      = nullptr;  // slip convergence measure (1 double)
@@ -234,7 +236,7 @@ struct Step {
         return KH_OK;
     }
 
-    static T *CP(kh_run *r) { return (T *)r->cphi; }
+    static PCache<T> CP(kh_run *r) { return PCache<T>{(T *)r->cphi, r->cidx, r->ccode, r->ccnt}; }
     static int pos_update(kh_run *r, int ab2) {
         KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
         r->pcache_valid = 0;
@@ -242,20 +244,20 @@ struct Step {
     }
     // scatter phi (Make_phi_particle_sum) and fill the per-particle cache
     static int scatter_phi(kh_run *r) {
-        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, 1, r->stream)));
+        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), 1, r->stream)));
         r->pcache_valid = 1;
         return KH_OK;
     }
     // the position integrator and the phi scatter in one launch
     static int move_scatter_phi(kh_run *r, int ab2) {
-        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, 1, r->stream, 1, ab2,
+        KH_LAUNCH("phi_scatter", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), 1, r->stream, 1, ab2,
                                                  r->jikan.dt_md, r->jikan.hdt_md)));
         r->pcache_valid = 1;
         return KH_OK;
     }
     static int ensure_cache(kh_run *r) {
         if (!r->pcache_valid) {
-            KH_LAUNCH("phi_cache", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, 0, r->stream)));
+            KH_LAUNCH("phi_cache", (phi_scatter<T>(r->pp, r->dp, PS(r), CP(r), 0, r->stream)));
             r->pcache_valid = 1;
         }
         return KH_OK;
@@ -267,7 +269,7 @@ struct Step {
     static int calc_f_hydro(kh_run *r) {
         KH_TRY(ensure_cache(r));
         T *u[3] = {U(r, 0), U(r, 1), U(r, 2)};
-        KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, u, PS(r), CP(r), r->cidx,
+        KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, u, PS(r), CP(r),
                                          (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->stream)));
         return KH_OK;
     }
@@ -297,7 +299,7 @@ struct Step {
     static int scatter_up(kh_run *r) {
         KH_TRY(ensure_cache(r));
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
-        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), CP(r), r->cidx, up3, r->stream)));
+        KH_LAUNCH("up_scatter", (up_scatter<T>(r->pp, r->dp, PS(r), CP(r), up3, r->stream)));
         return KH_OK;
     }
     static int make_up(kh_run *r) {
@@ -430,7 +432,7 @@ struct Step {
             double niv = 0.0, nio = 0.0;
             if (r->kbt > 0.) noise_amp(r, niv, nio);
             KH_LAUNCH("particle_step",
-                      (particle_step<T>(r->pp, r->dp, u, PS(r), CP(r), r->cidx, (-r->dx3 * r->rho) / r->jikan.dt_fluid,
+                      (particle_step<T>(r->pp, r->dp, u, PS(r), CP(r), (-r->dx3 * r->rho) / r->jikan.dt_fluid,
                                         ab2, ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->kbt > 0. ? 1 : 0, r->P.seed,
                                         r->P.stream_id, r->d_ts, niv, nio, up, r->stream)));
         }
@@ -608,7 +610,7 @@ int free_run(kh_run *r) {
     void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
                     r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
                     r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek, r->cphi, r->cidx,
-                    r->d_ts, r->slipeps, r->slipdc, r->xs};
+                    r->ccode, r->ccnt, r->d_ts, r->slipeps, r->slipdc, r->xs};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, r->stream);
     if (r->traj) (void)hipFreeAsync(r->traj, r->stream);
@@ -662,7 +664,7 @@ size_t run_footprint(const kh_params &P, int np) {
                 if (sqrt((i * P.dx) * (i * P.dx) + (j * P.dx) * (j * P.dx) + (k * P.dx) * (k * P.dx)) < rmax) npts++;
     const size_t npp = (size_t)(np > 0 ? np : 1);
     return 7 * 2 * esz * nspec + 12 * esz * nreal + npp * (sizeof(DevParticle) + 4 * sizeof(double)) +
-           npp * npts * (esz + sizeof(int)) +
+           npp * npts * (esz + 2 * sizeof(int)) + npp * sizeof(int) +
            npts * sizeof(int) + (size_t)P.nspecies * sizeof(DevSpecies) + 1024;
 }
 
@@ -817,6 +819,8 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     const size_t ncache = (size_t)(np > 0 ? np : 1) * sek.size();
     if (st == KH_OK) st = alloc(&r->cphi, r->esz * ncache);
     if (st == KH_OK) st = alloc((void **)&r->cidx, sizeof(int) * ncache);
+    if (st == KH_OK) st = alloc((void **)&r->ccode, sizeof(int) * ncache);
+    if (st == KH_OK) st = alloc((void **)&r->ccnt, sizeof(int) * (size_t)(np > 0 ? np : 1));
     if (st != KH_OK) return bail(st);
     hipStream_t  cs  = r->stream;
     const size_t psz = sizeof(DevParticle) * (size_t)(np > 0 ? np : 1);
