@@ -170,6 +170,18 @@ int kh_set_step_path(kh_run *run, int path);
  * particle records, output.cxx:268-375) without a host copy per step. */
 int kh_set_trajectory(kh_run *run, int nsteps);
 int kh_get_trajectory(kh_run *run, double *out, int nsteps);
+/* Output_udf frames (output.cxx:268-375, written at ts % GTS == 0 before the
+ * step of that ts, sp_3d_ns.cxx:863-950) kept on the device: frames at the
+ * next nframes step counts ts >= now with ts % every == 0 (the current state
+ * when ts % every == 0 already). Frame k is 1 + KH_FRAME_WORDS * np doubles:
+ * ts, then per particle the record's members in its order: R[3], R_raw[3],
+ * v[3], q[4] (q0 scalar), omega[3], f_hydro[3], torque_hydro[3], f_r[3],
+ * torque_r[3] (0), f_slip[3], torque_slip[3]; the forces and torques are the
+ * previous step's (the *_previous members the reference writes). The record's
+ * constant E = 1.0 is not stored. nframes = 0 stops recording. */
+#define KH_FRAME_WORDS 34
+int kh_set_output(kh_run *run, int nframes, int every);
+int kh_get_output(kh_run *run, double *out, int nframes);
 /* 1 when all work enqueued on the run (kh_step is asynchronous) has
  * completed, 0 while some is pending, a negative status on a device error. */
 int kh_query(kh_run *run);

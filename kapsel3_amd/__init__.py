@@ -24,7 +24,7 @@ EXPORTS = [
     "kh_set_time", "kh_get_time", "kh_get_field", "kh_set_field", "kh_init_fluid", "kh_step", "kh_apply",
     "kh_sync", "kh_A2a_k", "kh_A_k2a", "kh_set_step_path", "kh_set_graph", "kh_timing_enable", "kh_timing_reset", "kh_timing_get",
     "kh_timing_count", "kh_timing_name", "kh_strerror", "kh_last_error", "kh_set_trajectory", "kh_get_trajectory",
-    "kh_run_bytes", "kh_query", "kh_server_create", "kh_server_submit", "kh_server_poll", "kh_server_wait",
+    "kh_set_output", "kh_get_output", "kh_run_bytes", "kh_query", "kh_server_create", "kh_server_submit", "kh_server_poll", "kh_server_wait",
     "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy",
 ]
 
@@ -41,6 +41,15 @@ class ServerStats(C.Structure):
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
                 ("wall_s", C.c_double), ("runs_per_s", C.c_double), ("steps_per_s", C.c_double),
                 ("grid_point_steps_per_s", C.c_double)]
+
+
+# One particle of an Output_udf frame (output.cxx:268-375), members in the
+# record's order; forces and torques are the previous step's.
+FRAME_PARTICLE_DTYPE = np.dtype([
+    ("R", "f8", 3), ("R_raw", "f8", 3), ("v", "f8", 3), ("q", "f8", 4), ("omega", "f8", 3),
+    ("f_hydro", "f8", 3), ("torque_hydro", "f8", 3), ("f_r", "f8", 3), ("torque_r", "f8", 3),
+    ("f_slip", "f8", 3), ("torque_slip", "f8", 3)])
+assert FRAME_PARTICLE_DTYPE.itemsize == 34 * 8
 
 
 class KhError(RuntimeError):
@@ -84,6 +93,8 @@ def lib():
         L.kh_last_error.restype = C.c_char_p
         L.kh_set_trajectory.argtypes = [vp, C.c_int]
         L.kh_get_trajectory.argtypes = [vp, vp, C.c_int]
+        L.kh_set_output.argtypes = [vp, C.c_int, C.c_int]
+        L.kh_get_output.argtypes = [vp, vp, C.c_int]
         L.kh_run_bytes.argtypes = [C.POINTER(KhParams), C.c_int]
         L.kh_run_bytes.restype = C.c_longlong
         L.kh_query.argtypes = [vp]
@@ -231,6 +242,24 @@ class Run:
         out = np.zeros((n, self.np, 6))
         _check(lib().kh_get_trajectory(self.h, out.ctypes.data, n))
         return out
+
+    def set_output(self, nframes, every):
+        """Record Output_udf frames on the device at the next nframes step
+        counts ts with ts % every == 0 (including the current state when it
+        qualifies); 0 stops."""
+        _check(lib().kh_set_output(self.h, int(nframes), int(every)))
+        self._frames = int(nframes)
+
+    def get_output(self, nframes=None):
+        """(t, particles): t is (nframes,) step counts, particles an
+        (nframes, np) array of FRAME_PARTICLE_DTYPE records."""
+        n = getattr(self, "_frames", 0) if nframes is None else int(nframes)
+        words = 1 + 34 * self.np
+        raw = np.zeros((n, words))
+        _check(lib().kh_get_output(self.h, raw.ctypes.data, n))
+        t = raw[:, 0].astype(np.int64)
+        parts = np.ascontiguousarray(raw[:, 1:]).view(FRAME_PARTICLE_DTYPE).reshape(n, self.np)
+        return t, parts
 
     def idle(self):
         """True when every enqueued step has completed."""

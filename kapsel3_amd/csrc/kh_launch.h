@@ -49,6 +49,8 @@ hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStr
 hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s);
 hipError_t tick(int *d_ts, hipStream_t s);
+hipError_t frame(const PParams &pp, const DevParticle *P, const int *d_ts, int ts0, int every, int nframes, double *out,
+                 hipStream_t s);
 hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int base, int nslots, double *out,
                   hipStream_t s);
 // F_H gather, LJ + gravity, velocity (+ thermostat kick), u_p scatter in one

@@ -6,6 +6,9 @@
 
 namespace kh {
 
+// doubles per particle in one Output_udf frame (KH_FRAME_WORDS of the C ABI)
+constexpr int FRAME_WORDS = 34;
+
 // janus_propulsion values (enum JP, input.h; KH_JP_* of the C ABI)
 constexpr int JP_MOTOR = 0, JP_SLIP = 1, JP_OBSTACLE = 2;
 

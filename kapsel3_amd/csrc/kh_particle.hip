@@ -1099,6 +1099,39 @@ hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int 
     return hipGetLastError();
 }
 
+// One Output_udf frame (output.cxx:268-375): when the step counter ts has
+// reached ts0 + k*every (k < nframes), frame k = {ts, then per particle R,
+// R_raw, v, q (q0..q3), omega, f_hydro, torque_hydro, f_r, torque_r (0),
+// f_slip, torque_slip}, the forces and torques being the previous step's, as
+// the reference writes them.
+__global__ void k_frame(PParams pp, const DevParticle *__restrict__ P, const int *__restrict__ d_ts, int ts0,
+                        int every, int nframes, double *__restrict__ out) {
+    const int n  = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ts = *d_ts, dt = ts - ts0;
+    if (dt < 0 || dt % every != 0 || dt / every >= nframes) return;
+    double *f = out + (size_t)(dt / every) * (1 + (size_t)FRAME_WORDS * pp.np);
+    if (n == 0) f[0] = (double)ts;
+    if (n >= pp.np) return;
+    const DevParticle &q = P[n];
+    double            *o = f + 1 + (size_t)FRAME_WORDS * n;
+    const double *src[10] = {q.x, q.x_nopbc, q.v, q.om, q.fh_prev, q.th_prev, q.fr_prev, nullptr, q.fs_prev, q.ts_prev};
+    int           k       = 0;
+    for (int s = 0; s < 10; s++) {
+        for (int d = 0; d < 3; d++) o[k + d] = src[s] ? src[s][d] : 0.0;
+        k += 3;
+        if (s == 2) {  // q after v
+            for (int d = 0; d < 4; d++) o[k + d] = q.q[d];
+            k += 4;
+        }
+    }
+}
+hipError_t frame(const PParams &pp, const DevParticle *P, const int *d_ts, int ts0, int every, int nframes, double *out,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_frame, dim3(pblocks(pp.np > 0 ? pp.np : 1)), dim3(64), 0, s, pp, P, d_ts, ts0, every, nframes,
+                       out);
+    return hipGetLastError();
+}
+
 #define KH_INST(T)                                                                                               \
     template hipError_t phi_scatter<T>(const PParams &, DevParticle *, T *, const PCache<T> &, int, hipStream_t, int, \
                                        int, double, double);                                                       \

@@ -106,6 +106,8 @@ struct kh_run {
     double      *traj         = nullptr;  // per-step particle record (x, v) for traj_steps steps
     int          traj_steps   = 0;
     int          traj_base    = 0;        // ts when recording started
+    double      *frames       = nullptr;  // Output_udf frames (kh_set_output)
+    int          frame_n = 0, frame_every = 1, frame_ts0 = 0;
     // the steady-state step (ts >= 1) captured once as a HIP graph
     int              use_graph  = 1;
     bool             capturing  = false;
@@ -378,6 +380,9 @@ struct Step {
             KH_LAUNCH("tick", tick(r->d_ts, r->stream));
         if (r->traj)
             KH_LAUNCH("record", record(r->pp, r->dp, r->d_ts, r->traj_base, r->traj_steps, r->traj, r->stream));
+        if (r->frames)
+            KH_LAUNCH("frame", frame(r->pp, r->dp, r->d_ts, r->frame_ts0, r->frame_every, r->frame_n, r->frames,
+                                     r->stream));
         return KH_OK;
     }
 
@@ -614,6 +619,7 @@ int free_run(kh_run *r) {
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, r->stream);
     if (r->traj) (void)hipFreeAsync(r->traj, r->stream);
+    if (r->frames) (void)hipFreeAsync(r->frames, r->stream);
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto &p : r->pending) {
         (void)hipEventDestroy(p.a);
@@ -1122,6 +1128,44 @@ int kh_get_trajectory(kh_run *r, double *out, int nsteps) {
     if (nsteps == 0 || r->np == 0) return KH_OK;
     KH_HIP(hipMemcpyAsync(out, r->traj, sizeof(double) * 6 * (size_t)nsteps * (size_t)r->np, hipMemcpyDeviceToHost,
                           r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    return KH_OK;
+}
+
+static_assert(KH_FRAME_WORDS == FRAME_WORDS, "Output_udf frame layout");
+
+int kh_set_output(kh_run *r, int nframes, int every) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    if (nframes < 0 || every < 1) return fail(KH_ERR_ARG, "nframes %d, every %d", nframes, every);
+    drop_graph(r);  // the frame kernel joins (or leaves) the captured step
+    if (r->frames) {
+        KH_HIP(hipFreeAsync(r->frames, r->stream));
+        r->frames = nullptr;
+    }
+    r->frame_n = 0;
+    if (nframes == 0) return KH_OK;
+    const size_t bytes = sizeof(double) * (size_t)nframes * (1 + (size_t)KH_FRAME_WORDS * (size_t)r->np);
+    if (hipMallocAsync((void **)&r->frames, bytes, r->stream) != hipSuccess) {
+        r->frames = nullptr;
+        return fail(KH_ERR_NOMEM, "output buffer of %zu bytes", bytes);
+    }
+    // first frame at the first ts >= now with ts % every == 0 (the
+    // reference writes before the step of that ts, sp_3d_ns.cxx:863-950)
+    const int ts   = r->jikan.ts;
+    r->frame_ts0   = (ts + every - 1) / every * every;
+    r->frame_every = every;
+    r->frame_n     = nframes;
+    if (r->frame_ts0 == ts)  // the current state is frame 0
+        KH_LAUNCH("frame", frame(r->pp, r->dp, r->d_ts, r->frame_ts0, every, nframes, r->frames, r->stream));
+    return KH_OK;
+}
+
+int kh_get_output(kh_run *r, double *out, int nframes) {
+    if (!r || (!out && nframes > 0)) return fail(KH_ERR_ARG, "NULL argument");
+    if (nframes < 0 || nframes > r->frame_n) return fail(KH_ERR_ARG, "%d frames requested, %d set", nframes, r->frame_n);
+    if (nframes == 0) return KH_OK;
+    KH_HIP(hipMemcpyAsync(out, r->frames, sizeof(double) * (size_t)nframes * (1 + (size_t)KH_FRAME_WORDS * r->np),
+                          hipMemcpyDeviceToHost, r->stream));
     KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
 }
