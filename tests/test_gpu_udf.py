@@ -1,0 +1,42 @@
+"""Runs driven by a UDF deck: the deck's parameters and particles step the
+same as a run built directly from the same values, and a restart deck
+written from a run (resume = CONTINUE, Save_Restart_udf's record,
+resume.cxx:10-74) resumes it: the restored run follows the uninterrupted
+one to 1e-12 of each quantity's scale (atomic scatters reorder sums)."""
+import os
+
+import numpy as np
+import pytest
+
+from kapsel3_amd import udf
+
+from test_gpu_restart import _close
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECK = os.path.join(HERE, "golden", "udf", "sedimentation.udf")
+
+
+def test_deck_run_and_udf_restart(gpu, tmp_path):
+    print("udf deck 32^3 x 4 (motor + passive), restart through a UDF deck", flush=True)
+    dk = udf.read_deck(DECK)
+    parts = udf.deck_particles(dk)
+    a = gpu.Run(dk.params, dk.nparticles)
+    a.set_particles(parts)
+    a.init_fluid()
+    assert not udf.restore_deck_state(a, dk)      # resume NEW
+    a.step(4)
+    out = tmp_path / "restart.udf"
+    udf.save_restart_udf(a, DECK, out, include=os.path.join(HERE, "golden", "udf", "define.udf"))
+    a.step(5)
+    dk2 = udf.read_deck(out)
+    b = gpu.Run(dk2.params, dk2.nparticles)
+    assert udf.restore_deck_state(b, dk2)
+    assert b.get_time().ts == 4
+    b.step(5)
+    _close(b, a, 1e-12)
+    pa = a.get_particles()
+    assert np.isfinite(pa["v"]).all() and np.abs(pa["v"]).max() > 0
+    a.close()
+    b.close()

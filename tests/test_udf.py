@@ -1,0 +1,97 @@
+"""UDF deck reader (kapsel3_amd.udf) on the CPU: the grammar, the mapping of
+the keys input.cxx reads onto the step's parameters, and write -> read
+round trips. The reference's own example decks are parsed when its tree is
+present (this container only)."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from kapsel3_amd import udf
+from kapsel3_amd._abi import JAX_NONE, JAX_Z, JP_MOTOR, JP_NONE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECK = os.path.join(HERE, "golden", "udf", "sedimentation.udf")
+REF = "/root/reference"
+
+
+def test_grammar_and_values():
+    d = udf.read_udf(DECK)
+    assert d["mesh"] == {"NPX": 5, "NPY": 5, "NPZ": 5}
+    assert d["gravity"] == {"G": 1e-3, "G_direction": "-Z"}   # string comment continued over a line
+    spec = d["object_type"]["spherical_particle"]["Particle_spec"]
+    assert len(spec) == 2 and spec[0]["janus_propulsion"] == "TUMBLER"
+    assert "janus_slip_vel" not in spec[0]                     # data group ended early
+    assert spec[1]["janus_rotlet_dipole_C2"] == 0.0
+    assert d["object_type"]["chain"]["Chain_spec"] == []      # empty array
+    parts = d["switch"]["INIT_distribution"]["user_specify"]["Particles"]
+    assert parts[3]["q"] == {"q0": 0.5, "q1": 0.5, "q2": 0.5, "q3": 0.5}
+    assert d["resume"]["Calculation"] == "NEW"
+
+
+def test_deck_params():
+    dk = udf.read_deck(DECK)
+    c = dk.params.c
+    assert (c.nx, c.ny, c.nz) == (32, 32, 32)
+    # DT = factor / (NU KMAX2), KMAX2 = sum_d (2 pi / L_d * TRN_d)^2 (input.cxx:469-476, 518)
+    kmax2 = 3 * (2 * math.pi / 32 * 11) ** 2
+    assert c.dt == pytest.approx(0.5 / kmax2, rel=1e-15)
+    assert (c.kbt, c.a, c.a_xi, c.g, c.g_dir, c.epsilon) == (1e-3, 4.0, 2.0, 1e-3, 2, 1.0)
+    assert (c.rotation, c.lj_truncate, c.lj_powers, list(c.fix_cell)) == (1, 1, 0, [1, 1, 1])
+    assert (c.slip_iter_max, c.slip_tol) == (4, 5e-3)
+    assert dk.counts == [2, 2] and dk.mstep == 15
+    s0, s1 = c.species[0], c.species[1]
+    assert (s0.propulsion, s0.janus_axis, s0.mass_ratio) == (JP_MOTOR, JAX_Z, 2.0)
+    assert list(s0.janus_force) == [0.0, 0.0, 1e-3] and list(s0.janus_torque) == [0.0, 2e-4, 0.0]
+    assert (s1.propulsion, s1.janus_axis, s1.mass_ratio) == (JP_NONE, JAX_NONE, 1.5)
+    p = udf.deck_particles(dk)
+    assert list(p["spec"]) == [0, 0, 1, 1]
+    assert np.array_equal(p["x"][1], [20.1, 22.4, 9.7]) and np.array_equal(p["x_nopbc"], p["x"])
+    assert np.array_equal(p["q"][3], [0.5, 0.5, 0.5, 0.5])
+
+
+def test_unsupported_model_is_refused(tmp_path):
+    d, schema, classes = udf.read_udf_schema(DECK)
+    d["constitutive_eq"]["type"] = "Shear_Navier_Stokes"
+    out = tmp_path / "shear.udf"
+    udf.write_udf(out, d, schema, classes, include=os.path.join(HERE, "golden", "udf", "define.udf"))
+    with pytest.raises(udf.UdfError, match="Shear_Navier_Stokes"):
+        udf.read_deck(out)
+
+
+def test_write_read_round_trip(tmp_path):
+    d, schema, classes = udf.read_udf_schema(DECK)
+    d["resume"] = {"Calculation": "CONTINUE", "CONTINUE": {"Saved_Data": {
+        "jikan": {"ts": 7, "time": 0.35},
+        "Particles": [{"R": {"x": 1.0 / 3.0, "y": 2.0, "z": -1e-300}, "q": {"q0": 1.0, "q1": 0, "q2": 0, "q3": 0}}],
+        "Zeta": [[[{"zeta0": 0.1 * k, "zeta1": -k} for k in range(4)] for _ in range(2)] for _ in range(2)],
+        "uk_dc": {"x": 1.5, "y": 0.0, "z": -2.5}}}}
+    out = tmp_path / "deck.udf"
+    udf.write_udf(out, d, schema, classes, include=os.path.join(HERE, "golden", "udf", "define.udf"))
+    back = udf.read_udf(out)
+    saved = back["resume"]["CONTINUE"]["Saved_Data"]
+    assert saved["jikan"] == {"ts": 7, "time": 0.35}
+    assert saved["Particles"][0]["R"]["x"] == 1.0 / 3.0           # doubles survive exactly
+    assert saved["Particles"][0]["v_old"] == {"x": 0.0, "y": 0.0, "z": 0.0}   # schema default
+    assert saved["Zeta"][1][1][3] == {"zeta0": 0.1 * 3, "zeta1": -3.0}
+    # members the deck left out are written with the schema's zero value
+    d["object_type"]["spherical_particle"]["Particle_spec"][0].update(
+        janus_slip_vel=0.0, janus_slip_mode=0.0, janus_rotlet_C1=0.0, janus_rotlet_dipole_C2=0.0)
+    for k in d:
+        if k != "resume":
+            assert back[k] == d[k], k
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
+def test_reference_decks_parse():
+    files = sorted(glob.glob(os.path.join(REF, "**", "*.udf"), recursive=True))
+    assert files
+    for f in files:
+        udf.read_udf(f)
+    dk = udf.read_deck(os.path.join(REF, "Examples", "02", "gravity.udf"))
+    c = dk.params.c
+    assert (c.nx, c.ny, c.nz) == (256, 256, 512) and dk.nparticles == 3204
+    assert (c.dx, c.eta, c.g, c.a) == (0.1, 5.0, 980.665, 5.0)
+    assert dk.mstep == 50 * 500
