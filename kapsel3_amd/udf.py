@@ -467,8 +467,13 @@ def restore_deck_state(run, deck):
     if res.get("Calculation", "NEW") != "CONTINUE":
         return False
     sd = res["CONTINUE"]["Saved_Data"]
-    p = run.get_particles()
-    for k, sp in enumerate(sd["Particles"][:run.np]):
+    if len(sd["Particles"]) != run.np:
+        raise UdfError(f"restart record holds {len(sd['Particles'])} particles, the run has {run.np}")
+    # the record carries no species: it follows Particle_spec order, as at
+    # initialisation
+    p = new_particles(run.np)
+    p["spec"] = np.repeat(np.arange(len(deck.counts)), deck.counts)[:run.np]
+    for k, sp in enumerate(sd["Particles"]):
         for a, b in _SAVED:
             if a in sp:
                 p[k][b] = _vec(sp[a])
