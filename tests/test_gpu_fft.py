@@ -50,3 +50,18 @@ def test_forward_inverse_fp32(gpu, shape):
         assert np.abs(g - o).max() / np.abs(o).max() < 2e-6
         back = run.A_k2a(g)
         assert np.abs(back[:, :, :nz] - a[:, :, :nz]).max() < 2e-5
+
+
+@pytest.mark.parametrize("precision,tol", [(FP64, 1e-13), (MIXED, 2e-6)], ids=["fp64", "fp32"])
+def test_forward_inverse_512(gpu, precision, tol):
+    """The largest length the engines are built for (512: a full wavefront
+    per z line, one-workgroup-per-CU column tiles)."""
+    print(f"fft 512^3 precision {precision}", flush=True)
+    shape = (512, 512, 512)
+    a = _real_field(shape, 9)
+    with gpu.Run(Params(shape, precision=precision), 0) as run:
+        g = run.A2a_k(a)
+        o = kref.fft_forward(a)
+        assert np.abs(g - o).max() / np.abs(o).max() < tol
+        back = run.A_k2a(g)
+        assert np.abs(back[:, :, :512] - a[:, :, :512]).max() < (1e-12 if precision == FP64 else 2e-5)
