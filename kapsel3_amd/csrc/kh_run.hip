@@ -386,6 +386,17 @@ struct Step {
         return KH_OK;
     }
 
+    // Reset_phi / Reset_u (the previous fused step's z_force already left
+    // u_p and phi_sum zero), then the position update and the phi scatter
+    static int particles_begin(kh_run *r, int ab2) {
+        if (!(r->zclear && r->pclean)) {
+            for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
+            KH_TRY(zero(r, PS(r)));
+        }
+        r->pclean = false;
+        return move_scatter_phi(r, ab2);
+    }
+
     static int step_body(kh_run *r) {
         if (r->step_path == KH_PATH_REFERENCE_ORDER) return step_reference_order(r);
         const int   ab2 = r->jikan.ts != 0;
@@ -395,6 +406,13 @@ struct Step {
             KH_LAUNCH("zeta_clean", (zeta_clean<T>(g, Z(r, 0), Z(r, 1), s)));
             r->zeta_dirty = 0;
         }
+        // The particles move and scatter phi first (the position update
+        // reads only the particle state, so it commutes with the fluid
+        // part): phi_sum is then known when u* is formed, and u* is written
+        // only where it is read. Squirmer runs keep the reference's order
+        // and the whole u* field (the slip iteration reads it on the shell).
+        const bool early = !r->slip;
+        if (early) KH_TRY(particles_begin(r, ab2));
         cpx<T>  *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
         Comps<T> c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
         // buffer of each advection product (u2u3, u3u1, u1u2, u2^2-u3^2,
@@ -417,15 +435,8 @@ struct Step {
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
         ZIO<T> zio{{S(r, 0), S(r, 1), S(r, 2)}, {U(r, 0), U(r, 1), U(r, 2)}};
-        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z)));
-        // particles (Reset_phi / Reset_u, scatters, gather, forces, integrators);
-        // the previous fused step's z_force already left u_p and phi_sum zero
-        if (!(r->zclear && r->pclean)) {
-            for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
-            KH_TRY(zero(r, PS(r)));
-        }
-        r->pclean = false;
-        KH_TRY(move_scatter_phi(r, ab2));
+        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z, early ? PS(r) : nullptr)));
+        if (!early) KH_TRY(particles_begin(r, ab2));
         if (r->slip) {
             KH_TRY(calc_f_hydro(r));
             KH_TRY(slip_iteration(r, ab2, true));

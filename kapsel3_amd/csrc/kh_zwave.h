@@ -11,10 +11,11 @@ bool zw_supported(int L);
 // rows of L reals -> half spectra (all L/2+1 modes); spectral rows `pitch` complex apart
 template <typename T>
 hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s);
-// half spectra (modes kz < kzv, rest zero) -> rows of L reals * scale; grid y = component
+// half spectra (modes kz < kzv, rest zero) -> rows of L reals * scale; grid y = component.
+// mask (a real field, L >= 32): only 4-point vectors where it is nonzero are written
 template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
-                  hipStream_t s);
+                  hipStream_t s, const T *mask = nullptr);
 // inverse z (3) -> advection products -> forward z (5), modes kz < TRN_Z
 template <typename T>
 hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T scale, hipStream_t s);

@@ -164,6 +164,52 @@ __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T sc
     wave_sync();
 }
 
+// The same inverse z pass for ncomp components in one workgroup, writing a
+// real-space vector of 4 points only where the mask (phi_sum) has a nonzero
+// point: u* is read only there afterwards (the F_H gather at points with
+// phi > 0, the rigidity force where phi_sum != 0), so the rest of the field
+// is left as it was. The mask is read once for all components.
+template <typename T, int L, int E>
+__global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, const T *__restrict__ mask,
+                                                       long long nrows, T scale, int kzv, int pitch) {
+    using Geo = WaveGeom<T, L, E>;
+    using F   = WaveFft<T, L, E, false, false>;
+    using RS  = RowStage<T, L>;
+    static_assert(L >= 4 * Geo::G, "four points per lane per row");
+    constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
+    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    fill_twiddles<T, L, Geo::NT>(tw);
+    __syncthreads();
+    const WaveLane<T, L, E> wl(strips, nrows, pitch);
+    const long long         oa = (2 * wl.line) * L, ob = oa + L;
+    bool                    wa[NT4], wb[NT4];
+#pragma unroll
+    for (int t = 0; t < NT4; t++) {
+        const int z = 4 * (wl.g + Geo::G * t);
+        wa[t]       = wl.okA && nonzero(RS::ld(mask + oa + z));
+        wb[t]       = wl.okB && nonzero(RS::ld(mask + ob + z));
+    }
+#pragma unroll 1
+    for (int c = 0; c < ncomp; c++) {
+        cpx<T> v[F::E];
+        load_pair<T, L, F>(io.in[c], wl, kzv, v);
+        F::run(v, wl.buf, tw, wl.g);
+#pragma unroll
+        for (int i = 0; i < F::E; i++) wl.buf[WavePad<T>::at(F::out_point(wl.g, i))] = v[i];
+        wave_sync();
+        T *ra = io.out[c] + oa, *rb = io.out[c] + ob;
+#pragma unroll
+        for (int t = 0; t < NT4; t++) {
+            const int    z = 4 * (wl.g + Geo::G * t);
+            const cpx<T> a = wl.buf[WavePad<T>::at(z)], b = wl.buf[WavePad<T>::at(z + 1)],
+                         d = wl.buf[WavePad<T>::at(z + 2)], e = wl.buf[WavePad<T>::at(z + 3)];
+            if (wa[t]) RS::st(ra + z, {a.x * scale, b.x * scale, d.x * scale, e.x * scale});
+            if (wb[t]) RS::st(rb + z, {a.y * scale, b.y * scale, d.y * scale, e.y * scale});
+        }
+        wave_sync();
+    }
+}
+
 // Advection products (U2advection_k, operate_omega.cxx:31-47): inverse z
 // transforms of u1 (s0), u2 (s1), u3 (s2), the five products in registers,
 // forward z transforms into p0..p4. Two components are live at a time, so u2
@@ -337,7 +383,21 @@ hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, h
 
 template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
-                  hipStream_t s) {
+                  hipStream_t s, const T *mask) {
+    if (mask && L >= 32) {  // every length with 4 points per lane per row
+        switch (L) {
+#define KH_CASE(n)                                                                                              \
+    case n:                                                                                                     \
+        if constexpr (n >= 32) {                                                                                \
+            hipLaunchKernelGGL((k_zw_c2r_masked<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, io, ncomp, mask, nrows, scale, kzv, pitch); \
+            return hipGetLastError();                                                                           \
+        }                                                                                                       \
+        break;
+            KH_FOR_EACH_WL(KH_CASE)
+#undef KH_CASE
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (L) {
 #define KH_CASE(n)                                                                                              \
     case n:                                                                                                     \
@@ -382,7 +442,7 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
 
 #define KH_INST(T)                                                                                           \
     template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t);                    \
-    template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t);            \
+    template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t, const T *); \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
     template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,        \
                                     hipStream_t);
