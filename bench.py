@@ -171,11 +171,12 @@ def serve_bench(count, rate, nsteps, max_active, seed=2024):
     """Config 4: `count` independent runs (grid uniform in {32,48,64,96}^3,
     1-64 colloids a=4, gravity 1e-3, mixed precision, `nsteps` steps each,
     per-step trajectories out) arriving as a Poisson process at `rate`/s,
-    served by one continuous-batching Server."""
+    served by one continuous-batching Server. rate = None submits every run
+    at once (the loop's capacity: runs/s at saturation)."""
     import kapsel3_amd
     from kapsel3_amd import serving
     reqs = serving.requests(count, seed=seed)
-    times = serving.arrivals(count, rate, seed=seed + 1)
+    times = serving.arrivals(count, rate, seed=seed + 1) if rate else [0.0] * count
     with kapsel3_amd.Server(max_active=max_active, chunk_steps=8) as srv:
         t0 = time.perf_counter()
         serving.serve(srv, reqs, times, nsteps)
@@ -187,7 +188,7 @@ def serve_bench(count, rate, nsteps, max_active, seed=2024):
             "peak_bytes": st["peak_bytes"], "wall_s": wall, "runs_per_s": st["runs_per_s"],
             "steps_per_s": st["steps_per_s"], "grid_point_steps_per_s": st["grid_point_steps_per_s"],
             "workload": "grid uniform {32,48,64,96}^3, colloids uniform 1-64 (capped at 20% volume), a=4 xi=2, "
-                        "gravity 1e-3, mixed precision, Poisson arrivals"}
+                        "gravity 1e-3, mixed precision, " + ("Poisson arrivals" if rate else "all submitted at once")}
 
 
 def roofline_of(stats, n, nparticles, npts, sf, pass_ms_per_step, steps, traffic=None):
@@ -243,6 +244,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--serve-active", type=int, default=16)
+    ap.add_argument("--saturate-runs", type=int, default=256, help="runs submitted at once for the capacity figure")
+    ap.add_argument("--saturate-active", type=int, default=32)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_256_2000.json"),
                     help="measured HBM bytes per launch (JSON by precision and kernel)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -304,6 +307,8 @@ def main():
 
     if not a.no_serve:
         out["serving"] = serve_bench(a.serve_runs, a.serve_rate, a.serve_steps, a.serve_active)
+        if a.saturate_runs > 0:
+            out["serving_saturation"] = serve_bench(a.saturate_runs, None, a.serve_steps, a.saturate_active)
 
     if not a.no_cpu_baseline:
         sps, cores, secs = cpu_baseline(n, P, a.cpu_steps)
