@@ -96,6 +96,9 @@ __global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *
                                                        T *__restrict__ phisum, PCache<T> pc, int ab2, double dt,
                                                        double hdt) {
     const int n = blockIdx.x;
+    // the first Sekibun offset of this lane, loaded before the position
+    // update so its latency hides behind it; each round loads the next
+    int next = threadIdx.x < pp.np_domain ? pp.sek[threadIdx.x] : 0;
     double    xp[3];
     if constexpr (MOVE) {
         __shared__ double xs[3];
@@ -121,9 +124,9 @@ __global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *
         const int m    = m0 + threadIdx.x;
         T         ph   = (T)0;
         long long idx  = 0;
-        int       code = 0;
+        const int code = next;
+        if (m + PT_NT < pp.np_domain) next = pp.sek[m + PT_NT];
         if (m < pp.np_domain) {
-            code = pp.sek[m];
             PPoint q;
             if (sizeof(T) == 4 && pp.direct_dist) {  // FP32 distance of the offset
                 q              = sek_point<false>(pp, c, xp, code);
