@@ -167,6 +167,42 @@ def squirmer_bench(steps, n=256, count=1000):
     return out
 
 
+def output_bench(steps, every, n=256, nparticles=2000):
+    """Output_udf frames recorded on the device inside the captured step
+    (kh_set_output) at the config-3 workload, mixed: steps/s with a frame
+    every `every` steps against the same run without frames, and the one
+    batched device-to-host copy of all frames."""
+    import kapsel3_amd
+    from kapsel3_amd import workloads
+    from kapsel3_amd._abi import MIXED
+    P, parts = workloads.sedimentation(n, nparticles, kbt=1e-3, precision=MIXED, seed=3)
+    run = kapsel3_amd.Run(P, nparticles)
+    run.set_particles(parts)
+    run.init_fluid()
+    run.step(3)
+    res = {"workload": f"{n}^3, {nparticles} spheres, mixed", "steps": steps, "every": every}
+    for label in ("without", "with"):
+        nframes = steps // every
+        if label == "with":
+            ts = run.get_time().ts
+            run.set_output(nframes, every)
+            nframes = len(range((ts + every - 1) // every * every, ts + steps + 1, every))
+        run.step(1)  # re-capture the graph
+        run.sync()
+        t0 = time.perf_counter()
+        run.step(steps)
+        run.sync()
+        res[f"{label}_frames_steps_per_s"] = steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    t, fr = run.get_output()
+    res["copy_ms"] = (time.perf_counter() - t0) * 1e3
+    res["frames"] = int(len(t))
+    res["frame_bytes"] = int(fr.nbytes // max(len(t), 1) + 8)
+    res["overhead"] = res["without_frames_steps_per_s"] / res["with_frames_steps_per_s"] - 1.0
+    run.close()
+    return res
+
+
 def serve_bench(count, rate, nsteps, max_active, seed=2024):
     """Config 4: `count` independent runs (grid uniform in {32,48,64,96}^3,
     1-64 colloids a=4, gravity 1e-3, mixed precision, `nsteps` steps each,
@@ -240,6 +276,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-squirm", action="store_true", help="skip the squirmer suspension")
     ap.add_argument("--squirm-steps", type=int, default=10)
+    ap.add_argument("--no-output", action="store_true", help="skip the Output_udf frame recording figure")
+    ap.add_argument("--output-steps", type=int, default=50)
+    ap.add_argument("--output-every", type=int, default=5)
     ap.add_argument("--serve-runs", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-steps", type=int, default=200)
@@ -304,6 +343,8 @@ def main():
             out["accuracy_mixed_vs_fp64"] = accuracy(head_state, f64_state, a.warmup + 2 * a.steps, 5.0)
     if not a.no_squirm:
         out["squirmers"] = squirmer_bench(a.squirm_steps)
+    if not a.no_output:
+        out["output_frames"] = output_bench(a.output_steps, a.output_every)
 
     if not a.no_serve:
         out["serving"] = serve_bench(a.serve_runs, a.serve_rate, a.serve_steps, a.serve_active)
