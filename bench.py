@@ -140,7 +140,7 @@ def accuracy(mixed, fp64, steps, a):
 
 
 def squirmer_bench(steps, n=256, count=1000):
-    """Config 5: 1000 squirmers (B1 = 0.01, B2/B1 uniform in [-2, 2], random
+    """1000 squirmers at 256^3 (B1 = 0.01, B2/B1 uniform in [-2, 2], random
     orientations) at 256^3, steps/s of the whole step with the slip
     iteration, mixed and FP64 (after 2 warm-up steps)."""
     import kapsel3_amd
@@ -204,7 +204,7 @@ def output_bench(steps, every, n=256, nparticles=2000):
 
 
 def serve_bench(count, rate, nsteps, max_active, seed=2024):
-    """Config 4: `count` independent runs (grid uniform in {32,48,64,96}^3,
+    """Serving workload: `count` independent runs (grid uniform in {32,48,64,96}^3,
     1-64 colloids a=4, gravity 1e-3, mixed precision, `nsteps` steps each,
     per-step trajectories out) arriving as a Poisson process at `rate`/s,
     served by one continuous-batching Server. rate = None submits every run

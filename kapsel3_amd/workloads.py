@@ -56,7 +56,7 @@ def random_quaternions(n, seed):
 
 
 def plain_single(n=32, precision=FP64):
-    """Config 1: one neutral sphere a=4, xi=2 driven by a constant body force
+    """One neutral sphere a=4, xi=2 driven by a constant body force
     1e-3 along +z (a motor force, since gravity on a neutral sphere is zero,
     md_force.cxx:314), fluid at rest, dt 0.05, FIX_CELL on all axes."""
     sp = make_species(mass_ratio=1.0, propulsion=JP_MOTOR, janus_axis=JAX_Z, janus_force=(0.0, 0.0, 1e-3))
@@ -67,7 +67,7 @@ def plain_single(n=32, precision=FP64):
 
 
 def sedimentation(n=128, nparticles=200, a=5.0, kbt=0.0, precision=FP64, seed=1, mass_ratio=2.0, gap=1.0):
-    """Configs 2 and 3: spheres a=5, xi=2, LJ 12:6 repulsive (cutoff
+    """Sedimentation: spheres a=5, xi=2, LJ 12:6 repulsive (cutoff
     2^(1/6) 2a), eps=1, gravity 1e-3 along -z, rho_p/rho = 2, fluid and
     particles at rest, dt 0.05. kbt > 0 adds the thermostat (alpha = 1)."""
     sp = make_species(mass_ratio=mass_ratio, propulsion=JP_NONE)
@@ -78,7 +78,7 @@ def sedimentation(n=128, nparticles=200, a=5.0, kbt=0.0, precision=FP64, seed=1,
 
 
 def squirmers(n=256, nparticles=1000, a=5.0, b1=0.01, precision=MIXED, seed=5, slip_iter=4, slip_tol=0.005):
-    """Config 5: squirmers with B1 = 0.01 and per-particle B2/B1 uniform in
+    """Squirmers with B1 = 0.01 and per-particle B2/B1 uniform in
     [-2, 2] (one species per particle, as the reference keeps slip
     parameters per species), random orientations, no gravity, kT = 0."""
     rng = np.random.default_rng(seed)

@@ -160,9 +160,9 @@ PATHS = pytest.mark.parametrize("path", ["fused", "reference"])
 
 @PATHS
 def test_plain_single_sphere_fp64(gpu, path):
-    """Config 1 (32^3, one driven sphere) for 1001 steps (MSTEP+1,
+    """One driven sphere (32^3, a = 4) for 1001 steps (MSTEP+1,
     sp_3d_ns.cxx:857)."""
-    print(f"config1 1001 steps {path}", flush=True)
+    print(f"single driven sphere 32^3, 1001 steps {path}", flush=True)
     P, parts = workloads.plain_single(32)
     o, g = run_both(P, parts, 1001, path=path)
     po, pg = o.get_particles(), g.get_particles()
@@ -177,9 +177,9 @@ def test_plain_single_sphere_fp64(gpu, path):
 
 @PATHS
 def test_sedimentation_shrunk_fp64(gpu, path):
-    """Config 2 shrunk: 64^3, 24 spheres a=5 in contact range (LJ active),
+    """Sedimentation shrunk: 64^3, 24 spheres a=5 in contact range (LJ active),
     gravity, 60 steps."""
-    print(f"config2 shrunk 64^3 x 24 {path}", flush=True)
+    print(f"sedimentation 64^3 x 24 {path}", flush=True)
     P, parts = workloads.sedimentation(64, 24, gap=0.4, seed=3)
     o, g = run_both(P, parts, 60, path=path)
     cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)
@@ -189,8 +189,8 @@ def test_sedimentation_shrunk_fp64(gpu, path):
 
 @PATHS
 def test_thermal_shrunk_fp64(gpu, path):
-    """Config 3 physics (LJ + gravity + kT noise) at 48^3 (radix-3 FFT), 30 steps."""
-    print(f"config3 physics 48^3 {path}", flush=True)
+    """Thermal sedimentation physics (LJ + gravity + kT noise) at 48^3 (radix-3 FFT), 30 steps."""
+    print(f"thermal sedimentation 48^3 {path}", flush=True)
     P, parts = workloads.sedimentation(48, 12, kbt=1e-3, gap=0.5, seed=4)
     o, g = run_both(P, parts, 30, path=path)
     cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)
@@ -198,9 +198,9 @@ def test_thermal_shrunk_fp64(gpu, path):
 
 @pytest.mark.parametrize("n,np_,steps", [(128, 60, 6), (256, 40, 3)])
 def test_sedimentation_production_grid_fp64(gpu, n, np_, steps):
-    """Config 2 geometry at 128^3 and the full 256^3 grid: the z passes of
+    """Sedimentation geometry (a = 5, LJ, gravity) at 128^3 and the full 256^3 grid: the z passes of
     the production size (wavefront-scope engine) against the oracle."""
-    print(f"config2 {n}^3 x {np_}, {steps} steps", flush=True)
+    print(f"sedimentation {n}^3 x {np_}, {steps} steps", flush=True)
     P, parts = workloads.sedimentation(n, np_, gap=0.4, seed=5, kbt=1e-3)
     o, g = run_both(P, parts, steps)
     cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)

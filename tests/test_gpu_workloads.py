@@ -1,14 +1,14 @@
 """The benchmark configurations at their stated sizes.
 
-- config 2 (128^3, 200 spheres, LJ + gravity, FP64): the GPU against the
+- sedimentation (128^3, 200 spheres, LJ + gravity, FP64): the GPU against the
   oracle over 20 steps, <= 1e-10 of each quantity's scale (the oracle's
   1000-step run takes minutes on the host; 20 steps cover Euler, AB2 and
   contacts);
-- config 3 (256^3, 2000 spheres, kT = 1e-3): mixed against FP64 on the GPU
+- thermal sedimentation (256^3, 2000 spheres, kT = 1e-3): mixed against FP64 on the GPU
   over the full 1000 steps, particle velocity within 1e-4 of the velocity
   scale (the north-star accuracy bar), position within 1e-4 of the largest
   displacement;
-- config 5 (256^3, 1000 squirmers): mixed against FP64 over 100 steps.
+- squirmers (256^3, 1000 squirmers): mixed against FP64 over 100 steps.
 """
 import numpy as np
 import pytest
@@ -22,8 +22,8 @@ from test_gpu_step import cmp_particles, rel
 pytestmark = pytest.mark.gpu
 
 
-def test_config2_fp64_vs_oracle(gpu):
-    print("config 2: 128^3 x 200, 20 steps vs oracle", flush=True)
+def test_sedimentation_128_fp64_vs_oracle(gpu):
+    print("sedimentation 128^3 x 200, 20 steps vs oracle", flush=True)
     P, parts = workloads.sedimentation(128, 200, seed=2)
     o = kref.Run(P, 200)
     g = gpu.Run(P, 200)
@@ -48,8 +48,8 @@ def _gpu_run(gpu, P, parts, steps):
     return p
 
 
-def test_config3_mixed_vs_fp64_1000_steps(gpu):
-    print("config 3: 256^3 x 2000, kT 1e-3, 1000 steps mixed vs fp64", flush=True)
+def test_thermal_256_mixed_vs_fp64_1000_steps(gpu):
+    print("thermal sedimentation 256^3 x 2000, kT 1e-3, 1000 steps mixed vs fp64", flush=True)
     out = {}
     for prec in (FP64, MIXED):
         P, parts = workloads.sedimentation(256, 2000, kbt=1e-3, precision=prec, seed=3)
@@ -63,8 +63,8 @@ def test_config3_mixed_vs_fp64_1000_steps(gpu):
     assert np.isfinite(p32["v"]).all() and ev < 1e-4 and ex < 1e-4
 
 
-def test_config5_squirmers_mixed_vs_fp64(gpu):
-    print("config 5: 256^3 x 1000 squirmers, 100 steps mixed vs fp64", flush=True)
+def test_squirmers_256_mixed_vs_fp64(gpu):
+    print("squirmers 256^3 x 1000, 100 steps mixed vs fp64", flush=True)
     out = {}
     for prec in (FP64, MIXED):
         P, parts = workloads.squirmers(256, 1000, precision=prec)

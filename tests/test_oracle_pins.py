@@ -356,7 +356,7 @@ def test_lennard_jones_brute_force_many():
 
 
 def test_single_sphere_settles_at_periodic_array_drag():
-    """Whole step, config 1: the settling speed approaches Hasimoto's
+    """Whole step, one driven sphere at 32^3: the settling speed approaches Hasimoto's
     periodic-array value (U/U_St = 1 - 1.7601 c^1/3 + c - 1.5593 c^2)."""
     P, parts = workloads.plain_single(32)
     r = kref.Run(P, 1)

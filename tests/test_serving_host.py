@@ -1,4 +1,4 @@
-"""Host side of the serving workload (config 4): the seeded request stream
+"""Host side of the serving workload (grids 32-96^3, 1-64 spheres): the seeded request stream
 and the Poisson arrival process. No GPU."""
 import numpy as np
 
