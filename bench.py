@@ -291,7 +291,7 @@ def main():
                     help="also run a plain 256 MiB device copy (torch) so profiler byte counters can be scaled")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: one simulation runs on one GPU; --gpus {a.gpus} is not supported "
               "(independent runs shard across GPUs by launching one process per GPU)", file=sys.stderr)
         sys.exit(2)
     from kapsel3_amd._abi import FP64, MIXED

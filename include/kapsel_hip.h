@@ -41,7 +41,7 @@ typedef enum kh_status {
     KH_ERR_ARG         = -1, /* invalid argument or parameter */
     KH_ERR_HIP         = -2, /* HIP runtime error */
     KH_ERR_NOMEM       = -3, /* device allocation failed */
-    KH_ERR_UNSUPPORTED = -4, /* size or option this build does not provide */
+    KH_ERR_UNSUPPORTED = -4, /* size or option this library does not provide */
     KH_ERR_STATE       = -5, /* call made in the wrong state */
     KH_ERR_NONFINITE   = -6  /* non-finite particle state detected */
 } kh_status;

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product (kapsel3_amd/) links,
  * loads or calls this code; only tests/, the smoke check and the CPU-baseline
- * leg of bench.py use it, as the checker and as the timed CPU baseline.
+ * leg of bench.py use it: to check results and as the timed CPU baseline.
  *
  * It works on the reference's own data layout: every field is NX*NY*NZ_
  * doubles with NZ_ = 2*(NZ/2+1); real-space values live in k < NZ, spectral

@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, as the checker. The product never imports it.
+CPU-baseline leg of bench.py, to check results. The product never imports it.
 """
 import ctypes as C
 import os

@@ -1,5 +1,5 @@
 /*
- * ref_ooura_driver.c -- drives the reference's own Ooura FFT (fftsg3d.c,
+ * ref_ooura_entry.c -- C entry points to the reference's own Ooura FFT (fftsg3d.c,
  * fftsg.c) with exactly the call sequence of A2a_k / A_k2a's default branch
  * (fft_wrapper.h:641-681; workspace sizes from Init_fft_ooura,
  * fft_wrapper.cxx:78-87), so the oracle's FFT can be checked against it.

@@ -11,7 +11,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libkapsel_hip.so)")
-    # the oracle is a CPU library of the checker; build it if this tree lacks it
+    # the oracle is the CPU library the tests check against; build it if absent
     lib = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(lib):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
