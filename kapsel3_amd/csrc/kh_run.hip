@@ -111,8 +111,10 @@ struct kh_run {
     // the steady-state step (ts >= 1) captured once as a HIP graph
     int              use_graph  = 1;
     bool             capturing  = false;
-    hipGraph_t       graph      = nullptr;
+    hipGraph_t       graph      = nullptr;  // one steady-state step
     hipGraphExec_t   graph_exec = nullptr;
+    hipGraph_t       graph_k      = nullptr;  // GRAPH_STEPS steady-state steps
+    hipGraphExec_t   graph_exec_k = nullptr;
     int          np_domain = 0;
     CTime        jikan;
     int          step_path  = KH_PATH_FUSED;
@@ -618,11 +620,12 @@ void *field_buffer(kh_run *r, int field) {
     }
 }
 
+void drop_graph(kh_run *r);
+
 int free_run(kh_run *r) {
     if (!r) return KH_OK;
     if (r->stream) (void)hipStreamSynchronize(r->stream);
-    if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
-    if (r->graph) (void)hipGraphDestroy(r->graph);
+    drop_graph(r);
     void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
                     r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
                     r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek, r->cphi, r->cidx,
@@ -688,8 +691,12 @@ size_t run_footprint(const kh_params &P, int np) {
 void drop_graph(kh_run *r) {
     if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
     if (r->graph) (void)hipGraphDestroy(r->graph);
-    r->graph_exec = nullptr;
-    r->graph      = nullptr;
+    if (r->graph_exec_k) (void)hipGraphExecDestroy(r->graph_exec_k);
+    if (r->graph_k) (void)hipGraphDestroy(r->graph_k);
+    r->graph_exec   = nullptr;
+    r->graph        = nullptr;
+    r->graph_exec_k = nullptr;
+    r->graph_k      = nullptr;
 }
 
 }  // namespace
@@ -1074,12 +1081,16 @@ int kh_init_fluid(kh_run *r) {
     return r->mixed ? Step<float>::init_fluid(r) : Step<double>::init_fluid(r);
 }
 
-// Capture one steady-state step (ts >= 1, fused path, zeta clean) as a HIP
-// graph; replays then cost one launch per step.
-static int capture_step(kh_run *r) {
+// Steady-state steps (ts >= 1, fused path, zeta clean) are identical up to
+// the device step counter, so `count` of them captured back to back form
+// one HIP graph; replays then cost one launch per `count` steps. Small grids
+// (the serving loop) are bound by launch cost, hence the multi-step graph.
+constexpr int GRAPH_STEPS = 8;
+static int capture_steps(kh_run *r, int count, hipGraph_t *graph, hipGraphExec_t *exec) {
     KH_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
     r->capturing = true;
-    int st       = r->mixed ? Step<float>::step(r) : Step<double>::step(r);
+    int st       = KH_OK;
+    for (int k = 0; k < count && st == KH_OK; k++) st = r->mixed ? Step<float>::step(r) : Step<double>::step(r);
     r->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(r->stream, &g);
@@ -1088,27 +1099,36 @@ static int capture_step(kh_run *r) {
         return st;
     }
     if (e != hipSuccess) return fail(KH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    r->graph = g;
-    KH_HIP(hipGraphInstantiate(&r->graph_exec, r->graph, nullptr, nullptr, 0));
+    *graph = g;
+    KH_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     return KH_OK;
 }
 
 int kh_step(kh_run *r, int nsteps) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
-    for (int s = 0; s < nsteps; s++) {
+    for (int s = 0; s < nsteps;) {
         // (the slip loop decides its length on the host: not captured)
         const bool graph_ok = r->use_graph && !r->timing && r->step_path == KH_PATH_FUSED && r->jikan.ts > 0 &&
                               !r->zeta_dirty && !r->slip && (!r->zclear || r->pclean);
-        if (graph_ok) {
-            if (!r->graph_exec) KH_TRY(capture_step(r));
+        int done = 1;
+        if (graph_ok && nsteps - s >= GRAPH_STEPS) {
+            if (!r->graph_exec_k) KH_TRY(capture_steps(r, GRAPH_STEPS, &r->graph_k, &r->graph_exec_k));
+            KH_HIP(hipGraphLaunch(r->graph_exec_k, r->stream));
+            r->pcache_valid = 1;
+            done = GRAPH_STEPS;
+        } else if (graph_ok) {
+            if (!r->graph_exec) KH_TRY(capture_steps(r, 1, &r->graph, &r->graph_exec));
             KH_HIP(hipGraphLaunch(r->graph_exec, r->stream));
             r->pcache_valid = 1;
         } else {
             KH_TRY(r->mixed ? Step<float>::step(r) : Step<double>::step(r));
         }
-        r->jikan.time += r->jikan.dt_fluid;  // sp_3d_ns.cxx:955
-        r->jikan.ts++;
+        for (int k = 0; k < done; k++) {
+            r->jikan.time += r->jikan.dt_fluid;  // sp_3d_ns.cxx:955
+            r->jikan.ts++;
+        }
+        s += done;
     }
     return KH_OK;
 }

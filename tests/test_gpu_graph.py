@@ -1,6 +1,7 @@
 """Captured-graph steps against uncaptured steps of the same run.
 
-Steps with ts >= 1 replay one captured HIP graph by default. Each replay is
+Steps with ts >= 1 replay a captured HIP graph by default (one launch per
+8 steps when at least 8 are asked for, else one per step). Each replay is
 checked against the uncaptured step, with host transfers (eager work and
 synchronisations) between the replays, after a change of dt (which must
 re-capture) and after kh_apply (which must run one uncaptured step).
@@ -48,6 +49,9 @@ def test_graph_replays_match_uncaptured(gpu):
         _close(g, e)
     g.step(5)
     e.step(5)
+    _close(g, e)
+    g.step(17)  # two replays of the 8-step graph, then one single step
+    e.step(17)
     _close(g, e)
     g.close()
     e.close()
