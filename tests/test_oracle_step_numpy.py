@@ -97,7 +97,8 @@ def points(xp):
     return rm, r, phi_of(np.sqrt((d ** 2).sum(1)))
 
 
-def step(z0, z1, ukdc, P, ts):
+def fluid_ns(z0, z1, ukdc):
+    """The slaved-Euler NS step, FIX_CELL and u* (sp_3d_ns.cxx:41-57)."""
     nu = ETA / RHO
     # NS_solver_slavedEuler (fluid_solver.cxx:348-374) via Zeta_k2advection_k
     u = [inv(c) for c in zeta2uhat(z0 * KEEP, z1 * KEEP, ukdc)]
@@ -115,18 +116,43 @@ def step(z0, z1, ukdc, P, ts):
     ukdc = np.zeros(3)                                            # FIX_CELL x, y, z (sp_3d_ns.cxx:46-52)
     z0, z1 = z0 * KEEP, z1 * KEEP
     us = [inv(c) for c in zeta2uhat(z0, z1, ukdc)]                # u*
-    # positions (particle_solver.cxx:39-85): Euler first, then AB2
-    for q in P:
-        dx = DT * q["v"] if ts == 0 else 0.5 * DT * (3 * q["v"] - q["v_old"])
-        q["x_nopbc"] = q["x_nopbc"] + dx
-        q["x"] = np.fmod(q["x"] + dx + N * DX, N * DX)
-    # phi_sum (make_phi.cxx:214-260)
+    return z0, z1, ukdc, us
+
+
+def project(f):
+    """Solenoidal_u (operate_omega.h:105-112): FFT, 2/3 rule, k.f = 0."""
+    fh = [fwd(f[d]) * KEEP for d in range(3)]
+    kf = (KX * fh[0] + KY * fh[1] + KZ * fh[2]) * IK2
+    return [inv(fh[0] - KX * kf), inv(fh[1] - KY * kf), inv(fh[2] - KZ * kf)]
+
+
+def u2zeta(u):
+    """U2zeta_k (operate_omega.h:208-211, fft_wrapper.cxx:407-476)."""
+    uh = [fwd(u[d]) for d in range(3)]
+    w = [1j * (KY * uh[2] - KZ * uh[1]), 1j * (KZ * uh[0] - KX * uh[2]), 1j * (KX * uh[1] - KY * uh[0])]
+    z0, z1 = reduce_case(*w)
+    return z0, z1, np.array([uh[d][0, 0, 0].real for d in range(3)])
+
+
+def scatter_phi(P):
+    """Make_phi_particle_sum (make_phi.cxx:214-260): phi_sum and the points."""
     phis = np.zeros((N, N, N))
     pts = []
     for q in P:
         rm, r, ph = points(q["x"])
         np.add.at(phis, (rm[:, 0], rm[:, 1], rm[:, 2]), ph)
         pts.append((rm, r, ph))
+    return phis, pts
+
+
+def step(z0, z1, ukdc, P, ts):
+    z0, z1, ukdc, us = fluid_ns(z0, z1, ukdc)
+    # positions (particle_solver.cxx:39-85): Euler first, then AB2
+    for q in P:
+        dx = DT * q["v"] if ts == 0 else 0.5 * DT * (3 * q["v"] - q["v_old"])
+        q["x_nopbc"] = q["x_nopbc"] + dx
+        q["x"] = np.fmod(q["x"] + dx + N * DX, N * DX)
+    phis, pts = scatter_phi(P)
     phi = np.minimum(phis, 1.0)
     # F_H with u* at the new position and the old velocity (md_force.cxx:383-506)
     coef = -DX ** 3 * RHO / DT
@@ -161,14 +187,8 @@ def step(z0, z1, ukdc, P, ts):
             np.add.at(up[d], (rm[:, 0], rm[:, 1], rm[:, 2]), vr[:, d])
     # f = P(u_p - phi u*), u = u* + f, zeta = curl u (f_particle.h:35-84,
     # operate_omega.cxx:171-195, fft_wrapper.cxx:407-476)
-    fh = [fwd(up[d] - phi * us[d]) * KEEP for d in range(3)]
-    kf = (KX * fh[0] + KY * fh[1] + KZ * fh[2]) * IK2
-    fh = [fh[0] - KX * kf, fh[1] - KY * kf, fh[2] - KZ * kf]
-    uh = [fwd(us[d] + inv(fh[d])) for d in range(3)]
-    w = [1j * (KY * uh[2] - KZ * uh[1]), 1j * (KZ * uh[0] - KX * uh[2]), 1j * (KX * uh[1] - KY * uh[0])]
-    z0, z1 = reduce_case(*w)
-    ukdc = np.array([uh[d][0, 0, 0].real for d in range(3)])
-    return z0, z1, ukdc
+    f = project([up[d] - phi * us[d] for d in range(3)])
+    return u2zeta([us[d] + f[d] for d in range(3)])
 
 
 def spec(a):
