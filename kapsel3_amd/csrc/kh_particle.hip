@@ -44,9 +44,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Reduce K doubles over the workgroup; result valid in thread 0.
-template <int K>
+template <int K, int NT = PT_NT>
 __device__ __forceinline__ void block_sum(double *v) {
-    __shared__ double red[PT_NT / 64][K];
+    __shared__ double red[NT / 64][K];
     const int         lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -58,7 +58,7 @@ __device__ __forceinline__ void block_sum(double *v) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
             double t = 0.0;
-            for (int w = 0; w < PT_NT / 64; w++) t += red[w][k];
+            for (int w = 0; w < NT / 64; w++) t += red[w][k];
             v[k] = t;
         }
     }
@@ -91,8 +91,8 @@ __device__ __forceinline__ void angular2v(int rotation, const double *om, const 
 // particle (the position integrator of the same step, one lane).
 __device__ void position_one(const PParams &pp, DevParticle *P, int n, int ab2, double dt, double hdt);
 
-template <typename T, bool SCATTER, bool MOVE>
-__global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *__restrict__ P,
+template <typename T, bool SCATTER, bool MOVE, int NT>
+__global__ __launch_bounds__(NT) void k_phi_scatter(PParams pp, DevParticle *__restrict__ P,
                                                        T *__restrict__ phisum, PCache<T> pc, int ab2, double dt,
                                                        double hdt) {
     const int n = blockIdx.x;
@@ -113,19 +113,19 @@ __global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *
     }
     const PCell     c  = particle_cell(pp, xp);
     const long long cb = (long long)n * pp.np_domain;
-    // the nonzero points are packed in Sekibun order: per round of PT_NT
+    // the nonzero points are packed in Sekibun order: per round of NT
     // points, a wave's ballot gives each lane its rank among the wave's
     // nonzero points and the per-wave counts (double-buffered in LDS) the
     // wave's base
-    __shared__ int wcnt[2][PT_NT / 64];
+    __shared__ int wcnt[2][NT / 64];
     const int      lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int            off  = 0;
-    for (int m0 = 0, par = 0; m0 < pp.np_domain; m0 += PT_NT, par ^= 1) {
+    for (int m0 = 0, par = 0; m0 < pp.np_domain; m0 += NT, par ^= 1) {
         const int m    = m0 + threadIdx.x;
         T         ph   = (T)0;
         long long idx  = 0;
         const int code = next;
-        if (m + PT_NT < pp.np_domain) next = pp.sek[m + PT_NT];
+        if (m + NT < pp.np_domain) next = pp.sek[m + NT];
         if (m < pp.np_domain) {
             PPoint q;
             if (sizeof(T) == 4 && pp.direct_dist) {  // FP32 distance of the offset
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(PT_NT) void k_phi_scatter(PParams pp, DevParticle *
         __syncthreads();
         int base = off, tot = 0;
 #pragma unroll
-        for (int w = 0; w < PT_NT / 64; w++) {
+        for (int w = 0; w < NT / 64; w++) {
             const int k = wcnt[par][w];
             base += w < wave ? k : 0;
             tot += k;
@@ -171,7 +171,7 @@ __device__ __forceinline__ void sek_r(const PParams &pp, const PCell &c, int cod
 
 // Calc_f_hydro_correct_precision (md_force.cxx:383-506), non-rigid branch,
 // over the cached phi and grid index of each Sekibun point.
-template <typename T>
+template <typename T, int NT = PT_NT>
 __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__restrict__ P, int n,
                                               const T *__restrict__ u0, const T *__restrict__ u1,
                                               const T *__restrict__ u2, const T *__restrict__ phisum,
@@ -185,7 +185,7 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
     // per-point terms in the field precision T, sums in FP64
     const T vt[3] = {(T)vp[0], (T)vp[1], (T)vp[2]}, wt[3] = {(T)om[0], (T)om[1], (T)om[2]};
     const int cnt = pc.cnt[n];
-    for (int m = threadIdx.x; m < cnt; m += PT_NT) {
+    for (int m = threadIdx.x; m < cnt; m += NT) {
         const T   ph   = pc.phi[cb + m];
         const int idx  = pc.idx[cb + m];
         const int code = pc.code[cb + m];
@@ -214,7 +214,7 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
     }
     double v[6];
     for (int k = 0; k < 6; k++) v[k] = acc[k].val();
-    block_sum<6>(v);
+    block_sum<6, NT>(v);
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; d++) {
             P[n].fh[d] = coef * v[d];
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__re
 }
 
 // Make_u_particle_sum scatter (make_phi.cxx:313-354) over the cache
-template <typename T>
+template <typename T, int NT = PT_NT>
 __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const double *xp, const double *vp,
                                                  const double *om, const T *__restrict__ phisum,
                                                  const PCache<T> &pc, T *__restrict__ up0, T *__restrict__ up1,
@@ -244,7 +244,7 @@ __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const
     const long long cb    = (long long)n * pp.np_domain;
     const T vt[3] = {(T)vp[0], (T)vp[1], (T)vp[2]}, wt[3] = {(T)om[0], (T)om[1], (T)om[2]};
     const int cnt = pc.cnt[n];
-    for (int m = threadIdx.x; m < cnt; m += PT_NT) {
+    for (int m = threadIdx.x; m < cnt; m += NT) {
         int  idx, code;
         T    dmy_phi;
         bool alone;  // phi_sum is this particle's phi alone
@@ -400,6 +400,7 @@ __device__ __forceinline__ int lj_mult(const PParams &pp, const double *xi, cons
 // (md_force.cxx:306-324). One workgroup per particle i: its lanes split the
 // partners j, each pair giving F_i += f(r_ij) (x_i - x_j), the same value the
 // reference's +/- updates add; the lane sums are reduced in FP64.
+template <int NT = PT_NT>
 __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__restrict__ P, int i) {
     const double xi[3]  = {P[i].x[0], P[i].x[1], P[i].x[2]};
     const int    si     = P[i].spec;
@@ -411,7 +412,7 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
         // rounding of ties at half a box, hence the margin on the cutoff);
         // the rest take the reference's Distance0
         const double cut2 = pp.lj_cutoff * pp.lj_cutoff * (1.0 + 1e-9);
-        for (int j = threadIdx.x; j < pp.np; j += PT_NT) {
+        for (int j = threadIdx.x; j < pp.np; j += NT) {
             if (j == i) continue;
             if (obst_i && pp.xs[4 * j + 3] != 0.0) continue;
             const double xj[3] = {pp.xs[4 * j], pp.xs[4 * j + 1], pp.xs[4 * j + 2]};
@@ -438,7 +439,7 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
             }
         }
     }
-    block_sum<3>(f);
+    block_sum<3, NT>(f);
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; d++) P[i].fr[d] += f[d];
         if (pp.g != 0.0) P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
@@ -569,10 +570,12 @@ __device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise 
 // stage reads only this particle's state and inputs already complete when
 // the launch starts (u*, phi_sum, every particle's new position), so the
 // four reference loops need no grid-wide barrier between them.
-// 8 waves per SIMD (64 VGPRs): two thousand particle workgroups are then
-// all resident at once on 256 CUs
-template <typename T>
-__global__ __launch_bounds__(PT_NT, 8) void k_particle_step(PParams pp, DevParticle *__restrict__ P,
+// NT = 256: 8 waves per SIMD (64 VGPRs), so two thousand particle
+// workgroups are all resident at once on 256 CUs. Few particles leave the
+// chip idle and the step's time is one workgroup's chain of dependent
+// loads: NT = 1024 splits each particle's points over four times the lanes.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT, NT == PT_NT ? 8 : 1) void k_particle_step(PParams pp, DevParticle *__restrict__ P,
                                                          const T *__restrict__ u0, const T *__restrict__ u1,
                                                          const T *__restrict__ u2, const T *__restrict__ phisum,
                                                          PCache<T> pc, double coef, int ab2, double h, Noise nz,
@@ -580,8 +583,8 @@ __global__ __launch_bounds__(PT_NT, 8) void k_particle_step(PParams pp, DevParti
                                                          T *__restrict__ up2) {
     const int               n = blockIdx.x;
     __shared__ PStash<T>    stash;
-    f_hydro_block<T>(pp, P, n, u0, u1, u2, phisum, pc, coef, &stash);
-    forces_block(pp, P, n);
+    f_hydro_block<T, NT>(pp, P, n, u0, u1, u2, phisum, pc, coef, &stash);
+    forces_block<NT>(pp, P, n);
     __shared__ double st[9];
     if (threadIdx.x == 0) {
         velocity_one(pp, P, n, ab2, h, nz);
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(PT_NT, 8) void k_particle_step(PParams pp, DevParti
     }
     __syncthreads();
     // each lane reads back only the stash entries it wrote
-    up_scatter_block<T>(pp, n, st, st + 3, st + 6, phisum, pc, up0, up1, up2, &stash);
+    up_scatter_block<T, NT>(pp, n, st, st + 3, st + 6, phisum, pc, up0, up1, up2, &stash);
 }
 
 // ------------------------------------------------------------ squirmers
@@ -978,19 +981,33 @@ __global__ __launch_bounds__(PT_NT) void k_slip_conv(PParams pp, const DevPartic
 // ------------------------------------------------------------ launchers
 static inline unsigned pblocks(int n) { return (unsigned)((n + 63) / 64); }
 
+// Workgroup size of the phi scatter and the fused particle step: 1024 lanes
+// while the particles' workgroups (16 wavefronts, one per CU at the step's
+// register count) all fit on the chip at once, 256 beyond.
+constexpr int PT_WIDE = 1024;
+static inline bool pt_wide(const PParams &pp) { return pp.np <= 256; }
+
+template <typename T, int NT>
+static void launch_phi_scatter(const PParams &pp, DevParticle *P, T *phisum, const PCache<T> &pc, int scatter,
+                               hipStream_t s, int move, int ab2, double dt, double hdt) {
+    if (move)
+        hipLaunchKernelGGL((k_phi_scatter<T, true, true, NT>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc, ab2, dt,
+                           hdt);
+    else if (scatter)
+        hipLaunchKernelGGL((k_phi_scatter<T, true, false, NT>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc, ab2,
+                           dt, hdt);
+    else
+        hipLaunchKernelGGL((k_phi_scatter<T, false, false, NT>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc, ab2,
+                           dt, hdt);
+}
 template <typename T>
 hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, const PCache<T> &pc, int scatter, hipStream_t s,
                        int move, int ab2, double dt, double hdt) {
     if (pp.np > 0) {
-        if (move)
-            hipLaunchKernelGGL((k_phi_scatter<T, true, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, ab2,
-                               dt, hdt);
-        else if (scatter)
-            hipLaunchKernelGGL((k_phi_scatter<T, true, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, ab2,
-                               dt, hdt);
+        if (pt_wide(pp))
+            launch_phi_scatter<T, PT_WIDE>(pp, P, phisum, pc, scatter, s, move, ab2, dt, hdt);
         else
-            hipLaunchKernelGGL((k_phi_scatter<T, false, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc,
-                               ab2, dt, hdt);
+            launch_phi_scatter<T, PT_NT>(pp, P, phisum, pc, scatter, s, move, ab2, dt, hdt);
     }
     return hipGetLastError();
 }
@@ -1043,9 +1060,12 @@ hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const
                          double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream, const int *d_ts,
                          double niv, double nio, T *const up[3], hipStream_t s) {
     const Noise nz{noise, seed, stream, d_ts, niv, nio};
-    if (pp.np > 0)
-        hipLaunchKernelGGL(k_particle_step<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, pc,
-                           coef, ab2, h, nz, up[0], up[1], up[2]);
+    if (pp.np > 0 && pt_wide(pp))
+        hipLaunchKernelGGL((k_particle_step<T, PT_WIDE>), dim3(pp.np), dim3(PT_WIDE), 0, s, pp, P, u[0], u[1], u[2],
+                           phisum, pc, coef, ab2, h, nz, up[0], up[1], up[2]);
+    else if (pp.np > 0)
+        hipLaunchKernelGGL((k_particle_step<T, PT_NT>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2],
+                           phisum, pc, coef, ab2, h, nz, up[0], up[1], up[2]);
     return hipGetLastError();
 }
 template <typename T>
