@@ -349,7 +349,8 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
 bool zw_supported(int L) { return wave_fft_ok(L); }
 
 // points per lane of each pass (the advection pass holds three components
-// in registers, so it takes fewer points per lane)
+// in registers, so it may take fewer points per lane; the FP64 force pass,
+// which streams seven fields per point, runs faster at four)
 #ifndef KH_ADV_E32
 #define KH_ADV_E32 8
 #endif
@@ -360,6 +361,7 @@ template <typename T, int L>
 struct ZwE {
     static constexpr int pass = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 8>::value;
     static constexpr int adv  = WaveE<T, L, sizeof(T) == 4 ? KH_ADV_E32 : 8>::value;
+    static constexpr int force = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 4>::value;
 };
 
 template <typename T, int L, int E>
@@ -431,7 +433,7 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, g, up[0], up[1], up[2], \
+        hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::force>), dim3(zw_blocks<T, n, ZwE<T, n>::force>(nrows)), dim3(256), 0, s, g, up[0], up[1], up[2], \
                            phis, us[0], us[1], us[2], f[0], f[1], f[2], clear, tick);                               \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)
