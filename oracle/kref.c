@@ -4,7 +4,9 @@
  * A CPU FP64 restatement of Time_evolution_hydro (sp_3d_ns.cxx:41-194) for
  * SW_EQ == Navier_Stokes with spherical particles, on the reference's padded
  * field layout. Every function cites the reference lines it restates. Where
- * the reference's behaviour is replaced on purpose the comment says so:
+ * the reference's behaviour is replaced on purpose the comment says so.
+ * kr_set_shear switches the run to the Lees-Edwards step
+ * Time_evolution_hydro_OBL (sp_3d_ns.cxx:328-433). Replacements:
  *   - thermal noise: Philox4x32-10 keyed by (seed, stream, particle, step)
  *     instead of the process-global MT19937 (fluct.h:48-60), whose stream is
  *     not reproducible under OpenMP; the amplitude formula is the reference's.
@@ -51,6 +53,10 @@ struct kr_run {
     int        *sek;
     CTime       jikan;
     kr_fft3     fft;
+    /* Lees-Edwards (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) */
+    int     le, le_ac;
+    double  shear_rate, shear_freq, degree_oblique, shear_rate_eff, XYaspect, Ivolume;
+    double *ucp[3], *w1, *w3[3];
 };
 
 #define IM(r, i, j, k) ((((size_t)(i) * (r)->NY + (j)) * (r)->NZ_) + (k))
@@ -405,8 +411,7 @@ static void zeta_k2advection_k(kr_run *r, double **zeta, double *uk_dc, double *
 
 /* NS_solver_slavedEuler (fluid_solver.cxx:348-374); the kept-mode boxes
  * ijk_range_two_third_filter (fft_wrapper.cxx:172-183). */
-static void ns_solver_slaved_euler(kr_run *r) {
-    zeta_k2advection_k(r, r->zeta, r->uk_dc, r->f_ns0);
+static void slaved_euler_update(kr_run *r) {
     const double dmy0 = -r->NU * r->jikan.dt_fluid;
     const double dmy1 = 1. / r->NU;
     const int    TX = r->TRN_X, TY = r->TRN_Y, TZ = r->TRN_Z;
@@ -423,6 +428,10 @@ static void ns_solver_slaved_euler(kr_run *r) {
                     r->zeta[1][im] += dmy * (r->zeta[1][im] - dmy1 * r->IK2[im] * r->f_ns0[1][im]);
                 }
     }
+}
+static void ns_solver_slaved_euler(kr_run *r) {
+    zeta_k2advection_k(r, r->zeta, r->uk_dc, r->f_ns0);
+    slaved_euler_update(r);
 }
 
 /* Solenoidal_uk (operate_omega.cxx:171-195) */
@@ -694,8 +703,26 @@ static void reset_forces(kr_run *r) {
 
 static int is_obstacle(const kr_run *r, const Particle *p) { return r->spec[p->spec].propulsion == KH_JP_OBSTACLE; }
 
-/* MD_solver_position_Euler / _AB2 (particle_solver.cxx:39-85) with
- * MD_solver_orientation_Euler / _AB2 (particle_rotation_solver.h:23-29, 72-81) */
+/* MD_solver_orientation_Euler / _AB2 (particle_rotation_solver.h:23-29, 72-81) */
+static void orientation_update(const kr_run *r, Particle *p, int ab2) {
+    if (!ab2) {
+        quaternion dqdt;
+        p->q_old = p->q;
+        qdot_space(&dqdt, &p->q, p->omega);
+        qtn_add(&p->q, &dqdt, r->jikan.dt_md);
+        qtn_normalize(&p->q);
+    } else {
+        quaternion dqdt, dqdt_old;
+        qdot_space(&dqdt, &p->q, p->omega);
+        qdot_space(&dqdt_old, &p->q_old, p->omega_old);
+        p->q_old = p->q;
+        qtn_add(&p->q, &dqdt, 3.0 * r->jikan.hdt_md);
+        qtn_add(&p->q, &dqdt_old, -r->jikan.hdt_md);
+        qtn_normalize(&p->q);
+    }
+}
+
+/* MD_solver_position_Euler / _AB2 (particle_solver.cxx:39-85) */
 static void position_update(kr_run *r, int ab2) {
 #pragma omp parallel for
     for (int n = 0; n < r->np; n++) {
@@ -708,23 +735,7 @@ static void position_update(kr_run *r, int ab2) {
             p->x[d] += delta_x;
         }
         for (int d = 0; d < 3; d++) p->x[d] = fmod(p->x[d] + r->L[d], r->L[d]); /* PBC, periodic_boundary.h:9-15 */
-        if (r->P.rotation) {
-            if (!ab2) {
-                quaternion dqdt;
-                p->q_old = p->q;
-                qdot_space(&dqdt, &p->q, p->omega);
-                qtn_add(&p->q, &dqdt, r->jikan.dt_md);
-                qtn_normalize(&p->q);
-            } else {
-                quaternion dqdt, dqdt_old;
-                qdot_space(&dqdt, &p->q, p->omega);
-                qdot_space(&dqdt_old, &p->q_old, p->omega_old);
-                p->q_old = p->q;
-                qtn_add(&p->q, &dqdt, 3.0 * r->jikan.hdt_md);
-                qtn_add(&p->q, &dqdt_old, -r->jikan.hdt_md);
-                qtn_normalize(&p->q);
-            }
-        }
+        if (r->P.rotation) orientation_update(r, p, ab2);
     }
 }
 
@@ -1221,6 +1232,725 @@ static void slip_iteration(kr_run *r) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Lees-Edwards sheared step (SW_EQ == Shear_Navier_Stokes_Lees_Edwards) */
+/* ------------------------------------------------------------------ */
+/* The fluid lives on an oblique grid that follows the mean shear
+ * u_x = shear_rate_eff * (y - LY/2): velocities are contravariant
+ * components on that grid, and degree_oblique is the accumulated strain,
+ * wrapped to (-XYaspect, XYaspect) by a remap of the grid. Spherical,
+ * non-rigid particles; linear interpolation between the grids
+ * (SW_OBL_INT == linear_int). The shear-stress diagnostics (Hydro_force,
+ * Calc_Reynolds_shear_stress, dev_shear_stress_*) are not computed: they
+ * feed output only and change no state. */
+
+/* contra2co_single / co2contra_single (fft_wrapper.h:611-633) */
+static void contra2co_single(double g, double *v) {
+    double a = v[0], b = v[1];
+    v[0]     = a + g * b;
+    v[1]     = g * a + (1. + g * g) * b;
+}
+
+/* Zeta_k2u_k_OBL (fft_wrapper.cxx:690-768): contravariant u from zeta */
+static void zeta_k2u_k_obl(kr_run *r, double **zeta, const double *uk_dc, double **u) {
+    const double g = r->degree_oblique;
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int kk = 0; kk < r->HNZ_; kk++) {
+                size_t im = IM(r, i, j, 2 * kk);
+                double kx = r->WAVE[0] * r->KX[im], ky = r->WAVE[1] * r->KY[im], kz = r->WAVE[2] * r->KZ[im];
+                double ik2 = r->IK2[im];
+                double d1r = zeta[0][im], d1i = zeta[0][im + 1], d2r = zeta[1][im], d2i = zeta[1][im + 1];
+                double ore[3], oim[3];
+                if (r->KX[im] != 0) {
+                    ore[0] = -(1. / kx) * (ky * d1r + kz * d2r);
+                    oim[0] = -(1. / kx) * (ky * d1i + kz * d2i);
+                    ore[1] = d1r;
+                    oim[1] = d1i;
+                    ore[2] = d2r;
+                    oim[2] = d2i;
+                } else if (r->KZ[im] != 0) {
+                    double dmy = -(ky / kz);
+                    ore[0]     = d2r;
+                    oim[0]     = d2i;
+                    ore[1]     = d1r;
+                    oim[1]     = d1i;
+                    ore[2]     = dmy * d1r;
+                    oim[2]     = dmy * d1i;
+                } else {
+                    ore[0] = d1r;
+                    oim[0] = d1i;
+                    ore[1] = 0.0;
+                    oim[1] = 0.0;
+                    ore[2] = d2r;
+                    oim[2] = d2i;
+                }
+                contra2co_single(g, ore);
+                contra2co_single(g, oim);
+                kx *= ik2;
+                ky *= ik2;
+                kz *= ik2;
+                u[0][im]     = -(ky * oim[2] - kz * oim[1]);
+                u[0][im + 1] = (ky * ore[2] - kz * ore[1]);
+                u[1][im]     = -(kz * oim[0] - kx * oim[2]);
+                u[1][im + 1] = (kz * ore[0] - kx * ore[2]);
+                u[2][im]     = -(kx * oim[1] - ky * oim[0]);
+                u[2][im + 1] = (kx * ore[1] - ky * ore[0]);
+            }
+    u[0][0] = uk_dc[0];
+    u[1][0] = uk_dc[1];
+    u[2][0] = uk_dc[2];
+}
+
+/* Zeta_k2omega_k_OBL (fft_wrapper.cxx:820-865): contravariant omega */
+static void zeta_k2omega_k_obl(kr_run *r, double **zeta, double **omega) {
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ_; k++) {
+                size_t im = IM(r, i, j, k);
+                double kx = r->WAVE[0] * r->KX[im], ky = r->WAVE[1] * r->KY[im], kz = r->WAVE[2] * r->KZ[im];
+                double d1 = zeta[0][im], d2 = zeta[1][im];
+                if (r->KX[im] != 0) {
+                    omega[0][im] = -(1. / kx) * (ky * d1 + kz * d2);
+                    omega[1][im] = d1;
+                    omega[2][im] = d2;
+                } else if (r->KZ[im] != 0) {
+                    omega[0][im] = d2;
+                    omega[1][im] = d1;
+                    omega[2][im] = -(ky / kz) * d1;
+                } else {
+                    omega[0][im] = d1;
+                    omega[1][im] = 0.0;
+                    omega[2][im] = d2;
+                }
+            }
+    for (int d = 0; d < 3; d++) omega[d][0] = 0.0;
+}
+
+/* U_k2rotation_k (fft_wrapper.cxx:893-935): u <- i k x u */
+static void u_k2rotation_k(kr_run *r, double **u) {
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int kk = 0; kk < r->HNZ_; kk++) {
+                size_t im    = IM(r, i, j, 2 * kk);
+                double ks[3] = {r->KX[im] * r->WAVE[0], r->KY[im] * r->WAVE[1], r->KZ[im] * r->WAVE[2]};
+                double ur[3], ui[3];
+                for (int d = 0; d < 3; d++) {
+                    ur[d] = u[d][im];
+                    ui[d] = u[d][im + 1];
+                }
+                u[0][im]     = -(ks[1] * ui[2] - ks[2] * ui[1]);
+                u[0][im + 1] = (ks[1] * ur[2] - ks[2] * ur[1]);
+                u[1][im]     = -(ks[2] * ui[0] - ks[0] * ui[2]);
+                u[1][im + 1] = (ks[2] * ur[0] - ks[0] * ur[2]);
+                u[2][im]     = -(ks[0] * ui[1] - ks[1] * ui[0]);
+                u[2][im + 1] = (ks[0] * ur[1] - ks[1] * ur[0]);
+            }
+}
+
+/* Zeta_k2advection_k_OBL (operate_omega.cxx:100-155) with Zeta_k2u_cpuky
+ * (operate_omega.h:153-157), Zeta_k2omega_OBL (191-197) and
+ * Omega_k2zeta_k_OBL (fft_wrapper.cxx:265-289) */
+static void zeta_k2advection_k_obl(kr_run *r, double **zeta, double *uk_dc, double **adv) {
+    truncate_two_third(r, zeta[0]);
+    truncate_two_third(r, zeta[1]);
+    double **u = r->u, *w1 = r->w1, **w3 = r->w3;
+    zeta_k2u_k_obl(r, zeta, uk_dc, u);
+    memcpy(w1, u[1], sizeof(double) * r->M);
+    for (int d = 0; d < 3; d++) A_k2a(r, u[d]);
+    zeta_k2omega_k_obl(r, zeta, w3);
+    for (int d = 0; d < 3; d++) A_k2a(r, w3[d]);
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ; k++) {
+                size_t im = IM(r, i, j, k);
+                double u1 = u[0][im], u2 = u[1][im], u3 = u[2][im];
+                double o1 = w3[0][im], o2 = w3[1][im], o3 = w3[2][im];
+                u[0][im]  = u2 * o3 - u3 * o2;
+                u[1][im]  = u3 * o1 - u1 * o3;
+                u[2][im]  = u1 * o2 - u2 * o1;
+            }
+    for (int d = 0; d < 3; d++) A2a_k(r, u[d]);
+    const double s = r->shear_rate_eff, g = r->degree_oblique;
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ_; k++) {
+                size_t im = IM(r, i, j, k);
+                u[0][im] -= 2. * s * w1[im];
+                u[1][im] -= 2. * s * g * w1[im];
+            }
+    u_k2rotation_k(r, u);
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ_; k++) {
+                size_t im = IM(r, i, j, k);
+                if (r->KX[im] != 0) {
+                    adv[0][im] = u[1][im];
+                    adv[1][im] = u[2][im];
+                } else if (r->KZ[im] != 0) {
+                    adv[0][im] = u[1][im];
+                    adv[1][im] = u[0][im];
+                } else {
+                    adv[0][im] = u[0][im];
+                    adv[1][im] = u[2][im];
+                }
+            }
+}
+
+/* NS_solver_slavedEuler_Shear_OBL (fluid_solver.cxx:411-437) */
+static void ns_solver_obl(kr_run *r) {
+    zeta_k2advection_k_obl(r, r->zeta, r->uk_dc, r->f_ns0);
+    slaved_euler_update(r);
+}
+
+/* Update_K2_OBL (fluid_solver.h:342-360) */
+static void update_k2_obl(kr_run *r) {
+    const double g = r->degree_oblique;
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ_; k++) {
+                size_t im = IM(r, i, j, k);
+                r->K2[im] = SQ(r->WAVE[0] * r->KX[im]) + SQ(r->WAVE[1] * r->KY[im] - r->WAVE[0] * r->KX[im] * g) +
+                            SQ(r->WAVE[2] * r->KZ[im]);
+                r->IK2[im] = (r->K2[im] > 0.0) ? 1.0 / r->K2[im] : 0.0;
+            }
+}
+
+/* Reset_U_OBL (fluid_solver.h:312-340): the oblique grid at strain
+ * +-XYaspect re-expressed on the grid at strain 0 (flg 0: from +XYaspect). */
+static void reset_u_obl(kr_run *r, double **u, double **ucp, int flg) {
+    const double delta = (flg == 0 ? r->XYaspect : -r->XYaspect);
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++) {
+            double sign = j - r->NY / 2;
+            if (!(sign == 0)) sign = sign / fabs(sign);
+            int i_oblique = (int)(sign * (j - r->NY / 2)) * sign;
+            i_oblique     = (int)fmod(i + delta * i_oblique + 4. * r->NX, r->NX);
+            for (int k = 0; k < r->NZ; k++) {
+                size_t im = IM(r, i, j, k), io = IM(r, i_oblique, j, k);
+                u[0][io]  = ucp[0][im] + delta * ucp[1][im];
+                u[1][io]  = ucp[1][im];
+                u[2][io]  = ucp[2][im];
+            }
+        }
+}
+
+/* obl2orth (fft_wrapper.h:276-288) */
+static void obl2orth(const kr_run *r, int j, int i, int *i_plus, int *i_oblique, double *alpha, double *beta) {
+    const double g       = r->degree_oblique;
+    int          delta_j = j - r->NY / 2;
+    double       sign    = (g >= 0.0 ? (double)delta_j : -(double)delta_j);
+    if (!(delta_j == 0)) sign = sign / fabs(sign);
+    int io     = (int)((int)(sign * g * delta_j) * sign + sign);
+    *alpha     = (io - g * delta_j) * sign;
+    *beta      = 1.0 - *alpha;
+    *i_oblique = (int)fmod(i + io + 4.0 * r->NX, r->NX);
+    *i_plus    = (int)fmod(i + sign + 2.0 * r->NX, r->NX);
+}
+/* orth2obl (fft_wrapper.h:262-274) */
+static void orth2obl(const kr_run *r, int j, int i, int *i_oblique, int *i_oblique_plus, double *alpha, double *beta) {
+    const double g       = r->degree_oblique;
+    int          delta_j = j - r->NY / 2;
+    double       sign    = (g >= 0.0 ? (double)delta_j : -(double)delta_j);
+    if (!(delta_j == 0)) sign = sign / fabs(sign);
+    int io           = (int)((int)(sign * g * delta_j) * sign);
+    *alpha           = (g * delta_j - io) * sign;
+    *beta            = 1.0 - *alpha;
+    io               = (int)fmod(i + io + 4.0 * r->NX, r->NX);
+    *i_oblique       = io;
+    *i_oblique_plus  = (int)fmod(io + sign + 4.0 * r->NX, r->NX);
+}
+
+/* U_oblique2u (fft_wrapper.h:480-505): oblique grid and contravariant
+ * components -> Cartesian grid and components, plus the mean shear flow */
+static void u_oblique2u(kr_run *r, double **uu, int add_mean_flow) {
+    double **w = r->w3;
+    for (int d = 0; d < 3; d++) memcpy(w[d], uu[d], sizeof(double) * r->M);
+    const double g = r->degree_oblique, s = r->shear_rate_eff;
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++) {
+            int    i_plus, i_oblique;
+            double alpha, beta;
+            obl2orth(r, j, i, &i_plus, &i_oblique, &alpha, &beta);
+            for (int k = 0; k < r->NZ; k++) {
+                size_t im = IM(r, i, j, k), io = IM(r, i_oblique, j, k), ip = IM(r, i_plus, j, k);
+                for (int d = 0; d < 3; d++) uu[d][io] = beta * w[d][im] + alpha * w[d][ip];
+                uu[0][io] += (g * uu[1][io]);
+                if (add_mean_flow) uu[0][io] += s * (j - r->NY / 2.0);
+            }
+        }
+}
+
+/* U2u_oblique (fft_wrapper.h:448-478): Cartesian -> oblique (the mean
+ * shear flow is not removed) */
+static void u2u_oblique(kr_run *r, double **uu) {
+    double **w = r->w3;
+    for (int d = 0; d < 3; d++) memcpy(w[d], uu[d], sizeof(double) * r->M);
+    const double g = r->degree_oblique;
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++) {
+            int    i_oblique, i_oblique_plus;
+            double alpha, beta;
+            orth2obl(r, j, i, &i_oblique, &i_oblique_plus, &alpha, &beta);
+            for (int k = 0; k < r->NZ; k++) {
+                size_t im = IM(r, i, j, k), io = IM(r, i_oblique, j, k), ip = IM(r, i_oblique_plus, j, k);
+                for (int d = 0; d < 3; d++) uu[d][im] = beta * w[d][io] + alpha * w[d][ip];
+                uu[0][im] -= (g * uu[1][im]);
+            }
+        }
+}
+
+/* adj (fdm.h:52) */
+static int adj(int x, int i, int n) { return ((i + x) < 0) ? i + x + n : (((i + x) > n - 1) ? i + x - n : i + x); }
+
+/* Calc_shear_rate_eff (sp_3d_ns.h:189-203) with Calc_local_gradient_y_OBL
+ * (sp_3d_ns.h:166-187), summed serially in (i, j, k) order */
+static void calc_shear_rate_eff(kr_run *r) {
+    const double  g = r->degree_oblique, inv_2dx = 1. / (2. * r->DX);
+    const double *f = r->u[0];
+    double        s = 0.0;
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ; k++) {
+                int    ip1 = adj(1, i, r->NX), jp1 = adj(1, j, r->NY), im1 = adj(-1, i, r->NX), jm1 = adj(-1, j, r->NY);
+                double fp1 = 0.5 * ((1 - g) * f[IM(r, ip1, jp1, k)] + (1 + g) * f[IM(r, im1, jp1, k)]);
+                double fm1 = 0.5 * ((1 - g) * f[IM(r, im1, jm1, k)] + (1 + g) * f[IM(r, ip1, jm1, k)]);
+                s += (fp1 - fm1) * inv_2dx;
+            }
+    s *= r->Ivolume * r->DX3;
+    r->shear_rate_eff += s;
+}
+
+/* Solenoidal_uk_OBL (operate_omega.cxx:197-227) */
+static void solenoidal_uk_obl(kr_run *r, double **u) {
+    const double g = r->degree_oblique;
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ_; k++) {
+                size_t im = IM(r, i, j, k);
+                double kx = r->KX[im] * r->WAVE[0], ky = r->KY[im] * r->WAVE[1], kz = r->KZ[im] * r->WAVE[2];
+                double kxc = (1. + g * g) * r->KX[im] * r->WAVE[0] - g * r->KY[im] * r->WAVE[1];
+                double kyc = -g * r->KX[im] * r->WAVE[0] + r->KY[im] * r->WAVE[1];
+                double dmy = r->IK2[im] * (u[0][im] * kx + u[1][im] * ky + u[2][im] * kz);
+                u[0][im] -= dmy * kxc;
+                u[1][im] -= dmy * kyc;
+                u[2][im] -= dmy * kz;
+            }
+}
+
+/* U_k2zeta_k_OBL (fft_wrapper.cxx:526-589) */
+static void u_k2zeta_k_obl(kr_run *r, double **u, double **zeta, double *uk_dc) {
+    const double g = r->degree_oblique;
+    uk_dc[0]       = u[0][0];
+    uk_dc[1]       = u[1][0];
+    uk_dc[2]       = u[2][0];
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int kk = 0; kk < r->HNZ_; kk++) {
+                size_t im    = IM(r, i, j, 2 * kk);
+                double ks[3] = {r->KX[im] * r->WAVE[0], r->KY[im] * r->WAVE[1], r->KZ[im] * r->WAVE[2]};
+                double ur[3] = {u[0][im], u[1][im], u[2][im]}, ui[3] = {u[0][im + 1], u[1][im + 1], u[2][im + 1]};
+                contra2co_single(g, ur);
+                contra2co_single(g, ui);
+                double ore[3], oim[3];
+                ore[0] = -(ks[1] * ui[2] - ks[2] * ui[1]);
+                oim[0] = (ks[1] * ur[2] - ks[2] * ur[1]);
+                ore[1] = -(ks[2] * ui[0] - ks[0] * ui[2]);
+                oim[1] = (ks[2] * ur[0] - ks[0] * ur[2]);
+                ore[2] = -(ks[0] * ui[1] - ks[1] * ui[0]);
+                oim[2] = (ks[0] * ur[1] - ks[1] * ur[0]);
+                int a0, a1;
+                if (r->KX[im] != 0) {
+                    a0 = 1;
+                    a1 = 2;
+                } else if (r->KZ[im] != 0) {
+                    a0 = 1;
+                    a1 = 0;
+                } else {
+                    a0 = 0;
+                    a1 = 2;
+                }
+                zeta[0][im]     = ore[a0];
+                zeta[0][im + 1] = oim[a0];
+                zeta[1][im]     = ore[a1];
+                zeta[1][im + 1] = oim[a1];
+            }
+}
+
+/* PBC_OBL (periodic_boundary.h:26-44): returns the Y crossing (+1 up, -1 down) */
+static int pbc_obl(const kr_run *r, double *x, double *delta_vx) {
+    double signY = x[1];
+    x[1]         = fmod(x[1] + r->L[1], r->L[1]);
+    signY -= x[1];
+    int sign = (int)signY;
+    if (!(sign == 0)) sign = sign / abs(sign);
+    x[0] -= (double)sign * r->degree_oblique * r->L[1];
+    x[0]      = fmod(x[0] + r->L[0], r->L[0]);
+    x[2]      = fmod(x[2] + r->L[2], r->L[2]);
+    *delta_vx = -(double)sign * r->shear_rate_eff * r->L[1];
+    return sign;
+}
+
+/* Distance0_OBL (interaction.h:50-72) */
+static void distance0_obl(const kr_run *r, const double *x1, const double *x2, double *r12, double *x12) {
+    double dmy   = 0.0;
+    double signY = x2[1] - x1[1];
+    x12[1]       = x2[1] - x1[1];
+    x12[1] -= (double)Nint(x12[1] / r->L[1]) * r->L[1];
+    signY -= x12[1];
+    int sign = (int)signY;
+    if (!(sign == 0)) sign = sign / abs(sign);
+    dmy += SQ(x12[1]);
+    x12[0] = x2[0] - (x1[0] + (double)sign * r->degree_oblique * r->L[1]);
+    x12[0] -= (double)Nint(x12[0] / r->L[0]) * r->L[0];
+    dmy += SQ(x12[0]);
+    x12[2] = x2[2] - x1[2];
+    x12[2] -= (double)Nint(x12[2] / r->L[2]) * r->L[2];
+    dmy += SQ(x12[2]);
+    *r12 = sqrt(dmy);
+}
+static double distance_obl(const kr_run *r, const double *x1, const double *x2) {
+    double d, v[3];
+    distance0_obl(r, x1, x2, &d, v);
+    return d;
+}
+
+/* Relative_coord_check_stepover_Y (make_phi.h:229-268): the Sekibun point
+ * on the oblique grid; returns the Y crossing of the point */
+static int relative_coord_obl(const kr_run *r, const int *cell, const int *x_int, const double *residue, int *r_mesh,
+                              double *rv) {
+    const int N[3]  = {r->NX, r->NY, r->NZ};
+    double    signY = x_int[1] + cell[1];
+    r_mesh[1]       = (x_int[1] + cell[1] + N[1]) % N[1];
+    signY -= r_mesh[1];
+    int sign = (int)signY;
+    if (!(sign == 0)) sign = sign / abs(sign);
+    double xo = fmod(x_int[0] - sign * r->degree_oblique * N[1] + residue[0] / r->DX + cell[0] + 4 * N[0], N[0]);
+    r_mesh[0] = (int)xo;
+    double xr = xo - r_mesh[0];
+    r_mesh[2] = (x_int[2] + cell[2] + N[2]) % N[2];
+    rv[0]     = (double)cell[0] * r->DX - xr * r->DX;
+    rv[1]     = (double)cell[1] * r->DX - residue[1];
+    rv[2]     = (double)cell[2] * r->DX - residue[2];
+    return sign;
+}
+
+/* Make_phi_particle_sum_OBL (make_phi.cxx:262-311, 498-502) */
+static void make_phi_particle_sum_obl(kr_run *r, double *phi, double *phi_sum) {
+    for (int n = 0; n < r->np; n++) {
+        double xp[3] = {r->p[n].x[0], r->p[n].x[1], r->p[n].x[2]};
+        int    x_int[3];
+        double residue[3];
+        particle_cell(r, xp, x_int, residue);
+        for (int mesh = 0; mesh < r->NP_domain; mesh++) {
+            int    r_mesh[3];
+            double rv[3], x[3];
+            relative_coord_obl(r, r->sek + 3 * mesh, x_int, residue, r_mesh, rv);
+            for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * r->DX;
+            phi_sum[IM(r, r_mesh[0], r_mesh[1], r_mesh[2])] += Phi(r, distance_obl(r, x, xp));
+        }
+    }
+#pragma omp parallel for
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++)
+            for (int k = 0; k < r->NZ; k++) {
+                size_t im = IM(r, i, j, k);
+                phi[im]   = fmin(phi_sum[im], 1.0);
+            }
+}
+
+/* Make_u_particle_sum_OBL (make_phi.cxx:356-402, 510-514): a point across
+ * the Y boundary sees the particle's velocity shifted by the shear jump */
+static void make_u_particle_sum_obl(kr_run *r, double **up, const double *phi_sum) {
+    const double jump = r->shear_rate_eff * r->L[1];
+    for (int n = 0; n < r->np; n++) {
+        const Particle *pn = &r->p[n];
+        double          xp[3], vp[3], om[3];
+        for (int d = 0; d < 3; d++) {
+            xp[d] = pn->x[d];
+            vp[d] = pn->v[d];
+            om[d] = pn->omega[d];
+        }
+        int    x_int[3];
+        double residue[3];
+        particle_cell(r, xp, x_int, residue);
+        for (int mesh = 0; mesh < r->NP_domain; mesh++) {
+            int    r_mesh[3];
+            double rv[3], x[3], v_rot[3];
+            int    sign = relative_coord_obl(r, r->sek + 3 * mesh, x_int, residue, r_mesh, rv);
+            for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * r->DX;
+            size_t im      = IM(r, r_mesh[0], r_mesh[1], r_mesh[2]);
+            double dmy_phi = Phi(r, distance_obl(r, x, xp)) / fmax(phi_sum[im], 1.0);
+            angular2v(r, om, rv, v_rot);
+            up[0][im] += ((vp[0] - sign * jump + v_rot[0]) * dmy_phi);
+            up[1][im] += (vp[1] + v_rot[1]) * dmy_phi;
+            up[2][im] += (vp[2] + v_rot[2]) * dmy_phi;
+        }
+    }
+}
+
+/* Calc_f_hydro_correct_precision_OBL (md_force.cxx:508-645), non-rigid;
+ * the distance is |r| of the stepover-corrected relative coordinate */
+static void calc_f_hydro_obl(kr_run *r, const double *phi_sum, double **u) {
+    const double dmy = (-r->DX3 * r->RHO) / r->jikan.dt_fluid, jump = r->shear_rate_eff * r->L[1];
+#pragma omp parallel for
+    for (int n = 0; n < r->np; n++) {
+        Particle *pn = &r->p[n];
+        double    xp[3], vp[3], om[3], force[3] = {0, 0, 0}, torque[3] = {0, 0, 0};
+        for (int d = 0; d < 3; d++) {
+            xp[d] = pn->x[d];
+            vp[d] = pn->v[d];
+            om[d] = pn->omega[d];
+        }
+        int    x_int[3];
+        double residue[3];
+        particle_cell(r, xp, x_int, residue);
+        for (int mesh = 0; mesh < r->NP_domain; mesh++) {
+            int    r_mesh[3];
+            double rv[3], v_rot[3], fp[3];
+            int    sign = relative_coord_obl(r, r->sek + 3 * mesh, x_int, residue, r_mesh, rv);
+            double dmyR = sqrt(SQ(rv[0]) + SQ(rv[1]) + SQ(rv[2]));
+            size_t im   = IM(r, r_mesh[0], r_mesh[1], r_mesh[2]);
+            double dphi = Phi(r, dmyR) / fmax(phi_sum[im], 1.0);
+            angular2v(r, om, rv, v_rot);
+            fp[0] = ((vp[0] - sign * jump + v_rot[0]) - u[0][im]) * dphi;
+            fp[1] = ((vp[1] + v_rot[1]) - u[1][im]) * dphi;
+            fp[2] = ((vp[2] + v_rot[2]) - u[2][im]) * dphi;
+            for (int d = 0; d < 3; d++) force[d] += fp[d];
+            torque[0] += (rv[1] * fp[2] - rv[2] * fp[1]);
+            torque[1] += (rv[2] * fp[0] - rv[0] * fp[2]);
+            torque[2] += (rv[0] * fp[1] - rv[1] * fp[0]);
+        }
+        for (int d = 0; d < 3; d++) pn->f_hydro[d] = (dmy * force[d]);
+        if (r->P.rotation)
+            for (int d = 0; d < 3; d++) pn->torque_hydro[d] = (dmy * torque[d]);
+    }
+}
+
+/* Calc_f_Lennard_Jones_shear_cap_primitive (md_force.cxx:198-305) with
+ * Distance0_OBL and cap = DBL_MAX (md_force.h:82-84): pairs m > n, each
+ * particle's own sum added after its row */
+static void calc_f_lennard_jones_obl(kr_run *r) {
+    const double pair_cutoff = r->A_R_cutoff * r->LJ_dia;
+    for (int n = 0; n < r->np; n++) {
+        Particle *pn     = &r->p[n];
+        double    sum[3] = {0, 0, 0};
+        for (int m = n + 1; m < r->np; m++) {
+            Particle *pm = &r->p[m];
+            if (is_obstacle(r, pn) && is_obstacle(r, pm)) continue;
+            double rij, rv[3];
+            distance0_obl(r, pn->x, pm->x, &rij, rv);
+            if (rij < pair_cutoff) {
+                double dmy_r = fmin(DBL_MAX / rij, lennard_jones_f(r, rij));
+                for (int d = 0; d < 3; d++) {
+                    double fn = dmy_r * (-rv[d]);
+                    pm->fr[d] -= fn;
+                    sum[d] += fn;
+                }
+            }
+        }
+        for (int d = 0; d < 3; d++) pn->fr[d] += sum[d];
+    }
+}
+
+/* MD_solver_position_Euler_OBL / _AB2_OBL (particle_solver.cxx:283-334) */
+static void position_update_obl(kr_run *r, int ab2) {
+#pragma omp parallel for
+    for (int n = 0; n < r->np; n++) {
+        Particle *p = &r->p[n];
+        if (is_obstacle(r, p)) continue;
+        for (int d = 0; d < 3; d++) {
+            p->x_previous[d] = p->x[d];
+            double delta_x   = ab2 ? r->jikan.hdt_md * (3.0 * p->v[d] - p->v_old[d]) : r->jikan.dt_md * p->v[d];
+            p->x_nopbc[d] += delta_x;
+            p->x[d] += delta_x;
+        }
+        double delta_vx;
+        pbc_obl(r, p->x, &delta_vx);
+        p->v[0] += delta_vx;
+        p->v_old[0] += delta_vx;
+        if (r->P.rotation) orientation_update(r, p, ab2);
+    }
+}
+
+/* MD_solver_velocity_Euler_OBL / _AB2_hydro_OBL (particle_solver.cxx:336-432)
+ * with Force_OBL (particle_solver.h:113-126) */
+static void velocity_update_obl(kr_run *r, int ab2) {
+    if (r->P.lj_truncate >= 0) calc_f_lennard_jones_obl(r);
+    if (r->P.g != 0.0) add_f_gravity(r);
+#pragma omp parallel for
+    for (int n = 0; n < r->np; n++) {
+        Particle *p    = &r->p[n];
+        int       s    = p->spec;
+        double    h    = ab2 ? r->jikan.hdt_md : r->jikan.dt_md;
+        double    dmy  = h * r->IMASS[s];
+        double    dmyr = h * r->IMOI[s];
+        if (!is_obstacle(r, p)) {
+            double sf[3], st[3];
+            self_propulsion(r, p, sf, st);
+            for (int d = 0; d < 3; d++) {
+                p->v_old[d]     = p->v[d];
+                p->omega_old[d] = p->omega[d];
+                if (!ab2) {
+                    p->v[d] += (dmy * (p->f_hydro[d] + p->fr[d] + sf[d]));
+                    p->omega[d] += (dmyr * (p->torque_hydro[d] + p->torque_r[d] + st[d]));
+                } else {
+                    p->v[d] += dmy * (2.0 * (p->f_hydro[d] + sf[d]) + p->fr[d] + p->fr_previous[d]);
+                    p->omega[d] +=
+                        dmyr * (2.0 * (p->torque_hydro[d] + st[d]) + p->torque_r[d] + p->torque_r_previous[d]);
+                }
+            }
+        } else {
+            for (int d = 0; d < 3; d++) {
+                p->v_old[d]     = p->v[d];
+                p->v[d]         = 0.0;
+                p->omega_old[d] = p->omega[d];
+                p->omega[d]     = 0.0;
+            }
+        }
+        for (int d = 0; d < 3; d++)
+            p->momentum_depend_fr[d] = ab2 ? h * (p->fr[d] + p->fr_previous[d]) : h * p->fr[d];
+    }
+    reset_forces(r);
+}
+
+/* Time_evolution_hydro_OBL (sp_3d_ns.cxx:328-433) */
+static void time_evolution_hydro_obl(kr_run *r) {
+    ns_solver_obl(r);
+    if (r->fix_cell_any)
+        for (int d = 0; d < 3; d++)
+            if (r->P.fix_cell[d]) r->uk_dc[d] = 0.0;
+    zeta_k2u_k_obl(r, r->zeta, r->uk_dc, r->u);
+    for (int d = 0; d < 3; d++) A_k2a(r, r->u[d]);
+    if (!r->le_ac)
+        r->shear_rate_eff = r->shear_rate;
+    else
+        r->shear_rate_eff = r->shear_rate * cos(2. * KR_PI * r->shear_freq * r->jikan.dt_fluid * r->jikan.ts);
+    r->degree_oblique += r->shear_rate_eff * r->jikan.dt_fluid;
+    if (r->degree_oblique >= r->XYaspect || r->degree_oblique <= -r->XYaspect) {
+        int flg = r->degree_oblique >= r->XYaspect ? 0 : 1;
+        reset_u_obl(r, r->ucp, r->u, flg);
+        for (int d = 0; d < 3; d++) {
+            double *t = r->u[d];
+            r->u[d]   = r->ucp[d];
+            r->ucp[d] = t;
+        }
+        r->degree_oblique += flg == 0 ? -r->XYaspect : r->XYaspect;
+    }
+    update_k2_obl(r);
+    for (int d = 0; d < 3; d++) memcpy(r->ucp[d], r->u[d], sizeof(double) * r->M);
+    u_oblique2u(r, r->ucp, 1);
+    if (!r->le_ac) calc_shear_rate_eff(r);
+
+    position_update_obl(r, r->jikan.ts != 0);
+    reset_field(r, r->phi);
+    reset_field(r, r->phi_sum);
+    for (int d = 0; d < 3; d++) reset_field(r, r->up[d]);
+    make_phi_particle_sum_obl(r, r->phi, r->phi_sum);
+    calc_f_hydro_obl(r, r->phi_sum, r->ucp);
+    velocity_update_obl(r, r->jikan.ts != 0);
+    if (r->kBT > 0.) thermostat(r);
+    for (int d = 0; d < 3; d++) reset_field(r, r->up[d]);
+    make_u_particle_sum_obl(r, r->up, r->phi_sum);
+    make_f_particle_nonsole(r, r->f, r->ucp, r->up, r->phi);
+    u2u_oblique(r, r->f);
+    add_f_particle(r, r->u, r->f);
+    for (int d = 0; d < 3; d++) A2a_k(r, r->u[d]);
+    solenoidal_uk_obl(r, r->u);
+    u_k2zeta_k_obl(r, r->u, r->zeta, r->uk_dc);
+}
+
+int kr_set_shear(kr_run *r, double shear_rate, int ac, double frequency) {
+    if (!r) return -1;
+    if (!r->le) {
+        for (int d = 0; d < 3; d++) {
+            r->ucp[d] = (double *)calloc(r->M, sizeof(double));
+            r->w3[d]  = (double *)calloc(r->M, sizeof(double));
+        }
+        r->w1 = (double *)calloc(r->M, sizeof(double));
+    }
+    r->le         = 1;
+    r->le_ac      = ac ? 1 : 0;
+    r->shear_rate = shear_rate;
+    r->shear_freq = frequency;
+    r->XYaspect   = (double)r->NX / (double)r->NY; /* sp_3d_ns.cxx:772 */
+    r->Ivolume    = 1. / (r->L[0] * r->L[1] * r->L[2]);
+    /* input.cxx:549-566: the sheared runs' LJ diameter */
+    double dummy_pow = (r->P.lj_powers == 0) ? pow(2., 1. / 6.) : (r->P.lj_powers == 1) ? pow(2., 1. / 12.)
+                                                                                        : pow(2., 1. / 18.);
+    r->LJ_dia = fmin((r->SIGMA + r->XI) / dummy_pow, r->SIGMA);
+    update_k2_obl(r);
+    return 0;
+}
+void kr_get_shear_state(kr_run *r, double *degree_oblique, double *shear_rate_eff) {
+    *degree_oblique = r->degree_oblique;
+    *shear_rate_eff = r->shear_rate_eff;
+}
+/* resume.cxx:566-580: the strain and the K2 it implies */
+void kr_set_shear_state(kr_run *r, double degree_oblique, double shear_rate_eff) {
+    r->degree_oblique = degree_oblique;
+    r->shear_rate_eff = shear_rate_eff;
+    update_k2_obl(r);
+}
+
+int kr_le_apply(kr_run *r, int fn) {
+    if (!r->le) return -1;
+    switch (fn) {
+        case KR_LE_OBLIQUE2U: u_oblique2u(r, r->u, 1); break;
+        case KR_LE_U2OBLIQUE: u2u_oblique(r, r->u); break;
+        case KR_LE_RESET_UP:
+        case KR_LE_RESET_DOWN:
+            reset_u_obl(r, r->ucp, r->u, fn == KR_LE_RESET_UP ? 0 : 1);
+            for (int d = 0; d < 3; d++) {
+                double *t = r->u[d];
+                r->u[d]   = r->ucp[d];
+                r->ucp[d] = t;
+            }
+            break;
+        case KR_LE_ZETA_K2U:
+            zeta_k2u_k_obl(r, r->zeta, r->uk_dc, r->u);
+            for (int d = 0; d < 3; d++) A_k2a(r, r->u[d]);
+            break;
+        case KR_LE_U2ZETA_K:
+            for (int d = 0; d < 3; d++) A2a_k(r, r->u[d]);
+            u_k2zeta_k_obl(r, r->u, r->zeta, r->uk_dc);
+            break;
+        case KR_LE_SOLENOIDAL_U:
+            for (int d = 0; d < 3; d++) A2a_k(r, r->u[d]);
+            solenoidal_uk_obl(r, r->u);
+            for (int d = 0; d < 3; d++) A_k2a(r, r->u[d]);
+            break;
+        case KR_LE_NS_SOLVER: ns_solver_obl(r); break;
+        case KR_LE_SHEAR_RATE_EFF: calc_shear_rate_eff(r); break;
+        case KR_LE_POSITION_EULER: position_update_obl(r, 0); break;
+        case KR_LE_MAKE_PHI:
+            reset_field(r, r->phi);
+            reset_field(r, r->phi_sum);
+            make_phi_particle_sum_obl(r, r->phi, r->phi_sum);
+            break;
+        case KR_LE_MAKE_UP:
+            for (int d = 0; d < 3; d++) reset_field(r, r->up[d]);
+            make_u_particle_sum_obl(r, r->up, r->phi_sum);
+            break;
+        case KR_LE_F_HYDRO: calc_f_hydro_obl(r, r->phi_sum, r->u); break;
+        case KR_LE_LJ: calc_f_lennard_jones_obl(r); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* the step and its initial block                                       */
 /* ------------------------------------------------------------------ */
 /* Time_evolution_hydro (sp_3d_ns.cxx:41-194), Navier_Stokes branch */
@@ -1253,7 +1983,10 @@ static void time_evolution_hydro(kr_run *r) {
 
 void kr_step(kr_run *r, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
-        time_evolution_hydro(r);
+        if (r->le)
+            time_evolution_hydro_obl(r);
+        else
+            time_evolution_hydro(r);
         r->jikan.time += r->jikan.dt_fluid;
         r->jikan.ts++;
     }
@@ -1410,6 +2143,11 @@ void kr_destroy(kr_run *r) {
     double *all[] = {r->zeta[0], r->zeta[1], r->u[0], r->u[1], r->u[2], r->up[0], r->up[1], r->up[2], r->f[0],
                      r->f[1],    r->f[2],    r->phi,  r->phi_sum, r->f_ns0[0], r->f_ns0[1], r->K2, r->IK2};
     for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); a++) free(all[a]);
+    for (int d = 0; d < 3; d++) {
+        free(r->ucp[d]);
+        free(r->w3[d]);
+    }
+    free(r->w1);
     free(r->KX);
     free(r->KY);
     free(r->KZ);

@@ -37,6 +37,32 @@ int  kr_get_field(kr_run *r, int field, double *out);
 int  kr_set_field(kr_run *r, int field, const double *in);
 
 void kr_init_fluid(kr_run *r);
+
+/* Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards): after
+ * kr_set_shear, kr_step runs Time_evolution_hydro_OBL. DC shear (ac = 0)
+ * at shear_rate, or AC shear_rate*cos(2 pi frequency t). The strain
+ * degree_oblique and the effective rate shear_rate_eff are run state. */
+int  kr_set_shear(kr_run *r, double shear_rate, int ac, double frequency);
+void kr_get_shear_state(kr_run *r, double *degree_oblique, double *shear_rate_eff);
+void kr_set_shear_state(kr_run *r, double degree_oblique, double shear_rate_eff);
+/* single functions of the sheared step, on the run's state */
+enum {
+    KR_LE_OBLIQUE2U = 0, /* U_oblique2u(u), mean flow added     fft_wrapper.h:480-505 */
+    KR_LE_U2OBLIQUE,     /* U2u_oblique(u)                      fft_wrapper.h:448-478 */
+    KR_LE_RESET_UP,      /* Reset_U_OBL(flag 0) + Swap_mem      fluid_solver.h:312-340 */
+    KR_LE_RESET_DOWN,    /* Reset_U_OBL(flag 1) + Swap_mem */
+    KR_LE_ZETA_K2U,      /* Zeta_k2u_k_OBL + U_k2u              fft_wrapper.cxx:690-768 */
+    KR_LE_U2ZETA_K,      /* U2u_k + U_k2zeta_k_OBL              fft_wrapper.cxx:526-589 */
+    KR_LE_SOLENOIDAL_U,  /* U2u_k + Solenoidal_uk_OBL + U_k2u   operate_omega.cxx:197-227 */
+    KR_LE_NS_SOLVER,     /* NS_solver_slavedEuler_Shear_OBL     fluid_solver.cxx:411-437 */
+    KR_LE_SHEAR_RATE_EFF,/* Calc_shear_rate_eff                 sp_3d_ns.h:189-203 */
+    KR_LE_POSITION_EULER,/* MD_solver_position_Euler_OBL       particle_solver.cxx:283-307 */
+    KR_LE_MAKE_PHI,      /* Reset_phi x2 + Make_phi_particle_sum_OBL  make_phi.cxx:262-311 */
+    KR_LE_MAKE_UP,       /* Reset_u + Make_u_particle_sum_OBL   make_phi.cxx:356-402 */
+    KR_LE_F_HYDRO,       /* Calc_f_hydro_correct_precision_OBL on u  md_force.cxx:508-645 */
+    KR_LE_LJ             /* Calc_f_Lennard_Jones_OBL (adds to fr)     md_force.cxx:198-305 */
+};
+int kr_le_apply(kr_run *r, int fn);
 void kr_step(kr_run *r, int nsteps);
 int  kr_apply(kr_run *r, int fn);
 

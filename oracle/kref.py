@@ -49,8 +49,20 @@ def lib():
         L.kr_philox4x32_10.argtypes = [vp, vp, vp]
         L.kr_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, vp]
         L.kr_set_threads.argtypes = [C.c_int]
+        L.kr_set_shear.argtypes = [vp, C.c_double, C.c_int, C.c_double]
+        L.kr_get_shear_state.argtypes = [vp, vp, vp]
+        L.kr_get_shear_state.restype = None
+        L.kr_set_shear_state.argtypes = [vp, C.c_double, C.c_double]
+        L.kr_set_shear_state.restype = None
+        L.kr_le_apply.argtypes = [vp, C.c_int]
         _LIB = L
     return _LIB
+
+
+# single functions of the sheared step (enum in kref.h)
+LE_FN = {name: i for i, name in enumerate([
+    "oblique2u", "u2oblique", "reset_up", "reset_down", "zeta_k2u", "u2zeta_k", "solenoidal_u",
+    "ns_solver", "shear_rate_eff", "position_euler", "make_phi", "make_up", "f_hydro", "lj"])}
 
 
 def fft_forward(a):
@@ -159,6 +171,23 @@ class Run:
 
     def apply(self, fn):
         if lib().kr_apply(self.h, FN[fn]) != 0:
+            raise ValueError(fn)
+
+    def set_shear(self, rate, ac=False, frequency=0.0):
+        """Switch to the Lees-Edwards step at this shear rate."""
+        if lib().kr_set_shear(self.h, float(rate), int(bool(ac)), float(frequency)) != 0:
+            raise ValueError("kr_set_shear failed")
+
+    def shear_state(self):
+        g, s = C.c_double(), C.c_double()
+        lib().kr_get_shear_state(self.h, C.byref(g), C.byref(s))
+        return g.value, s.value
+
+    def set_shear_state(self, degree_oblique, shear_rate_eff):
+        lib().kr_set_shear_state(self.h, float(degree_oblique), float(shear_rate_eff))
+
+    def le_apply(self, fn):
+        if lib().kr_le_apply(self.h, LE_FN[fn]) != 0:
             raise ValueError(fn)
 
     def A2a_k(self, a):

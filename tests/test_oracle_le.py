@@ -1,0 +1,430 @@
+"""Pins of the oracle's Lees-Edwards step (Time_evolution_hydro_OBL,
+sp_3d_ns.cxx:328-433) without running the reference:
+
+- at zero shear it is the plain step (the oblique frame and the rotational
+  form of the advection term change nothing when the grid is not sheared);
+- the grid remap at unit strain (Reset_U_OBL) leaves the Cartesian field
+  unchanged, exactly;
+- the oblique <-> Cartesian interpolation matches a restatement from its
+  geometric definition (a Cartesian point (x, y) sits at oblique abscissa
+  x - strain*(y - LY/2));
+- a 2D shearing wave (Kelvin mode) keeps its vorticity and decays with the
+  time-dependent |k(t)|^2, continuing through a remap onto the wave vector
+  (kx, ky - kx);
+- spheres in steady shear rotate at -rate/2 and move with the local flow,
+  including one that straddles the sheared boundary;
+- scatter, gather and LJ pair forces agree with brute-force sums over the
+  Lees-Edwards images; boundary crossing and the effective-rate correction
+  have closed forms.
+
+The shearing-wave and sphere tests fail when the sign of the shear term
+in the advection (operate_omega.cxx:145-146) or of the velocity jump across
+the boundary (make_phi.cxx:392, md_force.cxx:614) is flipped."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import kref
+from kapsel3_amd import workloads
+from kapsel3_amd._abi import Params, make_species
+
+
+def _pad(a, n):
+    b = np.zeros((n, n, n + 2))
+    b[:, :, :n] = a
+    return b
+
+
+def _band_limited_u(n, kmax, amp, seed):
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n) * n
+    kz = np.arange(n // 2 + 1)
+    m = (np.abs(k)[:, None, None] < kmax) & (np.abs(k)[None, :, None] < kmax) & (kz[None, None, :] < kmax)
+    return [np.fft.irfftn(np.fft.rfftn(rng.normal(size=(n, n, n))) * m, s=(n, n, n), axes=(0, 1, 2)) * amp
+            for _ in range(3)]
+
+
+def _params(n, **kw):
+    base = dict(a=4.0, a_xi=2.0, dt=0.05, eta=0.8, rotation=1,
+                species=[make_species(mass_ratio=1.5), make_species(mass_ratio=2.0)])
+    base.update(kw)
+    return Params(n, **base)
+
+
+# ---------------------------------------------------------------- zero shear
+def test_zero_shear_is_the_plain_step():
+    """Three spheres (two within LJ range), gravity, rotation, FIX_CELL,
+    thermal noise and a random flow: six steps of the sheared step at rate 0
+    against the plain step. Each run gets the same velocity field through
+    its own zeta convention (KY before KZ in U_k2zeta_k, KZ before KY in
+    U_k2zeta_k_OBL)."""
+    n = 32
+    # the sheared step's LJ loop visits every pair once (md_force.cxx:198-305),
+    # so the plain step is compared with its pair count, not the link cell's
+    P = _params(n, g=1e-3, g_dir=2, lj_truncate=1, epsilon=1.0, kbt=1e-4, seed=3, fix_cell=(0, 0, 1),
+                lj_cell_compat=0)
+    pos = np.array([[10.2, 5.3, 16.1], [19.1, 5.9, 15.4], [15.0, 30.5, 8.0]])
+    p = workloads.particles_at(pos, spec=[0, 1, 0])
+    rng = np.random.default_rng(1)
+    p["v"] = rng.normal(0, 1e-3, (3, 3))
+    p["omega"] = rng.normal(0, 1e-3, (3, 3))
+    u = _band_limited_u(n, 6, 1e-3, seed=2)
+    runs = [kref.Run(P, 3), kref.Run(P, 3)]
+    a, b = runs
+    b.set_shear(0.0)
+    for d in range(3):
+        b.set_field("u%d" % d, _pad(u[d], n))
+    b.le_apply("solenoidal_u")                   # at strain 0 the plain projection
+    for d in range(3):
+        a.set_field("u%d" % d, b.get_field("u%d" % d))
+    a.apply("u2zeta_k")
+    b.le_apply("u2zeta_k")
+    for r in runs:
+        r.set_particles(p)
+        r.step(6)
+    pa, pb = runs[0].get_particles(), runs[1].get_particles()
+    for f in ("x", "v", "omega", "q", "fr_previous", "f_hydro_previous", "torque_hydro_previous"):
+        scale = max(np.abs(pa[f]).max(), 1e-30)
+        assert np.abs(pa[f] - pb[f]).max() <= 1e-12 * scale, f
+    assert runs[1].shear_state()[0] == 0.0
+    # the flow: each zeta -> u in its own convention (the 2/3 rule first:
+    # the sheared step leaves the discarded modes of f in zeta and drops
+    # them at the start of the next step)
+    ua, ub = [], []
+    for r, le, out in zip(runs, (False, True), (ua, ub)):
+        if le:
+            r.le_apply("ns_solver")                 # truncates zeta, then one more step of the fluid
+            r.le_apply("zeta_k2u")
+        else:
+            r.apply("ns_solver")
+            r.apply("zeta_k2u")
+        out.extend(r.get_field("u%d" % d)[:, :, :n] for d in range(3))
+    scale = max(np.abs(x).max() for x in ua)
+    assert max(np.abs(x - y).max() for x, y in zip(ua, ub)) <= 1e-12 * scale
+
+
+# ---------------------------------------------------------------- grids
+def _cartesian_from_oblique(w, g, rate):
+    """Sample the oblique field w[d][xi, y, z] at Cartesian grid points: the
+    point (x, y) sits at oblique abscissa x - g*(y - NY/2); linear
+    interpolation along x; contravariant -> Cartesian u_x = u^x + g u^y;
+    plus the mean flow rate*(y - NY/2)."""
+    n = w[0].shape[0]
+    x = np.arange(n)
+    out = []
+    for d in range(3):
+        o = np.empty_like(w[d])
+        for j in range(n):
+            pos = (x - g * (j - n / 2.0)) % n
+            i0 = np.floor(pos).astype(int)
+            t = pos - i0
+            o[:, j, :] = (1 - t)[:, None] * w[d][i0 % n, j, :] + t[:, None] * w[d][(i0 + 1) % n, j, :]
+        out.append(o)
+    out[0] = out[0] + g * out[1] + rate * (np.arange(n)[None, :, None] - n / 2.0)
+    return out
+
+
+def _oblique_from_cartesian(c, g):
+    n = c[0].shape[0]
+    x = np.arange(n)
+    out = []
+    for d in range(3):
+        o = np.empty_like(c[d])
+        for j in range(n):
+            pos = (x + g * (j - n / 2.0)) % n
+            i0 = np.floor(pos).astype(int)
+            t = pos - i0
+            o[:, j, :] = (1 - t)[:, None] * c[d][i0 % n, j, :] + t[:, None] * c[d][(i0 + 1) % n, j, :]
+        out.append(o)
+    out[0] = out[0] - g * out[1]
+    return out
+
+
+@pytest.mark.parametrize("g", [0.3, -0.45, 0.8, 0.0])
+def test_oblique_interpolation_matches_geometry(g):
+    n, rate = 16, 0.02
+    r = kref.Run(_params(n), 0)
+    r.set_shear(rate)
+    r.set_shear_state(g, rate)
+    rng = np.random.default_rng(7)
+    w = [rng.normal(size=(n, n, n)) for _ in range(3)]
+    for d in range(3):
+        r.set_field("u%d" % d, _pad(w[d], n))
+    r.le_apply("oblique2u")
+    want = _cartesian_from_oblique(w, g, rate)
+    for d in range(3):
+        np.testing.assert_allclose(r.get_field("u%d" % d)[:, :, :n], want[d], rtol=0, atol=1e-13)
+    for d in range(3):
+        r.set_field("u%d" % d, _pad(w[d], n))
+    r.le_apply("u2oblique")
+    want = _oblique_from_cartesian(w, g)
+    for d in range(3):
+        np.testing.assert_allclose(r.get_field("u%d" % d)[:, :, :n], want[d], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("g,fn", [(1.0, "reset_up"), (-1.0, "reset_down")])
+def test_remap_at_unit_strain_keeps_the_cartesian_field(g, fn):
+    n, rate = 16, 0.03
+    rng = np.random.default_rng(11)
+    w = [rng.normal(size=(n, n, n)) for _ in range(3)]
+    r = kref.Run(_params(n), 0)
+    r.set_shear(rate)
+    r.set_shear_state(g, rate)
+    for d in range(3):
+        r.set_field("u%d" % d, _pad(w[d], n))
+    r.le_apply("oblique2u")
+    before = [r.get_field("u%d" % d)[:, :, :n] for d in range(3)]
+    for d in range(3):
+        r.set_field("u%d" % d, _pad(w[d], n))
+    r.le_apply(fn)
+    r.set_shear_state(0.0, rate)
+    r.le_apply("oblique2u")
+    after = [r.get_field("u%d" % d)[:, :, :n] for d in range(3)]
+    for d in range(3):
+        np.testing.assert_array_equal(before[d], after[d])
+
+
+# ---------------------------------------------------------------- spectral
+def test_sheared_projection_and_zeta_round_trip():
+    """At strain 0.35: the projection makes the covariant divergence
+    k_i u^i vanish and is idempotent; u -> zeta -> u returns a projected u."""
+    n, g = 16, 0.35
+    r = kref.Run(_params(n), 0)
+    r.set_shear(0.01)
+    r.set_shear_state(g, 0.01)
+    u = _band_limited_u(n, 5, 1.0, seed=4)
+    for d in range(3):
+        r.set_field("u%d" % d, _pad(u[d], n))
+    r.le_apply("solenoidal_u")
+    p1 = [r.get_field("u%d" % d) for d in range(3)]
+    uh = [np.fft.rfftn(x[:, :, :n]) for x in p1]
+    k = np.fft.fftfreq(n) * n
+    kx, ky, kz = k[:, None, None], k[None, :, None], np.arange(n // 2 + 1)[None, None, :]
+    div = kx * uh[0] + ky * uh[1] + kz * uh[2]
+    assert np.abs(div).max() < 1e-12 * np.abs(uh[0]).max()
+    r.le_apply("solenoidal_u")
+    for d in range(3):
+        np.testing.assert_allclose(r.get_field("u%d" % d)[:, :, :n], p1[d][:, :, :n], atol=1e-13)
+    r.le_apply("u2zeta_k")
+    r.le_apply("zeta_k2u")
+    for d in range(3):
+        np.testing.assert_allclose(r.get_field("u%d" % d)[:, :, :n], p1[d][:, :, :n], atol=1e-12)
+
+
+def test_kelvin_mode_decays_with_the_sheared_wave_number():
+    """A 2D shearing wave u = A(-ky, kx, 0) sin(k.x) on 16^3 at rate 0.1:
+    its vorticity is conserved by the shear and decays by
+    exp(-nu |k(t)|^2 dt) per step with |k(t)|^2 = kx^2 + (ky - g kx)^2; at
+    the remap (strain 1 -> 0) the wave moves to (kx, ky - kx) and picks up
+    the phase exp(i pi kx) of the remap's centre row. 300 steps; the explicit
+    shear term costs O(dt) (0.3% here)."""
+    n, dt, eta, rate = 16, 0.05, 0.1, 0.1
+    r = kref.Run(_params(n, dt=dt, eta=eta), 0)
+    r.set_shear(rate)
+    w = 2 * np.pi / n
+    kx, ky = 1, 3
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    s = np.sin(w * (kx * X + ky * Y))
+    u = np.zeros((3, n, n, n + 2))
+    u[0, :, :, :n] = (1e-6 * -ky * s)[:, :, None]
+    u[1, :, :, :n] = (1e-6 * kx * s)[:, :, None]
+    for d in range(3):
+        r.set_field("u%d" % d, u[d])
+    r.le_apply("u2zeta_k")
+    z0, z1, _ = r.get_fluid()
+    amp0 = complex(z1[kx, ky, 0], z1[kx, ky, 1])
+    fac, g, shift, checked = 1.0, 0.0, 0, 0
+    for st in range(300):
+        fac *= math.exp(-eta * ((w * kx) ** 2 + (w * (ky - shift) - w * kx * g) ** 2) * dt)
+        r.step(1)
+        gn, sre = r.shear_state()
+        assert abs(sre - rate) < 1e-15
+        if gn < g:
+            shift += 1
+        g = gn
+        if st % 50 == 49:
+            z0, z1, _ = r.get_fluid()
+            jj = (ky - shift) % n
+            amp = complex(z1[kx, jj, 0], z1[kx, jj, 1]) / amp0
+            want = fac * (-1) ** (kx * shift)
+            assert abs(amp - want) < 5e-3 * abs(want), (st, amp, want)
+            rest = np.abs(z1).sum() + np.abs(z0).sum() - 2 * (abs(z1[kx, jj, 0]) + abs(z1[kx, jj, 1]))
+            assert rest < 1e-9 * abs(amp0)
+            checked += 1
+    assert shift == 1 and checked == 6
+
+
+def test_unperturbed_shear_stays_unperturbed():
+    n = 16
+    r = kref.Run(_params(n), 0)
+    r.set_shear(0.2)
+    r.step(250)                                      # two remaps
+    z0, z1, uk = r.get_fluid()
+    assert not z0.any() and not z1.any() and not uk.any()
+    g, s = r.shear_state()
+    assert s == 0.2 and abs(g - (250 * 0.2 * 0.05 - 2.0)) < 1e-12
+
+
+def test_effective_rate_correction_telescopes():
+    """Calc_shear_rate_eff adds the mean of a centred difference over a
+    periodic grid, which is zero up to rounding."""
+    n = 16
+    r = kref.Run(_params(n), 0)
+    r.set_shear(0.01)
+    r.set_shear_state(0.4, 0.01)
+    rng = np.random.default_rng(3)
+    for d in range(3):
+        r.set_field("u%d" % d, _pad(rng.normal(size=(n, n, n)), n))
+    r.le_apply("shear_rate_eff")
+    assert abs(r.shear_state()[1] - 0.01) < 1e-16
+
+
+# ---------------------------------------------------------------- particles
+def test_spheres_in_shear_rotate_at_half_the_rate_and_follow_the_flow():
+    """Neutrally buoyant spheres in 32^3 at rate 0.004: one at the centre,
+    one 8 above it, one straddling the sheared boundary (y = 30.5, its
+    domain reaches past y = 32). After 200 steps each spins at -rate/2 and
+    moves with the local mean flow rate*(y - 16), to the few per cent that
+    the periodic images and the finite radius allow."""
+    n, rate = 32, 0.004
+    P = Params(n, a=4.0, a_xi=2.0, dt=0.05, eta=1.0, rotation=1, species=[make_species(mass_ratio=1.0)])
+    pos = np.array([[16.0, 16.0, 16.0], [8.0, 24.0, 8.0], [24.0, 30.5, 24.0]])
+    r = kref.Run(P, 3)
+    r.set_particles(workloads.particles_at(pos))
+    r.set_shear(rate)
+    r.step(200)
+    q = r.get_particles()
+    spin = q["omega"][:, 2] / (-rate / 2)
+    assert np.all((spin > 0.93) & (spin < 1.05)), spin
+    assert np.abs(q["omega"][:, :2]).max() < 0.01 * rate           # neighbours tilt the axis a little
+    want = rate * (pos[:, 1] - n / 2)
+    assert abs(q["v"][0, 0]) < 0.02 * rate * 8
+    np.testing.assert_allclose(q["v"][1:, 0], want[1:], rtol=0.03)
+
+
+def test_boundary_crossing_closed_form():
+    """MD_solver_position_Euler_OBL + PBC_OBL: leaving through the top
+    (y > LY) re-enters at the bottom shifted by -strain*LX in x with v_x
+    lowered by rate*LY; through the bottom, the opposite."""
+    n, g, s, dt = 16, 0.3, 0.02, 0.05
+    r = kref.Run(_params(n, dt=dt), 2)
+    p = workloads.particles_at(np.array([[5.0, 15.9, 4.0], [9.0, 0.05, 4.0]]))
+    p["v"] = [[0.1, 4.0, 0.0], [0.2, -2.0, 0.0]]
+    r.set_particles(p)
+    r.set_shear(s)
+    r.set_shear_state(g, s)
+    r.le_apply("position_euler")
+    q = r.get_particles()
+    L = float(n)
+    np.testing.assert_allclose(q["x"][0], [(5.0 + dt * 0.1 - g * L) % L, 15.9 + dt * 4.0 - L, 4.0], atol=1e-13)
+    np.testing.assert_allclose(q["x"][1], [(9.0 + dt * 0.2 + g * L) % L, 0.05 - dt * 2.0 + L, 4.0], atol=1e-13)
+    np.testing.assert_allclose(q["v"][:, 0], [0.1 - s * L, 0.2 + s * L], atol=1e-15)
+    np.testing.assert_allclose(q["x_nopbc"][0], [5.0 + dt * 0.1, 15.9 + dt * 4.0, 4.0], atol=1e-13)
+
+
+def _nearest_image(X, xp, g, L):
+    """Displacement X - image of xp over the Lees-Edwards images
+    xp + (a L + m g L, m L, c L) (the row of boxes m above moves by m times
+    the strain in x), nearest one; returns (m, r)."""
+    best = None
+    for m in (-1, 0, 1):
+        for a in (-1, 0, 1, 2, -2):
+            for c in (-1, 0, 1):
+                img = xp + np.array([a * L + m * g * L, m * L, c * L])
+                d = X - img[None, :]
+                d2 = (d ** 2).sum(axis=1)
+                if best is None:
+                    best = (d2, np.full(len(X), m), d)
+                else:
+                    take = d2 < best[0]
+                    best = (np.where(take, d2, best[0]), np.where(take, m, best[1]), np.where(take[:, None], d, best[2]))
+    return best[1], best[2]
+
+
+def test_scatter_and_gather_against_the_images():
+    """phi_sum, u_p and F_H, T_H at strain 0.37, rate 0.013, for two
+    overlapping spheres one of which straddles the sheared boundary, against
+    brute-force sums over every grid point and the nearest Lees-Edwards
+    image (velocity of the image in row m: v + m rate LY)."""
+    n, g, rate = 24, 0.37, 0.013
+    P = _params(n, a=3.0)
+    xp = np.array([[7.3, 22.6, 11.2], [1.42, 1.9, 12.5]])
+    p = workloads.particles_at(xp)
+    rng = np.random.default_rng(5)
+    p["v"] = rng.normal(0, 0.01, (2, 3))
+    p["omega"] = rng.normal(0, 0.01, (2, 3))
+    r = kref.Run(P, 2)
+    r.set_particles(p)
+    r.set_shear(rate)
+    r.set_shear_state(g, rate)
+    r.le_apply("make_phi")
+    L = float(n)
+    I, J, K = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    X = np.stack([I.ravel(), J.ravel(), K.ravel()], axis=1).astype(float)
+    radius, xi = 3.0, 2.0
+    phis, ms, rs = [], [], []
+    for k in range(2):
+        m, d = _nearest_image(X, xp[k], g, L)
+        dist = np.sqrt((d ** 2).sum(axis=1))
+        phis.append(np.array([kref.phi(x, radius, 1.0, xi) for x in dist]))
+        ms.append(m)
+        rs.append(d)
+    phi_sum = (phis[0] + phis[1]).reshape(n, n, n)
+    got = r.get_field("phi_sum")[:, :, :n]
+    np.testing.assert_allclose(got, phi_sum, rtol=0, atol=1e-13)
+    assert (phi_sum > 1).any()                                   # the spheres overlap
+    assert any((m != 0)[phis[k] > 0].any() for k, m in enumerate(ms))   # and one crosses the boundary
+    r.le_apply("make_up")
+    norm = np.maximum(phi_sum.ravel(), 1.0)
+    up = np.zeros((3, n ** 3))
+    for k in range(2):
+        vimg = p["v"][k][None, :] + ms[k][:, None] * np.array([rate * L, 0, 0])[None, :]
+        vrot = np.cross(np.broadcast_to(p["omega"][k], rs[k].shape), rs[k])
+        up += ((vimg + vrot) * (phis[k] / norm)[:, None]).T
+    for d in range(3):
+        np.testing.assert_allclose(r.get_field("up%d" % d)[:, :, :n].ravel(), up[d], rtol=0, atol=1e-14)
+    u = rng.normal(0, 0.01, (3, n, n, n + 2))
+    for d in range(3):
+        r.set_field("u%d" % d, u[d])
+    r.le_apply("f_hydro")
+    q = r.get_particles()
+    uu = u[:, :, :, :n].reshape(3, -1).T
+    for k in range(2):
+        vimg = p["v"][k][None, :] + ms[k][:, None] * np.array([rate * L, 0, 0])[None, :]
+        vrot = np.cross(np.broadcast_to(p["omega"][k], rs[k].shape), rs[k])
+        fp = (vimg + vrot - uu) * (phis[k] / norm)[:, None]
+        F = -(1.0 / 0.05) * fp.sum(axis=0)
+        T = -(1.0 / 0.05) * np.cross(rs[k], fp).sum(axis=0)
+        np.testing.assert_allclose(q["f_hydro"][k], F, rtol=1e-11, atol=1e-13)
+        np.testing.assert_allclose(q["torque_hydro"][k], T, rtol=1e-11, atol=1e-13)
+
+
+def test_lennard_jones_pairs_across_the_sheared_boundary():
+    n, g = 32, 0.41
+    P = _params(n, a=4.0, lj_truncate=1, epsilon=1.0, species=[make_species()])
+    # particle 1 sits 2 to the right of particle 0's image in the row below
+    xp = np.array([[3.0, 31.0, 10.0], [(3.0 - g * n + 2.0) % n, 6.9, 10.5], [20.0, 16.0, 16.0], [28.5, 16.4, 15.0]])
+    r = kref.Run(P, 4)
+    r.set_particles(workloads.particles_at(xp))
+    r.set_shear(0.01)
+    r.set_shear_state(g, 0.01)
+    r.le_apply("lj")
+    got = r.get_particles()["fr"]
+    sigma = 8.0
+    cutoff = 2 ** (1 / 6) * min((sigma + 2.0) / 2 ** (1 / 6), sigma)
+    want = np.zeros((4, 3))
+    pairs = 0
+    for a in range(4):
+        for b in range(4):
+            if a == b:
+                continue
+            m, d = _nearest_image(xp[a][None, :], xp[b], g, float(n))
+            rv = -d[0]                                   # x_b(image) - x_a
+            rr = np.sqrt((rv ** 2).sum())
+            if rr < cutoff:
+                lj = 24.0 / rr ** 2 * (2 * (sigma / rr) ** 12 - (sigma / rr) ** 6)
+                want[a] += lj * (-rv)
+                pairs += 1
+    assert pairs == 4                                    # (0,1) across the boundary and (2,3)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
