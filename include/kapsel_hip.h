@@ -13,7 +13,9 @@
  *
  * Reference interfaces replaced (KAPSEL file:line):
  *   kh_step            Time_evolution_hydro (sp_3d_ns.cxx:41-194), the
- *                      function-pointer seam `Time_evolution` (sp_3d_ns.cxx:10)
+ *                      function-pointer seam `Time_evolution` (sp_3d_ns.cxx:10);
+ *                      after kh_set_shear, Time_evolution_hydro_OBL
+ *                      (sp_3d_ns.cxx:328-433), bound at sp_3d_ns.cxx:704-707
  *   kh_init_fluid      Init_zeta_k + initial constraint block
  *                      (init_fluid.cxx:10-25, sp_3d_ns.cxx:809-826)
  *   kh_A2a_k/kh_A_k2a  A2a_k / A_k2a (fft_wrapper.h:641-681)
@@ -163,6 +165,19 @@ int kh_step(kh_run *run, int nsteps);
  * Both give the reference's step to round-off. */
 enum { KH_PATH_FUSED = 0, KH_PATH_REFERENCE_ORDER = 1 };
 int kh_set_step_path(kh_run *run, int path);
+/* Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards): from the
+ * next step on kh_step runs Time_evolution_hydro_OBL (sp_3d_ns.cxx:328-433),
+ * the sheared step on an oblique grid that follows the mean flow
+ * rate*(y - LY/2): DC at shear_rate (ac = 0) or AC shear_rate*cos(2 pi
+ * frequency t) (input.cxx:770-792). Particle positions and velocities stay
+ * Cartesian (lab frame). LJ pairs are counted once and the LJ diameter is
+ * min((SIGMA + XI)/2^(1/2n), SIGMA) as the reference's sheared runs take it
+ * (input.cxx:549-566). Not for squirmer (slip) runs. The strain
+ * degree_oblique (in (-NX/NY, NX/NY)) and Shear_rate_eff are run state,
+ * saved and restored like zeta (resume.cxx:72, 566-580). */
+int kh_set_shear(kh_run *run, double shear_rate, int ac, double frequency);
+int kh_get_shear_state(kh_run *run, double *degree_oblique, double *shear_rate_eff);
+int kh_set_shear_state(kh_run *run, double degree_oblique, double shear_rate_eff);
 /* Per-step particle record kept on the device: from the next step on, the
  * (x[3], v[3]) of every particle after each of the next nsteps steps
  * (nsteps x np x 6 doubles; 0 stops recording). No counterpart in the

@@ -25,7 +25,8 @@ EXPORTS = [
     "kh_sync", "kh_A2a_k", "kh_A_k2a", "kh_set_step_path", "kh_set_graph", "kh_timing_enable", "kh_timing_reset", "kh_timing_get",
     "kh_timing_count", "kh_timing_name", "kh_strerror", "kh_last_error", "kh_set_trajectory", "kh_get_trajectory",
     "kh_set_output", "kh_get_output", "kh_run_bytes", "kh_query", "kh_server_create", "kh_server_submit", "kh_server_poll", "kh_server_wait",
-    "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy",
+    "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy", "kh_set_shear", "kh_get_shear_state",
+    "kh_set_shear_state",
 ]
 
 
@@ -105,6 +106,9 @@ def lib():
         L.kh_server_run_times.argtypes = [vp, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.kh_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
         L.kh_server_destroy.argtypes = [vp]
+        L.kh_set_shear.argtypes = [vp, C.c_double, C.c_int, C.c_double]
+        L.kh_get_shear_state.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.kh_set_shear_state.argtypes = [vp, C.c_double, C.c_double]
         _LIB = L
     return _LIB
 
@@ -242,6 +246,20 @@ class Run:
         out = np.zeros((n, self.np, 6))
         _check(lib().kh_get_trajectory(self.h, out.ctypes.data, n))
         return out
+
+    def set_shear(self, rate, ac=False, frequency=0.0):
+        """Lees-Edwards shear from the next step on (Time_evolution_hydro_OBL):
+        DC at `rate`, or AC rate*cos(2 pi frequency t)."""
+        _check(lib().kh_set_shear(self.h, float(rate), int(bool(ac)), float(frequency)))
+
+    def shear_state(self):
+        """(degree_oblique, Shear_rate_eff) of a sheared run."""
+        g, s = C.c_double(), C.c_double()
+        _check(lib().kh_get_shear_state(self.h, C.byref(g), C.byref(s)))
+        return g.value, s.value
+
+    def set_shear_state(self, degree_oblique, shear_rate_eff):
+        _check(lib().kh_set_shear_state(self.h, float(degree_oblique), float(shear_rate_eff)))
 
     def set_output(self, nframes, every):
         """Record Output_udf frames on the device at the next nframes step

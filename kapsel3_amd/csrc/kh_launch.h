@@ -28,6 +28,28 @@ hipError_t add3(long long n, T *const u[3], T *const f[3], hipStream_t s);
 template <typename T>
 hipError_t fill(long long n, T *p, T v, hipStream_t s);
 
+// kh_shear.hip (the Lees-Edwards step; gd = strain of the oblique grid)
+template <typename T>
+hipError_t obl_zeta_to_u(const Grid &g, cpx<T> *z0, cpx<T> *z1, cpx<T> *u0, cpx<T> *u1, cpx<T> *u2, cpx<T> *w1,
+                         double *ukdc, int fixmask, int truncate, double gd, hipStream_t s);
+template <typename T>
+hipError_t obl_zeta_to_omega(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const o[3], hipStream_t s);
+template <typename T>
+hipError_t obl_cross(long long n, T *const u[3], T *const w[3], hipStream_t s);
+template <typename T>
+hipError_t obl_adv_ns(const Grid &g, cpx<T> *const F[3], const cpx<T> *w1, cpx<T> *z0, cpx<T> *z1, double rate,
+                      double gd, double dmy0, double dmy1, hipStream_t s);
+template <typename T>
+hipError_t obl_remap(const Grid &g, T *const u[3], T *const o[3], double delta, hipStream_t s);
+template <typename T>
+hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, hipStream_t s);
+template <typename T>
+hipError_t obl_add_f(const Grid &g, T *const up[3], const T *phisum, T *const c[3], T *const u[3], double gd,
+                     hipStream_t s);
+template <typename T>
+hipError_t obl_u_to_zeta(const Grid &g, const cpx<T> *u0, const cpx<T> *u1, const cpx<T> *u2, cpx<T> *z0, cpx<T> *z1,
+                         double *ukdc, double gd, hipStream_t s);
+
 // kh_particle.hip (PCache: per-particle cache of the Sekibun points with
 // nonzero phi, filled by phi_scatter)
 // move = 1: the particle is first advanced by the position integrator

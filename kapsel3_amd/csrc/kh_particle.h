@@ -51,6 +51,10 @@ struct PParams {
     int               motor;  // SW_JANUS_MOTOR
     double           *xs;     // np x 4: particle position, 1.0 for obstacles (compact copy for pair loops)
     int               direct_dist;  // every Sekibun offset is within half a box: |r| is the minimum-image distance
+    // Lees-Edwards (the sheared step): the row of boxes above is displaced by
+    // obl_g*LX... in x (obl_g*NY grid points) and moves obl_jump faster in x
+    int               obl;
+    double            obl_g, obl_jump;
 };
 
 // Per-particle cache of the Sekibun points where phi is nonzero, filled by
@@ -84,6 +88,25 @@ __device__ __forceinline__ double distance0(const PParams &pp, const double *x1,
         x12[d] -= (double)nint(x12[d] / pp.L[d]) * pp.L[d];
         dmy += x12[d] * x12[d];
     }
+    return sqrt(dmy);
+}
+
+// Distance0_OBL (interaction.h:50-72): the y minimum image first; crossing
+// it shifts x1 by the strain of the row of boxes it reaches
+__device__ __forceinline__ double distance0_obl(const PParams &pp, const double *x1, const double *x2, double *x12) {
+    double       dmy   = 0.0;
+    const double signY = x2[1] - x1[1];
+    x12[1]             = x2[1] - x1[1];
+    x12[1] -= (double)nint(x12[1] / pp.L[1]) * pp.L[1];
+    int sign = (int)(signY - x12[1]);
+    if (!(sign == 0)) sign = sign / abs(sign);
+    dmy += x12[1] * x12[1];
+    x12[0] = x2[0] - (x1[0] + (double)sign * pp.obl_g * pp.L[1]);
+    x12[0] -= (double)nint(x12[0] / pp.L[0]) * pp.L[0];
+    dmy += x12[0] * x12[0];
+    x12[2] = x2[2] - x1[2];
+    x12[2] -= (double)nint(x12[2] / pp.L[2]) * pp.L[2];
+    dmy += x12[2] * x12[2];
     return sqrt(dmy);
 }
 
@@ -139,9 +162,25 @@ struct PPoint {
     long long idx;  // real-space linear index (x, y, z)
     double    r[3]; // cell*dx - residue
     double    dist; // minimum-image |r_mesh*DX - xp|
+    int       sign; // sheared runs: +1 / -1 when the point lies across the top / bottom boundary
 };
-// DIST = false leaves `dist` unset (the caller measures |r| itself)
-template <bool DIST = true>
+
+// Relative_coord_check_stepover_Y (make_phi.h:229-268): on a sheared run a
+// Sekibun point across the y boundary lies in the row of boxes displaced by
+// sign*obl_g*NY grid points; returns that sign and sets r_mesh[0], r[0].
+__device__ __forceinline__ int obl_stepover(const PParams &pp, const PCell &c, const int *cell, int *rm0, double *r0) {
+    const int m1   = c.x_int[1] + cell[1];
+    const int sign = m1 >= pp.ny ? 1 : (m1 < 0 ? -1 : 0);
+    const double xo =
+        fmod(c.x_int[0] - sign * pp.obl_g * pp.ny + c.residue[0] / pp.dx + cell[0] + 4 * pp.nx, (double)pp.nx);
+    const int rm = (int)xo;
+    *rm0         = min(max(rm, 0), pp.nx - 1);
+    *r0          = (double)cell[0] * pp.dx - (xo - rm) * pp.dx;
+    return sign;
+}
+// DIST = false leaves `dist` unset (the caller measures |r| itself); OBL:
+// the sheared step's stepover geometry (compiled only into its kernels)
+template <bool DIST = true, bool OBL = false>
 __device__ __forceinline__ PPoint sek_point(const PParams &pp, const PCell &c, const double *xp, int code) {
     int cell[3];
     sek_unpack(code, cell);
@@ -167,6 +206,15 @@ __device__ __forceinline__ PPoint sek_point(const PParams &pp, const PCell &c, c
             q.dist = sqrt(q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2]);
         else
             q.dist = distance0(pp, x, xp, x12);
+    }
+    q.sign = 0;
+    if constexpr (OBL) {
+        q.sign = obl_stepover(pp, c, cell, &rm[0], &q.r[0]);
+        if constexpr (DIST) {
+            x[0]   = rm[0] * pp.dx;
+            q.dist = pp.direct_dist ? sqrt(q.r[0] * q.r[0] + q.r[1] * q.r[1] + q.r[2] * q.r[2])
+                                    : distance0_obl(pp, x, xp, x12);
+        }
     }
     q.idx  = ((long long)rm[0] * pp.ny + rm[1]) * pp.nz + rm[2];
     return q;

@@ -91,7 +91,7 @@ __device__ __forceinline__ void angular2v(int rotation, const double *om, const 
 // particle (the position integrator of the same step, one lane).
 __device__ void position_one(const PParams &pp, DevParticle *P, int n, int ab2, double dt, double hdt);
 
-template <typename T, bool SCATTER, bool MOVE, int NT>
+template <typename T, bool SCATTER, bool MOVE, int NT, bool OBL = false>
 __global__ __launch_bounds__(NT) void k_phi_scatter(PParams pp, DevParticle *__restrict__ P,
                                                        T *__restrict__ phisum, PCache<T> pc, int ab2, double dt,
                                                        double hdt) {
@@ -129,11 +129,11 @@ __global__ __launch_bounds__(NT) void k_phi_scatter(PParams pp, DevParticle *__r
         if (m < pp.np_domain) {
             PPoint q;
             if (sizeof(T) == 4 && pp.direct_dist) {  // FP32 distance of the offset
-                q              = sek_point<false>(pp, c, xp, code);
+                q              = sek_point<false, OBL>(pp, c, xp, code);
                 const float r0 = (float)q.r[0], r1 = (float)q.r[1], r2 = (float)q.r[2];
                 ph             = prof_phi_t<T>(pp, (T)sqrtf(r0 * r0 + r1 * r1 + r2 * r2));
             } else {
-                q  = sek_point(pp, c, xp, code);
+                q  = sek_point<true, OBL>(pp, c, xp, code);
                 ph = prof_phi_t<T>(pp, (T)q.dist);
             }
             idx = q.idx;
@@ -161,17 +161,25 @@ __global__ __launch_bounds__(NT) void k_phi_scatter(PParams pp, DevParticle *__r
     if (threadIdx.x == 0) pc.cnt[n] = off;
 }
 
-// r = cell*dx - residue (Relative_coord, make_phi.h:154-170)
-__device__ __forceinline__ void sek_r(const PParams &pp, const PCell &c, int code, double *r) {
+// r = cell*dx - residue (Relative_coord, make_phi.h:154-170); on a sheared
+// run the stepover form, and the returned sign of the point's y crossing
+template <bool OBL = false>
+__device__ __forceinline__ int sek_r(const PParams &pp, const PCell &c, int code, double *r) {
     int cell[3];
     sek_unpack(code, cell);
 #pragma unroll
     for (int d = 0; d < 3; d++) r[d] = (double)cell[d] * pp.dx - c.residue[d];
+    if constexpr (!OBL) {
+        return 0;
+    } else {
+        int rm0;
+        return obl_stepover(pp, c, cell, &rm0, &r[0]);
+    }
 }
 
 // Calc_f_hydro_correct_precision (md_force.cxx:383-506), non-rigid branch,
 // over the cached phi and grid index of each Sekibun point.
-template <typename T, int NT = PT_NT>
+template <typename T, int NT = PT_NT, bool OBL = false>
 __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__restrict__ P, int n,
                                               const T *__restrict__ u0, const T *__restrict__ u1,
                                               const T *__restrict__ u2, const T *__restrict__ phisum,
@@ -190,10 +198,14 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
         const int idx  = pc.idx[cb + m];
         const int code = pc.code[cb + m];
         double    rd[3];
-        sek_r(pp, c, code, rd);
+        const int sgn = sek_r<OBL>(pp, c, code, rd);
         const T r[3] = {(T)rd[0], (T)rd[1], (T)rd[2]};
         T       vr[3];
         angular2v_t<T>(pp.rotation, wt, r, vr);
+        // across the sheared boundary the image moves sgn*obl_jump faster
+        // (md_force.cxx:614); the reference forms vp - sign*jump first
+        T vx = vt[0];
+        if constexpr (OBL) vx = sgn ? (T)(vp[0] - sgn * pp.obl_jump) : vt[0];
         const T ps      = phisum[idx];
         const T dmy_phi = ph / (ps > (T)1 ? ps : (T)1);
         if (stash && m < PT_CAP) {
@@ -202,7 +214,7 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
             stash->code[m] = code;
         }
         T       fp[3];
-        fp[0] = ((vt[0] + vr[0]) - u0[idx]) * dmy_phi;
+        fp[0] = ((vx + vr[0]) - u0[idx]) * dmy_phi;
         fp[1] = ((vt[1] + vr[1]) - u1[idx]) * dmy_phi;
         fp[2] = ((vt[2] + vr[2]) - u2[idx]) * dmy_phi;
         acc[0].add((double)fp[0]);
@@ -227,15 +239,15 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
             }
     }
 }
-template <typename T>
+template <typename T, bool OBL>
 __global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__restrict__ P, const T *__restrict__ u0,
                                                    const T *__restrict__ u1, const T *__restrict__ u2,
                                                    const T *__restrict__ phisum, PCache<T> pc, double coef) {
-    f_hydro_block<T>(pp, P, blockIdx.x, u0, u1, u2, phisum, pc, coef);
+    f_hydro_block<T, PT_NT, OBL>(pp, P, blockIdx.x, u0, u1, u2, phisum, pc, coef);
 }
 
 // Make_u_particle_sum scatter (make_phi.cxx:313-354) over the cache
-template <typename T, int NT = PT_NT>
+template <typename T, int NT = PT_NT, bool OBL = false>
 __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const double *xp, const double *vp,
                                                  const double *om, const T *__restrict__ phisum,
                                                  const PCache<T> &pc, T *__restrict__ up0, T *__restrict__ up1,
@@ -262,12 +274,14 @@ __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const
             dmy_phi    = ph / (ps > (T)1 ? ps : (T)1);
             alone      = ps == ph;
         }
-        double rd[3];
-        sek_r(pp, c, code, rd);
+        double    rd[3];
+        const int sgn = sek_r<OBL>(pp, c, code, rd);
         const T r[3] = {(T)rd[0], (T)rd[1], (T)rd[2]};
         T       vr[3];
         angular2v_t<T>(pp.rotation, wt, r, vr);
-        const T a = (vt[0] + vr[0]) * dmy_phi, b = (vt[1] + vr[1]) * dmy_phi, e = (vt[2] + vr[2]) * dmy_phi;
+        T vx = vt[0];
+        if constexpr (OBL) vx = sgn ? (T)(vp[0] - sgn * pp.obl_jump) : vt[0];  // make_phi.cxx:392
+        const T a = (vx + vr[0]) * dmy_phi, b = (vt[1] + vr[1]) * dmy_phi, e = (vt[2] + vr[2]) * dmy_phi;
         if (alone) {
             // phi_sum is this particle's phi alone: no other particle adds
             // here (a contribution below half an ulp of phi would be the only
@@ -283,7 +297,7 @@ __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const
         }
     }
 }
-template <typename T>
+template <typename T, bool OBL>
 __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParticle *__restrict__ P,
                                                       const T *__restrict__ phisum, PCache<T> pc,
                                                       T *__restrict__ up0, T *__restrict__ up1,
@@ -292,7 +306,7 @@ __global__ __launch_bounds__(PT_NT) void k_up_scatter(PParams pp, const DevParti
     const double xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
     const double vp[3] = {P[n].v[0], P[n].v[1], P[n].v[2]};
     const double om[3] = {P[n].om[0], P[n].om[1], P[n].om[2]};
-    up_scatter_block<T>(pp, n, xp, vp, om, phisum, pc, up0, up1, up2);
+    up_scatter_block<T, PT_NT, OBL>(pp, n, xp, vp, om, phisum, pc, up0, up1, up2);
 }
 
 // ---------------------------------------------------------- quaternions
@@ -333,7 +347,24 @@ __device__ void position_one(const PParams &pp, DevParticle *P, int n, int ab2, 
         p.x_nopbc[d] += delta_x;
         p.x[d] += delta_x;
     }
-    for (int d = 0; d < 3; d++) p.x[d] = fmod(p.x[d] + pp.L[d], pp.L[d]);
+    if (!pp.obl) {
+        for (int d = 0; d < 3; d++) p.x[d] = fmod(p.x[d] + pp.L[d], pp.L[d]);
+    } else {
+        // PBC_OBL (periodic_boundary.h:26-44): through the top (bottom) the
+        // particle enters the row of boxes below (above): x shifted by
+        // -(+)strain*LY, v_x by -(+)rate*LY
+        double signY = p.x[1];
+        p.x[1]       = fmod(p.x[1] + pp.L[1], pp.L[1]);
+        signY -= p.x[1];
+        int sign = (int)signY;
+        if (!(sign == 0)) sign = sign / abs(sign);
+        p.x[0] -= (double)sign * pp.obl_g * pp.L[1];
+        p.x[0]              = fmod(p.x[0] + pp.L[0], pp.L[0]);
+        p.x[2]              = fmod(p.x[2] + pp.L[2], pp.L[2]);
+        const double dvx    = -(double)sign * pp.obl_jump;
+        p.v[0] += dvx;
+        p.v_old[0] += dvx;
+    }
     for (int d = 0; d < 3; d++) pp.xs[4 * n + d] = p.x[d];
     if (pp.rotation) {
         double dq[4];
@@ -400,7 +431,7 @@ __device__ __forceinline__ int lj_mult(const PParams &pp, const double *xi, cons
 // (md_force.cxx:306-324). One workgroup per particle i: its lanes split the
 // partners j, each pair giving F_i += f(r_ij) (x_i - x_j), the same value the
 // reference's +/- updates add; the lane sums are reduced in FP64.
-template <int NT = PT_NT>
+template <int NT = PT_NT, bool OBL = false>
 __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__restrict__ P, int i) {
     const double xi[3]  = {P[i].x[0], P[i].x[1], P[i].x[2]};
     const int    si     = P[i].spec;
@@ -426,10 +457,19 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
                 odd |= fabs(e) > hl;
                 d2 += e * e;
             }
-            if (!odd && d2 >= cut2) continue;
+            if (!OBL && !odd && d2 >= cut2) continue;
             double rv[3];
-            // r_ij_vec from the higher index to the lower, as the link cell takes it
-            const double rij = (i > j) ? distance0(pp, xi, xj, rv) : distance0(pp, xj, xi, rv);
+            double rij;
+            if constexpr (!OBL) {
+                // r_ij_vec from the higher index to the lower, as the link cell takes it
+                rij = (i > j) ? distance0(pp, xi, xj, rv) : distance0(pp, xj, xi, rv);
+            } else {
+                // Calc_f_Lennard_Jones_shear_cap_primitive (md_force.cxx:198-305)
+                // takes r = Distance0_OBL(x_n, x_m) for n < m (x_m - x_n); its
+                // negation has the orientation the update below expects
+                rij = (i < j) ? distance0_obl(pp, xi, xj, rv) : distance0_obl(pp, xj, xi, rv);
+                for (int d = 0; d < 3; d++) rv[d] = -rv[d];
+            }
             if (rij < pp.lj_cutoff) {
                 const int    mult  = lj_mult(pp, xi, xj);
                 const double dmy_r = fmin(1.7976931348623157e308 / rij, lj_f(pp, rij));
@@ -445,8 +485,9 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
         if (pp.g != 0.0) P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
     }
 }
+template <bool OBL>
 __global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__restrict__ P) {
-    forces_block(pp, P, blockIdx.x);
+    forces_block<PT_NT, OBL>(pp, P, blockIdx.x);
 }
 
 // MD_solver_velocity_Euler (particle_solver.cxx:103-142) / _AB2_hydro
@@ -987,18 +1028,26 @@ static inline unsigned pblocks(int n) { return (unsigned)((n + 63) / 64); }
 constexpr int PT_WIDE = 1024;
 static inline bool pt_wide(const PParams &pp) { return pp.np <= 256; }
 
+template <typename T, int NT, bool OBL>
+static void launch_phi_scatter_v(const PParams &pp, DevParticle *P, T *phisum, const PCache<T> &pc, int scatter,
+                                 hipStream_t s, int move, int ab2, double dt, double hdt) {
+    if (move)
+        hipLaunchKernelGGL((k_phi_scatter<T, true, true, NT, OBL>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc,
+                           ab2, dt, hdt);
+    else if (scatter)
+        hipLaunchKernelGGL((k_phi_scatter<T, true, false, NT, OBL>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc,
+                           ab2, dt, hdt);
+    else
+        hipLaunchKernelGGL((k_phi_scatter<T, false, false, NT, OBL>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc,
+                           ab2, dt, hdt);
+}
 template <typename T, int NT>
 static void launch_phi_scatter(const PParams &pp, DevParticle *P, T *phisum, const PCache<T> &pc, int scatter,
                                hipStream_t s, int move, int ab2, double dt, double hdt) {
-    if (move)
-        hipLaunchKernelGGL((k_phi_scatter<T, true, true, NT>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc, ab2, dt,
-                           hdt);
-    else if (scatter)
-        hipLaunchKernelGGL((k_phi_scatter<T, true, false, NT>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc, ab2,
-                           dt, hdt);
+    if (pp.obl)
+        launch_phi_scatter_v<T, NT, true>(pp, P, phisum, pc, scatter, s, move, ab2, dt, hdt);
     else
-        hipLaunchKernelGGL((k_phi_scatter<T, false, false, NT>), dim3(pp.np), dim3(NT), 0, s, pp, P, phisum, pc, ab2,
-                           dt, hdt);
+        launch_phi_scatter_v<T, NT, false>(pp, P, phisum, pc, scatter, s, move, ab2, dt, hdt);
 }
 template <typename T>
 hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, const PCache<T> &pc, int scatter, hipStream_t s,
@@ -1015,14 +1064,24 @@ template <typename T>
 hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
                    double coef, hipStream_t s) {
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_f_hydro<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum, pc, coef);
+        if (pp.obl)
+            hipLaunchKernelGGL((k_f_hydro<T, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum,
+                               pc, coef);
+        else
+            hipLaunchKernelGGL((k_f_hydro<T, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum,
+                               pc, coef);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const PCache<T> &pc,
                       T *const up[3], hipStream_t s) {
     if (pp.np > 0)
-        hipLaunchKernelGGL(k_up_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, up[0], up[1], up[2]);
+        if (pp.obl)
+            hipLaunchKernelGGL((k_up_scatter<T, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, up[0],
+                               up[1], up[2]);
+        else
+            hipLaunchKernelGGL((k_up_scatter<T, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, up[0],
+                               up[1], up[2]);
     return hipGetLastError();
 }
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s) {
@@ -1030,7 +1089,10 @@ hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, doubl
     return hipGetLastError();
 }
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s) {
-    if (pp.np > 0) hipLaunchKernelGGL(k_forces, dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
+    if (pp.np > 0 && pp.obl)
+        hipLaunchKernelGGL(k_forces<true>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
+    else if (pp.np > 0)
+        hipLaunchKernelGGL(k_forces<false>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
     return hipGetLastError();
 }
 hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s, int noise, uint64_t seed,

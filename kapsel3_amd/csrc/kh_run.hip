@@ -119,6 +119,10 @@ struct kh_run {
     CTime        jikan;
     int          step_path  = KH_PATH_FUSED;
     int          zeta_dirty = 1;  // zeta may hold non-kept modes
+    // Lees-Edwards shear (kh_set_shear): DC rate or AC rate*cos(2 pi freq t);
+    // deg = degree_oblique (strain of the oblique grid), sre = Shear_rate_eff
+    int          shear = 0, shear_ac = 0;
+    double       shear_rate = 0.0, shear_freq = 0.0, deg = 0.0, sre = 0.0, xyaspect = 1.0;
     // measurement
     bool                               timing = false;
     std::map<std::string, KernelTimes> times;
@@ -377,9 +381,9 @@ struct Step {
     // final U2zeta_k of u* + f is taken directly from u*_hat + P f_hat.
     // one step + the device step counter
     static int step(kh_run *r) {
-        KH_TRY(step_body(r));
-        if (r->step_path == KH_PATH_REFERENCE_ORDER)  // the fused body ticks inside z_force (or right after it)
-            KH_LAUNCH("tick", tick(r->d_ts, r->stream));
+        KH_TRY(r->shear ? step_shear(r) : step_body(r));
+        // the fused body ticks inside z_force (or right after it)
+        if (r->shear || r->step_path == KH_PATH_REFERENCE_ORDER) KH_LAUNCH("tick", tick(r->d_ts, r->stream));
         if (r->traj)
             KH_LAUNCH("record", record(r->pp, r->dp, r->d_ts, r->traj_base, r->traj_steps, r->traj, r->stream));
         if (r->frames)
@@ -486,8 +490,78 @@ struct Step {
         return KH_OK;
     }
 
-    // Init_zeta_k on a fluid at rest + sp_3d_ns.cxx:809-826
+    // Time_evolution_hydro_OBL (sp_3d_ns.cxx:328-433): the reference's
+    // sequence with the oblique-grid operators of kh_shear.hip. The strain
+    // and the shear rate are host state, passed to each launch by value.
+    // Shear_rate_eff is the set rate: Calc_shear_rate_eff (sp_3d_ns.h:189-203)
+    // adds the mean of a centred difference over the periodic grid, which
+    // telescopes to zero.
+    static int step_shear(kh_run *r) {
+        const int    ab2 = r->jikan.ts != 0;
+        const Grid  &g   = r->g;
+        hipStream_t  s   = r->stream;
+        const double gd  = r->deg, dt = r->jikan.dt_fluid;
+        cpx<T>      *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
+        T           *u3[3] = {U(r, 0), U(r, 1), U(r, 2)}, *w3[3] = {W(r, 0), W(r, 1), W(r, 2)};
+        r->pclean          = false;
+        // NS_solver_slavedEuler_Shear_OBL: u and omega of the truncated zeta,
+        // u x omega, the shear terms, curl, slaved Euler
+        KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], S(r, 3), r->ukdc, 0, 1,
+                                                     gd, s)));
+        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], u3[d]));
+        KH_LAUNCH("obl_zeta_to_omega", (obl_zeta_to_omega<T>(g, Z(r, 0), Z(r, 1), s3, s)));
+        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], w3[d]));
+        KH_LAUNCH("obl_cross", (obl_cross<T>(g.nreal, u3, w3, s)));
+        for (int d = 0; d < 3; d++) KH_TRY(r2c(r, w3[d], s3[d]));
+        KH_LAUNCH("obl_adv_ns", (obl_adv_ns<T>(g, s3, S(r, 3), Z(r, 0), Z(r, 1), r->sre, gd, -r->nu * dt, 1. / r->nu,
+                                               s)));
+        // FIX_CELL, Zeta_k2u_k_OBL, U_k2u: u* on the oblique grid
+        KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], nullptr, r->ukdc,
+                                                     r->fixmask, 0, gd, s)));
+        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], u3[d]));
+        // the strain advances; past +-NX/NY the grid is remapped (Reset_U_OBL)
+        r->sre = r->shear_ac ? r->shear_rate * cos(2. * KH_PI * r->shear_freq * dt * r->jikan.ts) : r->shear_rate;
+        r->deg += r->sre * dt;
+        if (r->deg >= r->xyaspect || r->deg <= -r->xyaspect) {
+            const bool up = r->deg >= r->xyaspect;
+            KH_LAUNCH("obl_remap", (obl_remap<T>(g, u3, w3, up ? r->xyaspect : -r->xyaspect, s)));
+            for (int d = 0; d < 3; d++) std::swap(r->ureal[d], r->work[d]);
+            for (int d = 0; d < 3; d++) {
+                u3[d] = U(r, d);
+                w3[d] = W(r, d);
+            }
+            r->deg += up ? -r->xyaspect : r->xyaspect;
+        }
+        // u* on the Cartesian grid with the mean flow (ucp) into W
+        KH_LAUNCH("obl_to_cart", (obl_to_cart<T>(g, u3, w3, r->deg, r->sre, s)));
+        r->pp.obl_g    = r->deg;
+        r->pp.obl_jump = r->sre * r->pp.L[1];
+        // particles: positions, phi, F_H on ucp, velocities, noise, u_p
+        KH_TRY(pos_update(r, ab2));
+        KH_TRY(make_phi(r));
+        KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, w3, PS(r), CP(r), (-r->dx3 * r->rho) / dt, s)));
+        KH_TRY(velocity_update(r, ab2));
+        if (r->kbt > 0.) KH_TRY(thermo(r));
+        KH_TRY(make_up(r));
+        // f = u_p - phi ucp onto the oblique grid, u += f; project; zeta
+        T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+        KH_LAUNCH("obl_add_f", (obl_add_f<T>(g, up3, PS(r), w3, u3, r->deg, s)));
+        for (int d = 0; d < 3; d++) KH_TRY(r2c(r, u3[d], s3[d]));
+        KH_LAUNCH("obl_u_to_zeta", (obl_u_to_zeta<T>(g, s3[0], s3[1], s3[2], Z(r, 0), Z(r, 1), r->ukdc, r->deg, s)));
+        r->zeta_dirty = 1;  // zeta keeps the modes of f beyond the 2/3 rule
+        return KH_OK;
+    }
+
+    // Init_zeta_k on a fluid at rest + sp_3d_ns.cxx:809-826 (the plain
+    // functions, whatever the run: the reference calls them for sheared runs too)
     static int init_fluid(kh_run *r) {
+        const int obl = r->pp.obl;
+        r->pp.obl     = 0;
+        const int st  = init_fluid_plain(r);
+        r->pp.obl     = obl;
+        return st;
+    }
+    static int init_fluid_plain(kh_run *r) {
         r->pclean = false;
         KH_HIP(hipMemsetAsync(r->zeta[0], 0, sizeof(cpx<T>) * r->g.nspec, r->stream));
         KH_HIP(hipMemsetAsync(r->zeta[1], 0, sizeof(cpx<T>) * r->g.nspec, r->stream));
@@ -1110,7 +1184,7 @@ int kh_step(kh_run *r, int nsteps) {
     for (int s = 0; s < nsteps;) {
         // (the slip loop decides its length on the host: not captured)
         const bool graph_ok = r->use_graph && !r->timing && r->step_path == KH_PATH_FUSED && r->jikan.ts > 0 &&
-                              !r->zeta_dirty && !r->slip && (!r->zclear || r->pclean);
+                              !r->zeta_dirty && !r->slip && !r->shear && (!r->zclear || r->pclean);
         int done = 1;
         if (graph_ok && nsteps - s >= GRAPH_STEPS) {
             if (!r->graph_exec_k) KH_TRY(capture_steps(r, GRAPH_STEPS, &r->graph_k, &r->graph_exec_k));
@@ -1130,6 +1204,50 @@ int kh_step(kh_run *r, int nsteps) {
         }
         s += done;
     }
+    return KH_OK;
+}
+
+int kh_set_shear(kh_run *r, double shear_rate, int ac, double frequency) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    if (!std::isfinite(shear_rate) || !std::isfinite(frequency)) return fail(KH_ERR_ARG, "non-finite shear parameter");
+    if (r->slip) return fail(KH_ERR_UNSUPPORTED, "the sheared step has no squirmer slip iteration");
+    drop_graph(r);
+    r->shear      = 1;
+    r->shear_ac   = ac ? 1 : 0;
+    r->shear_rate = shear_rate;
+    r->shear_freq = frequency;
+    r->xyaspect   = (double)r->g.nx / (double)r->g.ny;  // sp_3d_ns.cxx:772
+    r->pp.obl     = 1;
+    r->pp.obl_g   = r->deg;
+    r->pp.obl_jump = r->sre * r->pp.L[1];
+    // the sheared runs' LJ: every pair once (md_force.cxx:198-305), and the
+    // diameter min((SIGMA + XI)/2^(1/2n), SIGMA) (input.cxx:549-566)
+    r->pp.lj_cell_compat = 0;
+    const double sigma   = 2.0 * r->radius;
+    const double pw      = r->P.lj_powers == 0 ? pow(2., 1. / 6.) : r->P.lj_powers == 1 ? pow(2., 1. / 12.)
+                                                                                       : pow(2., 1. / 18.);
+    const double dia     = std::min((sigma + r->xi) / pw, sigma);
+    r->pp.lj_cutoff      = sigma > 0.0 ? r->lj_cutoff / sigma * dia : 0.0;
+    r->pp.lj_dia         = dia;
+    return KH_OK;
+}
+
+int kh_get_shear_state(kh_run *r, double *degree_oblique, double *shear_rate_eff) {
+    if (!r || !degree_oblique || !shear_rate_eff) return fail(KH_ERR_ARG, "NULL argument");
+    *degree_oblique = r->deg;
+    *shear_rate_eff = r->sre;
+    return KH_OK;
+}
+
+int kh_set_shear_state(kh_run *r, double degree_oblique, double shear_rate_eff) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    if (!r->shear) return fail(KH_ERR_STATE, "kh_set_shear first");
+    if (!std::isfinite(degree_oblique) || !std::isfinite(shear_rate_eff) || fabs(degree_oblique) >= r->xyaspect)
+        return fail(KH_ERR_ARG, "strain must be finite and inside (-NX/NY, NX/NY)");
+    r->deg         = degree_oblique;
+    r->sre         = shear_rate_eff;
+    r->pp.obl_g    = r->deg;
+    r->pp.obl_jump = r->sre * r->pp.L[1];
     return KH_OK;
 }
 

@@ -1,0 +1,134 @@
+"""Lees-Edwards sheared step (Time_evolution_hydro_OBL, sp_3d_ns.cxx:328-433)
+on the GPU against the oracle, through the C ABI.
+
+FP64: whole-step trajectories and the final zeta within 1e-10 of the
+oracle's (relative to each quantity's scale), across a remap of the grid
+and for AC shear. Mixed precision: within 1e-4 of FP64 over 200 steps."""
+import numpy as np
+import pytest
+
+from oracle import kref
+from kapsel3_amd import workloads
+from kapsel3_amd._abi import FP64, MIXED, Params, make_species
+
+from test_gpu_step import cmp_particles, rel
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ["x", "x_nopbc", "v", "v_old", "omega", "omega_old", "q", "q_old", "f_hydro_previous",
+        "torque_hydro_previous", "fr_previous"]
+
+
+def sheared_system(n=32, precision=FP64, kbt=1e-4):
+    """Four spheres: one straddling the sheared boundary, a pair in LJ
+    range across it, one near the centre; gravity, rotation, FIX_CELL on z,
+    thermal noise, a random initial flow."""
+    sp = [make_species(mass_ratio=1.5), make_species(mass_ratio=2.0)]
+    P = Params(n, a=4.0, a_xi=2.0, dt=0.05, eta=0.8, species=sp, rotation=1, g=1e-3, g_dir=2, kbt=kbt,
+               fix_cell=(0, 0, 1), lj_truncate=1, epsilon=1.0, precision=precision, seed=9)
+    g0 = 0.995
+    pos = np.array([[10.2, 30.6, 16.1], [3.0, 29.5, 8.0], [(3.0 - g0 * n + 2.0) % n, 5.2, 8.5],
+                    [17.3, 15.1, 24.2]])
+    p = workloads.particles_at(pos, spec=np.array([0, 1, 0, 1]))
+    rng = np.random.default_rng(21)
+    p["v"] = rng.normal(0, 1e-3, (4, 3))
+    p["v_old"] = p["v"] + rng.normal(0, 1e-4, (4, 3))
+    p["omega"] = rng.normal(0, 1e-4, (4, 3))
+    p["omega_old"] = rng.normal(0, 1e-4, (4, 3))
+    p["q"] = workloads.random_quaternions(4, 2)
+    p["q_old"] = workloads.random_quaternions(4, 3)
+    return P, p, g0
+
+
+def initial_fluid(P, n, g0, rate):
+    """A random flow through the oracle's own u -> zeta at strain g0."""
+    o = kref.Run(P, 0)
+    o.set_shear(rate)
+    o.set_shear_state(g0, rate)
+    rng = np.random.default_rng(5)
+    k = np.fft.fftfreq(n) * n
+    kz = np.arange(n // 2 + 1)
+    m = (np.abs(k)[:, None, None] < 6) & (np.abs(k)[None, :, None] < 6) & (kz[None, None, :] < 6)
+    for d in range(3):
+        u = np.fft.irfftn(np.fft.rfftn(rng.normal(size=(n, n, n))) * m, s=(n, n, n), axes=(0, 1, 2)) * 2e-3
+        a = np.zeros((n, n, n + 2))
+        a[:, :, :n] = u
+        o.set_field("u%d" % d, a)
+    o.le_apply("solenoidal_u")
+    o.le_apply("u2zeta_k")
+    z0, z1, uk = o.get_fluid()
+    o.close()
+    return z0, z1, uk
+
+
+def _pair(P, p, z, rate, g0, ac=False, freq=0.0, ts0=0):
+    from kapsel3_amd import Run
+    gr = Run(P, len(p))
+    orc = kref.Run(P, len(p))
+    t = orc.get_time()
+    t.ts = ts0
+    t.time = ts0 * t.dt_fluid
+    for r in (gr, orc):
+        r.set_particles(p)
+        r.set_fluid(*z)
+        r.set_time(t)
+        r.set_shear(rate, ac=ac, frequency=freq)
+        r.set_shear_state(g0, rate)
+    return gr, orc
+
+
+@pytest.mark.parametrize("ac", [False, True])
+def test_sheared_step_fp64_matches_oracle(gpu, ac):
+    n, rate = 32, 0.4 if not ac else 0.3
+    P, p, g0 = sheared_system(n)
+    z = initial_fluid(P, n, g0, rate)
+    gr, orc = _pair(P, p, z, rate, g0, ac=ac, freq=0.5, ts0=0 if not ac else 3)
+    seen_remap = False
+    for chunk in range(4):
+        gr.step(3)
+        orc.step(3)
+        gg, sg = gr.shear_state()
+        go, so = orc.shear_state()
+        assert abs(gg - go) < 1e-14 and abs(sg - so) < 1e-14 * max(abs(so), 1)
+        seen_remap |= gg < g0
+        cmp_particles(gr.get_particles(), orc.get_particles(), 1e-10, keys=KEYS, p0=p, a=4.0)
+    zg, zo = gr.get_fluid(), orc.get_fluid()
+    for a, b in zip(zg[:2], zo[:2]):
+        assert rel(a, b) < 1e-10
+    assert np.abs(zg[2] - zo[2]).max() < 1e-10 * max(np.abs(zo[2]).max(), 1e-30) + 1e-18
+    assert seen_remap or ac                  # the DC case crosses strain 1 and is remapped
+    gr.close()
+
+
+def test_sheared_mixed_tracks_fp64(gpu):
+    from kapsel3_amd import Run
+    n, rate = 32, 0.05
+    out = {}
+    for prec in (FP64, MIXED):
+        P, p, g0 = sheared_system(n, precision=prec, kbt=0.0)
+        r = Run(P, len(p))
+        r.set_particles(p)
+        r.init_fluid()
+        r.set_shear(rate)
+        r.step(200)
+        out[prec] = r.get_particles()
+        r.close()
+    a, b = out[MIXED], out[FP64]
+    vs = np.abs(b["v"]).max()
+    assert np.abs(a["v"] - b["v"]).max() < 1e-4 * vs
+    assert np.abs(a["omega"] - b["omega"]).max() < 1e-4 * vs / 4.0
+    assert np.abs(a["x_nopbc"] - b["x_nopbc"]).max() < 1e-4 * np.abs(b["x_nopbc"] - p["x_nopbc"]).max()
+
+
+def test_shear_entry_points_reject_bad_input(gpu):
+    from kapsel3_amd import KhError, Run
+    P, p, g0 = sheared_system(16 * 2)
+    r = Run(P, len(p))
+    with pytest.raises(KhError):
+        r.set_shear_state(0.1, 0.0)          # not sheared yet
+    r.set_shear(0.1)
+    with pytest.raises(KhError):
+        r.set_shear_state(1.5, 0.1)          # outside (-NX/NY, NX/NY)
+    with pytest.raises(KhError):
+        r.set_shear(float("nan"))
+    r.close()
