@@ -167,6 +167,34 @@ def squirmer_bench(steps, n=256, count=1000):
     return out
 
 
+def shear_bench(steps, n=256, nparticles=2000, rate=1e-3):
+    """The Lees-Edwards sheared step (kh_set_shear) on the config-3
+    suspension: steps/s in mixed precision and FP64 after 2 warm-up steps."""
+    import kapsel3_amd
+    from kapsel3_amd import workloads
+    from kapsel3_amd._abi import FP64, MIXED
+    out = {"workload": f"{n}^3, {nparticles} spheres a=5 xi=2, LJ + gravity 1e-3 + kT 1e-3, "
+                       f"Lees-Edwards shear rate {rate:g}", "steps": steps}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+        P, parts = workloads.sedimentation(n, nparticles, kbt=1e-3, precision=prec, seed=3)
+        run = kapsel3_amd.Run(P, nparticles)
+        run.set_particles(parts)
+        run.init_fluid()
+        run.set_shear(rate)
+        run.step(2)
+        run.sync()
+        t0 = time.perf_counter()
+        run.step(steps)
+        run.sync()
+        t = time.perf_counter() - t0
+        p = run.get_particles()
+        run.close()
+        out[name] = {"value": steps / t, "unit": "steps/s", "ms_per_step": t / steps * 1e3,
+                     "finite": bool(np.isfinite(p["v"]).all())}
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def output_bench(steps, every, n=256, nparticles=2000):
     """Output_udf frames recorded on the device inside the captured step
     (kh_set_output) at the config-3 workload, mixed: steps/s with a frame
@@ -276,6 +304,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-squirm", action="store_true", help="skip the squirmer suspension")
     ap.add_argument("--squirm-steps", type=int, default=10)
+    ap.add_argument("--no-shear", action="store_true", help="skip the Lees-Edwards sheared suspension")
+    ap.add_argument("--shear-steps", type=int, default=20)
     ap.add_argument("--no-output", action="store_true", help="skip the Output_udf frame recording figure")
     ap.add_argument("--output-steps", type=int, default=50)
     ap.add_argument("--output-every", type=int, default=5)
@@ -343,6 +373,8 @@ def main():
             out["accuracy_mixed_vs_fp64"] = accuracy(head_state, f64_state, a.warmup + 2 * a.steps, 5.0)
     if not a.no_squirm:
         out["squirmers"] = squirmer_bench(a.squirm_steps)
+    if not a.no_shear:
+        out["sheared"] = shear_bench(a.shear_steps)
     if not a.no_output:
         out["output_frames"] = output_bench(a.output_steps, a.output_every)
 

@@ -197,9 +197,13 @@ class Run:
         z0, z1, uk = self.get_fluid()
         t = self.get_time()
         c = self.params.c
+        extra = {}
+        sh = getattr(self, "_shear", None)
+        if sh is not None:  # Lees-Edwards: the shear and its state (degree_oblique, Shear_rate_eff)
+            extra["shear"] = np.array([sh[0], float(sh[1]), sh[2], *self.shear_state()])
         np.savez(path, zeta0=z0, zeta1=z1, uk_dc=uk, particles=self.get_particles(),
                  ts=np.int64(t.ts), time=np.array([t.time, t.dt_fluid, t.hdt_fluid, t.dt_md, t.hdt_md]),
-                 grid=np.array([c.nx, c.ny, c.nz]), precision=np.int64(c.precision))
+                 grid=np.array([c.nx, c.ny, c.nz]), precision=np.int64(c.precision), **extra)
 
     def load_state(self, path):
         """Restore a state written by save_state (Force_restore_parameters +
@@ -216,6 +220,10 @@ class Run:
             self.set_fluid(f["zeta0"], f["zeta1"], f["uk_dc"])
             tm = f["time"]
             self.set_time(CTime(int(f["ts"]), *(float(v) for v in tm)))
+            if "shear" in f.files:
+                rate, ac, freq, deg, sre = (float(v) for v in f["shear"])
+                self.set_shear(rate, ac=ac != 0.0, frequency=freq)
+                self.set_shear_state(deg, sre)
 
     # ---- the step
     def init_fluid(self):
@@ -251,6 +259,7 @@ class Run:
         """Lees-Edwards shear from the next step on (Time_evolution_hydro_OBL):
         DC at `rate`, or AC rate*cos(2 pi frequency t)."""
         _check(lib().kh_set_shear(self.h, float(rate), int(bool(ac)), float(frequency)))
+        self._shear = (float(rate), bool(ac), float(frequency))
 
     def shear_state(self):
         """(degree_oblique, Shear_rate_eff) of a sheared run."""

@@ -6,9 +6,9 @@ positional values whose meaning comes from the schema of its included
 structs, `select` fields, `name[]` arrays). `read_udf` joins the two into a
 nested dict keyed by the schema's names, so `deck["mesh"]["NPX"]` reads the
 value input.cxx gets from `io_parser("mesh.NPX")`. `deck_params` then maps
-the keys input.cxx reads for the SPM step (Navier_Stokes fluid, spherical
-particles; input.cxx:742-760, 1235-1470, 1700-1905, 2305-2310) onto
-`Params`, and `deck_particles` / `restore_deck_state` give the initial
+the keys input.cxx reads for the SPM step (Navier_Stokes or
+Shear_Navier_Stokes_Lees_Edwards fluid, spherical particles; input.cxx:742-833,
+1235-1470, 1700-1905, 2305-2310) onto `Params` (and the deck's shear), and `deck_particles` / `restore_deck_state` give the initial
 particles and, for `resume.Calculation = CONTINUE`, the saved state
 (Save_Restart_udf's record, resume.cxx:10-74, 381-506).
 
@@ -343,11 +343,13 @@ class Deck:
     snapshots (GTS, Num_snap; MSTEP = GTS*Num_snap, the loop running
     MSTEP + 1 steps, sp_3d_ns.cxx:857), placement and orientation rules."""
 
-    def __init__(self, data, params, counts, gts, num_snap, distribution, orientation):
+    def __init__(self, data, params, counts, gts, num_snap, distribution, orientation, shear=None):
         self.data, self.params, self.counts = data, params, counts
         self.gts, self.num_snap = gts, num_snap
         self.mstep = gts * num_snap
         self.distribution, self.orientation = distribution, orientation
+        # (rate, ac, frequency) of a Shear_Navier_Stokes_Lees_Edwards deck
+        self.shear = shear
 
     @property
     def nparticles(self):
@@ -359,11 +361,22 @@ def _vec(v):
 
 
 def deck_params(data, precision=FP64, seed=12345):
-    """input.cxx's reading of a Navier_Stokes / spherical_particle deck."""
+    """input.cxx's reading of a Navier_Stokes or Shear_Navier_Stokes_Lees_Edwards
+    (input.cxx:796-833) / spherical_particle deck."""
     ce = data["constitutive_eq"]
-    if ce["type"] != "Navier_Stokes":
-        raise UdfError(f"constitutive_eq {ce['type']!r}: this step implements Navier_Stokes")
-    ns = ce["Navier_Stokes"]
+    if ce["type"] not in ("Navier_Stokes", "Shear_Navier_Stokes_Lees_Edwards"):
+        raise UdfError(f"constitutive_eq {ce['type']!r}: this step implements Navier_Stokes and "
+                       "Shear_Navier_Stokes_Lees_Edwards")
+    ns = ce[ce["type"]]
+    shear = None
+    if ce["type"] == "Shear_Navier_Stokes_Lees_Edwards":
+        ef = ns["External_field"]
+        if ef["type"] == "DC":
+            shear = (float(ef["DC"]["Shear_rate"]), False, 0.0)
+        elif ef["type"] == "AC":
+            shear = (float(ef["AC"]["Shear_rate"]), True, float(ef["AC"]["Frequency"]))
+        else:
+            raise UdfError(f"External_field {ef['type']!r}")
     ot = data["object_type"]
     if ot["type"] != "spherical_particle":
         raise UdfError(f"object_type {ot['type']!r}: this step implements spherical_particle")
@@ -404,7 +417,7 @@ def deck_params(data, precision=FP64, seed=12345):
                precision=precision, seed=seed)
     out = data["output"]
     return Deck(data, P, counts, int(out["GTS"]), int(out["Num_snap"]), sw["INIT_distribution"],
-                sw["INIT_orientation"])
+                sw["INIT_orientation"], shear)
 
 
 def read_deck(path, precision=FP64, seed=12345):
@@ -493,6 +506,12 @@ def restore_deck_state(run, deck):
     t = run.get_time()
     run.set_time(CTime(int(sd["jikan"]["ts"]), float(sd["jikan"]["time"]), t.dt_fluid, t.hdt_fluid, t.dt_md,
                        t.hdt_md))
+    if deck.shear is not None:
+        # the strain is restored (resume.cxx:562-567); Shear_rate_eff is not
+        # saved, so the resumed process starts it from its initial value 0
+        # until the first step sets it
+        run.set_shear(*deck.shear)
+        run.set_shear_state(float(sd.get("oblique", {}).get("degree_oblique", 0.0)), 0.0)
     return True
 
 
@@ -521,7 +540,8 @@ def restart_record(run):
             for p0, p1 in zip(z0.tolist(), z1.tolist())]
     return {"Calculation": "CONTINUE",
             "CONTINUE": {"Saved_Data": {"jikan": {"ts": int(t.ts), "time": float(t.time)}, "Particles": parts,
-                                        "Zeta": zeta, "uk_dc": v3(uk)}}}
+                                        "Zeta": zeta, "uk_dc": v3(uk),
+                                        "oblique": {"degree_oblique": float(run.shear_state()[0])}}}}
 
 
 def save_restart_udf(run, deck_path, out_path, include=None):

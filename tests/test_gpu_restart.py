@@ -26,10 +26,12 @@ def _close(a, b, tol):
     assert a.get_time().ts == b.get_time().ts
 
 
-def _resume(gpu, P, parts, tmp_path, before, after, tol):
+def _resume(gpu, P, parts, tmp_path, before, after, tol, shear=None):
     a = gpu.Run(P, len(parts))
     a.set_particles(parts)
     a.init_fluid()
+    if shear is not None:
+        a.set_shear(*shear)
     a.step(before)
     path = tmp_path / "state.npz"
     a.save_state(path)
@@ -39,6 +41,8 @@ def _resume(gpu, P, parts, tmp_path, before, after, tol):
     assert b.get_time().ts == before
     b.step(after)
     _close(b, a, tol)
+    if shear is not None:
+        assert b.shear_state() == a.shear_state()
     a.close()
     b.close()
 
@@ -48,6 +52,15 @@ def test_resume_thermal_sedimentation(gpu, tmp_path, precision, tol):
     print(f"restart 32^3 x 6 thermal, precision {precision}", flush=True)
     P, parts = workloads.sedimentation(32, 6, a=4, kbt=1e-3, gap=0.4, seed=11, precision=precision)
     _resume(gpu, P, parts, tmp_path, before=4, after=5, tol=tol)
+
+
+@pytest.mark.parametrize("ac", [False, True], ids=["dc", "ac"])
+def test_resume_sheared(gpu, tmp_path, ac):
+    """The Lees-Edwards state (rate, strain, Shear_rate_eff) travels with the
+    npz state: the resumed run continues exactly."""
+    print(f"restart 32^3 x 6 sheared, ac={ac}", flush=True)
+    P, parts = workloads.sedimentation(32, 6, a=4, kbt=1e-3, gap=0.4, seed=11, precision=FP64)
+    _resume(gpu, P, parts, tmp_path, before=4, after=5, tol=1e-12, shear=(0.05, ac, 0.2))
 
 
 def test_resume_squirmers_fp64(gpu, tmp_path):

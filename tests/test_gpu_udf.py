@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DECK = os.path.join(HERE, "golden", "udf", "sedimentation.udf")
+SHEAR = os.path.join(HERE, "golden", "udf", "shear.udf")
 
 
 def test_deck_run_and_udf_restart(gpu, tmp_path):
@@ -38,5 +39,35 @@ def test_deck_run_and_udf_restart(gpu, tmp_path):
     _close(b, a, 1e-12)
     pa = a.get_particles()
     assert np.isfinite(pa["v"]).all() and np.abs(pa["v"]).max() > 0
+    a.close()
+    b.close()
+
+
+def test_sheared_deck_and_udf_restart(gpu, tmp_path):
+    """A Lees-Edwards deck (the strain passes a remap on the way) restarted
+    through a UDF deck: the record carries degree_oblique
+    (resume.cxx:67-73); the resumed process starts Shear_rate_eff at 0 as
+    the reference's does, so the uninterrupted run is given the same."""
+    print("udf sheared deck 32^3 x 3, restart through a UDF deck", flush=True)
+    dk = udf.read_deck(SHEAR)
+    a = gpu.Run(dk.params, dk.nparticles)
+    a.set_particles(udf.deck_particles(dk))
+    a.init_fluid()
+    a.set_shear(*dk.shear)
+    a.set_shear_state(0.999, dk.shear[0])
+    a.step(4)
+    g4 = a.shear_state()[0]
+    assert 0.0 < g4 < 0.01                        # remapped
+    out = tmp_path / "restart.udf"
+    udf.save_restart_udf(a, SHEAR, out, include=os.path.join(HERE, "golden", "udf", "define.udf"))
+    a.set_shear_state(g4, 0.0)
+    a.step(5)
+    dk2 = udf.read_deck(out)
+    b = gpu.Run(dk2.params, dk2.nparticles)
+    assert udf.restore_deck_state(b, dk2)
+    assert b.shear_state() == (g4, 0.0)
+    b.step(5)
+    _close(b, a, 1e-12)
+    assert a.shear_state() == b.shear_state()
     a.close()
     b.close()

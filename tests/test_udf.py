@@ -61,6 +61,24 @@ def test_unsupported_model_is_refused(tmp_path):
         udf.read_deck(out)
 
 
+SHEAR = os.path.join(HERE, "golden", "udf", "shear.udf")
+
+
+def test_sheared_deck(tmp_path):
+    """Shear_Navier_Stokes_Lees_Edwards (input.cxx:796-833): the fluid block
+    of that model and its External_field give the shear."""
+    dk = udf.read_deck(SHEAR)
+    c = dk.params.c
+    assert dk.shear == (0.02, False, 0.0)
+    assert (c.nx, c.eta, c.kbt, c.dt, c.rotation) == (32, 1.0, 0.0, 0.05, 1)
+    assert dk.nparticles == 3 and tuple(c.fix_cell) == (0, 0, 0)
+    d, schema, classes = udf.read_udf_schema(SHEAR)
+    d["constitutive_eq"]["Shear_Navier_Stokes_Lees_Edwards"]["External_field"]["type"] = "AC"
+    out = tmp_path / "ac.udf"
+    udf.write_udf(out, d, schema, classes, include=os.path.join(HERE, "golden", "udf", "define.udf"))
+    assert udf.read_deck(out).shear == (0.02, True, 0.1)
+
+
 def test_write_read_round_trip(tmp_path):
     d, schema, classes = udf.read_udf_schema(DECK)
     d["resume"] = {"Calculation": "CONTINUE", "CONTINUE": {"Saved_Data": {
@@ -95,3 +113,5 @@ def test_reference_decks_parse():
     assert (c.nx, c.ny, c.nz) == (256, 256, 512) and dk.nparticles == 3204
     assert (c.dx, c.eta, c.g, c.a) == (0.1, 5.0, 980.665, 5.0)
     assert dk.mstep == 50 * 500
+    le = udf.read_deck(os.path.join(REF, "Examples", "07", "colloid_LE_v31_t01.udf"))
+    assert le.shear == (0.001, False, 0.0) and le.nparticles == 300 and le.params.c.nx == 64
