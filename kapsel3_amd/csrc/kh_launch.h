@@ -42,7 +42,8 @@ hipError_t obl_adv_ns(const Grid &g, cpx<T> *const F[3], const cpx<T> *w1, cpx<T
 template <typename T>
 hipError_t obl_remap(const Grid &g, T *const u[3], T *const o[3], double delta, hipStream_t s);
 template <typename T>
-hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, hipStream_t s);
+hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, const T *mask,
+                       hipStream_t s);
 template <typename T>
 hipError_t obl_add_f(const Grid &g, T *const up[3], const T *phisum, T *const c[3], T *const u[3], double gd,
                      hipStream_t s);

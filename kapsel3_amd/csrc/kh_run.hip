@@ -490,12 +490,15 @@ struct Step {
         return KH_OK;
     }
 
-    // Time_evolution_hydro_OBL (sp_3d_ns.cxx:328-433): the reference's
-    // sequence with the oblique-grid operators of kh_shear.hip. The strain
-    // and the shear rate are host state, passed to each launch by value.
-    // Shear_rate_eff is the set rate: Calc_shear_rate_eff (sp_3d_ns.h:189-203)
-    // adds the mean of a centred difference over the periodic grid, which
-    // telescopes to zero.
+    // Time_evolution_hydro_OBL (sp_3d_ns.cxx:328-433) with the oblique-grid
+    // operators of kh_shear.hip. The strain and the shear rate are host
+    // state passed to each launch by value; this step's values are known
+    // before it starts, so the particles move and scatter phi first (the
+    // position update reads only the particle state: it commutes with the
+    // fluid part), and the Cartesian u* and f = u_p - phi u* are then formed
+    // only where phi_sum is nonzero. Shear_rate_eff is the set rate:
+    // Calc_shear_rate_eff (sp_3d_ns.h:189-203) adds the mean of a centred
+    // difference over the periodic grid, which telescopes to zero.
     static int step_shear(kh_run *r) {
         const int    ab2 = r->jikan.ts != 0;
         const Grid  &g   = r->g;
@@ -504,8 +507,20 @@ struct Step {
         cpx<T>      *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
         T           *u3[3] = {U(r, 0), U(r, 1), U(r, 2)}, *w3[3] = {W(r, 0), W(r, 1), W(r, 2)};
         r->pclean          = false;
-        // NS_solver_slavedEuler_Shear_OBL: u and omega of the truncated zeta,
-        // u x omega, the shear terms, curl, slaved Euler
+        // this step's rate and strain; past +-NX/NY the grid is remapped
+        // (Reset_U_OBL) after the fluid part
+        const double sre = r->shear_ac ? r->shear_rate * cos(2. * KH_PI * r->shear_freq * dt * r->jikan.ts)
+                                       : r->shear_rate;
+        double       deg   = gd + sre * dt;
+        const int    remap = deg >= r->xyaspect ? 1 : (deg <= -r->xyaspect ? -1 : 0);
+        deg -= remap * r->xyaspect;
+        r->pp.obl_g    = deg;
+        r->pp.obl_jump = sre * r->pp.L[1];
+        KH_TRY(pos_update(r, ab2));
+        KH_TRY(make_phi(r));
+        // NS_solver_slavedEuler_Shear_OBL at the previous strain and rate:
+        // u and omega of the truncated zeta, u x omega, the shear terms,
+        // curl, slaved Euler
         KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], S(r, 3), r->ukdc, 0, 1,
                                                      gd, s)));
         for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], u3[d]));
@@ -519,35 +534,28 @@ struct Step {
         KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], nullptr, r->ukdc,
                                                      r->fixmask, 0, gd, s)));
         for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], u3[d]));
-        // the strain advances; past +-NX/NY the grid is remapped (Reset_U_OBL)
-        r->sre = r->shear_ac ? r->shear_rate * cos(2. * KH_PI * r->shear_freq * dt * r->jikan.ts) : r->shear_rate;
-        r->deg += r->sre * dt;
-        if (r->deg >= r->xyaspect || r->deg <= -r->xyaspect) {
-            const bool up = r->deg >= r->xyaspect;
-            KH_LAUNCH("obl_remap", (obl_remap<T>(g, u3, w3, up ? r->xyaspect : -r->xyaspect, s)));
+        if (remap) {
+            KH_LAUNCH("obl_remap", (obl_remap<T>(g, u3, w3, remap * r->xyaspect, s)));
             for (int d = 0; d < 3; d++) std::swap(r->ureal[d], r->work[d]);
             for (int d = 0; d < 3; d++) {
                 u3[d] = U(r, d);
                 w3[d] = W(r, d);
             }
-            r->deg += up ? -r->xyaspect : r->xyaspect;
         }
-        // u* on the Cartesian grid with the mean flow (ucp) into W
-        KH_LAUNCH("obl_to_cart", (obl_to_cart<T>(g, u3, w3, r->deg, r->sre, s)));
-        r->pp.obl_g    = r->deg;
-        r->pp.obl_jump = r->sre * r->pp.L[1];
-        // particles: positions, phi, F_H on ucp, velocities, noise, u_p
-        KH_TRY(pos_update(r, ab2));
-        KH_TRY(make_phi(r));
+        r->sre = sre;
+        r->deg = deg;
+        // u* on the Cartesian grid with the mean flow (ucp) into W, where phi_sum != 0
+        KH_LAUNCH("obl_to_cart", (obl_to_cart<T>(g, u3, w3, deg, sre, PS(r), s)));
+        // F_H on ucp, velocities, noise, u_p
         KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, w3, PS(r), CP(r), (-r->dx3 * r->rho) / dt, s)));
         KH_TRY(velocity_update(r, ab2));
         if (r->kbt > 0.) KH_TRY(thermo(r));
         KH_TRY(make_up(r));
         // f = u_p - phi ucp onto the oblique grid, u += f; project; zeta
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
-        KH_LAUNCH("obl_add_f", (obl_add_f<T>(g, up3, PS(r), w3, u3, r->deg, s)));
+        KH_LAUNCH("obl_add_f", (obl_add_f<T>(g, up3, PS(r), w3, u3, deg, s)));
         for (int d = 0; d < 3; d++) KH_TRY(r2c(r, u3[d], s3[d]));
-        KH_LAUNCH("obl_u_to_zeta", (obl_u_to_zeta<T>(g, s3[0], s3[1], s3[2], Z(r, 0), Z(r, 1), r->ukdc, r->deg, s)));
+        KH_LAUNCH("obl_u_to_zeta", (obl_u_to_zeta<T>(g, s3[0], s3[1], s3[2], Z(r, 0), Z(r, 1), r->ukdc, deg, s)));
         r->zeta_dirty = 1;  // zeta keeps the modes of f beyond the 2/3 rule
         return KH_OK;
     }

@@ -265,12 +265,15 @@ __device__ __forceinline__ void orth2obl(const Grid &g, double gd, int j, int i,
 }
 
 // U_oblique2u (fft_wrapper.h:480-505): oblique grid, contravariant
-// components -> Cartesian grid and components, plus the mean flow.
+// components -> Cartesian grid and components, plus the mean flow. With a
+// mask (phi_sum) only the points where it is nonzero are written: the
+// step reads this field nowhere else (F_H gathers and f = u_p - phi u
+// are zero-weighted there).
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_to_cart(Grid g, const T *__restrict__ u0, const T *__restrict__ u1,
                                                        const T *__restrict__ u2, T *__restrict__ c0,
                                                        T *__restrict__ c1, T *__restrict__ c2, double gd,
-                                                       double rate) {
+                                                       double rate, const T *__restrict__ mask) {
     const long long t = (long long)blockIdx.x * SH_NT + threadIdx.x;
     if (t >= g.nreal) return;
     const int       k    = (int)(t % g.nz);
@@ -280,6 +283,7 @@ __global__ __launch_bounds__(SH_NT) void k_obl_to_cart(Grid g, const T *__restri
     double          alpha, beta;
     obl2orth(g, gd, j, i, ip, io, alpha, beta);
     const long long qp = ((long long)ip * g.ny + j) * g.nz + k, qo = ((long long)io * g.ny + j) * g.nz + k;
+    if (mask && mask[qo] == (T)0) return;
     const double    a0 = beta * (double)u0[t] + alpha * (double)u0[qp];
     const double    a1 = beta * (double)u1[t] + alpha * (double)u1[qp];
     const double    a2 = beta * (double)u2[t] + alpha * (double)u2[qp];
@@ -295,7 +299,22 @@ __global__ __launch_bounds__(SH_NT) void k_obl_to_cart(Grid g, const T *__restri
 
 // Make_f_particle_dt_nonsole (f_particle.h:35-54) on the Cartesian grid,
 // U2u_oblique (fft_wrapper.h:448-478) onto the oblique grid and
-// Add_f_particle (f_particle.h:68-84), per oblique point.
+// Add_f_particle (f_particle.h:68-84), per oblique point. Where phi_sum is
+// zero u_p is zero too, so f = 0 there: such points are not read (the
+// Cartesian u* may be unwritten at them), and an oblique point whose two
+// source points both have f = 0 keeps its u (u + 0).
+template <typename T>
+__device__ __forceinline__ void f_at(const T *up0, const T *up1, const T *up2, const T *c0, const T *c1, const T *c2,
+                                     T ps, long long q, T *f) {
+    if (ps == (T)0) {
+        f[0] = f[1] = f[2] = (T)0;
+        return;
+    }
+    const T ph = fmin(ps, (T)1);
+    f[0]       = up0[q] - ph * c0[q];
+    f[1]       = up1[q] - ph * c1[q];
+    f[2]       = up2[q] - ph * c2[q];
+}
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_add_f(Grid g, const T *__restrict__ up0, const T *__restrict__ up1,
                                                      const T *__restrict__ up2, const T *__restrict__ phisum,
@@ -311,12 +330,14 @@ __global__ __launch_bounds__(SH_NT) void k_obl_add_f(Grid g, const T *__restrict
     double          alpha, beta;
     orth2obl(g, gd, j, i, io, iop, alpha, beta);
     const long long qa = ((long long)io * g.ny + j) * g.nz + k, qb = ((long long)iop * g.ny + j) * g.nz + k;
-    const T pa = fmin(phisum[qa], (T)1), pb = fmin(phisum[qb], (T)1);
-    const T fa0 = up0[qa] - pa * c0[qa], fa1 = up1[qa] - pa * c1[qa], fa2 = up2[qa] - pa * c2[qa];
-    const T fb0 = up0[qb] - pb * c0[qb], fb1 = up1[qb] - pb * c1[qb], fb2 = up2[qb] - pb * c2[qb];
-    T f0 = (T)(beta * (double)fa0 + alpha * (double)fb0);
-    const T f1 = (T)(beta * (double)fa1 + alpha * (double)fb1);
-    const T f2 = (T)(beta * (double)fa2 + alpha * (double)fb2);
+    const T         pa = phisum[qa], pb = phisum[qb];
+    if (pa == (T)0 && pb == (T)0) return;
+    T fa[3], fb[3];
+    f_at(up0, up1, up2, c0, c1, c2, pa, qa, fa);
+    f_at(up0, up1, up2, c0, c1, c2, pb, qb, fb);
+    T f0 = (T)(beta * (double)fa[0] + alpha * (double)fb[0]);
+    const T f1 = (T)(beta * (double)fa[1] + alpha * (double)fb[1]);
+    const T f2 = (T)(beta * (double)fa[2] + alpha * (double)fb[2]);
     f0 = (T)((double)f0 - gd * (double)f1);
     u0[t] += f0;
     u1[t] += f1;
@@ -411,9 +432,10 @@ hipError_t obl_remap(const Grid &g, T *const u[3], T *const o[3], double delta, 
     return hipGetLastError();
 }
 template <typename T>
-hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, hipStream_t s) {
+hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, const T *mask,
+                       hipStream_t s) {
     hipLaunchKernelGGL(k_obl_to_cart<T>, dim3(sh_blocks(g.nreal)), dim3(SH_NT), 0, s, g, u[0], u[1], u[2], c[0], c[1],
-                       c[2], gd, rate);
+                       c[2], gd, rate, mask);
     return hipGetLastError();
 }
 template <typename T>
@@ -440,7 +462,8 @@ hipError_t obl_u_to_zeta(const Grid &g, const cpx<T> *u0, const cpx<T> *u1, cons
     template hipError_t obl_adv_ns<T>(const Grid &, cpx<T> *const[3], const cpx<T> *, cpx<T> *, cpx<T> *, double,    \
                                       double, double, double, hipStream_t);                                          \
     template hipError_t obl_remap<T>(const Grid &, T *const[3], T *const[3], double, hipStream_t);                   \
-    template hipError_t obl_to_cart<T>(const Grid &, T *const[3], T *const[3], double, double, hipStream_t);         \
+    template hipError_t obl_to_cart<T>(const Grid &, T *const[3], T *const[3], double, double, const T *,         \
+                                       hipStream_t);                                                                 \
     template hipError_t obl_add_f<T>(const Grid &, T *const[3], const T *, T *const[3], T *const[3], double,         \
                                      hipStream_t);                                                                   \
     template hipError_t obl_u_to_zeta<T>(const Grid &, const cpx<T> *, const cpx<T> *, const cpx<T> *, cpx<T> *,     \
