@@ -499,6 +499,29 @@ struct Step {
     // only where phi_sum is nonzero. Shear_rate_eff is the set rate:
     // Calc_shear_rate_eff (sp_3d_ns.h:189-203) adds the mean of a centred
     // difference over the periodic grid, which telescopes to zero.
+    // Inverse transform of three 2/3-truncated spectra: the x pass on the
+    // kept (ky, kz) columns reading kept kx rows, the y pass on kz < TRN_Z
+    // reading kept ky rows, the z pass reading kz < TRN_Z.
+    static int c2r_kept3(kh_run *r, cpx<T> *const sp[3], T *const out[3]) {
+        const Grid &g = r->g;
+        Comps<T>    c{{sp[0], sp[1], sp[2], nullptr, nullptr}};
+        KH_LAUNCH("x_inv_kept", (cols_x<T, false>(g, c, 3, 1, 0, r->stream)));
+        KH_LAUNCH("y_inv_kept", (cols_y<T, false>(g, c, 3, 1, 0, r->stream)));
+        ZIO<T> zio{{sp[0], sp[1], sp[2]}, {out[0], out[1], out[2]}};
+        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, r->stream, g.trn_z)));
+        return KH_OK;
+    }
+    // Forward transform of three fields whose consumer reads kept modes only:
+    // y and x passes write only the kept rows of the kept columns.
+    static int r2c_kept3(kh_run *r, T *const in[3], cpx<T> *const sp[3]) {
+        const Grid &g = r->g;
+        for (int d = 0; d < 3; d++) KH_LAUNCH("fft_z_r2c", (fft_z_r2c<T>(g, in[d], sp[d], r->stream)));
+        Comps<T> c{{sp[0], sp[1], sp[2], nullptr, nullptr}};
+        KH_LAUNCH("y_fwd_kept", (cols_y<T, true>(g, c, 3, 0, 1, r->stream)));
+        KH_LAUNCH("x_fwd_kept", (cols_x<T, true>(g, c, 3, 0, 1, r->stream)));
+        return KH_OK;
+    }
+
     static int step_shear(kh_run *r) {
         const int    ab2 = r->jikan.ts != 0;
         const Grid  &g   = r->g;
@@ -521,19 +544,21 @@ struct Step {
         // NS_solver_slavedEuler_Shear_OBL at the previous strain and rate:
         // u and omega of the truncated zeta, u x omega, the shear terms,
         // curl, slaved Euler
+        // (zeta is truncated here, so u, omega and the kept modes of u x omega
+        // take the pruned transforms)
         KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], S(r, 3), r->ukdc, 0, 1,
                                                      gd, s)));
-        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], u3[d]));
+        KH_TRY(c2r_kept3(r, s3, u3));
         KH_LAUNCH("obl_zeta_to_omega", (obl_zeta_to_omega<T>(g, Z(r, 0), Z(r, 1), s3, s)));
-        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], w3[d]));
+        KH_TRY(c2r_kept3(r, s3, w3));
         KH_LAUNCH("obl_cross", (obl_cross<T>(g.nreal, u3, w3, s)));
-        for (int d = 0; d < 3; d++) KH_TRY(r2c(r, w3[d], s3[d]));
+        KH_TRY(r2c_kept3(r, w3, s3));
         KH_LAUNCH("obl_adv_ns", (obl_adv_ns<T>(g, s3, S(r, 3), Z(r, 0), Z(r, 1), r->sre, gd, -r->nu * dt, 1. / r->nu,
                                                s)));
         // FIX_CELL, Zeta_k2u_k_OBL, U_k2u: u* on the oblique grid
         KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], nullptr, r->ukdc,
                                                      r->fixmask, 0, gd, s)));
-        for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], u3[d]));
+        KH_TRY(c2r_kept3(r, s3, u3));
         if (remap) {
             KH_LAUNCH("obl_remap", (obl_remap<T>(g, u3, w3, remap * r->xyaspect, s)));
             for (int d = 0; d < 3; d++) std::swap(r->ureal[d], r->work[d]);

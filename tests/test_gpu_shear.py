@@ -132,3 +132,27 @@ def test_shear_entry_points_reject_bad_input(gpu):
     with pytest.raises(KhError):
         r.set_shear(float("nan"))
     r.close()
+
+
+@pytest.mark.parametrize("n,np_,steps", [(128, 60, 4), (256, 40, 2)])
+def test_sheared_production_grid_fp64(gpu, n, np_, steps):
+    """The sedimentation geometry (a = 5, LJ, gravity, noise) under shear at
+    128^3 and 256^3 (the production z-pass engines), started just below
+    unit strain so the grid is remapped during the run."""
+    from kapsel3_amd import Run
+    print(f"sheared {n}^3 x {np_}, {steps} steps", flush=True)
+    P, parts = workloads.sedimentation(n, np_, gap=0.4, seed=5, kbt=1e-3)
+    rate = 0.02
+    g0 = 1.0 - 1.5 * rate * P.c.dt
+    gr, orc = Run(P, np_), kref.Run(P, np_)
+    for r in (gr, orc):
+        r.set_particles(parts)
+        r.init_fluid()
+        r.set_shear(rate)
+        r.set_shear_state(g0, rate)
+        r.step(steps)
+    assert gr.shear_state()[0] < g0 and abs(gr.shear_state()[0] - orc.shear_state()[0]) < 1e-14
+    cmp_particles(gr.get_particles(), orc.get_particles(), 1e-10, keys=KEYS, p0=parts, a=5.0)
+    zo, zg = orc.get_fluid(), gr.get_fluid()
+    assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
+    gr.close()
