@@ -40,7 +40,9 @@ __device__ __forceinline__ void contra2co(double gd, double *v) {
 // u_hat(0) = uk_dc. With `truncate` the 2/3 rule is applied to zeta first,
 // in place (operate_omega.cxx:103-105); w1 (optional) receives a copy of
 // u_hat^y (Zeta_k2u_cpuky, operate_omega.h:153-157). fixmask zeroes uk_dc
-// components first (FIX_CELL, sp_3d_ns.cxx:339-345).
+// components first (FIX_CELL, sp_3d_ns.cxx:339-345). Only the kept modes of
+// u_hat and w1 are written: their consumers (the pruned inverse transform,
+// the slaved-Euler pass) read no others, and zeta is zero outside them.
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_u(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
                                                          cpx<T> *__restrict__ u0, cpx<T> *__restrict__ u1,
@@ -54,9 +56,11 @@ __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_u(Grid g, cpx<T> *__restr
     if (kz >= g.nzh) return;
     const int ikx = kx_of(g, i), iky = ky_of(g, j);
     cpx<T>    r0 = cmk((T)0, (T)0), r1 = r0, r2 = r0;
-    if (truncate && !kept_mode(g, ikx, iky, kz)) {
-        z0[idx] = z1[idx] = r0;
-    } else if (idx != 0) {
+    if (!kept_mode(g, ikx, iky, kz)) {
+        if (truncate) z0[idx] = z1[idx] = r0;
+        return;
+    }
+    if (idx != 0) {
         const double kx = g.wave_x * ikx, ky = g.wave_y * iky, kzz = g.wave_z * kz;
         const double K2 = k2_obl(g, ikx, iky, kz, gd), ik2 = K2 > 0.0 ? 1.0 / K2 : 0.0;
         const cpx<T> d1 = z0[idx], d2 = z1[idx];
@@ -108,7 +112,9 @@ __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_u(Grid g, cpx<T> *__restr
 }
 
 // Zeta_k2omega_k_OBL (fft_wrapper.cxx:820-865): contravariant omega_hat
-// (real coefficients on each of re and im; the real part of mode 0 zeroed)
+// (real coefficients on each of re and im; the real part of mode 0
+// zeroed), at the kept modes (zeta is truncated; the pruned inverse
+// transform reads no others)
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_omega(Grid g, const cpx<T> *__restrict__ z0,
                                                              const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ o0,
@@ -119,6 +125,7 @@ __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_omega(Grid g, const cpx<T
     sh_spec_index(g, idx, i, j, kz);
     if (kz >= g.nzh) return;
     const int    ikx = kx_of(g, i), iky = ky_of(g, j);
+    if (!kept_mode(g, ikx, iky, kz)) return;
     const double kx = g.wave_x * ikx, ky = g.wave_y * iky, kzz = g.wave_z * kz;
     const cpx<T> d1 = z0[idx], d2 = z1[idx];
     cpx<T>       a, b, c;
