@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--squirm-steps", type=int, default=10)
     ap.add_argument("--no-shear", action="store_true", help="skip the Lees-Edwards sheared suspension")
     ap.add_argument("--shear-steps", type=int, default=20)
+    ap.add_argument("--shear-only", action="store_true",
+                    help="run only the Lees-Edwards figure and print it (for profiling that step)")
     ap.add_argument("--no-output", action="store_true", help="skip the Output_udf frame recording figure")
     ap.add_argument("--output-steps", type=int, default=50)
     ap.add_argument("--output-every", type=int, default=5)
@@ -335,6 +337,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     npts = 1935  # Sekibun points of a = 5, xi = 2 (profile.cxx:22-65)
+    if a.shear_only:
+        print(json.dumps(shear_bench(a.shear_steps, n, P)))
+        return
 
     traffic = load_traffic(a.traffic) if (n, P) == (256, 2000) else None
 
