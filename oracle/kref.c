@@ -57,6 +57,9 @@ struct kr_run {
     int     le, le_ac;
     double  shear_rate, shear_freq, degree_oblique, shear_rate_eff, XYaspect, Ivolume;
     double *ucp[3], *w1, *w3[3];
+    double *le_hf; /* per particle: Hydro_force, Hydro_force_new sums, volume, Itrace, force_x */
+    double  le_stress_old, le_stress_new, le_reynolds, le_dev_lj, le_strain;
+    double  le_record[KR_LE_RECORD];
 };
 
 #define IM(r, i, j, k) ((((size_t)(i) * (r)->NY + (j)) * (r)->NZ_) + (k))
@@ -1700,14 +1703,21 @@ static void make_u_particle_sum_obl(kr_run *r, double **up, const double *phi_su
     }
 }
 
-/* Calc_f_hydro_correct_precision_OBL (md_force.cxx:508-645), non-rigid;
- * the distance is |r| of the stepover-corrected relative coordinate */
+/* Calc_f_hydro_correct_precision_OBL (md_force.cxx:508-745), non-rigid;
+ * the distance is |r| of the stepover-corrected relative coordinate. The
+ * stress fields Hydro_force / Hydro_force_new are summed as they are
+ * accumulated (the reference sums them over the grid afterwards,
+ * Calc_hydro_stress, fluid_solver.h:253-279): per particle in Sekibun
+ * order, particles in index order. */
 static void calc_f_hydro_obl(kr_run *r, const double *phi_sum, double **u) {
     const double dmy = (-r->DX3 * r->RHO) / r->jikan.dt_fluid, jump = r->shear_rate_eff * r->L[1];
+    double      *hf_old = r->le_hf, *hf_new = r->le_hf + r->np;
+    double      *vol = r->le_hf + 2 * r->np, *itr = r->le_hf + 3 * r->np, *f0 = r->le_hf + 4 * r->np;
 #pragma omp parallel for
     for (int n = 0; n < r->np; n++) {
         Particle *pn = &r->p[n];
         double    xp[3], vp[3], om[3], force[3] = {0, 0, 0}, torque[3] = {0, 0, 0};
+        double    rhop = r->spec[pn->spec].mass_ratio * r->RHO; /* RHO_particle, input.cxx:430 */
         for (int d = 0; d < 3; d++) {
             xp[d] = pn->x[d];
             vp[d] = pn->v[d];
@@ -1716,6 +1726,7 @@ static void calc_f_hydro_obl(kr_run *r, const double *phi_sum, double **u) {
         int    x_int[3];
         double residue[3];
         particle_cell(r, xp, x_int, residue);
+        double a = 0.0, vn = 0.0, it = 0.0;
         for (int mesh = 0; mesh < r->NP_domain; mesh++) {
             int    r_mesh[3];
             double rv[3], v_rot[3], fp[3];
@@ -1731,11 +1742,79 @@ static void calc_f_hydro_obl(kr_run *r, const double *phi_sum, double **u) {
             torque[0] += (rv[1] * fp[2] - rv[2] * fp[1]);
             torque[1] += (rv[2] * fp[0] - rv[0] * fp[2]);
             torque[2] += (rv[0] * fp[1] - rv[1] * fp[0]);
+            double ry = (r_mesh[1] + sign * r->L[1]);
+            a += fp[0] * ry * rhop; /* Hydro_force and Hydro_force_new, md_force.cxx:620-623 */
+            vn += dphi;
+            it += dphi * SQ(dmyR);
         }
+        it *= 2.0 / 3.0;
         for (int d = 0; d < 3; d++) pn->f_hydro[d] = (dmy * force[d]);
         if (r->P.rotation)
             for (int d = 0; d < 3; d++) pn->torque_hydro[d] = (dmy * torque[d]);
+        /* md_force.cxx:655-679: the momentum and rigid-motion corrections */
+        double ho = a, hn = a;
+        for (int mesh = 0; mesh < r->NP_domain; mesh++) {
+            int    r_mesh[3];
+            double rv[3];
+            int    sign = relative_coord_obl(r, r->sek + 3 * mesh, x_int, residue, r_mesh, rv);
+            double dmyR = sqrt(SQ(rv[0]) + SQ(rv[1]) + SQ(rv[2]));
+            size_t im   = IM(r, r_mesh[0], r_mesh[1], r_mesh[2]);
+            double dphi = Phi(r, dmyR) / fmax(phi_sum[im], 1.0);
+            double ry   = (r_mesh[1] + sign * r->L[1]);
+            ho -= (pn->momentum_depend_fr[0] / vn) * ry * dphi;
+            double sv = -r->RHO * force[0] / vn;
+            double sw = -r->RHO * (torque[1] * rv[2] - torque[2] * rv[1]) / it;
+            hn += (sv + sw) * ry * dphi;
+        }
+        hf_old[n] = ho;
+        hf_new[n] = hn;
+        vol[n]    = vn;
+        itr[n]    = it;
+        f0[n]     = force[0];
     }
+    /* md_force.cxx:681-745: the mean force, spread back by volume */
+    double sum_force = 0.0, sum_volume = 0.0;
+    for (int n = 0; n < r->np; n++) {
+        sum_force += f0[n];
+        sum_volume += vol[n];
+    }
+    sum_volume /= r->RHO;
+    double old = 0.0, nw = 0.0;
+    for (int n = 0; n < r->np; n++) {
+        const Particle *pn = &r->p[n];
+        double          xp[3] = {pn->x[0], pn->x[1], pn->x[2]}, residue[3], b = 0.0;
+        int             x_int[3];
+        particle_cell(r, xp, x_int, residue);
+        for (int mesh = 0; mesh < r->NP_domain; mesh++) {
+            int    r_mesh[3];
+            double rv[3];
+            int    sign = relative_coord_obl(r, r->sek + 3 * mesh, x_int, residue, r_mesh, rv);
+            double dmyR = sqrt(SQ(rv[0]) + SQ(rv[1]) + SQ(rv[2]));
+            size_t im   = IM(r, r_mesh[0], r_mesh[1], r_mesh[2]);
+            double dphi = Phi(r, dmyR) / fmax(phi_sum[im], 1.0);
+            b -= sum_force * (r_mesh[1] + sign * r->L[1]) * dphi / sum_volume;
+        }
+        old += hf_old[n] + b;
+        nw += hf_new[n];
+    }
+    /* Calc_hydro_stress (fluid_solver.h:253-279) */
+    const double k = (r->Ivolume * r->DX3) / r->jikan.dt_fluid;
+    r->le_stress_old = -old * k;
+    r->le_stress_new = -nw * k;
+}
+
+/* Calc_Reynolds_shear_stress (fluid_solver.h:165-183) on the Cartesian u */
+static void calc_reynolds_shear_stress(kr_run *r, double **u) {
+    double s = 0.0;
+    for (int i = 0; i < r->NX; i++)
+        for (int j = 0; j < r->NY; j++) {
+            double dy = (double)(j - r->NY / 2) * r->DX;
+            for (int k = 0; k < r->NZ; k++) {
+                size_t im = IM(r, i, j, k);
+                s += (u[0][im] - r->shear_rate_eff * dy) * (u[1][im]);
+            }
+        }
+    r->le_reynolds = -(r->Ivolume * r->DX3) * r->RHO * s;
 }
 
 /* Calc_f_Lennard_Jones_shear_cap_primitive (md_force.cxx:198-305) with
@@ -1743,6 +1822,7 @@ static void calc_f_hydro_obl(kr_run *r, const double *phi_sum, double **u) {
  * particle's own sum added after its row */
 static void calc_f_lennard_jones_obl(kr_run *r) {
     const double pair_cutoff = r->A_R_cutoff * r->LJ_dia;
+    double       ss0         = 0.0;
     for (int n = 0; n < r->np; n++) {
         Particle *pn     = &r->p[n];
         double    sum[3] = {0, 0, 0};
@@ -1758,10 +1838,12 @@ static void calc_f_lennard_jones_obl(kr_run *r) {
                     pm->fr[d] -= fn;
                     sum[d] += fn;
                 }
+                ss0 += (dmy_r * (-rv[0])) * rv[1]; /* the pair's shear stress, md_force.cxx:246 */
             }
         }
         for (int d = 0; d < 3; d++) pn->fr[d] += sum[d];
     }
+    r->le_dev_lj += ss0;
 }
 
 /* MD_solver_position_Euler_OBL / _AB2_OBL (particle_solver.cxx:283-334) */
@@ -1787,8 +1869,10 @@ static void position_update_obl(kr_run *r, int ab2) {
 /* MD_solver_velocity_Euler_OBL / _AB2_hydro_OBL (particle_solver.cxx:336-432)
  * with Force_OBL (particle_solver.h:113-126) */
 static void velocity_update_obl(kr_run *r, int ab2) {
+    r->le_dev_lj = 0.0; /* Force_OBL, particle_solver.h:113-126 */
     if (r->P.lj_truncate >= 0) calc_f_lennard_jones_obl(r);
     if (r->P.g != 0.0) add_f_gravity(r);
+    r->le_dev_lj *= r->Ivolume;
 #pragma omp parallel for
     for (int n = 0; n < r->np; n++) {
         Particle *p    = &r->p[n];
@@ -1859,6 +1943,7 @@ static void time_evolution_hydro_obl(kr_run *r) {
     for (int d = 0; d < 3; d++) reset_field(r, r->up[d]);
     make_phi_particle_sum_obl(r, r->phi, r->phi_sum);
     calc_f_hydro_obl(r, r->phi_sum, r->ucp);
+    calc_reynolds_shear_stress(r, r->ucp);
     velocity_update_obl(r, r->jikan.ts != 0);
     if (r->kBT > 0.) thermostat(r);
     for (int d = 0; d < 3; d++) reset_field(r, r->up[d]);
@@ -1871,6 +1956,26 @@ static void time_evolution_hydro_obl(kr_run *r) {
     u_k2zeta_k_obl(r, r->u, r->zeta, r->uk_dc);
 }
 
+/* Mean_shear_stress SHOW for Shear_Navier_Stokes_Lees_Edwards (sp_3d_ns.h:247-330),
+ * after the step and its time increment, with the strain update of
+ * sp_3d_ns.cxx:962-964 */
+static void le_shear_record(kr_run *r) {
+    r->le_strain += r->shear_rate_eff * r->jikan.dt_fluid;
+    const double fluid    = r->ETA * r->shear_rate_eff;
+    const double apparent = r->le_stress_new + r->le_reynolds + r->le_dev_lj + fluid + 0.0;
+    const double rec[KR_LE_RECORD] = {r->jikan.time,  r->shear_rate_eff, r->degree_oblique, r->le_strain,
+                                      r->le_dev_lj,   r->le_stress_old,  r->le_stress_new,  r->le_reynolds,
+                                      fluid,          0.0,               apparent,          apparent / r->shear_rate_eff};
+    memcpy(r->le_record, rec, sizeof(rec));
+}
+void kr_get_shear_record(kr_run *r, double out[KR_LE_RECORD]) { memcpy(out, r->le_record, sizeof(r->le_record)); }
+void kr_get_shear_diag(kr_run *r, double out[4]) {
+    out[0] = r->le_stress_old;
+    out[1] = r->le_stress_new;
+    out[2] = r->le_reynolds;
+    out[3] = r->le_dev_lj;
+}
+
 int kr_set_shear(kr_run *r, double shear_rate, int ac, double frequency) {
     if (!r) return -1;
     if (!r->le) {
@@ -1878,7 +1983,8 @@ int kr_set_shear(kr_run *r, double shear_rate, int ac, double frequency) {
             r->ucp[d] = (double *)calloc(r->M, sizeof(double));
             r->w3[d]  = (double *)calloc(r->M, sizeof(double));
         }
-        r->w1 = (double *)calloc(r->M, sizeof(double));
+        r->w1    = (double *)calloc(r->M, sizeof(double));
+        r->le_hf = (double *)calloc(5 * (size_t)(r->np > 0 ? r->np : 1), sizeof(double));
     }
     r->le         = 1;
     r->le_ac      = ac ? 1 : 0;
@@ -1944,7 +2050,12 @@ int kr_le_apply(kr_run *r, int fn) {
             make_u_particle_sum_obl(r, r->up, r->phi_sum);
             break;
         case KR_LE_F_HYDRO: calc_f_hydro_obl(r, r->phi_sum, r->u); break;
-        case KR_LE_LJ: calc_f_lennard_jones_obl(r); break;
+        case KR_LE_LJ:
+            r->le_dev_lj = 0.0;
+            calc_f_lennard_jones_obl(r);
+            r->le_dev_lj *= r->Ivolume;
+            break;
+        case KR_LE_REYNOLDS: calc_reynolds_shear_stress(r, r->u); break;
         default: return -1;
     }
     return 0;
@@ -1989,6 +2100,7 @@ void kr_step(kr_run *r, int nsteps) {
             time_evolution_hydro(r);
         r->jikan.time += r->jikan.dt_fluid;
         r->jikan.ts++;
+        if (r->le) le_shear_record(r);
     }
 }
 
@@ -2148,6 +2260,7 @@ void kr_destroy(kr_run *r) {
         free(r->w3[d]);
     }
     free(r->w1);
+    free(r->le_hf);
     free(r->KX);
     free(r->KY);
     free(r->KZ);

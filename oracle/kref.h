@@ -60,9 +60,19 @@ enum {
     KR_LE_MAKE_PHI,      /* Reset_phi x2 + Make_phi_particle_sum_OBL  make_phi.cxx:262-311 */
     KR_LE_MAKE_UP,       /* Reset_u + Make_u_particle_sum_OBL   make_phi.cxx:356-402 */
     KR_LE_F_HYDRO,       /* Calc_f_hydro_correct_precision_OBL on u  md_force.cxx:508-645 */
-    KR_LE_LJ             /* Calc_f_Lennard_Jones_OBL (adds to fr)     md_force.cxx:198-305 */
+    KR_LE_LJ,            /* Calc_f_Lennard_Jones_OBL (adds to fr)     md_force.cxx:198-305 */
+    KR_LE_REYNOLDS       /* Calc_Reynolds_shear_stress on u (Cartesian) fluid_solver.h:165-183 */
 };
 int kr_le_apply(kr_run *r, int fn);
+/* The record Mean_shear_stress prints after each sheared step
+ * (sp_3d_ns.h:247-330): time, shear_rate, degree_oblique, strain,
+ * lj_dev_stress, shear_stress_old, shear_stress_new, reynolds_stress,
+ * fluid_stress, interfacial_stress (0), apparent_stress, viscosity. */
+#define KR_LE_RECORD 12
+void kr_get_shear_record(kr_run *r, double out[KR_LE_RECORD]);
+/* the last stress sums: Hydro_force and Hydro_force_new stresses, the
+ * Reynolds stress, the LJ stress */
+void kr_get_shear_diag(kr_run *r, double out[4]);
 void kr_step(kr_run *r, int nsteps);
 int  kr_apply(kr_run *r, int fn);
 

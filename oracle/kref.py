@@ -55,6 +55,10 @@ def lib():
         L.kr_set_shear_state.argtypes = [vp, C.c_double, C.c_double]
         L.kr_set_shear_state.restype = None
         L.kr_le_apply.argtypes = [vp, C.c_int]
+        L.kr_get_shear_record.argtypes = [vp, vp]
+        L.kr_get_shear_record.restype = None
+        L.kr_get_shear_diag.argtypes = [vp, vp]
+        L.kr_get_shear_diag.restype = None
         _LIB = L
     return _LIB
 
@@ -62,7 +66,11 @@ def lib():
 # single functions of the sheared step (enum in kref.h)
 LE_FN = {name: i for i, name in enumerate([
     "oblique2u", "u2oblique", "reset_up", "reset_down", "zeta_k2u", "u2zeta_k", "solenoidal_u",
-    "ns_solver", "shear_rate_eff", "position_euler", "make_phi", "make_up", "f_hydro", "lj"])}
+    "ns_solver", "shear_rate_eff", "position_euler", "make_phi", "make_up", "f_hydro", "lj", "reynolds"])}
+# fields of the per-step stress record (Mean_shear_stress, sp_3d_ns.h:247-330)
+LE_RECORD = ["time", "shear_rate", "degree_oblique", "strain", "lj_dev_stress", "shear_stress_old",
+             "shear_stress_new", "reynolds_stress", "fluid_stress", "interfacial_stress", "apparent_stress",
+             "viscosity"]
 
 
 def fft_forward(a):
@@ -185,6 +193,16 @@ class Run:
 
     def set_shear_state(self, degree_oblique, shear_rate_eff):
         lib().kr_set_shear_state(self.h, float(degree_oblique), float(shear_rate_eff))
+
+    def shear_record(self):
+        o = np.zeros(len(LE_RECORD))
+        lib().kr_get_shear_record(self.h, o.ctypes.data)
+        return dict(zip(LE_RECORD, o))
+
+    def shear_diag(self):
+        o = np.zeros(4)
+        lib().kr_get_shear_diag(self.h, o.ctypes.data)
+        return o
 
     def le_apply(self, fn):
         if lib().kr_le_apply(self.h, LE_FN[fn]) != 0:

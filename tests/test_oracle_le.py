@@ -387,17 +387,40 @@ def test_scatter_and_gather_against_the_images():
     u = rng.normal(0, 0.01, (3, n, n, n + 2))
     for d in range(3):
         r.set_field("u%d" % d, u[d])
+    mdfr = rng.normal(0, 1e-3, (2, 3))
+    p["momentum_depend_fr"] = mdfr
+    r.set_particles(p)
     r.le_apply("f_hydro")
     q = r.get_particles()
     uu = u[:, :, :, :n].reshape(3, -1).T
+    # the stress sums of Hydro_force / Hydro_force_new (md_force.cxx:620-745):
+    # y of the point in the row the particle's image sits in, X_y - m LY
+    rho_p = [1.5, 1.5]                                          # species 0 of _params: mass ratio 1.5, RHO 1
+    h_old = h_new = b_all = f_all = v_all = 0.0
     for k in range(2):
         vimg = p["v"][k][None, :] + ms[k][:, None] * np.array([rate * L, 0, 0])[None, :]
         vrot = np.cross(np.broadcast_to(p["omega"][k], rs[k].shape), rs[k])
-        fp = (vimg + vrot - uu) * (phis[k] / norm)[:, None]
+        w = phis[k] / norm
+        fp = (vimg + vrot - uu) * w[:, None]
         F = -(1.0 / 0.05) * fp.sum(axis=0)
         T = -(1.0 / 0.05) * np.cross(rs[k], fp).sum(axis=0)
         np.testing.assert_allclose(q["f_hydro"][k], F, rtol=1e-11, atol=1e-13)
         np.testing.assert_allclose(q["torque_hydro"][k], T, rtol=1e-11, atol=1e-13)
+        ry = X[:, 1] - ms[k] * L
+        vol = w.sum()
+        itr = 2.0 / 3.0 * (w * (rs[k] ** 2).sum(axis=1)).sum()
+        f0, tq = fp[:, 0].sum(), np.cross(rs[k], fp).sum(axis=0)
+        a = rho_p[k] * (fp[:, 0] * ry).sum()
+        h_old += a - (mdfr[k, 0] / vol * ry * w).sum()
+        h_new += a + ((-f0 / vol - (tq[1] * rs[k][:, 2] - tq[2] * rs[k][:, 1]) / itr) * ry * w).sum()
+        b_all += (ry * w).sum()
+        f_all += f0
+        v_all += vol
+    h_old -= f_all / v_all * b_all
+    k_ = 1.0 / n ** 3 / 0.05
+    old, new = r.shear_diag()[:2]
+    assert abs(old - (-h_old * k_)) < 1e-11 * abs(h_old * k_)
+    assert abs(new - (-h_new * k_)) < 1e-11 * abs(h_new * k_)
 
 
 def test_lennard_jones_pairs_across_the_sheared_boundary():
@@ -428,3 +451,56 @@ def test_lennard_jones_pairs_across_the_sheared_boundary():
                 pairs += 1
     assert pairs == 4                                    # (0,1) across the boundary and (2,3)
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
+    # the pairs' shear stress (md_force.cxx:246, particle_solver.h:113-126):
+    # sum over pairs of F_x on the lower index times r_y, over the volume
+    ss = 0.0
+    for a in range(4):
+        for b in range(a + 1, 4):
+            m, d = _nearest_image(xp[a][None, :], xp[b], g, float(n))
+            rv = -d[0]
+            rr = np.sqrt((rv ** 2).sum())
+            if rr < cutoff:
+                lj = 24.0 / rr ** 2 * (2 * (sigma / rr) ** 12 - (sigma / rr) ** 6)
+                ss += lj * (-rv[0]) * rv[1]
+    assert abs(r.shear_diag()[3] - ss / n ** 3) < 1e-12 * abs(ss / n ** 3)
+
+
+def test_reynolds_stress_closed_form():
+    """Calc_Reynolds_shear_stress (fluid_solver.h:165-183) of the mean
+    shear flow plus u' = A (sin ky, sin ky, 0): -rho A^2 / 2 (the mean flow
+    itself carries none)."""
+    n, rate, A = 16, 0.03, 1e-2
+    r = kref.Run(_params(n, rho=1.3), 0)
+    r.set_shear(rate)
+    r.set_shear_state(0.2, rate)
+    j = np.arange(n)
+    s_ = np.sin(2 * np.pi * j / n)[None, :, None] * np.ones((n, n, n))
+    mean = rate * (j - n // 2)[None, :, None] * np.ones((n, n, n))
+    for comp, f in ((0, mean + A * s_), (1, A * s_), (2, 0 * s_)):
+        r.set_field("u%d" % comp, _pad(f, n))
+    r.le_apply("reynolds")
+    assert abs(r.shear_diag()[2] - (-1.3 * A * A / 2)) < 1e-15
+
+
+def test_einstein_viscosity_of_a_dilute_suspension():
+    """One neutrally buoyant sphere (a = 4, volume fraction 0.0082) in 32^3
+    under steady shear: the stress record (Mean_shear_stress, sp_3d_ns.h:
+    247-330) gives an apparent viscosity above the fluid's by 2.5 phi eta
+    (Einstein), within the 15 % that periodic images and the diffuse
+    interface allow; the old and new particle stresses agree, the Reynolds
+    stress is negligible, the strain accumulates rate * t."""
+    n, rate, steps = 32, 0.01, 400
+    P = Params(n, a=4.0, a_xi=2.0, dt=0.05, eta=1.0, rotation=1, lj_truncate=-1,
+               species=[make_species(mass_ratio=1.0)])
+    r = kref.Run(P, 1)
+    r.set_particles(workloads.particles_at(np.array([[16.0, 16.0, 16.0]])))
+    r.set_shear(rate)
+    r.step(steps)
+    rec = r.shear_record()
+    phi = 4.0 / 3.0 * np.pi * 4.0 ** 3 / n ** 3
+    extra = (rec["apparent_stress"] - rec["fluid_stress"]) / (1.0 * rate)
+    assert 0.95 * 2.5 * phi < extra < 1.15 * 2.5 * phi, extra
+    assert abs(rec["shear_stress_old"] - rec["shear_stress_new"]) < 0.01 * abs(rec["shear_stress_new"])
+    assert abs(rec["reynolds_stress"]) < 1e-3 * rec["fluid_stress"]
+    assert rec["fluid_stress"] == rate and abs(rec["strain"] - rate * steps * 0.05) < 1e-12
+    assert abs(rec["time"] - steps * 0.05) < 1e-12 and rec["viscosity"] == rec["apparent_stress"] / rate
