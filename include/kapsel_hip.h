@@ -178,6 +178,19 @@ int kh_set_step_path(kh_run *run, int path);
 int kh_set_shear(kh_run *run, double shear_rate, int ac, double frequency);
 int kh_get_shear_state(kh_run *run, double *degree_oblique, double *shear_rate_eff);
 int kh_set_shear_state(kh_run *run, double degree_oblique, double shear_rate_eff);
+/* The record Mean_shear_stress prints after every sheared step
+ * (sp_3d_ns.cxx:962-964, sp_3d_ns.h:247-330), kept on the device:
+ * time, shear_rate, degree_oblique, strain (Shear_strain_realized),
+ * lj_dev_stress, shear_stress_old, shear_stress_new (the Hydro_force sums of
+ * md_force.cxx:508-745), reynolds_stress (Calc_Reynolds_shear_stress,
+ * fluid_solver.h:165-183), fluid_stress (ETA * rate), interfacial_stress
+ * (0), apparent_stress, viscosity. kh_get_shear_record copies the last
+ * step's; kh_set_shear_history keeps the records of the next nsteps steps
+ * (0 stops), kh_get_shear_history copies them (nsteps x KH_SHEAR_RECORD). */
+#define KH_SHEAR_RECORD 12
+int kh_get_shear_record(kh_run *run, double out[KH_SHEAR_RECORD]);
+int kh_set_shear_history(kh_run *run, int nsteps);
+int kh_get_shear_history(kh_run *run, double *out, int nsteps);
 /* Per-step particle record kept on the device: from the next step on, the
  * (x[3], v[3]) of every particle after each of the next nsteps steps
  * (nsteps x np x 6 doubles; 0 stops recording). No counterpart in the

@@ -26,8 +26,13 @@ EXPORTS = [
     "kh_timing_count", "kh_timing_name", "kh_strerror", "kh_last_error", "kh_set_trajectory", "kh_get_trajectory",
     "kh_set_output", "kh_get_output", "kh_run_bytes", "kh_query", "kh_server_create", "kh_server_submit", "kh_server_poll", "kh_server_wait",
     "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy", "kh_set_shear", "kh_get_shear_state",
-    "kh_set_shear_state",
+    "kh_set_shear_state", "kh_get_shear_record", "kh_set_shear_history", "kh_get_shear_history",
 ]
+
+# fields of the sheared step's stress record (Mean_shear_stress, sp_3d_ns.h:247-330)
+SHEAR_RECORD = ["time", "shear_rate", "degree_oblique", "strain", "lj_dev_stress", "shear_stress_old",
+                "shear_stress_new", "reynolds_stress", "fluid_stress", "interfacial_stress", "apparent_stress",
+                "viscosity"]
 
 
 class ServerConfig(C.Structure):
@@ -109,6 +114,9 @@ def lib():
         L.kh_set_shear.argtypes = [vp, C.c_double, C.c_int, C.c_double]
         L.kh_get_shear_state.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.kh_set_shear_state.argtypes = [vp, C.c_double, C.c_double]
+        L.kh_get_shear_record.argtypes = [vp, vp]
+        L.kh_set_shear_history.argtypes = [vp, C.c_int]
+        L.kh_get_shear_history.argtypes = [vp, vp, C.c_int]
         _LIB = L
     return _LIB
 
@@ -269,6 +277,24 @@ class Run:
 
     def set_shear_state(self, degree_oblique, shear_rate_eff):
         _check(lib().kh_set_shear_state(self.h, float(degree_oblique), float(shear_rate_eff)))
+
+    def shear_record(self):
+        """The last sheared step's stress record as a dict (SHEAR_RECORD keys)."""
+        o = np.zeros(len(SHEAR_RECORD))
+        _check(lib().kh_get_shear_record(self.h, o.ctypes.data))
+        return dict(zip(SHEAR_RECORD, o))
+
+    def set_shear_history(self, nsteps):
+        """Keep the stress records of the next nsteps steps on the device."""
+        _check(lib().kh_set_shear_history(self.h, int(nsteps)))
+        self._shear_hist = int(nsteps)
+
+    def get_shear_history(self, nsteps=None):
+        """(nsteps, 12) array of stress records (columns as SHEAR_RECORD)."""
+        n = getattr(self, "_shear_hist", 0) if nsteps is None else int(nsteps)
+        out = np.zeros((n, len(SHEAR_RECORD)))
+        _check(lib().kh_get_shear_history(self.h, out.ctypes.data, n))
+        return out
 
     def set_output(self, nframes, every):
         """Record Output_udf frames on the device at the next nframes step

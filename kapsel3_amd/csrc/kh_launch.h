@@ -43,7 +43,8 @@ template <typename T>
 hipError_t obl_remap(const Grid &g, T *const u[3], T *const o[3], double delta, hipStream_t s);
 template <typename T>
 hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, const T *mask,
-                       hipStream_t s);
+                       double *acc, double dx, hipStream_t s);
+long long obl_to_cart_blocks(const Grid &g);
 template <typename T>
 hipError_t obl_add_f(const Grid &g, T *const up[3], const T *phisum, T *const c[3], T *const u[3], double gd,
                      hipStream_t s);
@@ -72,6 +73,10 @@ hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStr
 hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s);
 hipError_t tick(int *d_ts, hipStream_t s);
+// the sheared step's stress record from the per-particle partials and the
+// nacc Reynolds partials of k_obl_to_cart
+hipError_t obl_record(const PParams &pp, const double *acc, long long nacc, const ShearRec &h, double *last,
+                      double *hist, hipStream_t s);
 hipError_t frame(const PParams &pp, const DevParticle *P, const int *d_ts, int ts0, int every, int nframes, double *out,
                  hipStream_t s);
 hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int base, int nslots, double *out,

@@ -23,6 +23,7 @@ struct DevParticle {
     double v_slip[3], om_slip[3];
     double mass, mass_center[3], inertia[9];
     double smass, smc[3], sinertia[9], sdv[3], sdw[3];
+    double mdfr[3];  // momentum_depend_fr: the step's LJ + gravity impulse (sheared runs)
     int    spec, pad_;
 };
 
@@ -55,6 +56,15 @@ struct PParams {
     // obl_g*LX... in x (obl_g*NY grid points) and moves obl_jump faster in x
     int               obl;
     double            obl_g, obl_jump;
+    double            rho;     // RHO (the sheared step's stress sums)
+    double           *pstress; // sheared runs: np x 6 per-particle stress partials
+};
+
+// Host scalars of one sheared step's stress record (k_obl_record).
+struct ShearRec {
+    double time, rate, deg, strain, dt, eta;
+    double ivolume;   // 1/(LX LY LZ)
+    double ivol_dx3;  // DX^3/(LX LY LZ)
 };
 
 // Per-particle cache of the Sekibun points where phi is nonzero, filled by

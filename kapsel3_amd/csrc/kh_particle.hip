@@ -193,6 +193,10 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
     // per-point terms in the field precision T, sums in FP64
     const T vt[3] = {(T)vp[0], (T)vp[1], (T)vp[2]}, wt[3] = {(T)om[0], (T)om[1], (T)om[2]};
     const int cnt = pc.cnt[n];
+    // sheared runs: the Hydro_force sums (md_force.cxx:620-745) need, per
+    // particle, sum fp_x y', sum w, sum w |r|^2, sum y' w, sum r_y y' w and
+    // sum r_z y' w (w = phi/max(phi_sum,1), y' = r_mesh_y + sign LY)
+    Neu st[6];
     for (int m = threadIdx.x; m < cnt; m += NT) {
         const T   ph   = pc.phi[cb + m];
         const int idx  = pc.idx[cb + m];
@@ -223,20 +227,46 @@ __device__ __forceinline__ void f_hydro_block(const PParams &pp, DevParticle *__
         acc[3].add((double)(r[1] * fp[2] - r[2] * fp[1]));
         acc[4].add((double)(r[2] * fp[0] - r[0] * fp[2]));
         acc[5].add((double)(r[0] * fp[1] - r[1] * fp[0]));
+        if constexpr (OBL) {
+            const double ry = (double)((idx / pp.nz) % pp.ny) + sgn * pp.L[1], w = (double)dmy_phi;
+            st[0].add((double)fp[0] * ry);
+            st[1].add(w);
+            st[2].add(w * (rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]));
+            st[3].add(ry * w);
+            st[4].add(rd[1] * ry * w);
+            st[5].add(rd[2] * ry * w);
+        }
     }
-    double v[6];
+    constexpr int K = OBL ? 12 : 6;
+    double        v[K];
     for (int k = 0; k < 6; k++) v[k] = acc[k].val();
-    block_sum<6, NT>(v);
+    if constexpr (OBL)
+        for (int k = 0; k < 6; k++) v[6 + k] = st[k].val();
+    block_sum<K, NT>(v);
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; d++) {
             P[n].fh[d] = coef * v[d];
-            P[n].fs[d] = 0.0;
+            if constexpr (!OBL) P[n].fs[d] = 0.0;
         }
         if (pp.rotation)
             for (int d = 0; d < 3; d++) {
                 P[n].th[d] = coef * v[3 + d];
-                P[n].ts[d] = 0.0;
+                if constexpr (!OBL) P[n].ts[d] = 0.0;
             }
+        if constexpr (OBL) {
+            // per-particle partials of Hydro_force (old) and Hydro_force_new;
+            // the mean-force term and the sums over particles follow in
+            // k_obl_record
+            const double rhop = pp.species[P[n].spec].mass_ratio * pp.rho;
+            const double a = rhop * v[6], vol = v[7], itr = 2.0 / 3.0 * v[8], b = v[9];
+            double      *o = pp.pstress + 6 * (long long)n;
+            o[0]           = a - (P[n].mdfr[0] / vol) * b;
+            o[1]           = a + (-pp.rho * v[0] / vol) * b + (-pp.rho / itr) * (v[4] * v[11] - v[5] * v[10]);
+            o[2]           = b;
+            o[3]           = v[0];
+            o[4]           = vol;
+            o[5]           = 0.0;  // the LJ pairs' shear stress (k_forces)
+        }
     }
 }
 template <typename T, bool OBL>
@@ -436,7 +466,7 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
     const double xi[3]  = {P[i].x[0], P[i].x[1], P[i].x[2]};
     const int    si     = P[i].spec;
     const bool   obst_i = pp.species[si].propulsion == 2;
-    double       f[3]   = {0.0, 0.0, 0.0};
+    double       f[4]   = {0.0, 0.0, 0.0, 0.0};  // F_i; [3]: sheared runs, the pairs' shear stress
     if (pp.lj_on) {
         // pairs clearly beyond the cutoff are dropped by a division-free
         // minimum image (exact for positions inside the box, up to the
@@ -476,18 +506,64 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
                 const double sgn   = (i > j) ? -1.0 : 1.0;  // i>j: f_i += dmy_r*(-r); else f_i -= dmy_r*(-r)
                 for (int c = 0; c < mult; c++)
                     for (int d = 0; d < 3; d++) f[d] += sgn * (dmy_r * rv[d]);
+                // md_force.cxx:246: F_x on the lower index times r_y, each pair
+                // once (r = x_hi - x_lo = -rv here)
+                if constexpr (OBL)
+                    if (i < j) f[3] += (dmy_r * rv[0]) * (-rv[1]);
             }
         }
     }
-    block_sum<3, NT>(f);
+    block_sum<OBL ? 4 : 3, NT>(f);
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; d++) P[i].fr[d] += f[d];
         if (pp.g != 0.0) P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
+        if constexpr (OBL) pp.pstress[6 * (long long)i + 5] = f[3];
     }
 }
 template <bool OBL>
 __global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__restrict__ P) {
     forces_block<PT_NT, OBL>(pp, P, blockIdx.x);
+}
+
+// The record Mean_shear_stress prints after a sheared step (sp_3d_ns.h:
+// 247-330): the particle sums of Hydro_force / Hydro_force_new with the
+// mean-force term (md_force.cxx:681-745) as Calc_hydro_stress takes them
+// (fluid_solver.h:253-279), the LJ stress over the volume (Force_OBL,
+// particle_solver.h:113-126), the Reynolds stress from k_obl_to_cart's
+// per-workgroup partials. One workgroup.
+__global__ __launch_bounds__(PT_NT) void k_obl_record(PParams pp, const double *__restrict__ acc, long long nacc,
+                                                      ShearRec h, double *__restrict__ last,
+                                                      double *__restrict__ hist) {
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int n = threadIdx.x; n < pp.np; n += PT_NT)
+        for (int k = 0; k < 6; k++) v[k] += pp.pstress[6 * (long long)n + k];
+    for (long long b = threadIdx.x; b < nacc; b += PT_NT) v[6] += acc[b];
+    block_sum<7, PT_NT>(v);
+    if (threadIdx.x == 0) {
+        const double sum_volume = v[4] / pp.rho;
+        const double old = v[0] - (v[3] / sum_volume) * v[2];
+        double       rec[12];
+        rec[0]  = h.time;
+        rec[1]  = h.rate;
+        rec[2]  = h.deg;
+        rec[3]  = h.strain;
+        rec[4]  = v[5] * h.ivolume;                       // lj_dev_stress
+        rec[5]  = -old * h.ivol_dx3 / h.dt;               // shear_stress_temporal_old
+        rec[6]  = -v[1] * h.ivol_dx3 / h.dt;              // shear_stress_temporal_new
+        rec[7]  = -h.ivol_dx3 * pp.rho * v[6];            // reynolds_stress
+        rec[8]  = h.eta * h.rate;                         // fluid_stress
+        rec[9]  = 0.0;                                    // interfacial_stress
+        rec[10] = rec[6] + rec[7] + rec[4] + rec[8] + rec[9];  // apparent_stress
+        rec[11] = rec[10] / h.rate;                       // viscosity
+        for (int k = 0; k < 12; k++) last[k] = rec[k];
+        if (hist)
+            for (int k = 0; k < 12; k++) hist[k] = rec[k];
+    }
+}
+hipError_t obl_record(const PParams &pp, const double *acc, long long nacc, const ShearRec &h, double *last,
+                      double *hist, hipStream_t s) {
+    hipLaunchKernelGGL(k_obl_record, dim3(1), dim3(PT_NT), 0, s, pp, acc, nacc, h, last, hist);
+    return hipGetLastError();
 }
 
 // MD_solver_velocity_Euler (particle_solver.cxx:103-142) / _AB2_hydro
@@ -537,6 +613,10 @@ __device__ void velocity_one(const PParams &pp, DevParticle *__restrict__ P, int
             p.om[d]     = 0.0;
         }
     }
+    // hydro_stress of the sheared velocity updates (particle_solver.cxx:336-432):
+    // the LJ + gravity impulse of this update
+    if (pp.obl)
+        for (int d = 0; d < 3; d++) p.mdfr[d] = ab2 ? h * (p.fr[d] + p.fr_prev[d]) : h * p.fr[d];
     for (int d = 0; d < 3; d++) {
         p.fr_prev[d] = p.fr[d];
         p.fr[d]      = 0.0;

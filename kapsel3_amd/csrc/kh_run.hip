@@ -123,6 +123,13 @@ struct kh_run {
     // deg = degree_oblique (strain of the oblique grid), sre = Shear_rate_eff
     int          shear = 0, shear_ac = 0;
     double       shear_rate = 0.0, shear_freq = 0.0, deg = 0.0, sre = 0.0, xyaspect = 1.0;
+    double       strain = 0.0;                 // Shear_strain_realized
+    double      *shear_pstress = nullptr;      // np x 6 stress partials of the sheared gather and LJ
+    double      *shear_acc     = nullptr;      // Reynolds-stress partials (one per k_obl_to_cart workgroup)
+    long long    shear_nacc    = 0;
+    double      *shear_last    = nullptr;      // the last step's stress record (KH_SHEAR_RECORD doubles)
+    double      *shear_hist    = nullptr;      // records of shear_hist_n steps from ts = shear_hist_base
+    int          shear_hist_n = 0, shear_hist_base = 0;
     // measurement
     bool                               timing = false;
     std::map<std::string, KernelTimes> times;
@@ -384,6 +391,15 @@ struct Step {
         KH_TRY(r->shear ? step_shear(r) : step_body(r));
         // the fused body ticks inside z_force (or right after it)
         if (r->shear || r->step_path == KH_PATH_REFERENCE_ORDER) KH_LAUNCH("tick", tick(r->d_ts, r->stream));
+        if (r->shear) {
+            // Mean_shear_stress after the step (sp_3d_ns.cxx:962-964, sp_3d_ns.h:247-330)
+            const double dt = r->jikan.dt_fluid, vol = r->pp.L[0] * r->pp.L[1] * r->pp.L[2];
+            r->strain += r->sre * dt;
+            const ShearRec h{r->jikan.time + dt, r->sre, r->deg, r->strain, dt, r->eta, 1.0 / vol, r->dx3 / vol};
+            const int slot = r->jikan.ts - r->shear_hist_base;
+            double   *hist = (r->shear_hist && slot >= 0 && slot < r->shear_hist_n) ? r->shear_hist + 12 * slot : nullptr;
+            KH_LAUNCH("obl_record", obl_record(r->pp, r->shear_acc, r->shear_nacc, h, r->shear_last, hist, r->stream));
+        }
         if (r->traj)
             KH_LAUNCH("record", record(r->pp, r->dp, r->d_ts, r->traj_base, r->traj_steps, r->traj, r->stream));
         if (r->frames)
@@ -570,7 +586,7 @@ struct Step {
         r->sre = sre;
         r->deg = deg;
         // u* on the Cartesian grid with the mean flow (ucp) into W, where phi_sum != 0
-        KH_LAUNCH("obl_to_cart", (obl_to_cart<T>(g, u3, w3, deg, sre, PS(r), s)));
+        KH_LAUNCH("obl_to_cart", (obl_to_cart<T>(g, u3, w3, deg, sre, PS(r), r->shear_acc, r->P.dx, s)));
         // F_H on ucp, velocities, noise, u_p
         KH_LAUNCH("f_hydro", (f_hydro<T>(r->pp, r->dp, w3, PS(r), CP(r), (-r->dx3 * r->rho) / dt, s)));
         KH_TRY(velocity_update(r, ab2));
@@ -741,6 +757,8 @@ int free_run(kh_run *r) {
         if (b) (void)hipFreeAsync(b, r->stream);
     if (r->traj) (void)hipFreeAsync(r->traj, r->stream);
     if (r->frames) (void)hipFreeAsync(r->frames, r->stream);
+    for (double *b : {r->shear_pstress, r->shear_acc, r->shear_last, r->shear_hist})
+        if (b) (void)hipFreeAsync(b, r->stream);
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto &p : r->pending) {
         (void)hipEventDestroy(p.a);
@@ -997,6 +1015,7 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     pp.gravity_on_fluid = r->gravity_on_fluid;
     pp.motor           = r->motor;
     pp.xs              = r->xs;
+    pp.rho             = P->rho;
     {
         const int nh    = (int)P->a + (int)P->a_xi + 2, nmin = std::min(P->nx, std::min(P->ny, P->nz));
         pp.direct_dist = 2 * (nh + 2) <= nmin;
@@ -1048,6 +1067,7 @@ int kh_set_particles(kh_run *r, const Particle *p, int n) {
             d.smc[k] = s.surface_mass_center[k];
             d.sdv[k] = s.surface_dv[k];
             d.sdw[k] = s.surface_dw[k];
+            d.mdfr[k] = s.momentum_depend_fr[k];
             for (int l = 0; l < 3; l++) {
                 d.inertia[3 * k + l]  = s.inertia[k][l];
                 d.sinertia[3 * k + l] = s.surface_inertia[k][l];
@@ -1110,6 +1130,7 @@ int kh_get_particles(kh_run *r, Particle *p, int n) {
             s.surface_mass_center[k] = d.smc[k];
             s.surface_dv[k] = d.sdv[k];
             s.surface_dw[k] = d.sdw[k];
+            s.momentum_depend_fr[k] = d.mdfr[k];
             for (int l = 0; l < 3; l++) {
                 s.inertia[k][l]         = d.inertia[3 * k + l];
                 s.surface_inertia[k][l] = d.sinertia[3 * k + l];
@@ -1245,6 +1266,19 @@ int kh_set_shear(kh_run *r, double shear_rate, int ac, double frequency) {
     if (!std::isfinite(shear_rate) || !std::isfinite(frequency)) return fail(KH_ERR_ARG, "non-finite shear parameter");
     if (r->slip) return fail(KH_ERR_UNSUPPORTED, "the sheared step has no squirmer slip iteration");
     drop_graph(r);
+    if (!r->shear_last) {  // the stress record's device buffers
+        const size_t np = (size_t)(r->np > 0 ? r->np : 1);
+        r->shear_nacc   = obl_to_cart_blocks(r->g);
+        if (hipMallocAsync((void **)&r->shear_pstress, sizeof(double) * 6 * np, r->stream) != hipSuccess ||
+            hipMallocAsync((void **)&r->shear_acc, sizeof(double) * r->shear_nacc, r->stream) != hipSuccess ||
+            hipMallocAsync((void **)&r->shear_last, sizeof(double) * KH_SHEAR_RECORD, r->stream) != hipSuccess)
+            return fail(KH_ERR_NOMEM, "stress record buffers");
+        KH_HIP(hipMemsetAsync(r->shear_pstress, 0, sizeof(double) * 6 * np, r->stream));
+        KH_HIP(hipMemsetAsync(r->shear_acc, 0, sizeof(double) * r->shear_nacc, r->stream));
+        KH_HIP(hipMemsetAsync(r->shear_last, 0, sizeof(double) * KH_SHEAR_RECORD, r->stream));
+        r->pp.pstress = r->shear_pstress;
+    }
+    r->strain     = 0.0;
     r->shear      = 1;
     r->shear_ac   = ac ? 1 : 0;
     r->shear_rate = shear_rate;
@@ -1262,6 +1296,46 @@ int kh_set_shear(kh_run *r, double shear_rate, int ac, double frequency) {
     const double dia     = std::min((sigma + r->xi) / pw, sigma);
     r->pp.lj_cutoff      = sigma > 0.0 ? r->lj_cutoff / sigma * dia : 0.0;
     r->pp.lj_dia         = dia;
+    return KH_OK;
+}
+
+int kh_get_shear_record(kh_run *r, double out[KH_SHEAR_RECORD]) {
+    if (!r || !out) return fail(KH_ERR_ARG, "NULL argument");
+    if (!r->shear) return fail(KH_ERR_STATE, "not a sheared run (kh_set_shear)");
+    KH_HIP(hipMemcpyAsync(out, r->shear_last, sizeof(double) * KH_SHEAR_RECORD, hipMemcpyDeviceToHost, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    return KH_OK;
+}
+
+int kh_set_shear_history(kh_run *r, int nsteps) {
+    if (!r) return fail(KH_ERR_ARG, "NULL run");
+    if (!r->shear) return fail(KH_ERR_STATE, "not a sheared run (kh_set_shear)");
+    if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
+    if (r->shear_hist) {
+        KH_HIP(hipFreeAsync(r->shear_hist, r->stream));
+        r->shear_hist = nullptr;
+    }
+    r->shear_hist_n = 0;
+    if (nsteps == 0) return KH_OK;
+    const size_t bytes = sizeof(double) * KH_SHEAR_RECORD * (size_t)nsteps;
+    if (hipMallocAsync((void **)&r->shear_hist, bytes, r->stream) != hipSuccess) {
+        r->shear_hist = nullptr;
+        return fail(KH_ERR_NOMEM, "stress history of %zu bytes", bytes);
+    }
+    KH_HIP(hipMemsetAsync(r->shear_hist, 0, bytes, r->stream));
+    r->shear_hist_n    = nsteps;
+    r->shear_hist_base = r->jikan.ts;
+    return KH_OK;
+}
+
+int kh_get_shear_history(kh_run *r, double *out, int nsteps) {
+    if (!r || (!out && nsteps > 0)) return fail(KH_ERR_ARG, "NULL argument");
+    if (nsteps < 0 || nsteps > r->shear_hist_n) return fail(KH_ERR_ARG, "%d records requested, %d kept", nsteps,
+                                                            r->shear_hist_n);
+    if (nsteps > 0)
+        KH_HIP(hipMemcpyAsync(out, r->shear_hist, sizeof(double) * KH_SHEAR_RECORD * (size_t)nsteps,
+                              hipMemcpyDeviceToHost, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
 }
 
@@ -1314,6 +1388,7 @@ int kh_get_trajectory(kh_run *r, double *out, int nsteps) {
     return KH_OK;
 }
 
+static_assert(KH_SHEAR_RECORD == 12, "k_obl_record writes 12 doubles");
 static_assert(KH_FRAME_WORDS == FRAME_WORDS, "Output_udf frame layout");
 
 int kh_set_output(kh_run *r, int nframes, int every) {

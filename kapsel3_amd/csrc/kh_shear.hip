@@ -275,34 +275,59 @@ __device__ __forceinline__ void orth2obl(const Grid &g, double gd, int j, int i,
 // components -> Cartesian grid and components, plus the mean flow. With a
 // mask (phi_sum) only the points where it is nonzero are written: the
 // step reads this field nowhere else (F_H gathers and f = u_p - phi u
-// are zero-weighted there).
+// are zero-weighted there). With `acc`, every point's term of
+// Calc_Reynolds_shear_stress (fluid_solver.h:165-183),
+// (u_x - rate (j - NY/2) DX) u_y, is summed per workgroup into
+// acc[blockIdx.x] (one partial per workgroup, summed in a fixed order by
+// the record kernel).
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_to_cart(Grid g, const T *__restrict__ u0, const T *__restrict__ u1,
                                                        const T *__restrict__ u2, T *__restrict__ c0,
                                                        T *__restrict__ c1, T *__restrict__ c2, double gd,
-                                                       double rate, const T *__restrict__ mask) {
-    const long long t = (long long)blockIdx.x * SH_NT + threadIdx.x;
-    if (t >= g.nreal) return;
-    const int       k    = (int)(t % g.nz);
-    const long long rest = t / g.nz;
-    const int       j = (int)(rest % g.ny), i = (int)(rest / g.ny);
-    int             ip, io;
-    double          alpha, beta;
-    obl2orth(g, gd, j, i, ip, io, alpha, beta);
-    const long long qp = ((long long)ip * g.ny + j) * g.nz + k, qo = ((long long)io * g.ny + j) * g.nz + k;
-    if (mask && mask[qo] == (T)0) return;
-    const double    a0 = beta * (double)u0[t] + alpha * (double)u0[qp];
-    const double    a1 = beta * (double)u1[t] + alpha * (double)u1[qp];
-    const double    a2 = beta * (double)u2[t] + alpha * (double)u2[qp];
-    // the reference adds the two terms to the stored (T) value in turn
-    T x = (T)a0;
-    const T y = (T)a1;
-    x       = (T)((double)x + gd * (double)y);
-    x       = (T)((double)x + rate * (j - g.ny / 2.0));
-    c0[qo]  = x;
-    c1[qo]  = y;
-    c2[qo]  = (T)a2;
+                                                       double rate, const T *__restrict__ mask,
+                                                       double *__restrict__ acc, double dx) {
+    const long long t   = (long long)blockIdx.x * SH_NT + threadIdx.x;
+    double          rey = 0.0;
+    if (t < g.nreal) {
+        const int       k    = (int)(t % g.nz);
+        const long long rest = t / g.nz;
+        const int       j = (int)(rest % g.ny), i = (int)(rest / g.ny);
+        int             ip, io;
+        double          alpha, beta;
+        obl2orth(g, gd, j, i, ip, io, alpha, beta);
+        const long long qp = ((long long)ip * g.ny + j) * g.nz + k, qo = ((long long)io * g.ny + j) * g.nz + k;
+        const bool      put = !mask || mask[qo] != (T)0;
+        if (put || acc) {
+            const double a0 = beta * (double)u0[t] + alpha * (double)u0[qp];
+            const double a1 = beta * (double)u1[t] + alpha * (double)u1[qp];
+            // the reference adds the two terms to the stored (T) value in turn
+            T x = (T)a0;
+            const T y = (T)a1;
+            x       = (T)((double)x + gd * (double)y);
+            x       = (T)((double)x + rate * (j - g.ny / 2.0));
+            if (acc) rey = ((double)x - rate * ((double)(j - g.ny / 2) * dx)) * (double)y;
+            if (put) {
+                c0[qo] = x;
+                c1[qo] = y;
+                c2[qo] = (T)(beta * (double)u2[t] + alpha * (double)u2[qp]);
+            }
+        }
+    }
+    if (acc) {
+        __shared__ double red[SH_NT / 64];
+        for (int off = 32; off > 0; off >>= 1) rey += __shfl_down(rey, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rey;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = 0.0;
+            for (int w = 0; w < SH_NT / 64; w++) b += red[w];
+            acc[blockIdx.x] = b;
+        }
+    }
 }
+
+// workgroups of k_obl_to_cart: the length of its `acc` array
+long long obl_to_cart_blocks(const Grid &g) { return (long long)sh_blocks(g.nreal); }
 
 // Make_f_particle_dt_nonsole (f_particle.h:35-54) on the Cartesian grid,
 // U2u_oblique (fft_wrapper.h:448-478) onto the oblique grid and
@@ -440,9 +465,9 @@ hipError_t obl_remap(const Grid &g, T *const u[3], T *const o[3], double delta, 
 }
 template <typename T>
 hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, const T *mask,
-                       hipStream_t s) {
+                       double *acc, double dx, hipStream_t s) {
     hipLaunchKernelGGL(k_obl_to_cart<T>, dim3(sh_blocks(g.nreal)), dim3(SH_NT), 0, s, g, u[0], u[1], u[2], c[0], c[1],
-                       c[2], gd, rate, mask);
+                       c[2], gd, rate, mask, acc, dx);
     return hipGetLastError();
 }
 template <typename T>
@@ -470,7 +495,7 @@ hipError_t obl_u_to_zeta(const Grid &g, const cpx<T> *u0, const cpx<T> *u1, cons
                                       double, double, double, hipStream_t);                                          \
     template hipError_t obl_remap<T>(const Grid &, T *const[3], T *const[3], double, hipStream_t);                   \
     template hipError_t obl_to_cart<T>(const Grid &, T *const[3], T *const[3], double, double, const T *,         \
-                                       hipStream_t);                                                                 \
+                                       double *, double, hipStream_t);                                               \
     template hipError_t obl_add_f<T>(const Grid &, T *const[3], const T *, T *const[3], T *const[3], double,         \
                                      hipStream_t);                                                                   \
     template hipError_t obl_u_to_zeta<T>(const Grid &, const cpx<T> *, const cpx<T> *, const cpx<T> *, cpx<T> *,     \

@@ -83,7 +83,8 @@ def test_sheared_step_fp64_matches_oracle(gpu, ac):
     P, p, g0 = sheared_system(n)
     z = initial_fluid(P, n, g0, rate)
     gr, orc = _pair(P, p, z, rate, g0, ac=ac, freq=0.5, ts0=0 if not ac else 3)
-    seen_remap = False
+    seen_remap = seen_lj = False
+    gr.set_shear_history(12)
     for chunk in range(4):
         gr.step(3)
         orc.step(3)
@@ -92,6 +93,18 @@ def test_sheared_step_fp64_matches_oracle(gpu, ac):
         assert abs(gg - go) < 1e-14 and abs(sg - so) < 1e-14 * max(abs(so), 1)
         seen_remap |= gg < g0
         cmp_particles(gr.get_particles(), orc.get_particles(), 1e-10, keys=KEYS, p0=p, a=4.0)
+        # the stress record (Mean_shear_stress) of the step just taken
+        rg, ro = gr.shear_record(), orc.shear_record()
+        for k in ("time", "shear_rate", "degree_oblique", "strain", "fluid_stress"):
+            assert abs(rg[k] - ro[k]) <= 1e-13 * max(abs(ro[k]), 1.0), (k, rg[k], ro[k])
+        for k in ("lj_dev_stress", "shear_stress_old", "shear_stress_new", "reynolds_stress", "apparent_stress"):
+            assert abs(rg[k] - ro[k]) <= 1e-9 * max(abs(ro[k]), 1e-12), (k, rg[k], ro[k])
+        seen_lj |= abs(ro["lj_dev_stress"]) > 0
+        assert abs(ro["shear_stress_new"]) > 0
+    assert seen_lj                              # the LJ pair across the boundary contributes
+    hist = gr.get_shear_history()
+    assert np.array_equal(hist[-1], np.array(list(gr.shear_record().values())))
+    assert np.allclose(np.diff(hist[:, 0]), P.c.dt)
     zg, zo = gr.get_fluid(), orc.get_fluid()
     for a, b in zip(zg[:2], zo[:2]):
         assert rel(a, b) < 1e-10
