@@ -540,8 +540,9 @@ __global__ __launch_bounds__(PT_NT) void k_obl_record(PParams pp, const double *
     for (long long b = threadIdx.x; b < nacc; b += PT_NT) v[6] += acc[b];
     block_sum<7, PT_NT>(v);
     if (threadIdx.x == 0) {
+        // (no particles: the mean-force spread has no points, md_force.cxx:719-745)
         const double sum_volume = v[4] / pp.rho;
-        const double old = v[0] - (v[3] / sum_volume) * v[2];
+        const double old        = pp.np > 0 ? v[0] - (v[3] / sum_volume) * v[2] : v[0];
         double       rec[12];
         rec[0]  = h.time;
         rec[1]  = h.rate;

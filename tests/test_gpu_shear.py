@@ -169,3 +169,24 @@ def test_sheared_production_grid_fp64(gpu, n, np_, steps):
     zo, zg = orc.get_fluid(), gr.get_fluid()
     assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
     gr.close()
+
+
+def test_sheared_fluid_without_particles(gpu):
+    """Lees-Edwards with no particles: the flow of a random initial field
+    against the oracle; the stress record has no particle terms."""
+    from kapsel3_amd import Run
+    n, rate = 32, 0.3
+    P, _, g0 = sheared_system(n)
+    z = initial_fluid(P, n, g0, rate)
+    gr, orc = Run(P, 0), kref.Run(P, 0)
+    for r in (gr, orc):
+        r.set_fluid(*z)
+        r.set_shear(rate)
+        r.set_shear_state(g0, rate)
+        r.step(5)
+    zg, zo = gr.get_fluid(), orc.get_fluid()
+    assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
+    rg, ro = gr.shear_record(), orc.shear_record()
+    assert rg["shear_stress_old"] == 0.0 and rg["shear_stress_new"] == 0.0 and ro["shear_stress_new"] == 0.0
+    assert abs(rg["reynolds_stress"] - ro["reynolds_stress"]) <= 1e-9 * abs(ro["reynolds_stress"])
+    gr.close()
