@@ -132,6 +132,12 @@ enum kh_fn {
     KH_FN_SLIP_MOMENTUM,      /* Make_particle_momentum_factor (u)     operate_surface.cxx:12-153 */
     KH_FN_SLIP_SCATTER,       /* Reset_u + Make_force_u_slip_particle  operate_surface.cxx:200-363 */
     KH_FN_SLIP_ITERATION,     /* the slip loop + Swap_mem(u, up)       sp_3d_ns.cxx:121-161 */
+    /* sheared runs (kh_set_shear), at the run's strain and rate; on such a
+       run the particle functions above take the Lees-Edwards geometry */
+    KH_FN_OBL_NS_SOLVER,      /* NS_solver_slavedEuler_Shear_OBL fluid_solver.cxx:411-437 */
+    KH_FN_OBL_ZETA_K2U,       /* Zeta_k2u_k_OBL + U_k2u        fft_wrapper.cxx:690-768 */
+    KH_FN_OBL_U2ZETA_K,       /* U2u_k + Solenoidal_uk_OBL + U_k2zeta_k_OBL (sp_3d_ns.cxx:428-431) */
+    KH_FN_OBL_OBLIQUE2U,      /* Transform_obl_u(u, oblique2cartesian) fft_wrapper.h:480-505 */
     KH_FN_COUNT
 };
 

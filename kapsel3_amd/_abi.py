@@ -47,7 +47,8 @@ FIELD = {"u0": 0, "u1": 1, "u2": 2, "up0": 3, "up1": 4, "up2": 5, "phi": 6, "phi
 FN = {name: i for i, name in enumerate([
     "ns_solver", "zeta_k2u", "u2zeta_k", "position_euler", "position_ab2", "make_phi", "f_hydro",
     "velocity_euler", "velocity_ab2", "thermostat", "make_up", "f_particle_sole", "add_f_particle",
-    "slip_momentum", "slip_scatter", "slip_iteration"])}
+    "slip_momentum", "slip_scatter", "slip_iteration", "obl_ns_solver", "obl_zeta_k2u", "obl_u2zeta_k",
+    "obl_oblique2u"])}
 
 
 class KhSpecies(C.Structure):

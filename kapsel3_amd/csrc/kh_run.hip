@@ -625,6 +625,44 @@ struct Step {
         return KH_OK;
     }
 
+    // single functions of the sheared step on the run's state (strain and
+    // rate as kh_set_shear_state left them)
+    static int apply_obl(kh_run *r, int fn) {
+        const Grid &g      = r->g;
+        hipStream_t s      = r->stream;
+        cpx<T>     *s3[3]  = {S(r, 0), S(r, 1), S(r, 2)};
+        T          *u3[3]  = {U(r, 0), U(r, 1), U(r, 2)}, *w3[3] = {W(r, 0), W(r, 1), W(r, 2)};
+        const double dt    = r->jikan.dt_fluid;
+        switch (fn) {
+            case KH_FN_OBL_NS_SOLVER:  // Zeta_k2advection_k_OBL + slaved Euler, as in the step
+                KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], S(r, 3),
+                                                             r->ukdc, 0, 1, r->deg, s)));
+                KH_TRY(c2r_kept3(r, s3, u3));
+                KH_LAUNCH("obl_zeta_to_omega", (obl_zeta_to_omega<T>(g, Z(r, 0), Z(r, 1), s3, s)));
+                KH_TRY(c2r_kept3(r, s3, w3));
+                KH_LAUNCH("obl_cross", (obl_cross<T>(g.nreal, u3, w3, s)));
+                KH_TRY(r2c_kept3(r, w3, s3));
+                KH_LAUNCH("obl_adv_ns", (obl_adv_ns<T>(g, s3, S(r, 3), Z(r, 0), Z(r, 1), r->sre, r->deg,
+                                                       -r->nu * dt, 1. / r->nu, s)));
+                return KH_OK;
+            case KH_FN_OBL_ZETA_K2U:  // Zeta_k2u_k_OBL + U_k2u on every mode of zeta
+                KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], nullptr,
+                                                             r->ukdc, 0, -1, r->deg, s)));
+                for (int d = 0; d < 3; d++) KH_TRY(c2r(r, s3[d], u3[d]));
+                return KH_OK;
+            case KH_FN_OBL_U2ZETA_K:  // U2u_k + Solenoidal_uk_OBL + U_k2zeta_k_OBL
+                for (int d = 0; d < 3; d++) KH_TRY(r2c(r, u3[d], s3[d]));
+                KH_LAUNCH("obl_u_to_zeta", (obl_u_to_zeta<T>(g, s3[0], s3[1], s3[2], Z(r, 0), Z(r, 1), r->ukdc,
+                                                             r->deg, s)));
+                return KH_OK;
+            case KH_FN_OBL_OBLIQUE2U:  // U_oblique2u on u, in place
+                KH_LAUNCH("obl_to_cart", (obl_to_cart<T>(g, u3, w3, r->deg, r->sre, nullptr, nullptr, r->P.dx, s)));
+                for (int d = 0; d < 3; d++) std::swap(r->ureal[d], r->work[d]);
+                return KH_OK;
+        }
+        return fail(KH_ERR_ARG, "kh_apply: unknown function %d", fn);
+    }
+
     static int apply(kh_run *r, int fn) {
         r->zeta_dirty = 1;
         r->pclean     = false;
@@ -656,6 +694,12 @@ struct Step {
             case KH_FN_SLIP_ITERATION:
                 if (!r->slip) return fail(KH_ERR_STATE, "no slip species in this run");
                 return slip_iteration(r, r->jikan.ts != 0, false);
+            case KH_FN_OBL_NS_SOLVER:
+            case KH_FN_OBL_ZETA_K2U:
+            case KH_FN_OBL_U2ZETA_K:
+            case KH_FN_OBL_OBLIQUE2U:
+                if (!r->shear) return fail(KH_ERR_STATE, "not a sheared run (kh_set_shear)");
+                return apply_obl(r, fn);
             default: return fail(KH_ERR_ARG, "kh_apply: unknown function %d", fn);
         }
     }

@@ -43,6 +43,7 @@ __device__ __forceinline__ void contra2co(double gd, double *v) {
 // components first (FIX_CELL, sp_3d_ns.cxx:339-345). Only the kept modes of
 // u_hat and w1 are written: their consumers (the pruned inverse transform,
 // the slaved-Euler pass) read no others, and zeta is zero outside them.
+// truncate < 0: every mode is converted (zeta as given).
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_u(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
                                                          cpx<T> *__restrict__ u0, cpx<T> *__restrict__ u1,
@@ -56,7 +57,7 @@ __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_u(Grid g, cpx<T> *__restr
     if (kz >= g.nzh) return;
     const int ikx = kx_of(g, i), iky = ky_of(g, j);
     cpx<T>    r0 = cmk((T)0, (T)0), r1 = r0, r2 = r0;
-    if (!kept_mode(g, ikx, iky, kz)) {
+    if (truncate >= 0 && !kept_mode(g, ikx, iky, kz)) {
         if (truncate) z0[idx] = z1[idx] = r0;
         return;
     }
