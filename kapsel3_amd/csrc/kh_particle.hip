@@ -537,7 +537,13 @@ __global__ __launch_bounds__(PT_NT) void k_obl_record(PParams pp, const double *
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int n = threadIdx.x; n < pp.np; n += PT_NT)
         for (int k = 0; k < 6; k++) v[k] += pp.pstress[6 * (long long)n + k];
-    for (long long b = threadIdx.x; b < nacc; b += PT_NT) v[6] += acc[b];
+    // eight running sums keep eight loads of the partials in flight
+    double    a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long b    = threadIdx.x;
+    for (; b + 7 * PT_NT < nacc; b += 8 * PT_NT)
+        for (int u = 0; u < 8; u++) a[u] += acc[b + u * PT_NT];
+    for (; b < nacc; b += PT_NT) a[0] += acc[b];
+    v[6] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     block_sum<7, PT_NT>(v);
     if (threadIdx.x == 0) {
         // (no particles: the mean-force spread has no points, md_force.cxx:719-745)

@@ -562,7 +562,9 @@ struct Step {
         // curl, slaved Euler
         // (zeta is truncated here, so u, omega and the kept modes of u x omega
         // take the pruned transforms)
-        KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], S(r, 3), r->ukdc, 0, 1,
+        // (zeta's modes beyond the 2/3 rule are left as they are: nothing in
+        // the step reads them before U_k2zeta_k_OBL rewrites every mode)
+        KH_LAUNCH("obl_zeta_to_u", (obl_zeta_to_u<T>(g, Z(r, 0), Z(r, 1), s3[0], s3[1], s3[2], S(r, 3), r->ukdc, 0, 0,
                                                      gd, s)));
         KH_TRY(c2r_kept3(r, s3, u3));
         KH_LAUNCH("obl_zeta_to_omega", (obl_zeta_to_omega<T>(g, Z(r, 0), Z(r, 1), s3, s)));

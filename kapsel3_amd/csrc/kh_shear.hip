@@ -17,11 +17,33 @@ constexpr int SH_NT = 256;
 
 static inline unsigned sh_blocks(long long n) { return (unsigned)((n + SH_NT - 1) / SH_NT); }
 
-__device__ __forceinline__ void sh_spec_index(const Grid &g, long long idx, int &i, int &j, int &kz) {
-    kz             = (int)(idx % g.nzp);
-    long long rest = idx / g.nzp;
-    j              = (int)(rest % g.ny);
-    i              = (int)(rest / g.ny);
+// Grid axes are at most 512 points, so every element index fits 32 bits:
+// index arithmetic is unsigned 32-bit (a 64-bit division is a long
+// instruction sequence on the GPU).
+__device__ __forceinline__ void sh_spec_index(const Grid &g, unsigned idx, int &i, int &j, int &kz) {
+    kz                  = (int)(idx % (unsigned)g.nzp);
+    const unsigned rest = idx / (unsigned)g.nzp;
+    j                   = (int)(rest % (unsigned)g.ny);
+    i                   = (int)(rest / (unsigned)g.ny);
+}
+
+// t-th kept mode (|kx| < trn_x, |ky| < trn_y, kz < trn_z), kz fastest: a
+// launch over sh_nkept(g) threads visits each kept mode once.
+__device__ __forceinline__ void sh_kept_index(const Grid &g, unsigned t, int &i, int &j, int &kz) {
+    const unsigned nky  = (unsigned)nkept(g.trn_y);
+    kz                  = (int)(t % (unsigned)g.trn_z);
+    const unsigned rest = t / (unsigned)g.trn_z;
+    j                   = kept_index((int)(rest % nky), g.ny, g.trn_y);
+    i                   = kept_index((int)(rest / nky), g.nx, g.trn_x);
+}
+static inline long long sh_nkept(const Grid &g) { return (long long)(2 * g.trn_x - 1) * (2 * g.trn_y - 1) * g.trn_z; }
+
+// Real-space kernels walk whole rows (fixed i, j; k contiguous), SH_RPW rows
+// per workgroup, so a row's indices and interpolation weights are worked out
+// once per row.
+constexpr int SH_RPW = 8;
+static inline unsigned sh_row_blocks(const Grid &g) {
+    return (unsigned)(((long long)g.nx * g.ny + SH_RPW - 1) / SH_RPW);
 }
 
 __device__ __forceinline__ double k2_obl(const Grid &g, int ikx, int iky, int kz, double gd) {
@@ -43,21 +65,27 @@ __device__ __forceinline__ void contra2co(double gd, double *v) {
 // components first (FIX_CELL, sp_3d_ns.cxx:339-345). Only the kept modes of
 // u_hat and w1 are written: their consumers (the pruned inverse transform,
 // the slaved-Euler pass) read no others, and zeta is zero outside them.
-// truncate < 0: every mode is converted (zeta as given).
+// truncate < 0: every mode is converted (zeta as given). With truncate == 0
+// the launch covers the kept modes only (sh_kept_index).
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_u(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
                                                          cpx<T> *__restrict__ u0, cpx<T> *__restrict__ u1,
                                                          cpx<T> *__restrict__ u2, cpx<T> *__restrict__ w1,
                                                          double *__restrict__ ukdc, int fixmask, int truncate,
-                                                         double gd) {
-    const long long idx = (long long)blockIdx.x * SH_NT + threadIdx.x;
-    if (idx >= g.nspec) return;
+                                                         double gd, unsigned nthreads) {
+    const unsigned t = blockIdx.x * SH_NT + threadIdx.x;
+    if (t >= nthreads) return;
     int i, j, kz;
-    sh_spec_index(g, idx, i, j, kz);
-    if (kz >= g.nzh) return;
-    const int ikx = kx_of(g, i), iky = ky_of(g, j);
+    if (truncate == 0) {
+        sh_kept_index(g, t, i, j, kz);
+    } else {
+        sh_spec_index(g, t, i, j, kz);
+        if (kz >= g.nzh) return;
+    }
+    const unsigned idx = ((unsigned)i * g.ny + j) * g.nzp + kz;
+    const int      ikx = kx_of(g, i), iky = ky_of(g, j);
     cpx<T>    r0 = cmk((T)0, (T)0), r1 = r0, r2 = r0;
-    if (truncate >= 0 && !kept_mode(g, ikx, iky, kz)) {
+    if (truncate > 0 && !kept_mode(g, ikx, iky, kz)) {
         if (truncate) z0[idx] = z1[idx] = r0;
         return;
     }
@@ -119,14 +147,14 @@ __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_u(Grid g, cpx<T> *__restr
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_zeta_to_omega(Grid g, const cpx<T> *__restrict__ z0,
                                                              const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ o0,
-                                                             cpx<T> *__restrict__ o1, cpx<T> *__restrict__ o2) {
-    const long long idx = (long long)blockIdx.x * SH_NT + threadIdx.x;
-    if (idx >= g.nspec) return;
+                                                             cpx<T> *__restrict__ o1, cpx<T> *__restrict__ o2,
+                                                             unsigned nthreads) {
+    const unsigned t = blockIdx.x * SH_NT + threadIdx.x;
+    if (t >= nthreads) return;
     int i, j, kz;
-    sh_spec_index(g, idx, i, j, kz);
-    if (kz >= g.nzh) return;
-    const int    ikx = kx_of(g, i), iky = ky_of(g, j);
-    if (!kept_mode(g, ikx, iky, kz)) return;
+    sh_kept_index(g, t, i, j, kz);
+    const unsigned idx = ((unsigned)i * g.ny + j) * g.nzp + kz;
+    const int      ikx = kx_of(g, i), iky = ky_of(g, j);
     const double kx = g.wave_x * ikx, ky = g.wave_y * iky, kzz = g.wave_z * kz;
     const cpx<T> d1 = z0[idx], d2 = z1[idx];
     cpx<T>       a, b, c;
@@ -178,15 +206,13 @@ __global__ __launch_bounds__(SH_NT) void k_obl_adv_ns(Grid g, const cpx<T> *__re
                                                       const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2,
                                                       const cpx<T> *__restrict__ w1, cpx<T> *__restrict__ z0,
                                                       cpx<T> *__restrict__ z1, double rate, double gd, double dmy0,
-                                                      double dmy1) {
-    const long long t  = (long long)blockIdx.x * SH_NT + threadIdx.x;
-    const int       ky = nkept(g.trn_y);
-    if (t >= (long long)nkept(g.trn_x) * ky * g.trn_z) return;
-    const int       kz   = (int)(t % g.trn_z);
-    const long long rest = t / g.trn_z;
-    const int       j = kept_index((int)(rest % ky), g.ny, g.trn_y), i = kept_index((int)(rest / ky), g.nx, g.trn_x);
-    const int       ikx = kx_of(g, i), iky = ky_of(g, j);
-    const long long idx = ((long long)i * g.ny + j) * g.nzp + kz;
+                                                      double dmy1, unsigned nthreads) {
+    const unsigned t = blockIdx.x * SH_NT + threadIdx.x;
+    if (t >= nthreads) return;
+    int i, j, kz;
+    sh_kept_index(g, t, i, j, kz);
+    const int      ikx = kx_of(g, i), iky = ky_of(g, j);
+    const unsigned idx = ((unsigned)i * g.ny + j) * g.nzp + kz;
     const cpx<T>    w   = w1[idx];
     const double    s2  = 2. * rate;
     // F_hat (u x omega) less the shear terms, re/im
@@ -257,8 +283,9 @@ __device__ __forceinline__ void obl2orth(const Grid &g, double gd, int j, int i,
     const int io = (int)((int)(sign * gd * delta_j) * sign + sign);
     alpha        = (io - gd * delta_j) * sign;
     beta         = 1.0 - alpha;
-    i_oblique    = (int)fmod(i + io + 4.0 * g.nx, (double)g.nx);
-    i_plus       = (int)fmod(i + sign + 2.0 * g.nx, (double)g.nx);
+    // (the indices are integers: fmod of their sums is the integer remainder)
+    i_oblique    = (i + io + 4 * g.nx) % g.nx;
+    i_plus       = (i + (int)sign + 2 * g.nx) % g.nx;
 }
 __device__ __forceinline__ void orth2obl(const Grid &g, double gd, int j, int i, int &i_oblique,
                                          int &i_oblique_plus, double &alpha, double &beta) {
@@ -268,8 +295,8 @@ __device__ __forceinline__ void orth2obl(const Grid &g, double gd, int j, int i,
     const int io   = (int)((int)(sign * gd * delta_j) * sign);
     alpha          = (gd * delta_j - io) * sign;
     beta           = 1.0 - alpha;
-    i_oblique      = (int)fmod(i + io + 4.0 * g.nx, (double)g.nx);
-    i_oblique_plus = (int)fmod(i_oblique + sign + 4.0 * g.nx, (double)g.nx);
+    i_oblique      = (i + io + 4 * g.nx) % g.nx;
+    i_oblique_plus = (i_oblique + (int)sign + 4 * g.nx) % g.nx;
 }
 
 // U_oblique2u (fft_wrapper.h:480-505): oblique grid, contravariant
@@ -287,26 +314,28 @@ __global__ __launch_bounds__(SH_NT) void k_obl_to_cart(Grid g, const T *__restri
                                                        T *__restrict__ c1, T *__restrict__ c2, double gd,
                                                        double rate, const T *__restrict__ mask,
                                                        double *__restrict__ acc, double dx) {
-    const long long t   = (long long)blockIdx.x * SH_NT + threadIdx.x;
-    double          rey = 0.0;
-    if (t < g.nreal) {
-        const int       k    = (int)(t % g.nz);
-        const long long rest = t / g.nz;
-        const int       j = (int)(rest % g.ny), i = (int)(rest / g.ny);
-        int             ip, io;
-        double          alpha, beta;
+    const unsigned nrow = (unsigned)g.nx * g.ny;
+    double         rey  = 0.0;
+    for (int w = 0; w < SH_RPW; w++) {
+        const unsigned r = blockIdx.x * SH_RPW + w;
+        if (r >= nrow) break;  // the same row for every thread
+        const int i = (int)(r / (unsigned)g.ny), j = (int)(r - (unsigned)i * g.ny);
+        int       ip, io;
+        double    alpha, beta;
         obl2orth(g, gd, j, i, ip, io, alpha, beta);
-        const long long qp = ((long long)ip * g.ny + j) * g.nz + k, qo = ((long long)io * g.ny + j) * g.nz + k;
-        const bool      put = !mask || mask[qo] != (T)0;
-        if (put || acc) {
+        const unsigned rt = r * g.nz, rp = ((unsigned)ip * g.ny + j) * g.nz, ro = ((unsigned)io * g.ny + j) * g.nz;
+        for (int k = threadIdx.x; k < g.nz; k += SH_NT) {
+            const unsigned t = rt + k, qp = rp + k, qo = ro + k;
+            const bool     put = !mask || mask[qo] != (T)0;
+            if (!put && !acc) continue;
             const double a0 = beta * (double)u0[t] + alpha * (double)u0[qp];
             const double a1 = beta * (double)u1[t] + alpha * (double)u1[qp];
             // the reference adds the two terms to the stored (T) value in turn
-            T x = (T)a0;
+            T       x = (T)a0;
             const T y = (T)a1;
-            x       = (T)((double)x + gd * (double)y);
-            x       = (T)((double)x + rate * (j - g.ny / 2.0));
-            if (acc) rey = ((double)x - rate * ((double)(j - g.ny / 2) * dx)) * (double)y;
+            x         = (T)((double)x + gd * (double)y);
+            x         = (T)((double)x + rate * (j - g.ny / 2.0));
+            if (acc) rey += ((double)x - rate * ((double)(j - g.ny / 2) * dx)) * (double)y;
             if (put) {
                 c0[qo] = x;
                 c1[qo] = y;
@@ -328,7 +357,7 @@ __global__ __launch_bounds__(SH_NT) void k_obl_to_cart(Grid g, const T *__restri
 }
 
 // workgroups of k_obl_to_cart: the length of its `acc` array
-long long obl_to_cart_blocks(const Grid &g) { return (long long)sh_blocks(g.nreal); }
+long long obl_to_cart_blocks(const Grid &g) { return (long long)sh_row_blocks(g); }
 
 // Make_f_particle_dt_nonsole (f_particle.h:35-54) on the Cartesian grid,
 // U2u_oblique (fft_wrapper.h:448-478) onto the oblique grid and
@@ -338,7 +367,7 @@ long long obl_to_cart_blocks(const Grid &g) { return (long long)sh_blocks(g.nrea
 // source points both have f = 0 keeps its u (u + 0).
 template <typename T>
 __device__ __forceinline__ void f_at(const T *up0, const T *up1, const T *up2, const T *c0, const T *c1, const T *c2,
-                                     T ps, long long q, T *f) {
+                                     T ps, unsigned q, T *f) {
     if (ps == (T)0) {
         f[0] = f[1] = f[2] = (T)0;
         return;
@@ -354,27 +383,31 @@ __global__ __launch_bounds__(SH_NT) void k_obl_add_f(Grid g, const T *__restrict
                                                      const T *__restrict__ c0, const T *__restrict__ c1,
                                                      const T *__restrict__ c2, T *__restrict__ u0,
                                                      T *__restrict__ u1, T *__restrict__ u2, double gd) {
-    const long long t = (long long)blockIdx.x * SH_NT + threadIdx.x;
-    if (t >= g.nreal) return;
-    const int       k    = (int)(t % g.nz);
-    const long long rest = t / g.nz;
-    const int       j = (int)(rest % g.ny), i = (int)(rest / g.ny);
-    int             io, iop;
-    double          alpha, beta;
-    orth2obl(g, gd, j, i, io, iop, alpha, beta);
-    const long long qa = ((long long)io * g.ny + j) * g.nz + k, qb = ((long long)iop * g.ny + j) * g.nz + k;
-    const T         pa = phisum[qa], pb = phisum[qb];
-    if (pa == (T)0 && pb == (T)0) return;
-    T fa[3], fb[3];
-    f_at(up0, up1, up2, c0, c1, c2, pa, qa, fa);
-    f_at(up0, up1, up2, c0, c1, c2, pb, qb, fb);
-    T f0 = (T)(beta * (double)fa[0] + alpha * (double)fb[0]);
-    const T f1 = (T)(beta * (double)fa[1] + alpha * (double)fb[1]);
-    const T f2 = (T)(beta * (double)fa[2] + alpha * (double)fb[2]);
-    f0 = (T)((double)f0 - gd * (double)f1);
-    u0[t] += f0;
-    u1[t] += f1;
-    u2[t] += f2;
+    const unsigned nrow = (unsigned)g.nx * g.ny;
+    for (int w = 0; w < SH_RPW; w++) {
+        const unsigned r = blockIdx.x * SH_RPW + w;
+        if (r >= nrow) return;  // the same row for every thread
+        const int i = (int)(r / (unsigned)g.ny), j = (int)(r - (unsigned)i * g.ny);
+        int       io, iop;
+        double    alpha, beta;
+        orth2obl(g, gd, j, i, io, iop, alpha, beta);
+        const unsigned rt = r * g.nz, ra = ((unsigned)io * g.ny + j) * g.nz, rb = ((unsigned)iop * g.ny + j) * g.nz;
+        for (int k = threadIdx.x; k < g.nz; k += SH_NT) {
+            const unsigned t = rt + k, qa = ra + k, qb = rb + k;
+            const T        pa = phisum[qa], pb = phisum[qb];
+            if (pa == (T)0 && pb == (T)0) continue;
+            T fa[3], fb[3];
+            f_at(up0, up1, up2, c0, c1, c2, pa, qa, fa);
+            f_at(up0, up1, up2, c0, c1, c2, pb, qb, fb);
+            T       f0 = (T)(beta * (double)fa[0] + alpha * (double)fb[0]);
+            const T f1 = (T)(beta * (double)fa[1] + alpha * (double)fb[1]);
+            const T f2 = (T)(beta * (double)fa[2] + alpha * (double)fb[2]);
+            f0         = (T)((double)f0 - gd * (double)f1);
+            u0[t] += f0;
+            u1[t] += f1;
+            u2[t] += f2;
+        }
+    }
 }
 
 // Solenoidal_uk_OBL (operate_omega.cxx:197-227) and U_k2zeta_k_OBL
@@ -384,8 +417,8 @@ __global__ __launch_bounds__(SH_NT) void k_obl_u_to_zeta(Grid g, const cpx<T> *_
                                                          const cpx<T> *__restrict__ u1, const cpx<T> *__restrict__ u2,
                                                          cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
                                                          double *__restrict__ ukdc, double gd) {
-    const long long idx = (long long)blockIdx.x * SH_NT + threadIdx.x;
-    if (idx >= g.nspec) return;
+    const unsigned idx = blockIdx.x * SH_NT + threadIdx.x;
+    if (idx >= (unsigned)g.nspec) return;
     int i, j, kz;
     sh_spec_index(g, idx, i, j, kz);
     if (kz >= g.nzh) return;
@@ -435,14 +468,16 @@ __global__ __launch_bounds__(SH_NT) void k_obl_u_to_zeta(Grid g, const cpx<T> *_
 template <typename T>
 hipError_t obl_zeta_to_u(const Grid &g, cpx<T> *z0, cpx<T> *z1, cpx<T> *u0, cpx<T> *u1, cpx<T> *u2, cpx<T> *w1,
                          double *ukdc, int fixmask, int truncate, double gd, hipStream_t s) {
-    hipLaunchKernelGGL(k_obl_zeta_to_u<T>, dim3(sh_blocks(g.nspec)), dim3(SH_NT), 0, s, g, z0, z1, u0, u1, u2, w1,
-                       ukdc, fixmask, truncate, gd);
+    const long long n = truncate == 0 ? sh_nkept(g) : g.nspec;
+    hipLaunchKernelGGL(k_obl_zeta_to_u<T>, dim3(sh_blocks(n)), dim3(SH_NT), 0, s, g, z0, z1, u0, u1, u2, w1, ukdc,
+                       fixmask, truncate, gd, (unsigned)n);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t obl_zeta_to_omega(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T> *const o[3], hipStream_t s) {
-    hipLaunchKernelGGL(k_obl_zeta_to_omega<T>, dim3(sh_blocks(g.nspec)), dim3(SH_NT), 0, s, g, z0, z1, o[0], o[1],
-                       o[2]);
+    const long long n = sh_nkept(g);
+    hipLaunchKernelGGL(k_obl_zeta_to_omega<T>, dim3(sh_blocks(n)), dim3(SH_NT), 0, s, g, z0, z1, o[0], o[1], o[2],
+                       (unsigned)n);
     return hipGetLastError();
 }
 template <typename T>
@@ -453,9 +488,9 @@ hipError_t obl_cross(long long n, T *const u[3], T *const w[3], hipStream_t s) {
 template <typename T>
 hipError_t obl_adv_ns(const Grid &g, cpx<T> *const F[3], const cpx<T> *w1, cpx<T> *z0, cpx<T> *z1, double rate,
                       double gd, double dmy0, double dmy1, hipStream_t s) {
-    const long long nk = (long long)(2 * g.trn_x - 1) * (2 * g.trn_y - 1) * g.trn_z;
+    const long long nk = sh_nkept(g);
     hipLaunchKernelGGL(k_obl_adv_ns<T>, dim3(sh_blocks(nk)), dim3(SH_NT), 0, s, g, F[0], F[1], F[2], w1, z0, z1, rate,
-                       gd, dmy0, dmy1);
+                       gd, dmy0, dmy1, (unsigned)nk);
     return hipGetLastError();
 }
 template <typename T>
@@ -467,14 +502,14 @@ hipError_t obl_remap(const Grid &g, T *const u[3], T *const o[3], double delta, 
 template <typename T>
 hipError_t obl_to_cart(const Grid &g, T *const u[3], T *const c[3], double gd, double rate, const T *mask,
                        double *acc, double dx, hipStream_t s) {
-    hipLaunchKernelGGL(k_obl_to_cart<T>, dim3(sh_blocks(g.nreal)), dim3(SH_NT), 0, s, g, u[0], u[1], u[2], c[0], c[1],
+    hipLaunchKernelGGL(k_obl_to_cart<T>, dim3(sh_row_blocks(g)), dim3(SH_NT), 0, s, g, u[0], u[1], u[2], c[0], c[1],
                        c[2], gd, rate, mask, acc, dx);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t obl_add_f(const Grid &g, T *const up[3], const T *phisum, T *const c[3], T *const u[3], double gd,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_obl_add_f<T>, dim3(sh_blocks(g.nreal)), dim3(SH_NT), 0, s, g, up[0], up[1], up[2], phisum,
+    hipLaunchKernelGGL(k_obl_add_f<T>, dim3(sh_row_blocks(g)), dim3(SH_NT), 0, s, g, up[0], up[1], up[2], phisum,
                        c[0], c[1], c[2], u[0], u[1], u[2], gd);
     return hipGetLastError();
 }
