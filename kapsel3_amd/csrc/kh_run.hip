@@ -15,6 +15,7 @@
 #include "../../include/kapsel_hip.h"
 #include "kh_fused.h"
 #include "kh_launch.h"
+#include "kh_zwave.h"
 
 using namespace kh;
 
@@ -527,6 +528,26 @@ struct Step {
         KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, r->stream, g.trn_z)));
         return KH_OK;
     }
+    // The same inverse transform of omega's three spectra, then u x omega
+    // (contravariant u and omega, covariant product) into out: fused into
+    // the z pass where the wavefront engine has the length, else a separate
+    // pass. u holds the real-space u.
+    static int c2r_kept3_cross(kh_run *r, cpx<T> *const sp[3], T *const u[3], T *const out[3]) {
+        const Grid &g = r->g;
+        if (!zw_cross_supported(g.nz)) {
+            KH_TRY(c2r_kept3(r, sp, out));
+            KH_LAUNCH("obl_cross", (obl_cross<T>(g.nreal, u, out, r->stream)));
+            return KH_OK;
+        }
+        Comps<T> c{{sp[0], sp[1], sp[2], nullptr, nullptr}};
+        KH_LAUNCH("x_inv_kept", (cols_x<T, false>(g, c, 3, 1, 0, r->stream)));
+        KH_LAUNCH("y_inv_kept", (cols_y<T, false>(g, c, 3, 1, 0, r->stream)));
+        const T scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
+        ZIO<T>  zio{{sp[0], sp[1], sp[2]}, {out[0], out[1], out[2]}};
+        KH_LAUNCH("z_c2r_cross",
+                  (zw_c2r_cross<T>(g.nz, zio, u, (long long)g.nx * g.ny, scale, g.trn_z, g.nzp, r->stream)));
+        return KH_OK;
+    }
     // Forward transform of three fields whose consumer reads kept modes only:
     // y and x passes write only the kept rows of the kept columns.
     static int r2c_kept3(kh_run *r, T *const in[3], cpx<T> *const sp[3]) {
@@ -568,8 +589,7 @@ struct Step {
                                                      gd, s)));
         KH_TRY(c2r_kept3(r, s3, u3));
         KH_LAUNCH("obl_zeta_to_omega", (obl_zeta_to_omega<T>(g, Z(r, 0), Z(r, 1), s3, s)));
-        KH_TRY(c2r_kept3(r, s3, w3));
-        KH_LAUNCH("obl_cross", (obl_cross<T>(g.nreal, u3, w3, s)));
+        KH_TRY(c2r_kept3_cross(r, s3, u3, w3));
         KH_TRY(r2c_kept3(r, w3, s3));
         KH_LAUNCH("obl_adv_ns", (obl_adv_ns<T>(g, s3, S(r, 3), Z(r, 0), Z(r, 1), r->sre, gd, -r->nu * dt, 1. / r->nu,
                                                s)));
@@ -641,8 +661,7 @@ struct Step {
                                                              r->ukdc, 0, 1, r->deg, s)));
                 KH_TRY(c2r_kept3(r, s3, u3));
                 KH_LAUNCH("obl_zeta_to_omega", (obl_zeta_to_omega<T>(g, Z(r, 0), Z(r, 1), s3, s)));
-                KH_TRY(c2r_kept3(r, s3, w3));
-                KH_LAUNCH("obl_cross", (obl_cross<T>(g.nreal, u3, w3, s)));
+                KH_TRY(c2r_kept3_cross(r, s3, u3, w3));
                 KH_TRY(r2c_kept3(r, w3, s3));
                 KH_LAUNCH("obl_adv_ns", (obl_adv_ns<T>(g, s3, S(r, 3), Z(r, 0), Z(r, 1), r->sre, r->deg,
                                                        -r->nu * dt, 1. / r->nu, s)));

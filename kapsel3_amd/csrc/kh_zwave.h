@@ -17,6 +17,11 @@ template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
                   hipStream_t s, const T *mask = nullptr);
 // inverse z (3) -> advection products -> forward z (5), modes kz < TRN_Z
+// inverse z pass of omega fused with u x omega (lengths >= 32)
+bool zw_cross_supported(int L);
+template <typename T>
+hipError_t zw_c2r_cross(int L, const ZIO<T> &io, const T *const u[3], long long nrows, T scale, int kzv, int pitch,
+                        hipStream_t s);
 template <typename T>
 hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T scale, hipStream_t s);
 // f = up - min(phi_sum,1) u* -> forward z (3), modes kz < TRN_Z; with

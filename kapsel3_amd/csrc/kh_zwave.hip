@@ -210,6 +210,74 @@ __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, con
     }
 }
 
+// p, q, r = a x (x, y, z) per point of four-point vectors
+template <typename V>
+__device__ __forceinline__ void cross4(const V &a, const V &b, const V &c, const V &x, const V &y, const V &z, V &p,
+                                       V &q, V &r) {
+    p = {b.x * z.x - c.x * y.x, b.y * z.y - c.y * y.y, b.z * z.z - c.z * y.z, b.w * z.w - c.w * y.w};
+    q = {c.x * x.x - a.x * z.x, c.y * x.y - a.y * z.y, c.z * x.z - a.z * z.z, c.w * x.w - a.w * z.w};
+    r = {a.x * y.x - b.x * x.x, a.y * y.y - b.y * x.y, a.z * y.z - b.z * x.z, a.w * y.w - b.w * x.w};
+}
+
+// The inverse z pass of omega's three components fused with u x omega
+// (contravariant u and omega, covariant product: operate_omega.cxx:113-137).
+// A line's omega rows stay in registers, u is read once per point, and the
+// product is written to io.out: omega itself never reaches HBM.
+template <typename T, int L, int E>
+__global__ __launch_bounds__(256) void k_zw_c2r_cross(ZIO<T> io, const T *__restrict__ u0, const T *__restrict__ u1,
+                                                      const T *__restrict__ u2, long long nrows, T scale, int kzv,
+                                                      int pitch) {
+    using Geo = WaveGeom<T, L, E>;
+    using F   = WaveFft<T, L, E, false, false>;
+    using RS  = RowStage<T, L>;
+    using V   = typename RS::vec;
+    static_assert(L >= 4 * Geo::G, "four points per lane per row");
+    constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
+    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    fill_twiddles<T, L, Geo::NT>(tw);
+    __syncthreads();
+    const WaveLane<T, L, E> wl(strips, nrows, pitch);
+    const long long         oa = (2 * wl.line) * L, ob = oa + L;
+    V                       wa[3][NT4], wb[3][NT4];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cpx<T> v[F::E];
+        load_pair<T, L, F>(io.in[c], wl, kzv, v);
+        F::run(v, wl.buf, tw, wl.g);
+#pragma unroll
+        for (int i = 0; i < F::E; i++) wl.buf[WavePad<T>::at(F::out_point(wl.g, i))] = v[i];
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < NT4; t++) {
+            const int    z = 4 * (wl.g + Geo::G * t);
+            const cpx<T> a = wl.buf[WavePad<T>::at(z)], b = wl.buf[WavePad<T>::at(z + 1)],
+                         d = wl.buf[WavePad<T>::at(z + 2)], e = wl.buf[WavePad<T>::at(z + 3)];
+            wa[c][t] = {a.x * scale, b.x * scale, d.x * scale, e.x * scale};
+            wb[c][t] = {a.y * scale, b.y * scale, d.y * scale, e.y * scale};
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int t = 0; t < NT4; t++) {
+        const int z = 4 * (wl.g + Geo::G * t);
+        V         p, q, r;
+        if (wl.okA) {
+            const long long o = oa + z;
+            cross4(RS::ld(u0 + o), RS::ld(u1 + o), RS::ld(u2 + o), wa[0][t], wa[1][t], wa[2][t], p, q, r);
+            RS::st(io.out[0] + o, p);
+            RS::st(io.out[1] + o, q);
+            RS::st(io.out[2] + o, r);
+        }
+        if (wl.okB) {
+            const long long o = ob + z;
+            cross4(RS::ld(u0 + o), RS::ld(u1 + o), RS::ld(u2 + o), wb[0][t], wb[1][t], wb[2][t], p, q, r);
+            RS::st(io.out[0] + o, p);
+            RS::st(io.out[1] + o, q);
+            RS::st(io.out[2] + o, r);
+        }
+    }
+}
+
 // Advection products (U2advection_k, operate_omega.cxx:31-47): inverse z
 // transforms of u1 (s0), u2 (s1), u3 (s2), the five products in registers,
 // forward z transforms into p0..p4. Two components are live at a time, so u2
@@ -411,6 +479,26 @@ hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, 
     }
 }
 
+bool zw_cross_supported(int L) { return zw_supported(L) && L >= 32; }
+
+template <typename T>
+hipError_t zw_c2r_cross(int L, const ZIO<T> &io, const T *const u[3], long long nrows, T scale, int kzv, int pitch,
+                        hipStream_t s) {
+    switch (L) {
+#define KH_CASE(n)                                                                                              \
+    case n:                                                                                                     \
+        if constexpr (n >= 32) {                                                                                \
+            hipLaunchKernelGGL((k_zw_c2r_cross<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, io, u[0], u[1], u[2], nrows, scale, kzv, pitch); \
+            return hipGetLastError();                                                                           \
+        }                                                                                                       \
+        break;
+        KH_FOR_EACH_WL(KH_CASE)
+#undef KH_CASE
+        default: break;
+    }
+    return hipErrorInvalidValue;
+}
+
 template <typename T>
 hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T scale, hipStream_t s) {
     const long long nrows = (long long)g.nx * g.ny;
@@ -445,6 +533,7 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
 #define KH_INST(T)                                                                                           \
     template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t);                    \
     template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t, const T *); \
+    template hipError_t zw_c2r_cross<T>(int, const ZIO<T> &, const T *const[3], long long, T, int, int, hipStream_t); \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
     template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,        \
                                     hipStream_t);
