@@ -50,7 +50,7 @@ hipError_t obl_add_f(const Grid &g, T *const up[3], const T *phisum, T *const c[
                      hipStream_t s);
 template <typename T>
 hipError_t obl_u_to_zeta(const Grid &g, const cpx<T> *u0, const cpx<T> *u1, const cpx<T> *u2, cpx<T> *z0, cpx<T> *z1,
-                         double *ukdc, double gd, hipStream_t s);
+                         double *ukdc, double gd, hipStream_t s, int kept = 0);
 
 // kh_particle.hip (PCache: per-particle cache of the Sekibun points with
 // nonzero phi, filled by phi_scatter)

@@ -120,6 +120,7 @@ struct kh_run {
     CTime        jikan;
     int          step_path  = KH_PATH_FUSED;
     int          zeta_dirty = 1;  // zeta may hold non-kept modes
+    int          zeta_lazy  = 0;  // zeta's non-kept modes are still to be formed from ureal (settle_zeta)
     // Lees-Edwards shear (kh_set_shear): DC rate or AC rate*cos(2 pi freq t);
     // deg = degree_oblique (strain of the oblique grid), sre = Shear_rate_eff
     int          shear = 0, shear_ac = 0;
@@ -617,9 +618,26 @@ struct Step {
         // f = u_p - phi ucp onto the oblique grid, u += f; project; zeta
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
         KH_LAUNCH("obl_add_f", (obl_add_f<T>(g, up3, PS(r), w3, u3, deg, s)));
-        for (int d = 0; d < 3; d++) KH_TRY(r2c(r, u3[d], s3[d]));
-        KH_LAUNCH("obl_u_to_zeta", (obl_u_to_zeta<T>(g, s3[0], s3[1], s3[2], Z(r, 0), Z(r, 1), r->ukdc, deg, s)));
+        // the kept modes only: the next sheared step reads no others, and
+        // settle() forms the rest from u (left in ureal) when anything else
+        // is about to read or change them
+        KH_TRY(r2c_kept3(r, u3, s3));
+        KH_LAUNCH("obl_u_to_zeta", (obl_u_to_zeta<T>(g, s3[0], s3[1], s3[2], Z(r, 0), Z(r, 1), r->ukdc, deg, s, 1)));
         r->zeta_dirty = 1;  // zeta keeps the modes of f beyond the 2/3 rule
+        r->zeta_lazy  = 1;
+        return KH_OK;
+    }
+
+    // The rest of the sheared step's U_k2u + Solenoidal_uk_OBL +
+    // U_k2zeta_k_OBL: every mode of zeta from the u the step left in ureal,
+    // at the step's strain (the kept modes come out as the step formed them).
+    static int settle(kh_run *r) {
+        const Grid &g     = r->g;
+        cpx<T>     *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
+        for (int d = 0; d < 3; d++) KH_TRY(r2c(r, U(r, d), s3[d]));
+        KH_LAUNCH("obl_u_to_zeta",
+                  (obl_u_to_zeta<T>(g, s3[0], s3[1], s3[2], Z(r, 0), Z(r, 1), r->ukdc, r->deg, r->stream)));
+        r->zeta_lazy = 0;
         return KH_OK;
     }
 
@@ -1214,8 +1232,17 @@ int kh_get_particles(kh_run *r, Particle *p, int n) {
     return KH_OK;
 }
 
+// Forms zeta's modes that a sheared step left to be formed (Step::settle):
+// every entry point that reads zeta, changes ureal or the strain, or runs
+// anything but the sheared step calls it first.
+static int settle_zeta(kh_run *r) {
+    if (!r->zeta_lazy) return KH_OK;
+    return r->mixed ? Step<float>::settle(r) : Step<double>::settle(r);
+}
+
 int kh_set_fluid(kh_run *r, const double *z0, const double *z1, const double uk_dc[3]) {
     if (!r || !z0 || !z1 || !uk_dc) return fail(KH_ERR_ARG, "NULL argument");
+    r->zeta_lazy = 0;  // every mode is overwritten
     KH_TRY(upload_spec(r, r->zeta[0], z0));
     KH_TRY(upload_spec(r, r->zeta[1], z1));
     r->zeta_dirty = 1;
@@ -1225,6 +1252,7 @@ int kh_set_fluid(kh_run *r, const double *z0, const double *z1, const double uk_
 
 int kh_get_fluid(kh_run *r, double *z0, double *z1, double uk_dc[3]) {
     if (!r || !z0 || !z1 || !uk_dc) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(settle_zeta(r));
     KH_TRY(download_spec(r, z0, r->zeta[0]));
     KH_TRY(download_spec(r, z1, r->zeta[1]));
     KH_HIP(hipMemcpy(uk_dc, r->ukdc, 3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1249,6 +1277,7 @@ int kh_get_time(kh_run *r, CTime *t) {
 
 int kh_get_field(kh_run *r, int field, double *out) {
     if (!r || !out) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(settle_zeta(r));
     if (field == KH_FIELD_PHI) {  // phi = min(phi_sum, 1) (make_phi.cxx:248-259)
         KH_TRY(download_real(r, out, r->phisum));
         const size_t n = (size_t)r->g.nx * r->g.ny * 2 * r->g.nzh;
@@ -1262,6 +1291,7 @@ int kh_get_field(kh_run *r, int field, double *out) {
 
 int kh_set_field(kh_run *r, int field, const double *in) {
     if (!r || !in) return fail(KH_ERR_ARG, "NULL argument");
+    KH_TRY(settle_zeta(r));
     r->pclean = false;
     void *b = field_buffer(r, field);
     if (!b) return fail(KH_ERR_ARG, "field %d cannot be written", field);
@@ -1270,6 +1300,7 @@ int kh_set_field(kh_run *r, int field, const double *in) {
 
 int kh_init_fluid(kh_run *r) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
+    r->zeta_lazy = 0;  // every mode is overwritten
     r->zeta_dirty = 1;
     return r->mixed ? Step<float>::init_fluid(r) : Step<double>::init_fluid(r);
 }
@@ -1300,6 +1331,7 @@ static int capture_steps(kh_run *r, int count, hipGraph_t *graph, hipGraphExec_t
 int kh_step(kh_run *r, int nsteps) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
+    if (!r->shear) KH_TRY(settle_zeta(r));
     for (int s = 0; s < nsteps;) {
         // (the slip loop decides its length on the host: not captured)
         const bool graph_ok = r->use_graph && !r->timing && r->step_path == KH_PATH_FUSED && r->jikan.ts > 0 &&
@@ -1330,6 +1362,7 @@ int kh_set_shear(kh_run *r, double shear_rate, int ac, double frequency) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (!std::isfinite(shear_rate) || !std::isfinite(frequency)) return fail(KH_ERR_ARG, "non-finite shear parameter");
     if (r->slip) return fail(KH_ERR_UNSUPPORTED, "the sheared step has no squirmer slip iteration");
+    KH_TRY(settle_zeta(r));  // at the strain the last step left
     drop_graph(r);
     if (!r->shear_last) {  // the stress record's device buffers
         const size_t np = (size_t)(r->np > 0 ? r->np : 1);
@@ -1416,6 +1449,7 @@ int kh_set_shear_state(kh_run *r, double degree_oblique, double shear_rate_eff) 
     if (!r->shear) return fail(KH_ERR_STATE, "kh_set_shear first");
     if (!std::isfinite(degree_oblique) || !std::isfinite(shear_rate_eff) || fabs(degree_oblique) >= r->xyaspect)
         return fail(KH_ERR_ARG, "strain must be finite and inside (-NX/NY, NX/NY)");
+    KH_TRY(settle_zeta(r));  // at the strain the last step left
     r->deg         = degree_oblique;
     r->sre         = shear_rate_eff;
     r->pp.obl_g    = r->deg;
@@ -1513,6 +1547,7 @@ int kh_set_graph(kh_run *r, int on) {
 
 int kh_apply(kh_run *r, int fn) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
+    KH_TRY(settle_zeta(r));
     return r->mixed ? Step<float>::apply(r, fn) : Step<double>::apply(r, fn);
 }
 
@@ -1547,6 +1582,7 @@ int kh_A_k2a(kh_run *r, double *a) {
 int kh_set_step_path(kh_run *r, int path) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (path != KH_PATH_FUSED && path != KH_PATH_REFERENCE_ORDER) return fail(KH_ERR_ARG, "unknown step path %d", path);
+    KH_TRY(settle_zeta(r));
     r->step_path = path;
     drop_graph(r);
     return KH_OK;

@@ -412,16 +412,23 @@ __global__ __launch_bounds__(SH_NT) void k_obl_add_f(Grid g, const T *__restrict
 
 // Solenoidal_uk_OBL (operate_omega.cxx:197-227) and U_k2zeta_k_OBL
 // (fft_wrapper.cxx:526-589) per mode; uk_dc from the projected u_hat(0).
+// With `kept` the launch covers the kept modes only (sh_kept_index).
 template <typename T>
 __global__ __launch_bounds__(SH_NT) void k_obl_u_to_zeta(Grid g, const cpx<T> *__restrict__ u0,
                                                          const cpx<T> *__restrict__ u1, const cpx<T> *__restrict__ u2,
                                                          cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
-                                                         double *__restrict__ ukdc, double gd) {
-    const unsigned idx = blockIdx.x * SH_NT + threadIdx.x;
-    if (idx >= (unsigned)g.nspec) return;
+                                                         double *__restrict__ ukdc, double gd, int kept,
+                                                         unsigned nthreads) {
+    const unsigned t = blockIdx.x * SH_NT + threadIdx.x;
+    if (t >= nthreads) return;
     int i, j, kz;
-    sh_spec_index(g, idx, i, j, kz);
-    if (kz >= g.nzh) return;
+    if (kept) {
+        sh_kept_index(g, t, i, j, kz);
+    } else {
+        sh_spec_index(g, t, i, j, kz);
+        if (kz >= g.nzh) return;
+    }
+    const unsigned idx = ((unsigned)i * g.ny + j) * g.nzp + kz;
     const int    ikx = kx_of(g, i), iky = ky_of(g, j);
     const double kx = ikx * g.wave_x, ky = iky * g.wave_y, kzz = kz * g.wave_z;
     const double kxc = (1. + gd * gd) * ikx * g.wave_x - gd * iky * g.wave_y;
@@ -515,9 +522,10 @@ hipError_t obl_add_f(const Grid &g, T *const up[3], const T *phisum, T *const c[
 }
 template <typename T>
 hipError_t obl_u_to_zeta(const Grid &g, const cpx<T> *u0, const cpx<T> *u1, const cpx<T> *u2, cpx<T> *z0, cpx<T> *z1,
-                         double *ukdc, double gd, hipStream_t s) {
-    hipLaunchKernelGGL(k_obl_u_to_zeta<T>, dim3(sh_blocks(g.nspec)), dim3(SH_NT), 0, s, g, u0, u1, u2, z0, z1, ukdc,
-                       gd);
+                         double *ukdc, double gd, hipStream_t s, int kept) {
+    const long long n = kept ? sh_nkept(g) : g.nspec;
+    hipLaunchKernelGGL(k_obl_u_to_zeta<T>, dim3(sh_blocks(n)), dim3(SH_NT), 0, s, g, u0, u1, u2, z0, z1, ukdc, gd,
+                       kept, (unsigned)n);
     return hipGetLastError();
 }
 
@@ -535,7 +543,7 @@ hipError_t obl_u_to_zeta(const Grid &g, const cpx<T> *u0, const cpx<T> *u1, cons
     template hipError_t obl_add_f<T>(const Grid &, T *const[3], const T *, T *const[3], T *const[3], double,         \
                                      hipStream_t);                                                                   \
     template hipError_t obl_u_to_zeta<T>(const Grid &, const cpx<T> *, const cpx<T> *, const cpx<T> *, cpx<T> *,     \
-                                         cpx<T> *, double *, double, hipStream_t);
+                                         cpx<T> *, double *, double, hipStream_t, int);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST
