@@ -9,7 +9,9 @@ bool fft_length_supported(int L);
 
 // Single passes (each one kernel launch).
 template <typename T>
-hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s);
+// kzvalid >= 0: only kz < kzvalid is written (the wavefront engine; other
+// lengths write every kz)
+hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s, int kzvalid = -1);
 // kzvalid >= 0: modes kz >= kzvalid are taken as zero (not read)
 template <typename T>
 hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s, int kzvalid = -1);

@@ -233,8 +233,8 @@ static hipError_t launch_strided(int L, cpx<T> *data, long long lstride, int nco
 }
 
 template <typename T>
-hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s) {
-    if (zw_supported(g.nz)) return zw_r2c<T>(g.nz, in, out, (long long)g.nx * g.ny, g.nzp, s);
+hipError_t fft_z_r2c(const Grid &g, const T *in, cpx<T> *out, hipStream_t s, int kzvalid) {
+    if (zw_supported(g.nz)) return zw_r2c<T>(g.nz, in, out, (long long)g.nx * g.ny, g.nzp, s, kzvalid);
     return launch_z_r2c<T>(g.nz, in, out, (long long)g.nx * g.ny, g.nzp, s);
 }
 template <typename T>
@@ -281,7 +281,7 @@ hipError_t fft3d_c2r(const Grid &g, cpx<T> *spec_inout, T *real_out, hipStream_t
 }
 
 #define KH_INST(T)                                                                   \
-    template hipError_t fft_z_r2c<T>(const Grid &, const T *, cpx<T> *, hipStream_t); \
+    template hipError_t fft_z_r2c<T>(const Grid &, const T *, cpx<T> *, hipStream_t, int); \
     template hipError_t fft_z_c2r<T>(const Grid &, const cpx<T> *, T *, hipStream_t, int); \
     template hipError_t fft_z_c2r_multi<T>(const Grid &, const ZIO<T> &, int, hipStream_t, int, const T *); \
     template hipError_t fft_y<T, true>(const Grid &, cpx<T> *, hipStream_t);          \

@@ -553,7 +553,7 @@ struct Step {
     // y and x passes write only the kept rows of the kept columns.
     static int r2c_kept3(kh_run *r, T *const in[3], cpx<T> *const sp[3]) {
         const Grid &g = r->g;
-        for (int d = 0; d < 3; d++) KH_LAUNCH("fft_z_r2c", (fft_z_r2c<T>(g, in[d], sp[d], r->stream)));
+        for (int d = 0; d < 3; d++) KH_LAUNCH("fft_z_r2c", (fft_z_r2c<T>(g, in[d], sp[d], r->stream, g.trn_z)));
         Comps<T> c{{sp[0], sp[1], sp[2], nullptr, nullptr}};
         KH_LAUNCH("y_fwd_kept", (cols_y<T, true>(g, c, 3, 0, 1, r->stream)));
         KH_LAUNCH("x_fwd_kept", (cols_x<T, true>(g, c, 3, 0, 1, r->stream)));

@@ -10,7 +10,7 @@ namespace kh {
 bool zw_supported(int L);
 // rows of L reals -> half spectra (all L/2+1 modes); spectral rows `pitch` complex apart
 template <typename T>
-hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s);
+hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s, int kzv = -1);
 // half spectra (modes kz < kzv, rest zero) -> rows of L reals * scale; grid y = component.
 // mask (a real field, L >= 32): only 4-point vectors where it is nonzero are written
 template <typename T>

@@ -92,7 +92,7 @@ __device__ __forceinline__ void store_pair(const cpx<T> *v, const WaveLane<T, L,
 // ------------------------------------------------------------- kernels
 template <typename T, int L, int E>
 __global__ __launch_bounds__(256) void k_zw_r2c(const T *__restrict__ in, cpx<T> *__restrict__ out, long long nrows,
-                                                int pitch) {
+                                                int pitch, int kzv) {
     using Geo = WaveGeom<T, L, E>;
     using F   = WaveFft<T, L, E, true, true>;
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_zw_r2c(const T *__restrict__ in, cpx<T>
         v[i]        = cmk(wl.okA ? ra[n] : (T)0, wl.okB ? rb[n] : (T)0);
     }
     F::run(v, wl.buf, tw, wl.g);
-    store_pair<T, L, F>(v, wl, L / 2 + 1, out);
+    store_pair<T, L, F>(v, wl, kzv, out);
 }
 
 // Real rows of a line <-> its LDS strip as (row 2p, row 2p+1) pairs, four
@@ -439,11 +439,12 @@ static inline unsigned zw_blocks(long long nrows) {
 }
 
 template <typename T>
-hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s) {
+hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s, int kzv) {
+    if (kzv < 0 || kzv > L / 2 + 1) kzv = L / 2 + 1;
     switch (L) {
 #define KH_CASE(n)                                                                                   \
     case n:                                                                                          \
-        hipLaunchKernelGGL((k_zw_r2c<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, in, out, nrows, pitch); \
+        hipLaunchKernelGGL((k_zw_r2c<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, in, out, nrows, pitch, kzv); \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)
 #undef KH_CASE
@@ -531,7 +532,7 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
 }
 
 #define KH_INST(T)                                                                                           \
-    template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t);                    \
+    template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t, int);               \
     template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t, const T *); \
     template hipError_t zw_c2r_cross<T>(int, const ZIO<T> &, const T *const[3], long long, T, int, int, hipStream_t); \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
