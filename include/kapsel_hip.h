@@ -228,7 +228,9 @@ long long kh_run_bytes(const kh_params *params, int nparticles);
 int kh_set_graph(kh_run *run, int on);
 /* One function of the step on the device state (enum kh_fn). */
 int kh_apply(kh_run *run, int fn);
-/* Wait for the run's stream; reports errors raised by enqueued work. */
+/* Wait for the run's stream; reports errors raised by enqueued work. After
+   sheared steps it first forms the vorticity modes beyond the 2/3 rule that
+   they leave to be formed, so the device state is complete when it returns. */
 int kh_sync(kh_run *run);
 
 /* -------- standalone transforms (A2a_k / A_k2a semantics) --------

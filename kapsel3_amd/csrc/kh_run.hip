@@ -1553,6 +1553,7 @@ int kh_apply(kh_run *r, int fn) {
 
 int kh_sync(kh_run *r) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
+    KH_TRY(settle_zeta(r));
     KH_HIP(hipStreamSynchronize(r->stream));
     return KH_OK;
 }
