@@ -1,15 +1,16 @@
-"""Benchmark of the SPM step: steps/s at 256^3 with 2000 colloids, mixed
-(FP32 fields, FP64 particles) and FP64, on one MI355X.
+"""Benchmark of the SPM step: steps/s at 256^3 with 2000 colloids, FP64 and
+mixed (FP32 fields, FP64 particles), on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints ONE JSON line. `value` is the mixed-precision steps/s of the whole
-step (inputs resident in HBM, K steps between two device synchronisations).
-The FP64 step is timed the same way and reported beside it. Per-kernel
-times come from HIP events recorded around every launch on the run's own
-stream during the timed steps; `roofline` describes the kernel with the
-largest share of the step. `cpu_baseline` times the oracle (the FP64 CPU
-restatement, OpenMP) on this host for a bounded sample of the same workload.
+Prints ONE JSON line. `value` is the FP64 steps/s of the whole step (the
+reference's own precision; inputs resident in HBM, K steps between two
+device synchronisations); the mixed-precision step is timed the same way and
+reported beside it under "mixed". Per-kernel times come from HIP events
+recorded around every launch on the run's own stream during a second set of
+K steps; `roofline` describes the FP64 kernel with the largest share of the
+step. `cpu_baseline` times the oracle (the FP64 CPU restatement, OpenMP) on
+this host for a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -37,11 +38,13 @@ def step_bytes(n, p, npts, sf, sp):
     return sf * field_words(n) * 113 + sp * p * npts * 13
 
 
-def kernel_bytes(name, n, p, npts, sf):
+def kernel_bytes(name, n, p, npts, sf, nphi=None):
     """Algorithmic HBM bytes of one launch: every distinct input read once
     and every output written once, over the modes and points the kernel's
     function needs (2/3-kept modes |kx|,|ky| < T, kz < T with T = (N+2)/3;
-    Sekibun points of every particle). sf = bytes of one field scalar."""
+    Sekibun points of every particle). sf = bytes of one field scalar; nphi
+    = grid points where phi_sum > 0 (the only points where u* is read and
+    f = u_p - phi u* is nonzero; all points when not given)."""
     t = (n + 2) // 3
     kx, ky, kz = 2 * t - 1, 2 * t - 1, t
     cs = 2 * sf                      # one complex
@@ -50,6 +53,7 @@ def kernel_bytes(name, n, p, npts, sf):
     ycol = n * n * kz                # y/z lines of the kept kz planes
     real = n ** 3
     pt = p * npts
+    nphi = real if nphi is None else min(nphi, real)
     table = {
         # fused pipeline
         "x_zeta2u_inv": cs * (2 * kept + 3 * xcol),
@@ -59,10 +63,11 @@ def kernel_bytes(name, n, p, npts, sf):
         "x_fwd_adv": cs * (5 * xcol + 5 * kept),
         "adv_ns": cs * (5 + 2 + 2) * kept,
         "x_adv_ns": cs * (5 * xcol + 2 * kept + 2 * kept),
-        "z_c2r": cs * 3 * ycol + sf * 3 * real,
-        # phi_sum everywhere; u_p and u* only where some particle's phi > 0
-        # (at most the P NP Sekibun points): elsewhere f = 0
-        "z_force": sf * real + 6 * sf * min(real, pt) + cs * 3 * ycol,
+        # u* is read only where phi_sum > 0 (F_H gather, rigidity force):
+        # three spectra in, phi_sum everywhere, u* written at those points
+        "z_c2r": cs * 3 * ycol + sf * real + sf * 3 * nphi,
+        # phi_sum everywhere; u_p and u* only where phi_sum > 0: elsewhere f = 0
+        "z_force": sf * real + 6 * sf * nphi + cs * 3 * ycol,
         "y_fwd_force": cs * (3 * ycol + 3 * xcol),
         "x_f_zeta": cs * (3 * xcol + 2 * kept + 2 * kept),
         "fill": sf * real,
@@ -76,6 +81,26 @@ def kernel_bytes(name, n, p, npts, sf):
         "particle_step": 11 * sf * pt,
     }
     return table.get(name)
+
+
+def phi_points(x, n, a, hxi):
+    """Grid points (unit spacing, periodic n^3) where some sphere's profile
+    is nonzero: minimum-image distance below a + xi/2 (phi(r) > 0 exactly
+    there, profile.h:30-50)."""
+    rc = a + hxi
+    h = int(np.ceil(rc)) + 1
+    o = np.arange(-h, h + 1)
+    off = np.stack(np.meshgrid(o, o, o, indexing="ij"), -1).reshape(-1, 3)
+    base = np.floor(x).astype(np.int64)
+    hit = np.zeros(n ** 3, dtype=bool)
+    for i in range(0, len(x), 256):
+        g = base[i:i + 256, None, :] + off[None, :, :]            # candidate points
+        d = g - x[i:i + 256, None, :]
+        d -= np.rint(d / n) * n
+        inside = (d * d).sum(-1) < rc * rc
+        gi = np.mod(g[inside], n)
+        hit[(gi[:, 0] * n + gi[:, 1]) * n + gi[:, 2]] = True
+    return int(hit.sum())
 
 
 def time_gpu(precision, n, nparticles, steps, warmup, graph=True, seed=3):
@@ -169,7 +194,11 @@ def squirmer_bench(steps, n=256, count=1000):
 
 def shear_bench(steps, n=256, nparticles=2000, rate=1e-3):
     """The Lees-Edwards sheared step (kh_set_shear) on the config-3
-    suspension: steps/s in mixed precision and FP64 after 2 warm-up steps."""
+    suspension, mixed and FP64, after 2 warm-up steps. The sheared step
+    forms only zeta's 2/3-kept modes; the rest are formed once when the
+    state is read (settle: one full forward transform). `settle_ms` is that
+    completion timed on its own; `value` counts the steps alone (no state
+    read), `read_every_step` a caller that reads zeta after every step."""
     import kapsel3_amd
     from kapsel3_amd import workloads
     from kapsel3_amd._abi import FP64, MIXED
@@ -183,13 +212,24 @@ def shear_bench(steps, n=256, nparticles=2000, rate=1e-3):
         run.set_shear(rate)
         run.step(2)
         run.sync()
+        settle = []
+        for _ in range(3):  # the completion alone: queue drained first, then kh_sync settles
+            run.step(1)
+            while not run.idle():
+                time.sleep(1e-4)
+            t0 = time.perf_counter()
+            run.sync()
+            settle.append(time.perf_counter() - t0)
+        settle_s = float(np.median(settle))
         t0 = time.perf_counter()
         run.step(steps)
         run.sync()
-        t = time.perf_counter() - t0
+        t = time.perf_counter() - t0 - settle_s  # the timed chunk ends with one completion
         p = run.get_particles()
         run.close()
         out[name] = {"value": steps / t, "unit": "steps/s", "ms_per_step": t / steps * 1e3,
+                     "settle_ms": settle_s * 1e3,
+                     "read_every_step": {"value": 1.0 / (t / steps + settle_s), "unit": "steps/s"},
                      "finite": bool(np.isfinite(p["v"]).all())}
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out
@@ -255,24 +295,27 @@ def serve_bench(count, rate, nsteps, max_active, seed=2024):
                         "gravity 1e-3, mixed precision, " + ("Poisson arrivals" if rate else "all submitted at once")}
 
 
-def roofline_of(stats, n, nparticles, npts, sf, pass_ms_per_step, steps, traffic=None):
+def roofline_of(stats, n, nparticles, npts, sf, pass_ms_per_step, steps, traffic=None, nphi=None,
+                traffic_source=None):
     """Roofline entry of the kernel with the largest total time."""
     best = max(stats.items(), key=lambda kv: kv[1][1])
     name, (launches, ms) = best
     per_launch_ms = ms / launches
-    b = kernel_bytes(name, n, nparticles, npts, sf)
+    b = kernel_bytes(name, n, nparticles, npts, sf, nphi)
     achieved = (b / (per_launch_ms * 1e-3) / 1e9) if b else None
     share = ms / (pass_ms_per_step * steps)
+    t = (traffic or {}).get(name)
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": (traffic or {}).get(name),
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": t,
+            "traffic_source": traffic_source if t is not None else None,
             "launches": launches, "avg_us": per_launch_ms * 1e3, "bytes_per_launch": b,
             "share_of_step": share}
 
 
-def kernel_table(stats, n, nparticles, npts, sf):
+def kernel_table(stats, n, nparticles, npts, sf, nphi=None):
     out = {}
     for k, (c, ms) in sorted(stats.items()):
-        b = kernel_bytes(k, n, nparticles, npts, sf)
+        b = kernel_bytes(k, n, nparticles, npts, sf, nphi)
         us = ms / c * 1e3
         out[k] = {"launches": c, "avg_us": us, "bytes_per_launch": b,
                   "gbs": (b / (us * 1e-6) / 1e9) if b else None}
@@ -288,6 +331,30 @@ def load_traffic(path):
         return json.load(f)
 
 
+def copy_ceiling(mib=256, reps=10):
+    """A plain device-to-device copy of `mib` MiB (torch), timed with HIP
+    events: the bandwidth a pure streaming kernel reaches on this part (read
+    plus write bytes / time). Working set 2 x mib MiB, beyond the
+    last-level cache."""
+    import torch
+    src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"bytes": 2 * mib * (1 << 20), "avg_us": ms * 1e3, "GBps": 2 * mib * (1 << 20) / (ms * 1e-3) / 1e9,
+            "frac_of_peak": 2 * mib * (1 << 20) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,11 +362,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--particles", type=int, default=2000)
-    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--mixed-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true", help="time the mixed step only")
-    ap.add_argument("--fp64-only", action="store_true", help="time the FP64 step only (value = FP64 steps/s)")
+    ap.add_argument("--no-mixed", action="store_true", help="time the FP64 step only")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time the mixed step only (for profiling it; value is then the mixed steps/s)")
     ap.add_argument("--no-graph", action="store_true", help="run the timed steps uncaptured")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-squirm", action="store_true", help="skip the squirmer suspension")
@@ -319,8 +387,7 @@ def main():
     ap.add_argument("--saturate-active", type=int, default=32)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_256_2000.json"),
                     help="measured HBM bytes per launch (JSON by precision and kernel)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 256 MiB device copy (torch) so profiler byte counters can be scaled")
+    ap.add_argument("--no-copy-ceiling", action="store_true", help="skip the 256 MiB device copy measurement")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: one simulation runs on one GPU; --gpus {a.gpus} is not supported "
@@ -328,54 +395,59 @@ def main():
         sys.exit(2)
     from kapsel3_amd._abi import FP64, MIXED
     n, P = a.grid, a.particles
-    if a.copy_calibration:
-        import torch
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    ceiling = None if a.no_copy_ceiling else copy_ceiling()
     npts = 1935  # Sekibun points of a = 5, xi = 2 (profile.cxx:22-65)
     if a.shear_only:
         print(json.dumps(shear_bench(a.shear_steps, n, P)))
         return
 
     traffic = load_traffic(a.traffic) if (n, P) == (256, 2000) else None
+    tsrc = ("stored: profiles/traffic_256_2000.json, rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this "
+            "kernel scaled by a plain copy's reading (not measured in this run)")
 
     def measure(prec, name, sf, k):
         t, st, ok, tk, state = time_gpu(prec, n, P, k, a.warmup, graph=not a.no_graph)
         ms, kms = t / k * 1e3, tk / k * 1e3
-        table = kernel_table(st, n, P, npts, sf)
-        sb = step_bytes(n, P, npts, sf, sf)
+        nphi = phi_points(state[1]["x"], n, 5.0, 1.0) if P else 0
+        table = kernel_table(st, n, P, npts, sf, nphi)
         pruned = sum(v["bytes_per_launch"] * v["launches"] / k for v in table.values() if v["bytes_per_launch"])
         return {"value": k / t, "unit": "steps/s", "ms_per_step": ms, "steps": k, "finite": ok,
-                "step_roofline": {"reference_step_bytes": sb, "achieved": sb / (ms * 1e-3) / 1e9,
-                                  "frac": sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                  "pruned_step_bytes": pruned, "pruned_frac": pruned / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                  "peak": HBM_PEAK_GBS, "unit": "GB/s"},
-                "roofline": roofline_of(st, n, P, npts, sf, kms, k, (traffic or {}).get(name)),
+                "step_roofline": {
+                    "pipeline_step_bytes": pruned,
+                    "achieved": pruned / (ms * 1e-3) / 1e9, "frac": pruned / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac_of_copy_ceiling": (pruned / (ms * 1e-3) / 1e9 / ceiling["GBps"]) if ceiling else None,
+                    "phi_points": nphi,
+                    "reference_step_bytes_label": step_bytes(n, P, npts, sf, sf),
+                    "note": "pipeline_step_bytes: the algorithmic bytes of every launch of this pruned, fused step "
+                            "(kernels table); reference_step_bytes_label: the reference-order step's bytes (113 "
+                            "field passes + 13 accesses per Sekibun point), a label only, not a rate"},
+                "roofline": roofline_of(st, n, P, npts, sf, kms, k, (traffic or {}).get(name), nphi, tsrc),
                 "kernel_pass_ms_per_step": kms, "kernels": table, "_state": state}
 
-    head = measure(FP64, "fp64", 8, a.steps) if a.fp64_only else measure(MIXED, "mixed", 4, a.steps)
-    prec = "fp64" if a.fp64_only else "mixed"
+    if a.mixed_only:
+        head, prec, dt = measure(MIXED, "mixed", 4, a.steps), "mixed", "f32 fields / f64 particles"
+    else:
+        head, prec, dt = measure(FP64, "fp64", 8, a.steps), "fp64", "f64"
     out = {"metric": METRIC, "value": head["value"], "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64" if a.fp64_only else "f32 fields / f64 particles", "data": "synthetic",
+           "vs_baseline": None, "dtype": dt, "data": "synthetic",
            "config": {"workload": f"SPM step, {n}^3 grid, {P} spheres a=5 xi=2, LJ 12:6 + gravity 1e-3 + kT 1e-3, "
                                   f"dt 0.05, {prec} precision", "grid": n, "particles": P, "precision": prec},
            "roofline": head["roofline"], "cpu_baseline": None,
            "grid_point_steps_per_s": n ** 3 * head["value"], "finite": head["finite"], "graph": not a.no_graph,
-           "step_roofline": head["step_roofline"], "kernel_pass_ms_per_step": head["kernel_pass_ms_per_step"],
-           "kernels": head["kernels"]}
+           "step_roofline": head["step_roofline"], "copy_ceiling": ceiling,
+           "kernel_pass_ms_per_step": head["kernel_pass_ms_per_step"], "kernels": head["kernels"]}
     head_state = head.pop("_state")
-    if not a.no_fp64 and not a.fp64_only:
-        f64 = measure(FP64, "fp64", 8, a.fp64_steps or a.steps)
-        f64_state = f64.pop("_state")
-        out["fp64"] = f64
-        out["speedup_mixed_vs_fp64"] = out["value"] / f64["value"]
-        if (a.fp64_steps or a.steps) == a.steps:  # same trajectory length: mixed against FP64
-            out["accuracy_mixed_vs_fp64"] = accuracy(head_state, f64_state, a.warmup + 2 * a.steps, 5.0)
+    if not a.no_mixed and not a.mixed_only:
+        km = a.mixed_steps or a.steps
+        mx = measure(MIXED, "mixed", 4, km)
+        mx_state = mx.pop("_state")
+        mx["grid_point_steps_per_s"] = n ** 3 * mx["value"]
+        out["mixed"] = mx
+        out["speedup_mixed_vs_fp64"] = mx["value"] / out["value"]
+        if km == a.steps:  # same trajectory length: mixed against FP64
+            out["accuracy_mixed_vs_fp64"] = accuracy(mx_state, head_state, a.warmup + 2 * a.steps, 5.0)
     if not a.no_squirm:
         out["squirmers"] = squirmer_bench(a.squirm_steps)
     if not a.no_shear:

@@ -64,6 +64,11 @@ struct kr_run {
 
 #define IM(r, i, j, k) ((((size_t)(i) * (r)->NY + (j)) * (r)->NZ_) + (k))
 
+/* Planted mistake (kr_set_mutation, kref.h): 0 in every use but the tests
+ * that show a pin catches one. */
+static int g_mut = KR_MUT_NONE;
+void       kr_set_mutation(int m) { g_mut = (m > KR_MUT_NONE && m < KR_MUT_COUNT) ? m : KR_MUT_NONE; }
+
 /* ------------------------------------------------------------------ */
 /* small helpers: macro.h:44-52, interaction.h:113-132, lad3.h          */
 /* ------------------------------------------------------------------ */
@@ -217,6 +222,8 @@ static void angular2v(const kr_run *r, const double *omega, const double *rv, do
         v[0] = omega[1] * rv[2] - omega[2] * rv[1];
         v[1] = omega[2] * rv[0] - omega[0] * rv[2];
         v[2] = omega[0] * rv[1] - omega[1] * rv[0];
+        if (g_mut == KR_MUT_OMEGA_R_SIGN)
+            for (int d = 0; d < 3; d++) v[d] = -v[d];
     } else {
         v[0] = v[1] = v[2] = 0.0;
     }
@@ -317,6 +324,8 @@ static void u_k2zeta_k(kr_run *r, double **u, double **zeta, double *uk_dc) {
     uk_dc[0] = u[0][0];
     uk_dc[1] = u[1][0];
     uk_dc[2] = u[2][0];
+    if (g_mut == KR_MUT_UKDC_MEAN)
+        for (int d = 0; d < 3; d++) uk_dc[d] /= (double)r->NX * r->NY * r->NZ;
 #pragma omp parallel for
     for (int i = 0; i < r->NX; i++)
         for (int j = 0; j < r->NY; j++)
@@ -356,7 +365,7 @@ static void u2advection_k(kr_run *r, double **u, double **adv) {
             for (int k = 0; k < r->NZ; k++) {
                 size_t im = IM(r, i, j, k);
                 double u1 = u[0][im], u2 = u[1][im], u3 = u[2][im];
-                u[0][im]   = u2 * u3;
+                u[0][im]   = g_mut == KR_MUT_ADV_SIGN ? -(u2 * u3) : u2 * u3;
                 u[1][im]   = u3 * u1;
                 u[2][im]   = u1 * u2;
                 adv[0][im] = (u2 - u3) * (u2 + u3);
@@ -524,7 +533,7 @@ static void make_u_particle_sum(kr_run *r, double **up, const double *phi_sum) {
             for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * r->DX;
             size_t im      = IM(r, r_mesh[0], r_mesh[1], r_mesh[2]);
             double dmy     = distance(r, x, xp);
-            double dmy_phi = Phi(r, dmy) / fmax(phi_sum[im], 1.0);
+            double dmy_phi = Phi(r, dmy) / (g_mut == KR_MUT_UP_NO_MAX ? 1.0 : fmax(phi_sum[im], 1.0));
             angular2v(r, om, rv, v_rot);
             up[0][im] += ((vp[0] + v_rot[0]) * dmy_phi);
             up[1][im] += ((vp[1] + v_rot[1]) * dmy_phi);
@@ -535,7 +544,7 @@ static void make_u_particle_sum(kr_run *r, double **up, const double *phi_sum) {
 
 /* Calc_f_hydro_correct_precision (md_force.cxx:383-506), non-rigid branch */
 static void calc_f_hydro(kr_run *r, const double *phi_sum, double **u) {
-    const double dmy = (-r->DX3 * r->RHO) / r->jikan.dt_fluid;
+    const double dmy = (g_mut == KR_MUT_FH_SIGN ? r->DX3 * r->RHO : -r->DX3 * r->RHO) / r->jikan.dt_fluid;
 #pragma omp parallel for
     for (int n = 0; n < r->np; n++) {
         Particle *pn = &r->p[n];
@@ -556,7 +565,7 @@ static void calc_f_hydro(kr_run *r, const double *phi_sum, double **u) {
             double dmyR = distance(r, x, xp);
             angular2v(r, om, rv, v_rot);
             size_t im      = IM(r, r_mesh[0], r_mesh[1], r_mesh[2]);
-            double dmy_phi = Phi(r, dmyR) / fmax(phi_sum[im], 1.0);
+            double dmy_phi = Phi(r, dmyR) / (g_mut == KR_MUT_FH_NO_MAX ? 1.0 : fmax(phi_sum[im], 1.0));
             for (int d = 0; d < 3; d++) {
                 fp[d] = ((vp[d] + v_rot[d]) - u[d][im]) * dmy_phi;
                 force[d] += fp[d];
@@ -771,9 +780,10 @@ static void velocity_update(kr_run *r, int ab2) {
                     p->v[d] += (dmy * (p->f_hydro[d] + p->fr[d] + sf[d]));
                     p->omega[d] += (dmyr * (p->torque_hydro[d] + p->torque_r[d] + st[d]));
                 } else {
-                    p->v[d] += dmy * (2.0 * (p->f_hydro[d] + sf[d]) + p->fr[d] + p->fr_previous[d]);
+                    const double w = g_mut == KR_MUT_AB2_FH_WEIGHT ? 1.0 : 2.0;
+                    p->v[d] += dmy * (w * (p->f_hydro[d] + sf[d]) + p->fr[d] + p->fr_previous[d]);
                     p->omega[d] +=
-                        dmyr * (2.0 * (p->torque_hydro[d] + st[d]) + p->torque_r[d] + p->torque_r_previous[d]);
+                        dmyr * (w * (p->torque_hydro[d] + st[d]) + p->torque_r[d] + p->torque_r_previous[d]);
                 }
             }
         } else {
@@ -957,8 +967,9 @@ static void make_particle_momentum_factor(kr_run *r, double **u) {
                 for (int l = 0; l < 3; l++) M2[d][l] -= dmy_phi * rv[d] * rv[l];
             }
             if (r->spec[ps].propulsion == KH_JP_SLIP && less_than_mp(dmy_xi, r->HXI)) {
-                double slip_vel = r->spec[ps].slip_vel, slip_mode = r->slip_mode_half[ps];
-                double C1 = r->spec[ps].rotlet_c1, C2 = r->spec[ps].rotlet_c2;
+                double slip_vel = r->spec[ps].slip_vel,
+                       slip_mode = r->slip_mode_half[ps] * (g_mut == KR_MUT_SLIP_B2_FULL ? 2.0 : 1.0);
+                double C1 = r->spec[ps].rotlet_c1 * (g_mut == KR_MUT_SLIP_ROTLET ? 2.0 : 1.0), C2 = r->spec[ps].rotlet_c2;
                 double dmy_phi_s = (1.0 - dmy_phi) * DPhi_compact_sin_norm(r, dmy_r);
                 SM0 += dmy_phi_s;
                 double n_r[3], n_th[3], n_ta[3], sr, th, ta;
@@ -1055,8 +1066,9 @@ static void make_force_u_slip_particle(kr_run *r, double **up, double **u) {
             p->torque_slip[d] = 0.0;
         }
         if (r->spec[ps].propulsion != KH_JP_SLIP) continue;
-        double slip_vel = r->spec[ps].slip_vel, slip_mode = r->slip_mode_half[ps];
-        double C1 = r->spec[ps].rotlet_c1, C2 = r->spec[ps].rotlet_c2;
+        double slip_vel = r->spec[ps].slip_vel,
+                       slip_mode = r->slip_mode_half[ps] * (g_mut == KR_MUT_SLIP_B2_FULL ? 2.0 : 1.0);
+        double C1 = r->spec[ps].rotlet_c1 * (g_mut == KR_MUT_SLIP_ROTLET ? 2.0 : 1.0), C2 = r->spec[ps].rotlet_c2;
         double vp[3], om[3], dv[3], dw[3], xp[3];
         slip_droplet(vp, om, dv, dw, p);
         for (int d = 0; d < 3; d++) xp[d] = p->x[d];
@@ -1097,7 +1109,7 @@ static void make_force_u_slip_particle(kr_run *r, double **up, double **u) {
 
 /* Calc_f_slip_correct_precision (md_force.cxx:325-382) */
 static void calc_f_slip(kr_run *r, double **u) {
-    const double dmy = (r->DX3 * r->RHO) / r->jikan.dt_fluid;
+    const double dmy = (g_mut == KR_MUT_SLIP_FORCE_SIGN ? -1.0 : 1.0) * (r->DX3 * r->RHO) / r->jikan.dt_fluid;
 #pragma omp parallel for
     for (int n = 0; n < r->np; n++) {
         Particle *p = &r->p[n];
@@ -1194,8 +1206,9 @@ static void update_slip_velocity(kr_run *r, int iter) {
                 p->v_slip[d]     = p->v[d];
                 p->omega_slip[d] = p->omega[d];
             } else {
-                p->v_slip[d]     = 0.3 * p->v_slip[d] + 0.7 * p->v[d];
-                p->omega_slip[d] = 0.3 * p->omega_slip[d] + 0.7 * p->omega[d];
+                const double w   = g_mut == KR_MUT_SLIP_RELAX ? 0.5 : 0.7;
+                p->v_slip[d]     = (1.0 - w) * p->v_slip[d] + w * p->v[d];
+                p->omega_slip[d] = (1.0 - w) * p->omega_slip[d] + w * p->omega[d];
             }
         }
     }
@@ -1220,14 +1233,14 @@ static void slip_iteration(kr_run *r) {
             reset_forces(r); /* end_iter */
         } else {
             update_slip_velocity(r, iter);
-            for (int n = 0; n < r->np; n++) /* reset_iter */
+            for (int n = 0; n < r->np && g_mut != KR_MUT_SLIP_NO_RESET; n++) /* reset_iter */
                 for (int d = 0; d < 3; d++) {
                     r->p[n].v[d]     = r->p[n].v_old[d];
                     r->p[n].omega[d] = r->p[n].omega_old[d];
                 }
         }
     }
-    for (int d = 0; d < 3; d++) { /* Swap_mem(u, up) */
+    for (int d = 0; d < 3 && g_mut != KR_MUT_SLIP_FINAL_USTAR; d++) { /* Swap_mem(u, up) */
         double *t = r->u[d];
         r->u[d]   = r->up[d];
         r->up[d]  = t;
@@ -2066,21 +2079,29 @@ int kr_le_apply(kr_run *r, int fn) {
 /* ------------------------------------------------------------------ */
 /* Time_evolution_hydro (sp_3d_ns.cxx:41-194), Navier_Stokes branch */
 static void time_evolution_hydro(kr_run *r) {
+    const int ab2 = r->jikan.ts != 0 || g_mut == KR_MUT_NO_EULER_START;
+    if (g_mut == KR_MUT_FH_PRE_NS_U) zeta_k2u(r, r->zeta, r->uk_dc, r->f); /* u before the NS update */
     ns_solver_slaved_euler(r);
-    if (r->fix_cell_any)
+    if (r->fix_cell_any && g_mut != KR_MUT_NO_FIX_CELL)
         for (int d = 0; d < 3; d++)
             if (r->P.fix_cell[d]) r->uk_dc[d] = 0.0;
     truncate_two_third(r, r->zeta[0]);
     truncate_two_third(r, r->zeta[1]);
     zeta_k2u(r, r->zeta, r->uk_dc, r->u);
-    position_update(r, r->jikan.ts != 0);
+    if (g_mut == KR_MUT_FH_OLD_X) { /* F_H at the positions before the move */
+        reset_field(r, r->phi);
+        reset_field(r, r->phi_sum);
+        make_phi_particle_sum(r, r->phi, r->phi_sum);
+        calc_f_hydro(r, r->phi_sum, r->u);
+    }
+    position_update(r, ab2);
     reset_field(r, r->phi);
     reset_field(r, r->phi_sum);
     make_phi_particle_sum(r, r->phi, r->phi_sum);
     for (int d = 0; d < 3; d++) reset_field(r, r->up[d]);
-    calc_f_hydro(r, r->phi_sum, r->u);
+    if (g_mut != KR_MUT_FH_OLD_X) calc_f_hydro(r, r->phi_sum, g_mut == KR_MUT_FH_PRE_NS_U ? r->f : r->u);
     if (!r->sw_janus_slip)
-        velocity_update(r, r->jikan.ts != 0);
+        velocity_update(r, ab2);
     else
         slip_iteration(r);
     if (r->kBT > 0.) thermostat(r);

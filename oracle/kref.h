@@ -96,6 +96,33 @@ int    kr_sekibun_cell(const kr_run *r, int *out3);
 void kr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void kr_noise_normals(uint64_t seed, uint32_t stream, int particle, int ts, double out6[6]);
 
+/* Planted mistakes, for the tests that show a pin catches one (CPU only).
+ * kr_set_mutation(m) makes the oracle commit mistake m in its step from the
+ * next call on (process-wide); KR_MUT_NONE restores it. Nothing but those
+ * tests sets it. */
+enum {
+    KR_MUT_NONE = 0,
+    KR_MUT_FH_PRE_NS_U,      /* F_H from the velocity before the NS update, not u* */
+    KR_MUT_OMEGA_R_SIGN,     /* rigid velocity v - omega x r (Angular2v sign)     */
+    KR_MUT_FH_NO_MAX,        /* F_H weight phi instead of phi / max(phi_sum, 1)   */
+    KR_MUT_UP_NO_MAX,        /* u_p weight phi instead of phi / max(phi_sum, 1)   */
+    KR_MUT_NO_FIX_CELL,      /* FIX_CELL zeroing of uk_dc skipped                 */
+    KR_MUT_ADV_SIGN,         /* one advection product (u2 u3) with its sign flipped */
+    KR_MUT_AB2_FH_WEIGHT,    /* AB2 velocity update with 1 F_H instead of 2 F_H   */
+    KR_MUT_NO_EULER_START,   /* AB2 on the first step too                         */
+    KR_MUT_FH_OLD_X,         /* F_H at the positions before the move              */
+    KR_MUT_FH_SIGN,          /* F_H prefactor +rho dx^3/dt                         */
+    KR_MUT_UKDC_MEAN,        /* uk_dc stored as the mean, not the sum, of u       */
+    KR_MUT_SLIP_RELAX,       /* slip relaxation 0.5 instead of 0.7                */
+    KR_MUT_SLIP_B2_FULL,     /* B2/B1 not halved                                  */
+    KR_MUT_SLIP_FORCE_SIGN,  /* slip force with its sign flipped                  */
+    KR_MUT_SLIP_FINAL_USTAR, /* after the slip loop u stays u* (no Swap_mem)      */
+    KR_MUT_SLIP_ROTLET,      /* rotlet C1 doubled                                 */
+    KR_MUT_SLIP_NO_RESET,    /* no velocity reset between slip iterations         */
+    KR_MUT_COUNT
+};
+void kr_set_mutation(int m);
+
 int kr_threads(void);
 void kr_set_threads(int n);
 

@@ -49,6 +49,8 @@ def lib():
         L.kr_philox4x32_10.argtypes = [vp, vp, vp]
         L.kr_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, vp]
         L.kr_set_threads.argtypes = [C.c_int]
+        L.kr_set_mutation.argtypes = [C.c_int]
+        L.kr_set_mutation.restype = None
         L.kr_set_shear.argtypes = [vp, C.c_double, C.c_int, C.c_double]
         L.kr_get_shear_state.argtypes = [vp, vp, vp]
         L.kr_get_shear_state.restype = None
@@ -71,6 +73,27 @@ LE_FN = {name: i for i, name in enumerate([
 LE_RECORD = ["time", "shear_rate", "degree_oblique", "strain", "lj_dev_stress", "shear_stress_old",
              "shear_stress_new", "reynolds_stress", "fluid_stress", "interfacial_stress", "apparent_stress",
              "viscosity"]
+
+
+# planted mistakes (kr_set_mutation, kref.h), in enum order
+MUTATIONS = ["none", "fh_pre_ns_u", "omega_r_sign", "fh_no_max", "up_no_max", "no_fix_cell", "adv_sign",
+             "ab2_fh_weight", "no_euler_start", "fh_old_x", "fh_sign", "ukdc_mean", "slip_relax", "slip_b2_full",
+             "slip_force_sign", "slip_final_ustar", "slip_rotlet", "slip_no_reset"]
+
+
+class mutation:
+    """Context manager: the oracle commits the named planted mistake inside
+    the block (tests of the pins only)."""
+
+    def __init__(self, name):
+        self.m = MUTATIONS.index(name)
+
+    def __enter__(self):
+        lib().kr_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().kr_set_mutation(0)
 
 
 def fft_forward(a):

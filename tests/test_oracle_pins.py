@@ -180,6 +180,52 @@ def test_slaved_euler_taylor_green_decays_exactly():
     assert np.abs(z1a[m] / z1b[m] - math.exp(-eta * k2 * dt)).max() < 1e-13
 
 
+def test_ns_solver_random_flow_against_spectral_brute_force():
+    """NS_solver_slavedEuler (fluid_solver.cxx:348-374) with the advection
+    term of U2advection_k (operate_omega.cxx:12-85) on a random solenoidal
+    flow with 2/3-truncated modes: vorticity w = ik x u evolves as
+    w' = exp(-nu k^2 dt) w + (1 - exp(-nu k^2 dt)) Omega / (nu k^2) with
+    Omega = -FFT(curl((u.grad) u)), (u.grad) u_i = d_j(u_i u_j) from the
+    products in real space (alias-free on the kept modes). Compared through
+    u' = ik x w' / k^2 plus the unchanged mean."""
+    n, eta, dt = 16, 0.8, 0.05
+    r = _run_fields(n, eta=eta, dt=dt)
+    kx, ky, kz = kvec(n)
+    K = np.stack(np.broadcast_arrays(kx, ky, kz))
+    trn = (n + 2) // 3
+    ki = [np.rint(k / (2 * np.pi / n)).astype(int) for k in (kx, ky, kz)]
+    kept = (np.abs(ki[0]) < trn) & (np.abs(ki[1]) < trn) & (ki[2] < trn)
+    rng = np.random.default_rng(11)
+    k2 = (K ** 2).sum(0)
+    k2s = np.where(k2 == 0, 1.0, k2)
+    F = (rng.standard_normal((3,) + kept.shape) + 1j * rng.standard_normal((3,) + kept.shape)) * kept
+    F = F - K * (K * F).sum(0) / k2s                                   # solenoidal
+    u = np.stack([np.fft.irfftn(F[d], s=(n, n, n), axes=(0, 1, 2)) for d in range(3)])
+    u = 0.3 * u / np.abs(u).max() + np.array([0.02, -0.01, 0.03])[:, None, None, None]
+    _set_velocity(r, n, list(u))
+    r.apply("ns_solver")
+    r.apply("zeta_k2u")
+    got = np.stack([r.get_field(f"u{d}")[:, :, :n] for d in range(3)])
+    U = np.stack([np.fft.rfftn(u[d]) for d in range(3)])
+    P = {(i, j): np.fft.rfftn(u[i] * u[j]) for i in range(3) for j in range(3)}
+    adv = np.stack([sum(1j * K[j] * P[(i, j)] for j in range(3)) for i in range(3)])
+    omega = -1j * np.cross(K, adv, axis=0)                             # -FFT(curl adv)
+    w = 1j * np.cross(K, U, axis=0)
+    e = np.exp(-eta * k2 * dt)
+    w2 = (e * w + (1 - e) * omega / (eta * k2s)) * kept
+    U2 = 1j * np.cross(K, w2, axis=0) / k2s
+    U2[:, 0, 0, 0] = U[:, 0, 0, 0]
+    want = np.stack([np.fft.irfftn(U2[d], s=(n, n, n), axes=(0, 1, 2)) for d in range(3)])
+    change = np.abs(want - u).max()
+    assert change > 1e-3                                                # the step does something
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    # the nonlinear term matters here: without it the answer is off
+    UL = 1j * np.cross(K, e * w * kept, axis=0) / k2s
+    UL[:, 0, 0, 0] = U[:, 0, 0, 0]
+    lin = np.stack([np.fft.irfftn(UL[d], s=(n, n, n), axes=(0, 1, 2)) for d in range(3)])
+    assert np.abs(lin - want).max() > 1e-4 * np.abs(want).max()
+
+
 def test_projection_solenoidal_and_idempotent():
     """Solenoidal_u (operate_omega.h:105-112): k.f = 0, only kept modes,
     P(P f) = P f."""
