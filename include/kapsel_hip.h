@@ -197,6 +197,35 @@ int kh_set_shear_state(kh_run *run, double degree_oblique, double shear_rate_eff
 int kh_get_shear_record(kh_run *run, double out[KH_SHEAR_RECORD]);
 int kh_set_shear_history(kh_run *run, int nsteps);
 int kh_get_shear_history(kh_run *run, double *out, int nsteps);
+/* Rigid composite bodies (SW_PT == rigid; rigid.h, make_phi.cxx:589-801).
+ * Body k is made of the particles (beads) cum[k] .. cum[k+1]-1
+ * (Rigid_Particle_Cumul), its species that of its first bead; the beads'
+ * x_nopbc must hold coordinates that place each body in one piece (the
+ * reference reads them unwrapped, init_particle.cxx:455-487), x the same
+ * wrapped into the box. free_flags[k]: 1 "free", 0 "fix" (Rigid_motion;
+ * NULL: all free). v0, w0: initial body velocities, nb x 3, space frame
+ * (NULL: at rest, as the reference starts them). Call after
+ * kh_set_particles and before kh_init_fluid: it runs the reference's
+ * initial block -- centres and masses from the phi-weighted bead volume,
+ * inertia tensors, principal frames (the bead orientations q), bead
+ * velocities -- and from then on kh_step moves bodies: per-body sums of the
+ * beads' hydrodynamic force and torque about the centre, LJ between beads of
+ * different bodies only (with lj_cell_compat the reference's pair
+ * bookkeeping, md_force.cxx:134-153), gravity on the body volume, Euler's
+ * equations in the body frame (calc_Rigid_VOGs, rigid.h:467-559),
+ * positions and orientations (solver_Rigid_Position, rigid.h:333-392).
+ * Not for kT > 0, slip species, sheared runs or ROTATION off
+ * (KH_ERR_UNSUPPORTED). */
+#define KH_RIGID_WORDS 32
+int kh_set_rigid(kh_run *run, int nbodies, const int *cum, const int *free_flags, const double *v0,
+                 const double *w0);
+/* Per body, KH_RIGID_WORDS doubles: xG[3], xG_nopbc[3], vG[3], omegaG[3],
+ * the previous step's hydrodynamic force[3] and torque[3] and LJ + gravity
+ * force[3] and torque[3] (the RigidParticles members of Output_udf,
+ * output.cxx:324-373), mass, principal moments[3], orientation q[4] (q0
+ * scalar). */
+int kh_get_rigid(kh_run *run, double *out, int nbodies);
+
 /* Per-step particle record kept on the device: from the next step on, the
  * (x[3], v[3]) of every particle after each of the next nsteps steps
  * (nsteps x np x 6 doubles; 0 stops recording). No counterpart in the
@@ -239,6 +268,7 @@ int kh_sync(kh_run *run);
  * 1/(NX*NY*NZ); the kz=0 and kz=NZ/2 planes are taken as Hermitian. */
 int kh_A2a_k(kh_run *run, double *a);
 int kh_A_k2a(kh_run *run, double *a);
+
 
 /* -------- measurement -------- */
 /* When enabled, every kernel launch of the run is bracketed by HIP events
@@ -283,6 +313,14 @@ typedef struct {
 int kh_server_create(const kh_server_config *cfg, kh_server **out);
 int kh_server_submit(kh_server *srv, const kh_params *params, const Particle *particles, int nparticles,
                      int nsteps, double *traj_out, long long *id);
+/* As kh_server_submit, and the run's Output_udf frames (output.cxx:268-375,
+ * as kh_set_output records them) at ts = 0, frame_every, 2 frame_every, ...
+ * up to nsteps are copied to frames_out at completion:
+ * (nsteps / frame_every + 1) frames of 1 + KH_FRAME_WORDS * np doubles.
+ * The reference writes a frame every GTS steps of its loop
+ * (sp_3d_ns.cxx:863-950); frame_every is that GTS. */
+int kh_server_submit_output(kh_server *srv, const kh_params *params, const Particle *particles, int nparticles,
+                            int nsteps, double *traj_out, int frame_every, double *frames_out, long long *id);
 /* 1 done, 0 pending, < 0 the run's failure status (kh_last_error has it) */
 int kh_server_poll(kh_server *srv, long long id);
 /* as poll, blocking up to timeout_s seconds (0 on timeout) */

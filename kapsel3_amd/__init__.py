@@ -12,8 +12,8 @@ import os
 import numpy as np
 
 from ._abi import (CTime, FIELD, FN, FP64, JAX_NONE, JAX_X, JAX_Y, JAX_Z, JP_MOTOR, JP_NONE,  # noqa: F401
-                   JP_OBSTACLE, JP_SLIP, MIXED, PARTICLE_DTYPE, KhParams, KhSpecies, Params, make_species,
-                   new_particles)
+                   JP_OBSTACLE, JP_SLIP, MIXED, PARTICLE_DTYPE, RIGID_DTYPE, KhParams, KhSpecies, Params,
+                   make_species, new_particles)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkapsel_hip.so")
@@ -27,6 +27,7 @@ EXPORTS = [
     "kh_set_output", "kh_get_output", "kh_run_bytes", "kh_query", "kh_server_create", "kh_server_submit", "kh_server_poll", "kh_server_wait",
     "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy", "kh_set_shear", "kh_get_shear_state",
     "kh_set_shear_state", "kh_get_shear_record", "kh_set_shear_history", "kh_get_shear_history",
+    "kh_server_submit_output", "kh_set_rigid", "kh_get_rigid",
 ]
 
 # fields of the sheared step's stress record (Mean_shear_stress, sp_3d_ns.h:247-330)
@@ -88,6 +89,8 @@ def lib():
         L.kh_set_graph.argtypes = [vp, C.c_int]
         L.kh_A2a_k.argtypes = [vp, vp]
         L.kh_A_k2a.argtypes = [vp, vp]
+        L.kh_set_rigid.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.kh_get_rigid.argtypes = [vp, vp, C.c_int]
         L.kh_timing_enable.argtypes = [vp, C.c_int]
         L.kh_timing_reset.argtypes = [vp]
         L.kh_timing_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
@@ -106,6 +109,8 @@ def lib():
         L.kh_query.argtypes = [vp]
         L.kh_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
         L.kh_server_submit.argtypes = [vp, C.POINTER(KhParams), vp, C.c_int, C.c_int, vp, C.POINTER(C.c_longlong)]
+        L.kh_server_submit_output.argtypes = [vp, C.POINTER(KhParams), vp, C.c_int, C.c_int, vp, C.c_int, vp,
+                                              C.POINTER(C.c_longlong)]
         L.kh_server_poll.argtypes = [vp, C.c_longlong]
         L.kh_server_wait.argtypes = [vp, C.c_longlong, C.c_double]
         L.kh_server_run_times.argtypes = [vp, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -321,6 +326,24 @@ class Run:
     def sync(self):
         _check(lib().kh_sync(self.h))
 
+    def set_rigid(self, cum, free=None, v0=None, w0=None):
+        """Rigid composite bodies (kh_set_rigid): body k is made of the
+        particles cum[k] .. cum[k+1]-1 (their x_nopbc unwrapped); free[k] 1
+        "free" / 0 "fix"; v0, w0 (nb, 3) initial body velocities."""
+        cum = np.ascontiguousarray(cum, dtype=np.int32)
+        fr = None if free is None else np.ascontiguousarray(free, dtype=np.int32)
+        v = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64)
+        w = None if w0 is None else np.ascontiguousarray(w0, dtype=np.float64)
+        _check(lib().kh_set_rigid(self.h, len(cum) - 1, cum.ctypes.data, None if fr is None else fr.ctypes.data,
+                                  None if v is None else v.ctypes.data, None if w is None else w.ctypes.data))
+        self.nb = len(cum) - 1
+
+    def get_rigid(self):
+        """The bodies' state (RIGID_DTYPE records)."""
+        out = np.zeros(self.nb, dtype=RIGID_DTYPE)
+        _check(lib().kh_get_rigid(self.h, out.ctypes.data, self.nb))
+        return out
+
     def A2a_k(self, a):
         a = np.ascontiguousarray(a, dtype=np.float64).copy()
         _check(lib().kh_A2a_k(self.h, a.ctypes.data))
@@ -370,14 +393,36 @@ class Server:
         self.h = h
         self._keep = {}
 
-    def submit(self, P, particles, nsteps):
+    def submit(self, P, particles, nsteps, frame_every=0):
+        """Queue a run; returns (id, trajectory array). With frame_every > 0
+        the run's Output_udf frames every frame_every steps (ts = 0 to
+        nsteps) are kept too: frames(id) returns them once it completes."""
         parts = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
         traj = np.zeros((nsteps, len(parts), 6))
         rid = C.c_longlong()
-        _check(lib().kh_server_submit(self.h, C.byref(P.c), parts.ctypes.data, len(parts), nsteps,
-                                      traj.ctypes.data, C.byref(rid)))
-        self._keep[rid.value] = (P, parts, traj)  # alive until the run completes
+        if frame_every > 0:
+            raw = np.zeros((nsteps // frame_every + 1, 1 + 34 * len(parts)))
+            _check(lib().kh_server_submit_output(self.h, C.byref(P.c), parts.ctypes.data, len(parts), nsteps,
+                                                 traj.ctypes.data, int(frame_every), raw.ctypes.data,
+                                                 C.byref(rid)))
+        else:
+            raw = None
+            _check(lib().kh_server_submit(self.h, C.byref(P.c), parts.ctypes.data, len(parts), nsteps,
+                                          traj.ctypes.data, C.byref(rid)))
+        self._keep[rid.value] = (P, parts, traj, raw)  # alive until the run completes
         return rid.value, traj
+
+    def frames(self, rid):
+        """(t, particles) of a completed run submitted with frame_every:
+        t the step counts, particles (nframes, np) FRAME_PARTICLE_DTYPE."""
+        raw = self._keep[rid][3]
+        if raw is None:
+            raise KhError(f"run {rid} was submitted without frames")
+        if not self.poll(rid):
+            raise KhError(f"run {rid} has not completed")
+        np_ = len(self._keep[rid][1])
+        t = raw[:, 0].astype(np.int64)
+        return t, np.ascontiguousarray(raw[:, 1:]).view(FRAME_PARTICLE_DTYPE).reshape(len(t), np_)
 
     def poll(self, rid):
         return _check(lib().kh_server_poll(self.h, rid)) == 1

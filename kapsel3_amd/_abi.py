@@ -37,6 +37,12 @@ assert PARTICLE_DTYPE.fields["x"][1] == 8 and PARTICLE_DTYPE.fields["v"][1] == 8
 assert PARTICLE_DTYPE.fields["f_hydro"][1] == 152 and PARTICLE_DTYPE.fields["omega"][1] == 368
 assert PARTICLE_DTYPE.fields["q"][1] == 728
 
+# one rigid body (kh_get_rigid, KH_RIGID_WORDS doubles)
+RIGID_DTYPE = np.dtype([("xG", "f8", 3), ("xG_nopbc", "f8", 3), ("vG", "f8", 3), ("omegaG", "f8", 3),
+                        ("f_hydro", "f8", 3), ("torque_hydro", "f8", 3), ("f_r", "f8", 3), ("torque_r", "f8", 3),
+                        ("mass", "f8"), ("Ibody", "f8", 3), ("q", "f8", 4)])
+assert RIGID_DTYPE.itemsize == 32 * 8
+
 # enum JP / JAX (input.h:48-49)
 JP_MOTOR, JP_SLIP, JP_OBSTACLE, JP_NONE = 0, 1, 2, 3
 JAX_X, JAX_Y, JAX_Z, JAX_NONE = 0, 1, 2, 3

@@ -23,7 +23,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unknown-pragmas",
 ]
 
-SOURCES = ["kh_fft.hip", "kh_zwave.hip", "kh_fused.hip", "kh_spectral.hip", "kh_shear.hip", "kh_particle.hip", "kh_run.hip", "kh_server.hip"]
+SOURCES = ["kh_fft.hip", "kh_zwave.hip", "kh_fused.hip", "kh_spectral.hip", "kh_shear.hip", "kh_particle.hip", "kh_run.hip",
+           "kh_server.hip"]
 
 
 def _compile(src):

@@ -66,6 +66,13 @@ hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, 
                       T *const up[3], hipStream_t s);
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s);
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s);
+// rigid bodies: solver_Rigid_Position + update_Orientation +
+// update_Particle_Configuration, and calc_Rigid_VOGs + set_Particle_Velocities
+// + reset_Forces (one workgroup per body)
+hipError_t rigid_position(const PParams &pp, DevParticle *P, const RigidArgs &ra, int ab2, double dt, double hdt,
+                          hipStream_t s);
+hipError_t rigid_velocity(const PParams &pp, DevParticle *P, const RigidArgs &ra, int ab2, double dt, double hdt,
+                          hipStream_t s);
 // noise = 1: the thermostat's kick follows the update in the same launch
 hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s, int noise = 0,
                     uint64_t seed = 0, uint32_t stream = 0, const int *d_ts = nullptr, double niv = 0.0,

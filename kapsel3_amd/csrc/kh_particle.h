@@ -58,6 +58,26 @@ struct PParams {
     double            obl_g, obl_jump;
     double            rho;     // RHO (the sheared step's stress sums)
     double           *pstress; // sheared runs: np x 6 per-particle stress partials
+    // rigid composite bodies: body of each bead (nullptr: spherical
+    // particles), and per bead the LJ force from partners of higher index
+    const int        *bid;
+    double           *flo;
+};
+
+// One rigid body (rigid.h's per-body state, input.cxx:1655-1695).
+struct DevBody {
+    double xG[3], xGn[3], xGp[3];            // xGs, xGs_nopbc, xGs_previous
+    double vG[3], vGo[3], wG[3], wGo[3];     // velocityGs(_old), omegaGs(_old)
+    double fGp[3], tGp[3], fGrp[3], tGrp[3]; // the previous step's forceGs, torqueGs, forceGrs, torqueGrs
+    double M, Ib[3];                         // Rigid_Masses, principal moments (Rigid_Moments_body diagonal)
+    int    first, last, free_, spec;         // beads [first, last), Rigid_motion, RigidID_Components
+};
+struct RigidArgs {
+    DevBody      *B;
+    int           nb;
+    double       *GR;   // np x 3 GRvecs (space frame, from the centre to the bead)
+    const double *GRb;  // np x 3 GRvecs_body
+    int           compat;  // the reference's LJ torque bookkeeping (md_force.cxx:148-153)
 };
 
 // Host scalars of one sheared step's stress record (k_obl_record).

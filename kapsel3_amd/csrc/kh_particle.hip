@@ -461,12 +461,19 @@ __device__ __forceinline__ int lj_mult(const PParams &pp, const double *xi, cons
 // (md_force.cxx:306-324). One workgroup per particle i: its lanes split the
 // partners j, each pair giving F_i += f(r_ij) (x_i - x_j), the same value the
 // reference's +/- updates add; the lane sums are reduced in FP64.
-template <int NT = PT_NT, bool OBL = false>
+// RIGID: beads of rigid bodies -- pairs within one body are skipped
+// (rigid_chain, interaction.h:18-24), gravity acts on the bodies, and the
+// forces from partners of higher index are also kept apart (pp.flo) for
+// the reference's body torque bookkeeping.
+template <int NT = PT_NT, bool OBL = false, bool RIGID = false>
 __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__restrict__ P, int i) {
     const double xi[3]  = {P[i].x[0], P[i].x[1], P[i].x[2]};
     const int    si     = P[i].spec;
     const bool   obst_i = pp.species[si].propulsion == 2;
-    double       f[4]   = {0.0, 0.0, 0.0, 0.0};  // F_i; [3]: sheared runs, the pairs' shear stress
+    // F_i; [3]: sheared runs, the pairs' shear stress; rigid beads: [3..5]
+    // the part from partners of higher index
+    double f[RIGID ? 6 : 4] = {};
+    const int bi = RIGID ? pp.bid[i] : -1;
     if (pp.lj_on) {
         // pairs clearly beyond the cutoff are dropped by a division-free
         // minimum image (exact for positions inside the box, up to the
@@ -476,6 +483,7 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
         for (int j = threadIdx.x; j < pp.np; j += NT) {
             if (j == i) continue;
             if (obst_i && pp.xs[4 * j + 3] != 0.0) continue;
+            if (RIGID && pp.bid[j] == bi) continue;
             const double xj[3] = {pp.xs[4 * j], pp.xs[4 * j + 1], pp.xs[4 * j + 2]};
             double       d2    = 0.0;
             bool         odd   = false;  // a position outside the box
@@ -505,7 +513,10 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
                 const double dmy_r = fmin(1.7976931348623157e308 / rij, lj_f(pp, rij));
                 const double sgn   = (i > j) ? -1.0 : 1.0;  // i>j: f_i += dmy_r*(-r); else f_i -= dmy_r*(-r)
                 for (int c = 0; c < mult; c++)
-                    for (int d = 0; d < 3; d++) f[d] += sgn * (dmy_r * rv[d]);
+                    for (int d = 0; d < 3; d++) {
+                        f[d] += sgn * (dmy_r * rv[d]);
+                        if (RIGID && j > i) f[3 + d] += sgn * (dmy_r * rv[d]);
+                    }
                 // md_force.cxx:246: F_x on the lower index times r_y, each pair
                 // once (r = x_hi - x_lo = -rv here)
                 if constexpr (OBL)
@@ -513,16 +524,20 @@ __device__ __forceinline__ void forces_block(const PParams &pp, DevParticle *__r
             }
         }
     }
-    block_sum<OBL ? 4 : 3, NT>(f);
+    block_sum<RIGID ? 6 : (OBL ? 4 : 3), NT>(f);
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; d++) P[i].fr[d] += f[d];
-        if (pp.g != 0.0) P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
+        if constexpr (RIGID) {
+            for (int d = 0; d < 3; d++) pp.flo[3 * (long long)i + d] = f[3 + d];
+        } else if (pp.g != 0.0) {
+            P[i].fr[pp.g_dir] -= pp.gravity_on_fluid * (pp.species[si].mass_ratio - 1.0);
+        }
         if constexpr (OBL) pp.pstress[6 * (long long)i + 5] = f[3];
     }
 }
-template <bool OBL>
+template <bool OBL, bool RIGID = false>
 __global__ __launch_bounds__(PT_NT) void k_forces(PParams pp, DevParticle *__restrict__ P) {
-    forces_block<PT_NT, OBL>(pp, P, blockIdx.x);
+    forces_block<PT_NT, OBL, RIGID>(pp, P, blockIdx.x);
 }
 
 // The record Mean_shear_stress prints after a sheared step (sp_3d_ns.h:
@@ -1176,7 +1191,9 @@ hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, doubl
     return hipGetLastError();
 }
 hipError_t forces(const PParams &pp, DevParticle *P, hipStream_t s) {
-    if (pp.np > 0 && pp.obl)
+    if (pp.np > 0 && pp.bid)
+        hipLaunchKernelGGL((k_forces<false, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
+    else if (pp.np > 0 && pp.obl)
         hipLaunchKernelGGL(k_forces<true>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
     else if (pp.np > 0)
         hipLaunchKernelGGL(k_forces<false>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P);
@@ -1192,6 +1209,189 @@ hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t
                       double nio, hipStream_t s) {
     const Noise nz{1, seed, stream, d_ts, niv, nio};
     if (pp.np > 0) hipLaunchKernelGGL(k_thermostat, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, nz);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- rigid bodies
+__device__ __forceinline__ void cross3(double *c, const double *a, const double *b) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// solver_Rigid_Position (rigid.h:333-392) with update_Orientation (271-290)
+// and update_Particle_Configuration (159-178): lane 0 moves the body's
+// centre and turns its first bead's quaternion (MD_solver_orientation_*,
+// from that bead's omega / omega_old); then the lanes place the beads.
+__global__ __launch_bounds__(64) void k_rigid_position(PParams pp, DevParticle *__restrict__ P, RigidArgs ra, int ab2,
+                                                       double dt, double hdt) {
+    DevBody &B = ra.B[blockIdx.x];
+    __shared__ double sq[4], sqo[4], sx[3], sv[3], sw[3];
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; d++) {
+            B.xGp[d]             = B.xG[d];
+            const double delta_x = ab2 ? hdt * (3.0 * B.vG[d] - B.vGo[d]) : dt * B.vG[d];
+            B.xG[d] += delta_x;
+            B.xGn[d] += delta_x;
+            B.xG[d] = fmod(B.xG[d] + pp.L[d], pp.L[d]);
+        }
+        DevParticle &f = P[B.first];
+        double       dq[4];
+        if (!ab2) {
+            for (int k = 0; k < 4; k++) f.q_old[k] = f.q[k];
+            qdot_space(dq, f.q, f.om);
+            for (int k = 0; k < 4; k++) f.q[k] += dq[k] * dt;
+        } else {
+            double dqo[4];
+            qdot_space(dq, f.q, f.om);
+            qdot_space(dqo, f.q_old, f.om_old);
+            for (int k = 0; k < 4; k++) f.q_old[k] = f.q[k];
+            for (int k = 0; k < 4; k++) f.q[k] += dq[k] * (3.0 * hdt);
+            for (int k = 0; k < 4; k++) f.q[k] += dqo[k] * (-hdt);
+        }
+        qtn_normalize(f.q);
+        for (int k = 0; k < 4; k++) {
+            sq[k]  = f.q[k];
+            sqo[k] = f.q_old[k];
+        }
+        for (int d = 0; d < 3; d++) {
+            sx[d] = B.xG[d];
+            sv[d] = B.vG[d];
+            sw[d] = B.wG[d];
+        }
+    }
+    __syncthreads();
+    for (int n = B.first + threadIdx.x; n < B.last; n += 64) {
+        DevParticle &p = P[n];
+        if (n != B.first)
+            for (int k = 0; k < 4; k++) {
+                p.q_old[k] = sqo[k];
+                p.q[k]     = sq[k];
+            }
+        double gr[3], wxr[3];
+        body2space(gr, sq, ra.GRb + 3 * (long long)n);
+        cross3(wxr, sw, gr);
+        for (int d = 0; d < 3; d++) {
+            ra.GR[3 * (long long)n + d] = gr[d];
+            p.x_prev[d]                 = p.x[d];
+            p.x[d]                      = fmod(sx[d] + gr[d] + pp.L[d], pp.L[d]);
+            p.om_old[d]                 = p.om[d];
+            p.v_old[d]                  = p.v[d];
+            p.om[d]                     = sw[d];
+            p.v[d]                      = sv[d] + wxr[d];
+            pp.xs[4 * n + d]            = p.x[d];
+        }
+    }
+}
+
+// calc_Rigid_VOGs (rigid.h:467-559) after Force: lane 0 sums its beads'
+// hydrodynamic force and torque about the centre (forceGs, torqueGs: GR x
+// f_hydro + torque_hydro per bead, md_force.cxx:467-505) and LJ force and
+// torque (forceGrs, torqueGrs, md_force.cxx:134-153), adds gravity on the
+// body's volume (md_force.cxx:316-323), then integrates the centre velocity
+// and Euler's equations in the body frame (particle_rotation_solver.h:
+// 106-184); the lanes then give the beads the body's motion
+// (set_Particle_Velocities, rigid.h:218-229) and roll their forces
+// (reset_Forces, particle_solver.cxx:11-37).
+__global__ __launch_bounds__(64) void k_rigid_velocity(PParams pp, DevParticle *__restrict__ P, RigidArgs ra,
+                                                       int ab2, double dt, double hdt) {
+    DevBody &B = ra.B[blockIdx.x];
+    __shared__ double sv[3], sw[3];
+    if (threadIdx.x == 0) {
+        double fG[3] = {0, 0, 0}, tG[3] = {0, 0, 0}, fGr[3] = {0, 0, 0}, tGr[3] = {0, 0, 0};
+        for (int n = B.first; n < B.last; n++) {
+            const DevParticle &p  = P[n];
+            const double      *gr = ra.GR + 3 * (long long)n;
+            double             gxf[3], gxr[3], fl[3];
+            cross3(gxf, gr, p.fh);
+            for (int d = 0; d < 3; d++) {
+                fG[d] += p.fh[d];
+                tG[d] += gxf[d] + p.th[d];
+                fGr[d] += p.fr[d];
+                // the reference adds GR x f_i to the body of the partner of lower
+                // index too, whose bead takes -f_i: per bead, the forces from
+                // partners of higher index enter with their sign flipped
+                fl[d] = ra.compat ? p.fr[d] - 2.0 * pp.flo[3 * (long long)n + d] : p.fr[d];
+            }
+            cross3(gxr, gr, fl);
+            for (int d = 0; d < 3; d++) tGr[d] += gxr[d];
+        }
+        const DevSpecies &sp = pp.species[B.spec];
+        if (pp.g != 0.0) fGr[pp.g_dir] -= pp.g * pp.rho * (B.M / (sp.mass_ratio * pp.rho)) * (sp.mass_ratio - 1.0);
+        for (int d = 0; d < 3; d++) {
+            B.vGo[d] = B.vG[d];
+            B.wGo[d] = B.wG[d];
+        }
+        const double *q = P[B.first].q, *qo = P[B.first].q_old;
+        double        wb[3], tb[3], tbo[3], tq[3], nw[3];
+        if (!ab2) {
+            if (B.free_)
+                for (int d = 0; d < 3; d++) B.vG[d] += (dt / B.M) * (fG[d] + fGr[d]);
+            for (int d = 0; d < 3; d++) tq[d] = tG[d] + tGr[d];
+            space2body(wb, q, B.wG);
+            space2body(tb, q, tq);
+            for (int d = 0; d < 3; d++) nw[d] = wb[d];
+            if (B.free_) {
+                nw[0] += (dt / B.Ib[0]) * (tb[0] + (B.Ib[1] - B.Ib[2]) * wb[1] * wb[2]);
+                nw[1] += (dt / B.Ib[1]) * (tb[1] + (B.Ib[2] - B.Ib[0]) * wb[2] * wb[0]);
+                nw[2] += (dt / B.Ib[2]) * (tb[2] + (B.Ib[0] - B.Ib[1]) * wb[0] * wb[1]);
+            }
+        } else {
+            if (B.free_)
+                for (int d = 0; d < 3; d++) B.vG[d] += (hdt / B.M) * (2.0 * fG[d] + fGr[d] + B.fGrp[d]);
+            for (int d = 0; d < 3; d++) tq[d] = 2.0 * tG[d] + tGr[d];
+            space2body(wb, q, B.wG);
+            space2body(tb, q, tq);
+            space2body(tbo, qo, B.tGrp);
+            for (int d = 0; d < 3; d++) nw[d] = wb[d];
+            if (B.free_) {
+                nw[0] += (hdt / B.Ib[0]) * ((tb[0] + tbo[0]) + 2.0 * (B.Ib[1] - B.Ib[2]) * wb[1] * wb[2]);
+                nw[1] += (hdt / B.Ib[1]) * ((tb[1] + tbo[1]) + 2.0 * (B.Ib[2] - B.Ib[0]) * wb[2] * wb[0]);
+                nw[2] += (hdt / B.Ib[2]) * ((tb[2] + tbo[2]) + 2.0 * (B.Ib[0] - B.Ib[1]) * wb[0] * wb[1]);
+            }
+        }
+        body2space(B.wG, q, nw);
+        for (int d = 0; d < 3; d++) {
+            B.fGp[d]  = fG[d];
+            B.tGp[d]  = tG[d];
+            B.fGrp[d] = fGr[d];
+            B.tGrp[d] = tGr[d];
+            sv[d]     = B.vG[d];
+            sw[d]     = B.wG[d];
+        }
+    }
+    __syncthreads();
+    for (int n = B.first + threadIdx.x; n < B.last; n += 64) {
+        DevParticle &p = P[n];
+        double       wxr[3];
+        cross3(wxr, sw, ra.GR + 3 * (long long)n);
+        for (int d = 0; d < 3; d++) {
+            p.om[d]      = sw[d];
+            p.v[d]       = sv[d] + wxr[d];
+            p.fr_prev[d] = p.fr[d];
+            p.fr[d]      = 0.0;
+            p.fh_prev[d] = p.fh[d];
+            p.fh[d]      = 0.0;
+            p.fs_prev[d] = p.fs[d];
+            p.fs[d]      = 0.0;
+            p.tr_prev[d] = p.tr[d];
+            p.tr[d]      = 0.0;
+            p.th_prev[d] = p.th[d];
+            p.th[d]      = 0.0;
+            p.ts_prev[d] = p.ts[d];
+            p.ts[d]      = 0.0;
+        }
+    }
+}
+
+hipError_t rigid_position(const PParams &pp, DevParticle *P, const RigidArgs &ra, int ab2, double dt, double hdt,
+                          hipStream_t s) {
+    if (ra.nb > 0) hipLaunchKernelGGL(k_rigid_position, dim3(ra.nb), dim3(64), 0, s, pp, P, ra, ab2, dt, hdt);
+    return hipGetLastError();
+}
+hipError_t rigid_velocity(const PParams &pp, DevParticle *P, const RigidArgs &ra, int ab2, double dt, double hdt,
+                          hipStream_t s) {
+    if (ra.nb > 0) hipLaunchKernelGGL(k_rigid_velocity, dim3(ra.nb), dim3(64), 0, s, pp, P, ra, ab2, dt, hdt);
     return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -132,6 +133,12 @@ struct kh_run {
     double      *shear_last    = nullptr;      // the last step's stress record (KH_SHEAR_RECORD doubles)
     double      *shear_hist    = nullptr;      // records of shear_hist_n steps from ts = shear_hist_base
     int          shear_hist_n = 0, shear_hist_base = 0;
+    // rigid composite bodies (kh_set_rigid)
+    int              rigid = 0;
+    RigidArgs        ra{};
+    int             *bid   = nullptr;   // np: body of each bead
+    double          *flo   = nullptr;   // np x 3: LJ force from partners of higher index
+    std::vector<int> sek_host;          // the packed Sekibun offsets (host copy, for the initial block)
     // measurement
     bool                               timing = false;
     std::map<std::string, KernelTimes> times;
@@ -255,7 +262,11 @@ struct Step {
 
     static PCache<T> CP(kh_run *r) { return PCache<T>{(T *)r->cphi, r->cidx, r->ccode, r->ccnt}; }
     static int pos_update(kh_run *r, int ab2) {
-        KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
+        if (r->rigid)
+            KH_LAUNCH("rigid_position", rigid_position(r->pp, r->dp, r->ra, ab2, r->jikan.dt_md, r->jikan.hdt_md,
+                                                       r->stream));
+        else
+            KH_LAUNCH("position", position(r->pp, r->dp, ab2, r->jikan.dt_md, r->jikan.hdt_md, r->stream));
         r->pcache_valid = 0;
         return KH_OK;
     }
@@ -301,6 +312,11 @@ struct Step {
     // kick follows in the same launch
     static int velocity_update(kh_run *r, int ab2, bool noise = false) {
         KH_LAUNCH("forces", forces(r->pp, r->dp, r->stream));
+        if (r->rigid) {  // calc_Rigid_VOGs + set_Particle_Velocities (particle_solver.cxx:137-140, 274-277)
+            KH_LAUNCH("rigid_velocity", rigid_velocity(r->pp, r->dp, r->ra, ab2, r->jikan.dt_md, r->jikan.hdt_md,
+                                                       r->stream));
+            return KH_OK;
+        }
         double niv = 0.0, nio = 0.0;
         if (noise) noise_amp(r, niv, nio);
         KH_LAUNCH("velocity", velocity(r->pp, r->dp, ab2, ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->stream,
@@ -418,6 +434,10 @@ struct Step {
             KH_TRY(zero(r, PS(r)));
         }
         r->pclean = false;
+        if (r->rigid) {  // the bodies move, then their beads scatter phi
+            KH_TRY(pos_update(r, ab2));
+            return scatter_phi(r);
+        }
         return move_scatter_phi(r, ab2);
     }
 
@@ -466,6 +486,10 @@ struct Step {
             KH_TRY(slip_iteration(r, ab2, true));
             for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
             if (r->kbt > 0.) KH_TRY(thermo(r));
+            KH_TRY(scatter_up(r));
+        } else if (r->rigid) {  // bead F_H, LJ, the bodies' update, u_p from the beads' rigid motion
+            KH_TRY(calc_f_hydro(r));
+            KH_TRY(velocity_update(r, ab2));
             KH_TRY(scatter_up(r));
         } else {  // F_H, LJ + gravity, velocity (+ noise), u_p scatter: one launch
             T     *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *up[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
@@ -842,6 +866,8 @@ int free_run(kh_run *r) {
     if (r->frames) (void)hipFreeAsync(r->frames, r->stream);
     for (double *b : {r->shear_pstress, r->shear_acc, r->shear_last, r->shear_hist})
         if (b) (void)hipFreeAsync(b, r->stream);
+    for (void *b : {(void *)r->ra.B, (void *)r->ra.GR, (void *)r->ra.GRb, (void *)r->bid, (void *)r->flo})
+        if (b) (void)hipFreeAsync(b, r->stream);
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto &p : r->pending) {
         (void)hipEventDestroy(p.a);
@@ -908,6 +934,204 @@ void drop_graph(kh_run *r) {
 }
 
 }  // namespace
+
+// ------------------------------------------------------ rigid bodies (host)
+// The initial block of a rigid run (init_particle.cxx:455-487) in FP64 on
+// the host, once: its sums run in the reference's order (beads in order,
+// each over its Sekibun points), so the principal frame it finds is the
+// reference's.
+namespace rigid_init {
+
+struct Geo {
+    int    n[3];
+    double dx, L[3], radius, hxi;
+};
+int    nint_h(double a) { return a >= 0.0 ? (int)(a + .5) : (int)(a - .5); }
+// Distance (interaction.h:125-132): minimum-image |x2 - x1|
+double dist_h(const Geo &g, const double *x1, const double *x2) {
+    double s = 0.0;
+    for (int d = 0; d < 3; d++) {
+        double e = x2[d] - x1[d];
+        e -= (double)nint_h(e / g.L[d]) * g.L[d];
+        s += e * e;
+    }
+    return sqrt(s);
+}
+// Phi (profile.h:30-50)
+double phi_h(const Geo &g, double x) {
+    auto H = [&](double y) { return y > 0 ? exp(-(g.dx / y) * (g.dx / y)) : 0.0; };
+    const double a1 = g.radius + g.hxi - x, a2 = x - g.radius + g.hxi;
+    const double h1 = H(a1);
+    return h1 / (h1 + H(a2));
+}
+// Particle_cell + Relative_coord (make_phi.h:114-170) for one Sekibun offset:
+// grid index, r = cell dx - residue, grid position
+struct Pt {
+    long long idx;
+    double    r[3], x[3];
+};
+Pt point_h(const Geo &g, const double *xp, int code) {
+    const int cell[3] = {(code & 255) - 128, ((code >> 8) & 255) - 128, ((code >> 16) & 255) - 128};
+    Pt        p;
+    int       rm[3];
+    for (int d = 0; d < 3; d++) {
+        const double dmy   = xp[d] / g.dx;
+        const int    x_int = (int)dmy;
+        const double res   = (dmy - (double)x_int) * g.dx;
+        rm[d]              = (x_int + cell[d] + g.n[d]) % g.n[d];
+        p.r[d]             = (double)cell[d] * g.dx - res;
+        p.x[d]             = rm[d] * g.dx;
+    }
+    p.idx = ((long long)rm[0] * g.n[1] + rm[1]) * g.n[2] + rm[2];
+    return p;
+}
+void cross(double *c, const double *a, const double *b) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+// qtn_qvq_prod (quaternion.h:178-185) with q (BODY2SPACE) or its conjugate
+void rotate(double *o, const double *q, const double *v, bool to_body) {
+    const double s = q[0], a = to_body ? -q[1] : q[1], b = to_body ? -q[2] : q[2], c = to_body ? -q[3] : q[3];
+    const double qs2 = s * s - (a * a + b * b + c * c), qdotp = a * v[0] + b * v[1] + c * v[2];
+    o[0] = (qs2 * v[0] + 2.0 * (qdotp * a + s * (b * v[2] - c * v[1])));
+    o[1] = (qs2 * v[1] + 2.0 * (qdotp * b + s * (c * v[0] - a * v[2])));
+    o[2] = (qs2 * v[2] + 2.0 * (qdotp * c + s * (a * v[1] - b * v[0])));
+}
+// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (the scheme
+// of Numerical Recipes section 11.1 as jacobi, matrix_diagonal.cxx:56-172,
+// applies it: entries scaled by their largest magnitude, thresholds for the
+// first three sweeps, negligible elements dropped after four); eigenvectors
+// returned as rows, by decreasing eigenvalue (jacobi_ordered, 25-53).
+bool jacobi(const double In[3][3], double V[3][3], double ev[3]) {
+    double A[3][3], b[3], z[3], mval = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) mval = std::max(mval, fabs(In[i][j]));
+    const double minv = mval > 0.0 ? 1.0 / mval : 1.0;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) {
+            A[i][j] = In[i][j] * minv;
+            V[i][j] = i == j ? 1.0 : 0.0;
+        }
+        b[i] = ev[i] = A[i][i];
+        z[i]         = 0.0;
+    }
+    auto rot = [](double M[3][3], double sn, double tau, int i, int j, int k, int l) {
+        const double g = M[i][j], h = M[k][l];
+        M[i][j]        = g - sn * (h + g * tau);
+        M[k][l]        = h + sn * (g - h * tau);
+    };
+    const double tol = 10.0 * 2.220446049250313e-16;  // TOL_MP (macro.h:36-37)
+    for (int it = 1; it <= 50; it++) {
+        const double sm = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+        if (!(sm > tol || -tol > sm)) {
+            for (int i = 0; i < 2; i++)
+                for (int j = i + 1; j < 3; j++) std::swap(V[i][j], V[j][i]);
+            for (int i = 0; i < 2; i++) {  // by decreasing eigenvalue, the last of equals
+                int    k   = i;
+                double top = ev[k];
+                for (int j = i; j < 3; j++)
+                    if (ev[j] >= top) {
+                        k   = j;
+                        top = ev[k];
+                    }
+                if (k != i) {
+                    ev[k] = ev[i];
+                    ev[i] = top;
+                    for (int j = 0; j < 3; j++) std::swap(V[k][j], V[i][j]);
+                }
+            }
+            for (int i = 0; i < 3; i++) ev[i] *= mval;
+            return true;
+        }
+        const double tresh = it < 4 ? 0.2 * sm / 9.0 : 0.0;
+        for (int ip = 0; ip < 2; ip++)
+            for (int iq = ip + 1; iq < 3; iq++) {
+                const double g = 100.0 * fabs(A[ip][iq]);
+                if (it > 4 && g <= 1.0e-17 * fabs(ev[ip]) && g <= 1.0e-17 * fabs(ev[iq])) {
+                    A[ip][iq] = 0.0;
+                } else if (fabs(A[ip][iq]) > tresh) {
+                    double h = ev[iq] - ev[ip], t;
+                    if (g <= 1.0e-17 * fabs(h)) {
+                        t = A[ip][iq] / h;
+                    } else {
+                        const double theta = 0.5 * h / A[ip][iq];
+                        t                  = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+                        if (theta < 0.0) t = -t;
+                    }
+                    const double c = 1.0 / sqrt(1 + t * t), sn = t * c, tau = sn / (1.0 + c);
+                    h = t * A[ip][iq];
+                    z[ip] -= h;
+                    z[iq] += h;
+                    ev[ip] -= h;
+                    ev[iq] += h;
+                    A[ip][iq] = 0.0;
+                    for (int j = 0; j < ip; j++) rot(A, sn, tau, j, ip, j, iq);
+                    for (int j = ip + 1; j < iq; j++) rot(A, sn, tau, ip, j, j, iq);
+                    for (int j = iq + 1; j < 3; j++) rot(A, sn, tau, ip, j, iq, j);
+                    for (int j = 0; j < 3; j++) rot(V, sn, tau, j, ip, j, iq);
+                }
+            }
+        for (int i = 0; i < 3; i++) {
+            b[i] += z[i];
+            ev[i] = b[i];
+            z[i]  = 0.0;
+        }
+    }
+    return false;
+}
+// The body frame from its principal axes: ex along the axis of the
+// smallest moment, ey the largest's made orthogonal to it, ez = ex x ey
+// (M_coordinate_frame, lad3.h:318-354); rows V[2] <- ex, V[0] <- ey, V[1] <- ez
+void frame(double V[3][3]) {
+    double ex[3], ey[3], ez[3], *u = V[2], *v = V[0];
+    double nrm = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int i = 0; i < 3; i++) ex[i] = u[i] / nrm;
+    const double dot = ex[0] * v[0] + ex[1] * v[1] + ex[2] * v[2];
+    for (int i = 0; i < 3; i++) ey[i] = v[i] - dot * ex[i];
+    nrm = sqrt(ey[0] * ey[0] + ey[1] * ey[1] + ey[2] * ey[2]);
+    for (int i = 0; i < 3; i++) ey[i] /= nrm;
+    cross(ez, ex, ey);
+    nrm = sqrt(ez[0] * ez[0] + ez[1] * ez[1] + ez[2] * ez[2]);
+    for (int i = 0; i < 3; i++) {
+        V[2][i] = ex[i];
+        V[0][i] = ey[i];
+        V[1][i] = ez[i] / nrm;
+    }
+}
+// Rotation matrix -> unit quaternion (s, v) from its largest q_i^2
+// component (Shepperd's method; rm_rqtn, quaternion.h:491-549)
+void to_quaternion(const double R[3][3], double *q) {
+    double qq[4] = {1.0 + R[0][0] + R[1][1] + R[2][2], 1.0 + R[0][0] - R[1][1] - R[2][2],
+                    1.0 - R[0][0] + R[1][1] - R[2][2], 1.0 - R[0][0] - R[1][1] + R[2][2]};
+    int    i     = 0;
+    for (int d = 1; d < 4; d++)
+        if (qq[d] > qq[i]) i = d;
+    qq[i]            = sqrt(qq[i]) / 2.0;
+    const double dmy = 1.0 / (4.0 * qq[i]);
+    const double sk[3] = {R[2][1] - R[1][2], R[0][2] - R[2][0], R[1][0] - R[0][1]};  // 4 s v
+    const double sy[3] = {R[1][0] + R[0][1], R[0][2] + R[2][0], R[2][1] + R[1][2]};  // 4 v_a v_b
+    if (i == 0) {
+        qq[1] = sk[0] * dmy;
+        qq[2] = sk[1] * dmy;
+        qq[3] = sk[2] * dmy;
+    } else if (i == 1) {
+        qq[0] = sk[0] * dmy;
+        qq[2] = sy[0] * dmy;
+        qq[3] = sy[1] * dmy;
+    } else if (i == 2) {
+        qq[0] = sk[1] * dmy;
+        qq[1] = sy[0] * dmy;
+        qq[3] = sy[2] * dmy;
+    } else {
+        qq[0] = sk[2] * dmy;
+        qq[1] = sy[1] * dmy;
+        qq[2] = sy[2] * dmy;
+    }
+    for (int k = 0; k < 4; k++) q[k] = qq[k];
+}
+
+}  // namespace rigid_init
 
 // ================================================================== C ABI
 extern "C" {
@@ -1021,6 +1245,7 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
                     if (dmy < max_radius) sek.push_back(sek_pack(i, j, k));
                 }
         r->np_domain = (int)sek.size();
+        r->sek_host  = sek;
     }
 
     // device memory
@@ -1167,6 +1392,12 @@ int kh_set_particles(kh_run *r, const Particle *p, int n) {
         d.spec = s.spec;
     }
     r->pcache_valid = 0;
+    if (r->rigid) {  // new particles: their bodies are set again with kh_set_rigid
+        r->rigid  = 0;
+        r->pp.bid = nullptr;
+        r->pp.flo = nullptr;
+        drop_graph(r);
+    }
     std::vector<double> xs(4 * (size_t)(n > 0 ? n : 1), 0.0);
     for (int i = 0; i < n; i++) {
         for (int k = 0; k < 3; k++) xs[4 * i + k] = h[i].x[k];
@@ -1523,6 +1754,193 @@ int kh_get_output(kh_run *r, double *out, int nframes) {
     KH_HIP(hipMemcpyAsync(out, r->frames, sizeof(double) * (size_t)nframes * (1 + (size_t)KH_FRAME_WORDS * r->np),
                           hipMemcpyDeviceToHost, r->stream));
     KH_HIP(hipStreamSynchronize(r->stream));
+    return KH_OK;
+}
+
+int kh_set_rigid(kh_run *r, int nb, const int *cum, const int *free_flags, const double *v0, const double *w0) {
+    using namespace rigid_init;
+    if (!r || !cum) return fail(KH_ERR_ARG, "NULL argument");
+    if (nb < 1 || cum[0] != 0 || cum[nb] != r->np) return fail(KH_ERR_ARG, "bead offsets must run from 0 to np");
+    for (int k = 0; k < nb; k++)
+        if (cum[k + 1] <= cum[k]) return fail(KH_ERR_ARG, "body %d has no beads", k);
+    if (r->kbt > 0. || r->slip || r->shear || !r->P.rotation)
+        return fail(KH_ERR_UNSUPPORTED, "rigid bodies need ROTATION on, kT = 0, no slip species and no shear");
+    std::vector<DevParticle> h((size_t)r->np);
+    KH_HIP(hipMemcpyAsync(h.data(), r->dp, sizeof(DevParticle) * h.size(), hipMemcpyDeviceToHost, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    const Geo g{{r->g.nx, r->g.ny, r->g.nz}, r->P.dx, {r->pp.L[0], r->pp.L[1], r->pp.L[2]}, r->radius, r->hxi};
+    std::vector<DevBody> B((size_t)nb);
+    std::vector<int>     bid((size_t)r->np);
+    std::vector<double>  GR(3 * (size_t)r->np), GRb(3 * (size_t)r->np);
+    for (int k = 0; k < nb; k++) {
+        DevBody &b = B[k];
+        memset(&b, 0, sizeof b);
+        b.first = cum[k];
+        b.last  = cum[k + 1];
+        b.spec  = h[cum[k]].spec;
+        b.free_ = free_flags ? (free_flags[k] != 0) : 1;
+        // init_set_xGs (rigid.h:25-38): the trial centre, from the
+        // unwrapped bead coordinates; init_set_PBC (58-80)
+        for (int n = b.first; n < b.last; n++) {
+            bid[n] = k;
+            for (int d = 0; d < 3; d++) b.xG[d] += h[n].x_nopbc[d];
+        }
+        for (int d = 0; d < 3; d++) {
+            b.xG[d] /= (double)(b.last - b.first);
+            b.xGn[d] = b.xGp[d] = b.xG[d];
+            b.xG[d]             = fmod(b.xG[d] + g.L[d], g.L[d]);
+        }
+        for (int n = b.first; n < b.last; n++)
+            for (int d = 0; d < 3; d++) h[n].x_prev[d] = h[n].x_nopbc[d];
+    }
+    auto set_gr = [&]() {  // init_set_GRvecs (rigid.h:44-54)
+        for (int n = 0; n < r->np; n++)
+            for (int d = 0; d < 3; d++) GR[3 * n + d] = h[n].x_nopbc[d] - B[bid[n]].xGn[d];
+    };
+    set_gr();
+    // Make_phi_particle_sum (make_phi.cxx:214-260) over every bead
+    std::unordered_map<long long, double> phisum;
+    for (int n = 0; n < r->np; n++)
+        for (int code : r->sek_host) {
+            const Pt p = point_h(g, h[n].x, code);
+            phisum[p.idx] += phi_h(g, dist_h(g, p.x, h[n].x));
+        }
+    // phi/max(phi_sum,1)-weighted mass, first moment and inertia of a body
+    // about its current centre (make_phi.cxx:589-630, 674-738)
+    auto moments = [&](int k, double &m, double *c, double J[3][3]) {
+        m = c[0] = c[1] = c[2] = 0.0;
+        for (int d = 0; d < 3; d++) J[d][0] = J[d][1] = J[d][2] = 0.0;
+        for (int n = B[k].first; n < B[k].last; n++)
+            for (int code : r->sek_host) {
+                const Pt     p = point_h(g, h[n].x, code);
+                const double w = phi_h(g, dist_h(g, p.x, h[n].x)) / std::max(phisum[p.idx], 1.0);
+                const double R[3] = {GR[3 * n] + p.r[0], GR[3 * n + 1] + p.r[1], GR[3 * n + 2] + p.r[2]};
+                m += w;
+                for (int d = 0; d < 3; d++) c[d] += w * R[d];
+                J[0][0] += w * (R[1] * R[1] + R[2] * R[2]);
+                J[0][1] += w * (-R[0] * R[1]);
+                J[0][2] += w * (-R[0] * R[2]);
+                J[1][0] += w * (-R[1] * R[0]);
+                J[1][1] += w * (R[0] * R[0] + R[2] * R[2]);
+                J[1][2] += w * (-R[1] * R[2]);
+                J[2][0] += w * (-R[2] * R[0]);
+                J[2][1] += w * (-R[2] * R[1]);
+                J[2][2] += w * (R[0] * R[0] + R[1] * R[1]);
+            }
+    };
+    for (int k = 0; k < nb; k++) {  // Make_phi_rigid_mass: mass, centre
+        double m, c[3], J[3][3];
+        moments(k, m, c, J);
+        B[k].M = m * r->dx3 * r->species[B[k].spec].mass_ratio * r->rho;
+        for (int d = 0; d < 3; d++) {
+            B[k].xGn[d] += c[d] / m;
+            B[k].xG[d] = fmod(B[k].xGn[d] + g.L[d], g.L[d]);
+        }
+    }
+    set_gr();
+    for (int k = 0; k < nb; k++) {  // Make_phi_rigid_inertia, init_Rigid_Coordinates (rigid.h:121-156)
+        double m, c[3], J[3][3], V[3][3], ev[3], R[3][3], q[4];
+        moments(k, m, c, J);
+        const double rho_p = r->species[B[k].spec].mass_ratio * r->rho;
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 3; e++) J[d][e] = J[d][e] * r->dx3 * rho_p;
+        if (!jacobi(J, V, ev)) return fail(KH_ERR_ARG, "body %d: inertia diagonalisation did not converge", k);
+        frame(V);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) R[j][i] = V[i][j];
+        to_quaternion(R, q);
+        // principal moments: the diagonal of R^T J R
+        for (int d = 0; d < 3; d++) {
+            double col[3] = {J[0][d], J[1][d], J[2][d]}, rc[3];
+            rotate(rc, q, col, true);
+            for (int e = 0; e < 3; e++) R[e][d] = rc[e];
+        }
+        for (int d = 0; d < 3; d++) {
+            double row[3] = {R[d][0], R[d][1], R[d][2]}, rr[3];
+            rotate(rr, q, row, true);
+            B[k].Ib[d] = rr[d];
+        }
+        for (int n = B[k].first; n < B[k].last; n++) {
+            for (int a = 0; a < 4; a++) h[n].q[a] = q[a];
+            rotate(&GRb[3 * n], q, &GR[3 * n], true);
+        }
+        // init_set_vGs (rigid.h:230-240): initial body velocities (space frame)
+        for (int d = 0; d < 3; d++) {
+            B[k].vG[d] = v0 ? v0[3 * k + d] : 0.0;
+            B[k].wG[d] = w0 ? w0[3 * k + d] : 0.0;
+        }
+    }
+    for (int n = 0; n < r->np; n++) {  // set_Particle_Velocities (rigid.h:218-229)
+        const DevBody &b = B[bid[n]];
+        double         wxr[3];
+        cross(wxr, b.wG, &GR[3 * n]);
+        for (int d = 0; d < 3; d++) {
+            h[n].om[d] = b.wG[d];
+            h[n].v[d]  = b.vG[d] + wxr[d];
+        }
+    }
+    // device side
+    drop_graph(r);
+    for (void *b : {(void *)r->ra.B, (void *)r->ra.GR, (void *)r->ra.GRb, (void *)r->bid, (void *)r->flo})
+        if (b) KH_HIP(hipFreeAsync(b, r->stream));
+    r->ra    = RigidArgs{};
+    r->bid   = nullptr;
+    r->flo   = nullptr;
+    r->rigid = 0;
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        if (hipMallocAsync(p, bytes, r->stream) != hipSuccess) {
+            *p = nullptr;
+            return fail(KH_ERR_NOMEM, "rigid body buffers");
+        }
+        return KH_OK;
+    };
+    KH_TRY(alloc((void **)&r->ra.B, sizeof(DevBody) * (size_t)nb));
+    KH_TRY(alloc((void **)&r->ra.GR, sizeof(double) * GR.size()));
+    KH_TRY(alloc((void **)&r->ra.GRb, sizeof(double) * GRb.size()));
+    KH_TRY(alloc((void **)&r->bid, sizeof(int) * bid.size()));
+    KH_TRY(alloc((void **)&r->flo, sizeof(double) * 3 * (size_t)r->np));
+    KH_HIP(hipMemcpyAsync(r->ra.B, B.data(), sizeof(DevBody) * B.size(), hipMemcpyHostToDevice, r->stream));
+    KH_HIP(hipMemcpyAsync(r->ra.GR, GR.data(), sizeof(double) * GR.size(), hipMemcpyHostToDevice, r->stream));
+    KH_HIP(hipMemcpyAsync((void *)r->ra.GRb, GRb.data(), sizeof(double) * GRb.size(), hipMemcpyHostToDevice,
+                          r->stream));
+    KH_HIP(hipMemcpyAsync(r->bid, bid.data(), sizeof(int) * bid.size(), hipMemcpyHostToDevice, r->stream));
+    KH_HIP(hipMemsetAsync(r->flo, 0, sizeof(double) * 3 * (size_t)r->np, r->stream));
+    KH_HIP(hipMemcpyAsync(r->dp, h.data(), sizeof(DevParticle) * h.size(), hipMemcpyHostToDevice, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    r->ra.nb      = nb;
+    r->ra.compat  = r->P.lj_cell_compat;
+    r->pp.bid     = r->bid;
+    r->pp.flo     = r->flo;
+    r->rigid      = 1;
+    r->pcache_valid = 0;
+    return KH_OK;
+}
+
+int kh_get_rigid(kh_run *r, double *out, int nb) {
+    if (!r || (!out && nb > 0)) return fail(KH_ERR_ARG, "NULL argument");
+    if (!r->rigid || nb != r->ra.nb) return fail(KH_ERR_ARG, "%d bodies requested, the run has %d", nb, r->ra.nb);
+    std::vector<DevBody>     B((size_t)nb);
+    std::vector<DevParticle> h((size_t)r->np);
+    KH_HIP(hipMemcpyAsync(B.data(), r->ra.B, sizeof(DevBody) * B.size(), hipMemcpyDeviceToHost, r->stream));
+    KH_HIP(hipMemcpyAsync(h.data(), r->dp, sizeof(DevParticle) * h.size(), hipMemcpyDeviceToHost, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    for (int k = 0; k < nb; k++) {
+        const DevBody &b = B[k];
+        double        *o = out + (size_t)KH_RIGID_WORDS * k;
+        for (int d = 0; d < 3; d++) {
+            o[d]      = b.xG[d];
+            o[3 + d]  = b.xGn[d];
+            o[6 + d]  = b.vG[d];
+            o[9 + d]  = b.wG[d];
+            o[12 + d] = b.fGp[d];
+            o[15 + d] = b.tGp[d];
+            o[18 + d] = b.fGrp[d];
+            o[21 + d] = b.tGrp[d];
+            o[25 + d] = b.Ib[d];
+        }
+        o[24] = b.M;
+        for (int a = 0; a < 4; a++) o[28 + a] = h[b.first].q[a];
+    }
     return KH_OK;
 }
 

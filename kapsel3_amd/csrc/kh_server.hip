@@ -8,8 +8,9 @@
 // at a time (after the first step each step is one captured-graph launch),
 // so the work of runs of different sizes overlaps on the GPU. A run's
 // trajectory is recorded on the device every step and copied to the
-// client's buffer once, when the run completes. Latency is measured from
-// submission to that copy.
+// client's buffer once, when the run completes; so are its Output_udf frames
+// when the client asked for them. Latency is measured from submission to
+// that copy.
 #include <stdio.h>
 #include <string.h>
 
@@ -43,6 +44,9 @@ struct Request {
     std::vector<Particle>      parts;
     int                        nsteps = 0;
     double                    *traj_out = nullptr;
+    int                        frame_every = 0;   // Output_udf frames every this many steps (0: none)
+    int                        nframes     = 0;
+    double                    *frames_out  = nullptr;
     long long                  bytes    = 0;
     long long                  grid     = 0;
     clk::time_point            t_submit, t_admit, t_done;
@@ -82,6 +86,10 @@ void finish(kh_server *S, Request *q, int status, const char *err) {
             status = KH_ERR_HIP;
             err    = kh_last_error();
         }
+        if (status == 0 && q->frames_out && kh_get_output(q->run, q->frames_out, q->nframes) != KH_OK) {
+            status = KH_ERR_HIP;
+            err    = kh_last_error();
+        }
         kh_run_destroy(q->run);
         q->run = nullptr;
     }
@@ -108,6 +116,7 @@ int start(Request *q) {
     if (st == KH_OK) st = kh_set_particles(q->run, q->parts.data(), (int)q->parts.size());
     if (st == KH_OK) st = kh_init_fluid(q->run);
     if (st == KH_OK) st = kh_set_trajectory(q->run, q->nsteps);
+    if (st == KH_OK && q->frames_out) st = kh_set_output(q->run, q->nframes, q->frame_every);
     return st;
 }
 
@@ -209,11 +218,14 @@ int kh_server_create(const kh_server_config *cfg, kh_server **out) {
     return KH_OK;
 }
 
-int kh_server_submit(kh_server *S, const kh_params *P, const Particle *parts, int np, int nsteps, double *traj_out,
-                     long long *id) {
+static int submit(kh_server *S, const kh_params *P, const Particle *parts, int np, int nsteps, double *traj_out,
+                  int frame_every, double *frames_out, long long *id) {
     if (!S || !P || (!parts && np > 0) || !id) return kh_set_error(KH_ERR_ARG, "NULL argument");
     if (nsteps < 1) return kh_set_error(KH_ERR_ARG, "nsteps must be >= 1");
-    const long long bytes = kh_run_bytes(P, np) + (long long)sizeof(double) * 6 * nsteps * np;
+    if (frames_out && frame_every < 1) return kh_set_error(KH_ERR_ARG, "frame_every must be >= 1");
+    const int nframes = frames_out ? nsteps / frame_every + 1 : 0;
+    const long long bytes = kh_run_bytes(P, np) + (long long)sizeof(double) * 6 * nsteps * np +
+                            (long long)sizeof(double) * nframes * (1 + (long long)KH_FRAME_WORDS * np);
     if (bytes < 0) return kh_set_error(KH_ERR_ARG, "bad run parameters");
     if (bytes > S->budget) return kh_set_error(KH_ERR_NOMEM, "run needs more device memory than the server's budget");
     auto q     = std::make_unique<Request>();
@@ -221,9 +233,12 @@ int kh_server_submit(kh_server *S, const kh_params *P, const Particle *parts, in
     q->species.assign(P->species, P->species + P->nspecies);
     q->P.species = q->species.data();
     q->parts.assign(parts, parts + np);
-    q->nsteps   = nsteps;
-    q->traj_out = traj_out;
-    q->bytes    = bytes;
+    q->nsteps      = nsteps;
+    q->traj_out    = traj_out;
+    q->frame_every = frames_out ? frame_every : 0;
+    q->nframes     = nframes;
+    q->frames_out  = frames_out;
+    q->bytes       = bytes;
     q->grid     = (long long)P->nx * P->ny * P->nz;
     q->t_submit = clk::now();
     std::lock_guard<std::mutex> lk(S->mu);
@@ -238,6 +253,17 @@ int kh_server_submit(kh_server *S, const kh_params *P, const Particle *parts, in
     S->all[q->id] = std::move(q);
     S->cv.notify_all();
     return KH_OK;
+}
+
+int kh_server_submit(kh_server *S, const kh_params *P, const Particle *parts, int np, int nsteps, double *traj_out,
+                     long long *id) {
+    return submit(S, P, parts, np, nsteps, traj_out, 0, nullptr, id);
+}
+
+int kh_server_submit_output(kh_server *S, const kh_params *P, const Particle *parts, int np, int nsteps,
+                            double *traj_out, int frame_every, double *frames_out, long long *id) {
+    if (!frames_out) return kh_set_error(KH_ERR_ARG, "NULL frames_out");
+    return submit(S, P, parts, np, nsteps, traj_out, frame_every, frames_out, id);
 }
 
 int kh_server_poll(kh_server *S, long long id) {

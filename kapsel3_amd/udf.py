@@ -228,10 +228,27 @@ def read_udf(path):
 
 def read_udf_schema(path):
     """read_udf, also returning the schema: (data, schema, classes)."""
+    data, schema, classes, _ = _read(path)
+    return data, schema, classes
+
+
+def read_udf_records(path):
+    """Every `\\begin{data}` block of a UDF file as its own dict, in file
+    order: the records an output file holds (Output_udf appends one per
+    frame with newRecord, output.cxx:268-375)."""
+    return _read(path)[3]
+
+
+def _read(path):
     with open(path) as f:
-        text = _sections(f.read(), os.path.dirname(os.path.abspath(path)), {os.path.abspath(path)})
+        return parse_udf_text(_sections(f.read(), os.path.dirname(os.path.abspath(path)), {os.path.abspath(path)}))
+
+
+def parse_udf_text(text):
+    """UDF text with its includes already expanded -> (data, schema,
+    classes, records)."""
     s = _Stream(_tokens(text))
-    schema, classes, data = {}, {}, {}
+    schema, classes, data, records = {}, {}, {}, []
     depth_header = 0
     while s.peek()[0] is not None:
         kind, v = s.next()
@@ -251,6 +268,8 @@ def read_udf_schema(path):
         elif v == "\\begin" and sec in ("global_def", "def"):
             _parse_def(s, schema, classes, sec)
         elif v == "\\begin" and sec == "data":
+            rec = {}
+            records.append(rec)
             while not (s.peek()[0] == "cmd" and s.peek()[1] == "\\end"):
                 _, name = s.next()
                 while s.skip("[]"):
@@ -260,12 +279,12 @@ def read_udf_schema(path):
                 if name not in schema:
                     raise UdfError(f"{name} has no definition")
                 dims, typ = schema[name]
-                data[name] = _bind(value, dims, typ, classes)
+                data[name] = rec[name] = _bind(value, dims, typ, classes)
         elif v == "\\end":
             continue
         else:
             raise UdfError(f"unknown section {v}{{{sec}}}")
-    return data, schema, classes
+    return data, schema, classes, records
 
 
 # ------------------------------------------------------------------ write

@@ -32,6 +32,17 @@ static const double kT_snap_o = 1.22;  /* input.cxx:311 */
 #define EPSILON_MP DBL_EPSILON         /* macro.h:36-38 */
 #define TOL_MP (10.0 * EPSILON_MP)
 
+/* one rigid body: rigid.h's per-body globals (input.cxx:1655-1695) */
+typedef struct {
+    double xG[3], xGn[3], xGp[3];    /* xGs, xGs_nopbc, xGs_previous          */
+    double vG[3], vGo[3], wG[3], wGo[3];
+    double fG[3], tG[3], fGr[3], tGr[3]; /* forceGs, torqueGs (hydro), forceGrs, torqueGrs (LJ, gravity) */
+    double fGp[3], tGp[3], fGrp[3], tGrp[3];
+    double M, Ib[3], Mom[9];         /* Rigid_Masses, Rigid_Moments_body diagonal, Rigid_Moments */
+    int    free_;                    /* Rigid_motion "free" (1) or "fix" (0) */
+    int    spec;                     /* RigidID_Components */
+} kr_body;
+
 struct kr_run {
     kh_params   P;
     kh_species *spec;
@@ -60,6 +71,12 @@ struct kr_run {
     double *le_hf; /* per particle: Hydro_force, Hydro_force_new sums, volume, Itrace, force_x */
     double  le_stress_old, le_stress_new, le_reynolds, le_dev_lj, le_strain;
     double  le_record[KR_LE_RECORD];
+    /* rigid composite bodies (SW_PT == rigid, kr_set_rigid) */
+    int      nb;    /* bodies; 0: spherical particles */
+    int     *bcum;  /* Rigid_Particle_Cumul: beads of body b are bcum[b] .. bcum[b+1]-1 */
+    int     *bid;   /* Particle_RigidID */
+    kr_body *body;
+    double (*GR)[3], (*GRb)[3]; /* GRvecs, GRvecs_body */
 };
 
 #define IM(r, i, j, k) ((((size_t)(i) * (r)->NY + (j)) * (r)->NZ_) + (k))
@@ -68,6 +85,9 @@ struct kr_run {
  * that show a pin catches one. */
 static int g_mut = KR_MUT_NONE;
 void       kr_set_mutation(int m) { g_mut = (m > KR_MUT_NONE && m < KR_MUT_COUNT) ? m : KR_MUT_NONE; }
+
+static void rigid_position(kr_run *r, int ab2);
+static void rigid_velocity(kr_run *r, int ab2);
 
 /* ------------------------------------------------------------------ */
 /* small helpers: macro.h:44-52, interaction.h:113-132, lad3.h          */
@@ -584,6 +604,19 @@ static void calc_f_hydro(kr_run *r, const double *phi_sum, double **u) {
                 pn->torque_slip[d]  = 0.0;
             }
     }
+    /* rigid: forceGs += sum of the beads' f_hydro, torqueGs += sum of
+     * (GR + r) x f over their points = GR x f_hydro + torque_hydro per bead
+     * (md_force.cxx:467-473, 496-505), in bead order */
+    for (int n = 0; n < r->np && r->nb; n++) {
+        kr_body        *b = &r->body[r->bid[n]];
+        const Particle *pn = &r->p[n];
+        double          gxf[3];
+        v_cross(gxf, r->GR[n], pn->f_hydro, 1.0);
+        for (int d = 0; d < 3; d++) {
+            b->fG[d] += pn->f_hydro[d];
+            b->tG[d] += gxf[d] + pn->torque_hydro[d];
+        }
+    }
 }
 
 /* Make_f_particle_dt_nonsole (f_particle.h:35-54) */
@@ -665,23 +698,48 @@ static void calc_f_lennard_jones(kr_run *r) {
             Particle *pi = &r->p[i], *pj = &r->p[j];
             if (r->spec[pi->spec].propulsion == KH_JP_OBSTACLE && r->spec[pj->spec].propulsion == KH_JP_OBSTACLE)
                 continue; /* obstacle_chain, interaction.h:29-35 */
+            if (r->nb && r->bid[i] == r->bid[j]) continue; /* rigid_chain, interaction.h:18-24 */
             double rij, rv[3];
             distance0(r, pi->x, pj->x, &rij, rv);
             if (rij < pair_cutoff) {
                 int mult = lj_multiplicity(r, pi->x, pj->x);
                 double dmy_r = fmin(DBL_MAX / rij, lennard_jones_f(r, rij));
-                for (int c = 0; c < mult; c++)
+                for (int c = 0; c < mult; c++) {
+                    double fi[3];
                     for (int d = 0; d < 3; d++) {
-                        double fi = dmy_r * (-rv[d]);
-                        pi->fr[d] += fi;
-                        pj->fr[d] -= fi;
+                        fi[d] = dmy_r * (-rv[d]);
+                        pi->fr[d] += fi[d];
+                        pj->fr[d] -= fi[d];
                     }
+                    if (r->nb) { /* forceGrs, torqueGrs (md_force.cxx:134-153) */
+                        kr_body *bi = &r->body[r->bid[i]], *bj = &r->body[r->bid[j]];
+                        double   ti[3], tj[3];
+                        v_cross(ti, r->GR[i], fi, 1.0);
+                        /* the reference adds GR_j x f_i to body j, whose bead takes -f_i;
+                         * lj_cell_compat = 0 gives it GR_j x (-f_i) */
+                        v_cross(tj, r->GR[j], fi, r->P.lj_cell_compat ? 1.0 : -1.0);
+                        for (int d = 0; d < 3; d++) {
+                            bi->fGr[d] += fi[d];
+                            bj->fGr[d] -= fi[d];
+                            bi->tGr[d] += ti[d];
+                            bj->tGr[d] += tj[d];
+                        }
+                    }
+                }
             }
         }
 }
 
-/* Add_f_gravity (md_force.cxx:306-324), non-rigid branch */
+/* Add_f_gravity (md_force.cxx:306-324) */
 static void add_f_gravity(kr_run *r) {
+    if (r->nb) { /* rigid branch: on each body's volume */
+        for (int k = 0; k < r->nb; k++) {
+            kr_body     *b     = &r->body[k];
+            const double ratio = r->spec[b->spec].mass_ratio;
+            b->fGr[r->P.g_dir] -= r->P.g * r->RHO * (b->M / (ratio * r->RHO)) * (ratio - 1.0);
+        }
+        return;
+    }
     for (int n = 0; n < r->np; n++)
         r->p[n].fr[r->P.g_dir] -= r->Gravity_on_fluid * (r->spec[r->p[n].spec].mass_ratio - 1.0);
 }
@@ -736,6 +794,10 @@ static void orientation_update(const kr_run *r, Particle *p, int ab2) {
 
 /* MD_solver_position_Euler / _AB2 (particle_solver.cxx:39-85) */
 static void position_update(kr_run *r, int ab2) {
+    if (r->nb) { /* SW_PT == rigid */
+        rigid_position(r, ab2);
+        return;
+    }
 #pragma omp parallel for
     for (int n = 0; n < r->np; n++) {
         Particle *p = &r->p[n];
@@ -763,6 +825,11 @@ static void self_propulsion(const kr_run *r, const Particle *p, double *f, doubl
 /* MD_solver_velocity_Euler (103-142) and MD_solver_velocity_AB2_hydro (239-281) */
 static void velocity_update(kr_run *r, int ab2) {
     force(r);
+    if (r->nb) { /* calc_Rigid_VOGs + set_Particle_Velocities (particle_solver.cxx:137-140, 274-277) */
+        rigid_velocity(r, ab2);
+        reset_forces(r);
+        return;
+    }
 #pragma omp parallel for
     for (int n = 0; n < r->np; n++) {
         Particle *p    = &r->p[n];
@@ -796,6 +863,446 @@ static void velocity_update(kr_run *r, int ab2) {
         }
     }
     reset_forces(r);
+}
+
+/* ------------------------------------------------------------------ */
+/* rigid composite bodies (SW_PT == rigid)                              */
+/* ------------------------------------------------------------------ */
+static void pbc3(const kr_run *r, double *x) { /* PBC (periodic_boundary.h:9-15) */
+    for (int d = 0; d < 3; d++) x[d] = fmod(x[d] + r->L[d], r->L[d]);
+}
+
+/* MD_solver_omega_Euler / _AB2 (particle_rotation_solver.h:106-184):
+ * Euler's equations in the body frame with the body's principal moments */
+static void omega_euler(double *w, const double *tq, const double *Ib, const quaternion *q, int free_, double dt) {
+    double wb[3], tb[3];
+    space2body(wb, w, q);
+    space2body(tb, tq, q);
+    double nw[3] = {wb[0], wb[1], wb[2]};
+    if (free_) {
+        nw[0] += (dt / Ib[0]) * (tb[0] + (Ib[1] - Ib[2]) * wb[1] * wb[2]);
+        nw[1] += (dt / Ib[1]) * (tb[1] + (Ib[2] - Ib[0]) * wb[2] * wb[0]);
+        nw[2] += (dt / Ib[2]) * (tb[2] + (Ib[0] - Ib[1]) * wb[0] * wb[1]);
+    }
+    body2space(w, nw, q);
+}
+static void omega_ab2(double *w, const double *tq, const double *tq_old, const double *Ib, const quaternion *q,
+                      const quaternion *q_old, int free_, double hdt) {
+    double wb[3], tb[3], tbo[3];
+    space2body(wb, w, q);
+    space2body(tb, tq, q);
+    space2body(tbo, tq_old, q_old);
+    double nw[3] = {wb[0], wb[1], wb[2]};
+    if (free_) {
+        nw[0] += (hdt / Ib[0]) * ((tb[0] + tbo[0]) + 2.0 * (Ib[1] - Ib[2]) * wb[1] * wb[2]);
+        nw[1] += (hdt / Ib[1]) * ((tb[1] + tbo[1]) + 2.0 * (Ib[2] - Ib[0]) * wb[2] * wb[0]);
+        nw[2] += (hdt / Ib[2]) * ((tb[2] + tbo[2]) + 2.0 * (Ib[0] - Ib[1]) * wb[0] * wb[1]);
+    }
+    body2space(w, nw, q);
+}
+
+/* set_Particle_Velocities (rigid.h:218-229): beads move with their body */
+static void rigid_bead_velocities(kr_run *r) {
+    for (int n = 0; n < r->np; n++) {
+        const kr_body *b = &r->body[r->bid[n]];
+        double         wxr[3];
+        v_cross(wxr, b->wG, r->GR[n], 1.0);
+        for (int d = 0; d < 3; d++) {
+            r->p[n].omega[d] = b->wG[d];
+            r->p[n].v[d]     = b->vG[d] + wxr[d];
+        }
+    }
+}
+
+/* solver_Rigid_Position (rigid.h:333-392), update_Orientation (271-290;
+ * the space-frame moments it also rotates feed only the thermostat and
+ * the stress, neither of which a rigid run here computes) and
+ * update_Particle_Configuration (rigid.h:159-178) */
+static void rigid_position(kr_run *r, int ab2) {
+    for (int k = 0; k < r->nb; k++) {
+        kr_body *b = &r->body[k];
+        for (int d = 0; d < 3; d++) {
+            b->xGp[d]      = b->xG[d];
+            double delta_x = ab2 ? r->jikan.hdt_md * (3.0 * b->vG[d] - b->vGo[d]) : r->jikan.dt_md * b->vG[d];
+            b->xG[d] += delta_x;
+            b->xGn[d] += delta_x;
+        }
+        pbc3(r, b->xG);
+        Particle *first = &r->p[r->bcum[k]];
+        orientation_update(r, first, ab2);
+        for (int n = r->bcum[k] + 1; n < r->bcum[k + 1]; n++) {
+            r->p[n].q_old = first->q_old;
+            r->p[n].q     = first->q;
+        }
+        for (int n = r->bcum[k]; n < r->bcum[k + 1]; n++) body2space(r->GR[n], r->GRb[n], &first->q);
+    }
+    for (int n = 0; n < r->np; n++) {
+        const kr_body *b = &r->body[r->bid[n]];
+        Particle      *p = &r->p[n];
+        double         wxr[3];
+        v_cross(wxr, b->wG, r->GR[n], 1.0);
+        for (int d = 0; d < 3; d++) {
+            p->x_previous[d] = p->x[d];
+            p->x[d]          = b->xG[d] + r->GR[n][d];
+            p->omega_old[d]  = p->omega[d];
+            p->v_old[d]      = p->v[d];
+            p->omega[d]      = b->wG[d];
+            p->v[d]          = b->vG[d] + wxr[d];
+        }
+        pbc3(r, p->x);
+    }
+}
+
+/* calc_Rigid_VOGs (rigid.h:467-559), then set_Particle_Velocities */
+static void rigid_velocity(kr_run *r, int ab2) {
+    for (int k = 0; k < r->nb; k++) {
+        kr_body *b = &r->body[k];
+        for (int d = 0; d < 3; d++) {
+            b->vGo[d] = b->vG[d];
+            b->wGo[d] = b->wG[d];
+        }
+    }
+    for (int k = 0; k < r->nb; k++) {
+        kr_body        *b     = &r->body[k];
+        const Particle *first = &r->p[r->bcum[k]];
+        if (!ab2) {
+            const double dmy = r->jikan.dt_md / b->M;
+            if (b->free_)
+                for (int d = 0; d < 3; d++) b->vG[d] += dmy * (b->fG[d] + b->fGr[d]);
+            double tq[3] = {b->tG[0] + b->tGr[0], b->tG[1] + b->tGr[1], b->tG[2] + b->tGr[2]};
+            omega_euler(b->wG, tq, b->Ib, &first->q, b->free_, r->jikan.dt_md);
+        } else {
+            const double dmy = r->jikan.hdt_md / b->M;
+            if (b->free_)
+                for (int d = 0; d < 3; d++) b->vG[d] += dmy * (2.0 * b->fG[d] + b->fGr[d] + b->fGrp[d]);
+            double tq[3] = {2.0 * b->tG[0] + b->tGr[0], 2.0 * b->tG[1] + b->tGr[1], 2.0 * b->tG[2] + b->tGr[2]};
+            omega_ab2(b->wG, tq, b->tGrp, b->Ib, &first->q, &first->q_old, b->free_, r->jikan.hdt_md);
+        }
+    }
+    for (int k = 0; k < r->nb; k++) {
+        kr_body *b = &r->body[k];
+        for (int d = 0; d < 3; d++) {
+            b->fGp[d]  = b->fG[d];
+            b->fGrp[d] = b->fGr[d];
+            b->tGp[d]  = b->tG[d];
+            b->tGrp[d] = b->tGr[d];
+            b->fG[d] = b->fGr[d] = b->tG[d] = b->tGr[d] = 0.0;
+        }
+    }
+    rigid_bead_velocities(r);
+}
+
+/* jacobi (matrix_diagonal.cxx:56-172): cyclic Jacobi eigen-decomposition
+ * of a symmetric 3x3 matrix (Numerical Recipes' scheme: thresholds for the
+ * first three sweeps, negligible elements dropped after four), the
+ * eigenvectors returned as rows, ordered by decreasing eigenvalue
+ * (jacobi_ordered, 25-53) */
+static void jrot(double A[3][3], double s, double tau, int i, int j, int k, int l) {
+    double g = A[i][j], h = A[k][l];
+    A[i][j]  = g - s * (h + g * tau);
+    A[k][l]  = h + s * (g - h * tau);
+}
+static int jacobi3(const double In[3][3], double V[3][3], double dv[3]) {
+    const int n = 3;
+    double    A[3][3], b[3], z[3], mval = 0.0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) mval = fmax(mval, fabs(In[i][j]));
+    const double minv = mval > 0.0 ? 1.0 / mval : 1.0;
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) {
+            A[i][j] = In[i][j] * minv;
+            V[i][j] = 0.0;
+        }
+        V[i][i] = 1.0;
+        b[i] = dv[i] = A[i][i];
+        z[i]         = 0.0;
+    }
+    for (int it = 1; it <= 50; it++) {
+        double sm = 0.0;
+        for (int ip = 0; ip < n - 1; ip++)
+            for (int iq = ip + 1; iq < n; iq++) sm += fabs(A[ip][iq]);
+        if (!(sm > TOL_MP || -TOL_MP > sm)) { /* zero_mp(sm) */
+            for (int i = 0; i < n - 1; i++)
+                for (int j = i + 1; j < n; j++) {
+                    double t = V[i][j];
+                    V[i][j]  = V[j][i];
+                    V[j][i]  = t;
+                }
+            for (int i = 0; i < n - 1; i++) { /* selection by decreasing eigenvalue, the last of equals */
+                int    k   = i;
+                double dmy = dv[k];
+                for (int j = i; j < n; j++)
+                    if (dv[j] >= dmy) {
+                        k   = j;
+                        dmy = dv[k];
+                    }
+                if (k != i) {
+                    dv[k] = dv[i];
+                    dv[i] = dmy;
+                    for (int j = 0; j < n; j++) {
+                        double t = V[k][j];
+                        V[k][j]  = V[i][j];
+                        V[i][j]  = t;
+                    }
+                }
+            }
+            for (int i = 0; i < n; i++) dv[i] *= mval;
+            return 0;
+        }
+        const double tresh = it < 4 ? 0.2 * sm / (n * n) : 0.0;
+        for (int ip = 0; ip < n - 1; ip++)
+            for (int iq = ip + 1; iq < n; iq++) {
+                const double g = 100.0 * fabs(A[ip][iq]);
+                if (it > 4 && g <= 1.0e-17 * fabs(dv[ip]) && g <= 1.0e-17 * fabs(dv[iq])) {
+                    A[ip][iq] = 0.0;
+                } else if (fabs(A[ip][iq]) > tresh) {
+                    double h = dv[iq] - dv[ip], t;
+                    if (g <= 1.0e-17 * fabs(h))
+                        t = A[ip][iq] / h;
+                    else {
+                        const double theta = 0.5 * h / A[ip][iq];
+                        t                  = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+                        if (theta < 0.0) t = -t;
+                    }
+                    const double c = 1.0 / sqrt(1 + t * t), sn = t * c, tau = sn / (1.0 + c);
+                    h = t * A[ip][iq];
+                    z[ip] -= h;
+                    z[iq] += h;
+                    dv[ip] -= h;
+                    dv[iq] += h;
+                    A[ip][iq] = 0.0;
+                    for (int j = 0; j < ip; j++) jrot(A, sn, tau, j, ip, j, iq);
+                    for (int j = ip + 1; j < iq; j++) jrot(A, sn, tau, ip, j, j, iq);
+                    for (int j = iq + 1; j < n; j++) jrot(A, sn, tau, ip, j, iq, j);
+                    for (int j = 0; j < n; j++) jrot(V, sn, tau, j, ip, j, iq);
+                }
+            }
+        for (int ip = 0; ip < n; ip++) {
+            b[ip] += z[ip];
+            dv[ip] = b[ip];
+            z[ip]  = 0.0;
+        }
+    }
+    return -1;
+}
+
+/* M_coordinate_frame (lad3.h:318-354): ex = u/|u|, ey = v orthogonalised,
+ * ez = ex x ey, written back into u, v, w */
+static void coordinate_frame(double *u, double *v, double *w) {
+    double ex[3], ey[3], ez[3], norm = sqrt(SQ(u[0]) + SQ(u[1]) + SQ(u[2]));
+    for (int i = 0; i < 3; i++) ex[i] = u[i] / norm;
+    double dmy = ex[0] * v[0] + ex[1] * v[1] + ex[2] * v[2];
+    for (int i = 0; i < 3; i++) ey[i] = v[i] - dmy * ex[i];
+    norm = sqrt(SQ(ey[0]) + SQ(ey[1]) + SQ(ey[2]));
+    for (int i = 0; i < 3; i++) ey[i] /= norm;
+    ez[0] = ex[1] * ey[2] - ex[2] * ey[1];
+    ez[1] = ex[2] * ey[0] - ex[0] * ey[2];
+    ez[2] = ex[0] * ey[1] - ex[1] * ey[0];
+    norm  = sqrt(SQ(ez[0]) + SQ(ez[1]) + SQ(ez[2]));
+    for (int i = 0; i < 3; i++) {
+        u[i] = ex[i];
+        v[i] = ey[i];
+        w[i] = ez[i] / norm;
+    }
+}
+
+/* rm_rqtn (quaternion.h:491-549): rotation matrix -> quaternion from its
+ * largest q_i^2 component */
+static void rm_to_qtn(quaternion *q, const double R[3][3]) {
+    double qq[4] = {1.0 + R[0][0] + R[1][1] + R[2][2], 1.0 + R[0][0] - R[1][1] - R[2][2],
+                    1.0 - R[0][0] + R[1][1] - R[2][2], 1.0 - R[0][0] - R[1][1] + R[2][2]};
+    int    i     = 0;
+    double mm    = qq[0];
+    for (int d = 1; d < 4; d++)
+        if (qq[d] > mm) {
+            mm = qq[d];
+            i  = d;
+        }
+    double dmy;
+    switch (i) {
+        case 0:
+            qq[0] = sqrt(qq[0]) / 2.0;
+            dmy   = 1.0 / (4.0 * qq[0]);
+            qq[1] = (R[2][1] - R[1][2]) * dmy;
+            qq[2] = (R[0][2] - R[2][0]) * dmy;
+            qq[3] = (R[1][0] - R[0][1]) * dmy;
+            break;
+        case 1:
+            qq[1] = sqrt(qq[1]) / 2.0;
+            dmy   = 1.0 / (4.0 * qq[1]);
+            qq[0] = (R[2][1] - R[1][2]) * dmy;
+            qq[2] = (R[1][0] + R[0][1]) * dmy;
+            qq[3] = (R[0][2] + R[2][0]) * dmy;
+            break;
+        case 2:
+            qq[2] = sqrt(qq[2]) / 2.0;
+            dmy   = 1.0 / (4.0 * qq[2]);
+            qq[0] = (R[0][2] - R[2][0]) * dmy;
+            qq[1] = (R[1][0] + R[0][1]) * dmy;
+            qq[3] = (R[2][1] + R[1][2]) * dmy;
+            break;
+        default:
+            qq[3] = sqrt(qq[3]) / 2.0;
+            dmy   = 1.0 / (4.0 * qq[3]);
+            qq[0] = (R[1][0] - R[0][1]) * dmy;
+            qq[1] = (R[2][0] + R[0][2]) * dmy;
+            qq[2] = (R[2][1] + R[1][2]) * dmy;
+            break;
+    }
+    q->s    = qq[0];
+    q->v[0] = qq[1];
+    q->v[1] = qq[2];
+    q->v[2] = qq[3];
+}
+
+/* phi-weighted sums of one body over its beads' Sekibun points
+ * (Make_phi_rigid_mass / Make_phi_rigid_inertia, make_phi.cxx:589-630,
+ * 674-738): mass m, first moment c about the current centre, inertia I */
+static void rigid_phi_moments(kr_run *r, int k, double *m, double *c, double J[3][3]) {
+    *m = c[0] = c[1] = c[2] = 0.0;
+    for (int d = 0; d < 3; d++) J[d][0] = J[d][1] = J[d][2] = 0.0;
+    for (int n = r->bcum[k]; n < r->bcum[k + 1]; n++) {
+        double xp[3] = {r->p[n].x[0], r->p[n].x[1], r->p[n].x[2]}, residue[3];
+        int    x_int[3];
+        particle_cell(r, xp, x_int, residue);
+        for (int mesh = 0; mesh < r->NP_domain; mesh++) {
+            int    r_mesh[3];
+            double rv[3], x[3];
+            relative_coord(r, r->sek + 3 * mesh, x_int, residue, r_mesh, rv);
+            for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * r->DX;
+            const double w = Phi(r, distance(r, x, xp)) / fmax(r->phi_sum[IM(r, r_mesh[0], r_mesh[1], r_mesh[2])], 1.0);
+            const double R[3] = {r->GR[n][0] + rv[0], r->GR[n][1] + rv[1], r->GR[n][2] + rv[2]};
+            *m += w;
+            for (int d = 0; d < 3; d++) c[d] += w * R[d];
+            J[0][0] += w * (R[1] * R[1] + R[2] * R[2]);
+            J[0][1] += w * (-R[0] * R[1]);
+            J[0][2] += w * (-R[0] * R[2]);
+            J[1][0] += w * (-R[1] * R[0]);
+            J[1][1] += w * (R[0] * R[0] + R[2] * R[2]);
+            J[1][2] += w * (-R[1] * R[2]);
+            J[2][0] += w * (-R[2] * R[0]);
+            J[2][1] += w * (-R[2] * R[1]);
+            J[2][2] += w * (R[0] * R[0] + R[1] * R[1]);
+        }
+    }
+}
+
+/* The initial block of a rigid run (init_particle.cxx:455-487): trial
+ * centres from the unwrapped bead coordinates, the phi-weighted centre
+ * and mass, bead vectors, inertia, principal frame, bead velocities. */
+int kr_set_rigid(kr_run *r, int nb, const int *cum, const int *free_flags, const double *v0, const double *w0) {
+    if (nb < 1 || !cum || cum[0] != 0 || cum[nb] != r->np || !r->P.rotation || r->kBT > 0. || r->sw_janus_slip ||
+        r->le)
+        return -1;
+    for (int k = 0; k < nb; k++)
+        if (cum[k + 1] <= cum[k]) return -1;
+    free(r->bcum);
+    free(r->bid);
+    free(r->body);
+    free(r->GR);
+    free(r->GRb);
+    r->nb   = nb;
+    r->bcum = (int *)malloc(sizeof(int) * (size_t)(nb + 1));
+    r->bid  = (int *)malloc(sizeof(int) * (size_t)r->np);
+    r->body = (kr_body *)calloc((size_t)nb, sizeof(kr_body));
+    r->GR   = (double(*)[3])calloc((size_t)r->np, sizeof(double[3]));
+    r->GRb  = (double(*)[3])calloc((size_t)r->np, sizeof(double[3]));
+    memcpy(r->bcum, cum, sizeof(int) * (size_t)(nb + 1));
+    for (int k = 0; k < nb; k++) {
+        kr_body *b = &r->body[k];
+        b->spec    = r->p[cum[k]].spec;
+        b->free_   = free_flags ? (free_flags[k] != 0) : 1;
+        for (int n = cum[k]; n < cum[k + 1]; n++) r->bid[n] = k;
+        /* init_set_xGs (rigid.h:25-38) on the unwrapped coordinates */
+        for (int n = cum[k]; n < cum[k + 1]; n++)
+            for (int d = 0; d < 3; d++) b->xG[d] += r->p[n].x_nopbc[d];
+        for (int d = 0; d < 3; d++) b->xG[d] /= (double)(cum[k + 1] - cum[k]);
+        /* init_set_PBC (rigid.h:58-80) */
+        for (int d = 0; d < 3; d++) b->xGn[d] = b->xGp[d] = b->xG[d];
+        for (int n = cum[k]; n < cum[k + 1]; n++)
+            for (int d = 0; d < 3; d++) r->p[n].x_previous[d] = r->p[n].x_nopbc[d];
+        pbc3(r, b->xG);
+    }
+    /* init_set_GRvecs (rigid.h:44-54) */
+    for (int n = 0; n < r->np; n++)
+        for (int d = 0; d < 3; d++) r->GR[n][d] = r->p[n].x_nopbc[d] - r->body[r->bid[n]].xGn[d];
+    reset_field(r, r->phi);
+    reset_field(r, r->phi_sum);
+    make_phi_particle_sum(r, r->phi, r->phi_sum);
+    for (int k = 0; k < nb; k++) { /* Make_phi_rigid_mass */
+        kr_body     *b = &r->body[k];
+        double       m, c[3], J[3][3];
+        const double rho_p = r->spec[b->spec].mass_ratio * r->RHO;
+        rigid_phi_moments(r, k, &m, c, J);
+        b->M = m * r->DX3 * rho_p;
+        for (int d = 0; d < 3; d++) b->xGn[d] += c[d] / m;
+        for (int d = 0; d < 3; d++) b->xG[d] = b->xGn[d];
+        pbc3(r, b->xG);
+    }
+    for (int n = 0; n < r->np; n++)
+        for (int d = 0; d < 3; d++) r->GR[n][d] = r->p[n].x_nopbc[d] - r->body[r->bid[n]].xGn[d];
+    for (int k = 0; k < nb; k++) { /* Make_phi_rigid_inertia, init_Rigid_Coordinates (rigid.h:121-156) */
+        kr_body     *b = &r->body[k];
+        double       m, c[3], J[3][3], V[3][3], ev[3], R[3][3];
+        const double rho_p = r->spec[b->spec].mass_ratio * r->RHO;
+        rigid_phi_moments(r, k, &m, c, J);
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 3; e++) b->Mom[3 * d + e] = J[d][e] = J[d][e] * r->DX3 * rho_p;
+        if (jacobi3(J, V, ev)) return -1;
+        coordinate_frame(V[2], V[0], V[1]);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) R[j][i] = V[i][j];
+        quaternion q;
+        rm_to_qtn(&q, R);
+        for (int n = cum[k]; n < cum[k + 1]; n++) r->p[n].q = q;
+        /* Rigid_Moments_body = R^T I R (rigid_body.cxx:26-60); its diagonal */
+        for (int d = 0; d < 3; d++) {
+            double col[3] = {J[0][d], J[1][d], J[2][d]}, rc[3];
+            space2body(rc, col, &q);
+            for (int e = 0; e < 3; e++) R[e][d] = rc[e]; /* R := R^T I, by columns */
+        }
+        for (int d = 0; d < 3; d++) {
+            double row[3] = {R[d][0], R[d][1], R[d][2]}, rr[3];
+            space2body(rr, row, &q);
+            b->Ib[d] = rr[d];
+        }
+        for (int n = cum[k]; n < cum[k + 1]; n++) space2body(r->GRb[n], r->GR[n], &q);
+    }
+    /* init_set_vGs (rigid.h:230-240): the bodies' initial velocities (space
+     * frame; the reference starts them at rest) carried to the beads */
+    for (int k = 0; k < nb; k++)
+        for (int d = 0; d < 3; d++) {
+            r->body[k].vG[d] = v0 ? v0[3 * k + d] : 0.0;
+            r->body[k].wG[d] = w0 ? w0[3 * k + d] : 0.0;
+        }
+    rigid_bead_velocities(r);
+    return 0;
+}
+
+/* per body: xG, xG_nopbc, vG, omegaG, the previous step's hydrodynamic and
+ * LJ + gravity force and torque, mass, principal moments, orientation */
+void kr_get_rigid(kr_run *r, double *out) {
+    for (int k = 0; k < r->nb; k++) {
+        const kr_body  *b = &r->body[k];
+        const Particle *f = &r->p[r->bcum[k]];
+        double         *o = out + (size_t)KH_RIGID_WORDS * k;
+        for (int d = 0; d < 3; d++) {
+            o[d]      = b->xG[d];
+            o[3 + d]  = b->xGn[d];
+            o[6 + d]  = b->vG[d];
+            o[9 + d]  = b->wG[d];
+            o[12 + d] = b->fGp[d];
+            o[15 + d] = b->tGp[d];
+            o[18 + d] = b->fGrp[d];
+            o[21 + d] = b->tGrp[d];
+            o[25 + d] = b->Ib[d];
+        }
+        o[24] = b->M;
+        o[28] = f->q.s;
+        o[29] = f->q.v[0];
+        o[30] = f->q.v[1];
+        o[31] = f->q.v[2];
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -2293,6 +2800,11 @@ void kr_destroy(kr_run *r) {
     free(r->MOI);
     free(r->IMOI);
     free(r->slip_mode_half);
+    free(r->bcum);
+    free(r->bid);
+    free(r->body);
+    free(r->GR);
+    free(r->GRb);
     kr_fft3_free(&r->fft);
     free(r);
 }

@@ -96,6 +96,17 @@ int    kr_sekibun_cell(const kr_run *r, int *out3);
 void kr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void kr_noise_normals(uint64_t seed, uint32_t stream, int particle, int ts, double out6[6]);
 
+/* Rigid composite bodies (SW_PT == rigid): nb bodies, body k made of the
+ * beads cum[k] .. cum[k+1]-1 (Rigid_Particle_Cumul) of the particles already
+ * set, whose x_nopbc hold unwrapped coordinates of each body; free_flags[k]
+ * 1 "free", 0 "fix" (Rigid_motion); v0, w0 the bodies' initial velocities
+ * (nb x 3, space frame; NULL: at rest). Runs the initial block of
+ * init_particle.cxx:455-487. -1 for kT > 0, slip species, sheared runs or
+ * ROTATION off. */
+int  kr_set_rigid(kr_run *r, int nb, const int *cum, const int *free_flags, const double *v0, const double *w0);
+/* nb x KH_RIGID_WORDS doubles (the layout of kh_get_rigid) */
+void kr_get_rigid(kr_run *r, double *out);
+
 /* Planted mistakes, for the tests that show a pin catches one (CPU only).
  * kr_set_mutation(m) makes the oracle commit mistake m in its step from the
  * next call on (process-wide); KR_MUT_NONE restores it. Nothing but those

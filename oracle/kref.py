@@ -49,6 +49,9 @@ def lib():
         L.kr_philox4x32_10.argtypes = [vp, vp, vp]
         L.kr_noise_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, vp]
         L.kr_set_threads.argtypes = [C.c_int]
+        L.kr_set_rigid.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.kr_get_rigid.argtypes = [vp, vp]
+        L.kr_get_rigid.restype = None
         L.kr_set_mutation.argtypes = [C.c_int]
         L.kr_set_mutation.restype = None
         L.kr_set_shear.argtypes = [vp, C.c_double, C.c_int, C.c_double]
@@ -240,6 +243,24 @@ class Run:
         a = np.ascontiguousarray(a, dtype=np.float64).copy()
         lib().kr_A_k2a(self.h, a.ctypes.data)
         return a
+
+    def set_rigid(self, cum, free=None, v0=None, w0=None):
+        """Rigid bodies: body k = particles cum[k]..cum[k+1]-1 (kr_set_rigid)."""
+        cum = np.ascontiguousarray(cum, dtype=np.int32)
+        fr = None if free is None else np.ascontiguousarray(free, dtype=np.int32)
+        v = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64)
+        w = None if w0 is None else np.ascontiguousarray(w0, dtype=np.float64)
+        self._rigid = (cum, fr, v, w)
+        if lib().kr_set_rigid(self.h, len(cum) - 1, cum.ctypes.data, None if fr is None else fr.ctypes.data,
+                              None if v is None else v.ctypes.data, None if w is None else w.ctypes.data) != 0:
+            raise ValueError("kr_set_rigid rejected the bodies")
+        self.nb = len(cum) - 1
+
+    def get_rigid(self):
+        from kapsel3_amd._abi import RIGID_DTYPE
+        out = np.zeros(self.nb, dtype=RIGID_DTYPE)
+        lib().kr_get_rigid(self.h, out.ctypes.data)
+        return out
 
     def sekibun(self):
         n = lib().kr_sekibun_count(self.h)

@@ -5,6 +5,7 @@ recorded trajectory equals the states kh_step produces."""
 import numpy as np
 import pytest
 
+from oracle import kref
 from kapsel3_amd import serving, workloads
 from kapsel3_amd._abi import MIXED, FP64
 
@@ -29,24 +30,67 @@ def test_trajectory_record_matches_steps(gpu):
 
 
 def test_served_runs_match_alone(gpu):
-    print("served mix of 24 runs vs alone", flush=True)
-    reqs = serving.requests(24, seed=11, sizes=(32, 48, 64), pmax=12, precision=MIXED)
-    with gpu.Server(max_active=8, chunk_steps=4) as srv:
-        mix = [srv.submit(P, parts, 40) for P, parts in reqs]
+    """The serving workload's sizes (32, 48, 64 and 96^3, a = 4, up to 64
+    colloids, gravity) for its 200 steps: 24 runs in one mix, three of them
+    (one per size class up to 96^3) again alone."""
+    print("served mix of 24 runs (32-96^3) x 200 steps vs alone", flush=True)
+    reqs = serving.requests(24, seed=11, sizes=(32, 48, 64, 96), pmax=64, precision=MIXED)
+    with gpu.Server(max_active=8, chunk_steps=8) as srv:
+        mix = [srv.submit(P, parts, 200) for P, parts in reqs]
         for rid, _ in mix:
             srv.wait(rid)
         st = srv.stats()
     assert st["completed"] == 24 and st["failed"] == 0 and st["peak_active"] > 1
-    for k in (0, 7, 19):  # the same runs, each alone on a fresh server
+    sizes = [P.c.nx for P, _ in reqs]
+    picks = [sizes.index(n) for n in (32, 64, 96)]
+    for k in picks:  # the same runs, each alone on a fresh server
         P, parts = reqs[k]
+        print(f"  alone: run {k}, {P.c.nx}^3 x {len(parts)}", flush=True)
         with gpu.Server(max_active=1) as srv:
-            rid, alone = srv.submit(P, parts, 40)
+            rid, alone = srv.submit(P, parts, 200)
             srv.wait(rid)
         got = mix[k][1]
         vs = max(np.abs(alone[..., 3:]).max(), 1e-12)
         disp = max(np.abs(alone[..., :3] - parts["x"][None]).max(), 1e-12)
         assert np.abs(got[..., 3:] - alone[..., 3:]).max() <= 1e-5 * vs
         assert np.abs(got[..., :3] - alone[..., :3]).max() <= 1e-5 * disp
+
+
+def test_served_fp64_run_per_size_matches_oracle(gpu):
+    """One FP64 run of each serving size (32, 48, 64, 96^3; a = 4, xi = 2,
+    gravity 1e-3, seeded non-overlapping colloids) served together for 200
+    steps: every step's (x, v) against the oracle's trajectory of the same
+    run (Time_evolution_hydro, sp_3d_ns.cxx:41-194, looped as 857-956)."""
+    runs = []
+    for n, npart, seed in ((32, 24, 21), (48, 40, 22), (64, 64, 23), (96, 64, 24)):
+        P, parts = workloads.sedimentation(n, npart, a=4.0, kbt=0.0, precision=FP64, seed=seed, gap=1.0)
+        runs.append((P, parts))
+    print("served FP64 runs 32/48/64/96^3 x 200 steps", flush=True)
+    with gpu.Server(max_active=4, chunk_steps=8) as srv:
+        out = [srv.submit(P, parts, 200) for P, parts in runs]
+        for rid, _ in out:
+            srv.wait(rid)
+    for (P, parts), (_, traj) in zip(runs, out):
+        n = P.c.nx
+        print(f"  oracle {n}^3 x {len(parts)}", flush=True)
+        o = kref.Run(P, len(parts))
+        o.set_particles(parts)
+        o.init_fluid()
+        ref = np.zeros_like(traj)
+        for s in range(200):
+            o.step(1)
+            p = o.get_particles()
+            ref[s, :, :3], ref[s, :, 3:] = p["x"], p["v"]
+        o.close()
+        vs = np.abs(ref[..., 3:]).max()
+        d = ref[..., :3] - parts["x"][None]
+        d -= np.rint(d / n) * n
+        disp = np.abs(d).max()
+        dx = traj[..., :3] - ref[..., :3]
+        dx -= np.rint(dx / n) * n
+        assert vs > 0 and disp > 0
+        assert np.abs(traj[..., 3:] - ref[..., 3:]).max() < 1e-10 * vs, n
+        assert np.abs(dx).max() < 1e-10 * disp, n
 
 
 def test_admission_by_budget(gpu):

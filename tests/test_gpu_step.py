@@ -196,6 +196,19 @@ def test_thermal_shrunk_fp64(gpu, path):
     cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts)
 
 
+@PATHS
+def test_serving_size_96_fp64(gpu, path):
+    """The largest serving grid, 96^3 (radix-3 columns and a 96-point z
+    pass), with the serving physics (a = 4, xi = 2, 64 colloids, gravity
+    1e-3): 20 whole steps against the oracle."""
+    print(f"serving size 96^3 x 64, 20 steps {path}", flush=True)
+    P, parts = workloads.sedimentation(96, 64, a=4.0, kbt=0.0, gap=1.0, seed=24)
+    o, g = run_both(P, parts, 20, path=path)
+    cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=parts, a=4.0)
+    zo, zg = o.get_fluid(), g.get_fluid()
+    assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
+
+
 @pytest.mark.parametrize("n,np_,steps", [(128, 60, 6), (256, 40, 3)])
 def test_sedimentation_production_grid_fp64(gpu, n, np_, steps):
     """Sedimentation geometry (a = 5, LJ, gravity) at 128^3 and the full 256^3 grid: the z passes of

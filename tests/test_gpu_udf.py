@@ -8,6 +8,7 @@ import os
 import numpy as np
 import pytest
 
+from oracle import kref
 from kapsel3_amd import udf
 
 from test_gpu_restart import _close
@@ -71,3 +72,32 @@ def test_sheared_deck_and_udf_restart(gpu, tmp_path):
     assert a.shear_state() == b.shear_state()
     a.close()
     b.close()
+
+
+def test_udf_restart_deck_resumes_gpu_and_oracle_alike(gpu, tmp_path):
+    """One restart deck (written from a GPU run after 6 steps) restored into
+    a fresh GPU run and into the oracle (Force_restore_parameters,
+    resume.cxx:507-587): 10 more steps agree to 1e-10, AB2 history and the
+    noise counter included."""
+    from test_gpu_step import cmp_particles, rel
+    print("udf restart deck -> GPU and oracle, 10 steps", flush=True)
+    dk = udf.read_deck(DECK)
+    a = gpu.Run(dk.params, dk.nparticles)
+    a.set_particles(udf.deck_particles(dk))
+    a.init_fluid()
+    a.step(6)
+    out = tmp_path / "restart.udf"
+    udf.save_restart_udf(a, DECK, out, include=os.path.join(HERE, "golden", "udf", "define.udf"))
+    a.close()
+    dk2 = udf.read_deck(out)
+    g = gpu.Run(dk2.params, dk2.nparticles)
+    o = kref.Run(dk2.params, dk2.nparticles)
+    assert udf.restore_deck_state(g, dk2) and udf.restore_deck_state(o, dk2)
+    p0 = o.get_particles()
+    g.step(10)
+    o.step(10)
+    assert g.get_time().ts == o.get_time().ts == 16
+    cmp_particles(g.get_particles(), o.get_particles(), 1e-10, p0=p0, a=4.0)
+    zg, zo = g.get_fluid(), o.get_fluid()
+    assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
+    g.close()
