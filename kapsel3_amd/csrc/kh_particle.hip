@@ -1165,25 +1165,27 @@ hipError_t phi_scatter(const PParams &pp, DevParticle *P, T *phisum, const PCach
 template <typename T>
 hipError_t f_hydro(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
                    double coef, hipStream_t s) {
-    if (pp.np > 0)
+    if (pp.np > 0) {
         if (pp.obl)
             hipLaunchKernelGGL((k_f_hydro<T, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum,
                                pc, coef);
         else
             hipLaunchKernelGGL((k_f_hydro<T, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], phisum,
                                pc, coef);
+    }
     return hipGetLastError();
 }
 template <typename T>
 hipError_t up_scatter(const PParams &pp, const DevParticle *P, const T *phisum, const PCache<T> &pc,
                       T *const up[3], hipStream_t s) {
-    if (pp.np > 0)
+    if (pp.np > 0) {
         if (pp.obl)
             hipLaunchKernelGGL((k_up_scatter<T, true>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, up[0],
                                up[1], up[2]);
         else
             hipLaunchKernelGGL((k_up_scatter<T, false>), dim3(pp.np), dim3(PT_NT), 0, s, pp, P, phisum, pc, up[0],
                                up[1], up[2]);
+    }
     return hipGetLastError();
 }
 hipError_t position(const PParams &pp, DevParticle *P, int ab2, double dt, double hdt, hipStream_t s) {
