@@ -460,8 +460,8 @@ struct Step {
         cpx<T>  *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
         Comps<T> c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
         // buffer of each advection product (u2u3, u3u1, u1u2, u2^2-u3^2,
-        // u3^2-u1^2): z_adv reads u1, u2, u3 from S0, S1, S2 (u2 twice) and
-        // writes each product only into a buffer it has finished reading
+        // u3^2-u1^2): z_adv reads u1, u2, u3 from S0, S1, S2 and writes
+        // each product only into a buffer it has finished reading
         cpx<T>  *q5[5] = {S(r, 1), S(r, 2), S(r, 0), S(r, 4), S(r, 3)};
         Comps<T> cq5{{q5[0], q5[1], q5[2], q5[3], q5[4]}};
         // NS_solver_slavedEuler: zeta -> u -> products -> -Omega -> slaved Euler

@@ -280,11 +280,13 @@ __global__ __launch_bounds__(256) void k_zw_c2r_cross(ZIO<T> io, const T *__rest
 
 // Advection products (U2advection_k, operate_omega.cxx:31-47): inverse z
 // transforms of u1 (s0), u2 (s1), u3 (s2), the five products in registers,
-// forward z transforms into p0..p4. Two components are live at a time, so u2
+// forward z transforms into p0..p4. With LIVE3 all three components stay in
+// registers and each is read once; otherwise two are live at a time, so u2
 // is read and transformed twice; the order is u1 u2 -> p2, then u3 -> p1, p4,
-// then u2 again -> p0, p3. A product buffer may therefore be an input buffer
-// that is no longer read: p2 = s0, p1 = s2, p0 = s1 are allowed.
-template <typename T, int L, int E>
+// then u2 again -> p0, p3. Either way a line's inputs are read before its
+// products are written, so a product buffer may be an input buffer:
+// p2 = s0, p1 = s2, p0 = s1 are allowed.
+template <typename T, int L, int E, bool LIVE3>
 __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict__ s0, const cpx<T> *__restrict__ s1,
                                                 const cpx<T> *__restrict__ s2, cpx<T> *__restrict__ p0,
                                                 cpx<T> *__restrict__ p1, cpx<T> *__restrict__ p2,
@@ -298,6 +300,43 @@ __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, (long long)g.nx * g.ny, g.nzp);
     const int            kzv = g.trn_z;
+    if constexpr (LIVE3) {
+        cpx<T> a[E], b[E], c[E], v[E];
+        load_pair<T, L, Fi>(s0, wl, kzv, a);
+        load_pair<T, L, Fi>(s1, wl, kzv, b);
+        load_pair<T, L, Fi>(s2, wl, kzv, c);
+        Fi::run(a, wl.buf, tw, wl.g);
+        Fi::run(b, wl.buf, tw, wl.g);
+        Fi::run(c, wl.buf, tw, wl.g);
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            a[i] = cscale(a[i], scale);
+            b[i] = cscale(b[i], scale);
+            c[i] = cscale(c[i], scale);
+            v[i] = cmk(a[i].x * b[i].x, a[i].y * b[i].y);  // u1 u2
+        }
+        Ff::run(v, wl.buf, tw, wl.g);
+        store_pair<T, L, Ff>(v, wl, kzv, p2);
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = cmk(c[i].x * a[i].x, c[i].y * a[i].y);  // u3 u1
+        Ff::run(v, wl.buf, tw, wl.g);
+        store_pair<T, L, Ff>(v, wl, kzv, p1);
+#pragma unroll
+        for (int i = 0; i < E; i++)  // u3^2 - u1^2
+            v[i] = cmk((c[i].x - a[i].x) * (c[i].x + a[i].x), (c[i].y - a[i].y) * (c[i].y + a[i].y));
+        Ff::run(v, wl.buf, tw, wl.g);
+        store_pair<T, L, Ff>(v, wl, kzv, p4);
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = cmk(b[i].x * c[i].x, b[i].y * c[i].y);  // u2 u3
+        Ff::run(v, wl.buf, tw, wl.g);
+        store_pair<T, L, Ff>(v, wl, kzv, p0);
+#pragma unroll
+        for (int i = 0; i < E; i++)  // u2^2 - u3^2
+            v[i] = cmk((b[i].x - c[i].x) * (b[i].x + c[i].x), (b[i].y - c[i].y) * (b[i].y + c[i].y));
+        Ff::run(v, wl.buf, tw, wl.g);
+        store_pair<T, L, Ff>(v, wl, kzv, p3);
+        return;
+    }
     cpx<T> ua[E], ub[E], v[E];
     load_pair<T, L, Fi>(s0, wl, kzv, ua);
     Fi::run(ua, wl.buf, tw, wl.g);
@@ -425,11 +464,21 @@ bool zw_supported(int L) { return wave_fft_ok(L); }
 #ifndef KH_PASS_E32
 #define KH_PASS_E32 8
 #endif
+#ifndef KH_ADV_E64
+#define KH_ADV_E64 8
+#endif
+// the advection pass keeps all three components live (each read once) at
+// this many points per lane or fewer: at 8 (FP32 and FP64) the extra
+// registers cost less than the second read and transform of u2
+#ifndef KH_ADV_LIVE3_E
+#define KH_ADV_LIVE3_E 8
+#endif
 template <typename T, int L>
 struct ZwE {
-    static constexpr int pass = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 8>::value;
-    static constexpr int adv  = WaveE<T, L, sizeof(T) == 4 ? KH_ADV_E32 : 8>::value;
-    static constexpr int force = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 4>::value;
+    static constexpr int  pass  = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 8>::value;
+    static constexpr int  adv   = WaveE<T, L, sizeof(T) == 4 ? KH_ADV_E32 : KH_ADV_E64>::value;
+    static constexpr bool live3 = adv <= KH_ADV_LIVE3_E;
+    static constexpr int  force = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 4>::value;
 };
 
 template <typename T, int L, int E>
@@ -506,7 +555,7 @@ hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T sca
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                \
     case n:                                                                                                       \
-        hipLaunchKernelGGL((k_zw_adv<T, n, ZwE<T, n>::adv>), dim3(zw_blocks<T, n, ZwE<T, n>::adv>(nrows)), dim3(256), 0, s, g, s3[0], s3[1], s3[2], \
+        hipLaunchKernelGGL((k_zw_adv<T, n, ZwE<T, n>::adv, ZwE<T, n>::live3>), dim3(zw_blocks<T, n, ZwE<T, n>::adv>(nrows)), dim3(256), 0, s, g, s3[0], s3[1], s3[2], \
                            p5[0], p5[1], p5[2], p5[3], p5[4], scale);                                             \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)

@@ -83,9 +83,19 @@ def test_rigid_steps_against_oracle(gpu, path, compat):
     bo, bg = o.get_rigid(), g.get_rigid()
     assert np.abs(bo["f_r"]).max() > 0          # the bodies touch
     cmp_bodies(bg, bo, 1e-10)
-    cmp_particles(g.get_particles(), o.get_particles(), 1e-10,
-                  ["x", "v", "omega", "q", "q_old", "v_old", "omega_old", "fr_previous", "f_hydro_previous"],
-                  p0=p0, a=4.0)
+    pg, po = g.get_particles(), o.get_particles()
+    cmp_particles(pg, po, 1e-10,
+                  ["v", "omega", "q", "q_old", "v_old", "omega_old", "fr_previous", "f_hydro_previous"], a=4.0)
+    # bead positions against the beads' own displacement (the rigid position
+    # update places beads by x only and leaves x_nopbc as it was,
+    # update_Particle_Configuration, rigid.h:168-185)
+    d0 = po["x"] - p0["x"]
+    d0 -= np.rint(d0 / N) * N
+    dx = pg["x"] - po["x"]
+    dx -= np.rint(dx / N) * N
+    assert np.abs(d0).max() > 1e-3
+    assert np.abs(dx).max() <= 1e-10 * np.abs(d0).max()
+    assert np.array_equal(pg["x_nopbc"], po["x_nopbc"])
     zo, zg = o.get_fluid(), g.get_fluid()
     assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10
 
