@@ -269,6 +269,21 @@ int kh_sync(kh_run *run);
 int kh_A2a_k(kh_run *run, double *a);
 int kh_A_k2a(kh_run *run, double *a);
 
+/* kh_A2a_k without a run and without any device allocation. No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). FP64 forward transform, in place, of a host array in the
+ * reference layout (NX*NY*NZ_ doubles, NZ_ = NZ + 2), the result as
+ * kh_A2a_k gives it. Its staging (the real field and its spectrum, FP64,
+ * spectral rows padded to 128 bytes) lives in a fixed device array of
+ * KH_NOALLOC_MAX_BYTES that each GPU holds until the process ends; larger
+ * grids (more than 128^3 or so) return KH_ERR_UNSUPPORTED. Runs on the
+ * current device's null stream and returns when the result is in `a`.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race. */
+#define KH_NOALLOC_MAX_BYTES (64ll << 20)
+int kh_A2a_k_noalloc(double *a, int nx, int ny, int nz);
+
 
 /* -------- measurement -------- */
 /* When enabled, every kernel launch of the run is bracketed by HIP events

@@ -27,7 +27,7 @@ EXPORTS = [
     "kh_set_output", "kh_get_output", "kh_run_bytes", "kh_query", "kh_server_create", "kh_server_submit", "kh_server_poll", "kh_server_wait",
     "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy", "kh_set_shear", "kh_get_shear_state",
     "kh_set_shear_state", "kh_get_shear_record", "kh_set_shear_history", "kh_get_shear_history",
-    "kh_server_submit_output", "kh_set_rigid", "kh_get_rigid",
+    "kh_server_submit_output", "kh_set_rigid", "kh_get_rigid", "kh_A2a_k_noalloc",
 ]
 
 # fields of the sheared step's stress record (Mean_shear_stress, sp_3d_ns.h:247-330)
@@ -89,6 +89,7 @@ def lib():
         L.kh_set_graph.argtypes = [vp, C.c_int]
         L.kh_A2a_k.argtypes = [vp, vp]
         L.kh_A_k2a.argtypes = [vp, vp]
+        L.kh_A2a_k_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.kh_set_rigid.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.kh_get_rigid.argtypes = [vp, vp, C.c_int]
         L.kh_timing_enable.argtypes = [vp, C.c_int]
@@ -131,6 +132,18 @@ def _check(status):
         L = lib()
         raise KhError(f"{L.kh_strerror(status).decode()}: {L.kh_last_error().decode()}")
     return status
+
+
+def A2a_k_noalloc(a):
+    """Forward FP64 transform of a host array in the reference layout
+    (NX, NY, NZ+2) without a run and without any device allocation
+    (kh_A2a_k_noalloc: fixed device scratch of KH_NOALLOC_MAX_BYTES)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    if a.ndim != 3:
+        raise KhError("A2a_k_noalloc takes an (NX, NY, NZ+2) array")
+    nx, ny, nz2 = a.shape
+    _check(lib().kh_A2a_k_noalloc(a.ctypes.data, nx, ny, nz2 - 2))
+    return a
 
 
 class Run:

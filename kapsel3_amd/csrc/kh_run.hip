@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -2036,6 +2037,86 @@ int kh_timing_count(kh_run *r) {
 const char *kh_timing_name(kh_run *r, int i) {
     if (!r || i < 0 || i >= (int)r->names.size()) return nullptr;
     return r->names[i].c_str();
+}
+
+// ------------------------------------------------- allocation-free transform
+// kh_A2a_k_noalloc: the forward transform of kh_A2a_k without a run and
+// without any device allocation. Its staging (the real field and its
+// spectrum) lives in `scratch`, a file-scope device array of fixed size that
+// each GPU the library runs on holds until the process ends. Host code never
+// passes the array's name to a kernel or a copy (on the host it is only a
+// stand-in): the device address is taken once per device with
+// hipGetSymbolAddress and kept in noalloc_ptr. Every call on one GPU uses
+// the same array, so calls that overlap on one GPU race.
+}  // extern "C"
+
+namespace {
+constexpr size_t KH_NOALLOC_MAX = KH_NOALLOC_MAX_BYTES;  // bytes of scratch per device
+constexpr int    KH_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[KH_NOALLOC_MAX];
+std::mutex noalloc_mu;
+void      *noalloc_ptr[KH_MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device (taken on the first
+// call there)
+int noalloc_scratch(void **out) {
+    int dev = 0;
+    KH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= KH_MAX_DEVICES) return fail(KH_ERR_UNSUPPORTED, "device %d beyond %d", dev, KH_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        KH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(KH_ERR_HIP, "hipGetSymbolAddress returned a null address");
+        noalloc_ptr[dev] = p;
+    }
+    *out = noalloc_ptr[dev];
+    return KH_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int kh_A2a_k_noalloc(double *a, int nx, int ny, int nz) {
+    if (!a) return fail(KH_ERR_ARG, "NULL argument");
+    if (nx < 2 || ny < 2 || nz < 2 || (nz & 1))
+        return fail(KH_ERR_ARG, "grid %d x %d x %d: sizes must be at least 2, NZ even", nx, ny, nz);
+    if (!fft_length_supported(nx) || !fft_length_supported(ny) || !fft_length_supported(nz))
+        return fail(KH_ERR_UNSUPPORTED, "grid %d x %d x %d: lengths must be 2^a 3^b from 8 to 512", nx, ny, nz);
+    Grid g{};
+    g.nx  = nx;
+    g.ny  = ny;
+    g.nz  = nz;
+    g.nzh = nz / 2 + 1;
+    g.nzp = (g.nzh + 7) / 8 * 8;  // FP64 spectral rows on 128-byte boundaries
+    g.hnx = nx / 2;
+    g.hny = ny / 2;
+    g.trn_x = (nx + 2) / 3;
+    g.trn_y = (ny + 2) / 3;
+    g.trn_z = (nz + 2) / 3;
+    g.wave_x = 2. * KH_PI / nx;
+    g.wave_y = 2. * KH_PI / ny;
+    g.wave_z = 2. * KH_PI / nz;
+    g.nreal  = (long long)nx * ny * nz;
+    g.nspec  = (long long)nx * ny * g.nzp;
+    const size_t real_bytes = ((size_t)g.nreal * sizeof(double) + 255) / 256 * 256;
+    const size_t need       = real_bytes + (size_t)g.nspec * sizeof(cpx<double>);
+    if (need > KH_NOALLOC_MAX)
+        return fail(KH_ERR_UNSUPPORTED, "grid %d x %d x %d needs %zu bytes of scratch, more than the %zu available",
+                    nx, ny, nz, need, KH_NOALLOC_MAX);
+    void *base = nullptr;
+    KH_TRY(noalloc_scratch(&base));
+    double      *real = (double *)base;
+    cpx<double> *spec = (cpx<double> *)((char *)base + real_bytes);
+    const int    nz_  = 2 * g.nzh;
+    hipStream_t  s    = nullptr;  // the null stream: nothing is created
+    KH_HIP(hipMemcpy2DAsync(real, sizeof(double) * nz, a, sizeof(double) * nz_, sizeof(double) * nz,
+                            (size_t)nx * ny, hipMemcpyHostToDevice, s));
+    KH_HIP(fft3d_r2c<double>(g, real, spec, s));
+    KH_HIP(hipMemcpy2DAsync(a, sizeof(double) * nz_, spec, sizeof(cpx<double>) * g.nzp, sizeof(double) * nz_,
+                            (size_t)nx * ny, hipMemcpyDeviceToHost, s));
+    KH_HIP(hipStreamSynchronize(s));
+    return KH_OK;
 }
 
 }  // extern "C"
