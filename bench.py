@@ -167,7 +167,8 @@ def accuracy(mixed, fp64, steps, a):
 def squirmer_bench(steps, n=256, count=1000):
     """1000 squirmers at 256^3 (B1 = 0.01, B2/B1 uniform in [-2, 2], random
     orientations) at 256^3, steps/s of the whole step with the slip
-    iteration, mixed and FP64 (after 2 warm-up steps)."""
+    iteration, mixed and FP64 (after 10 warm-up steps), and the mean number
+    of slip iterations per step in the timed steps."""
     import kapsel3_amd
     from kapsel3_amd import workloads
     from kapsel3_amd._abi import FP64, MIXED
@@ -178,15 +179,18 @@ def squirmer_bench(steps, n=256, count=1000):
         run = kapsel3_amd.Run(P, count)
         run.set_particles(parts)
         run.init_fluid()
-        run.step(2)
+        run.step(10)  # the Euler step, then both captured graphs (8 steps and 1) are built
         run.sync()
+        it0, lo0 = run.slip_iterations()
         t0 = time.perf_counter()
         run.step(steps)
         run.sync()
         t = time.perf_counter() - t0
+        it1, lo1 = run.slip_iterations()
         p = run.get_particles()
         run.close()
         out[name] = {"value": steps / t, "unit": "steps/s", "ms_per_step": t / steps * 1e3,
+                     "mean_slip_iterations": (it1 - it0) / max(lo1 - lo0, 1),
                      "finite": bool(np.isfinite(p["v"]).all())}
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out
@@ -371,7 +375,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="run the timed steps uncaptured")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-squirm", action="store_true", help="skip the squirmer suspension")
-    ap.add_argument("--squirm-steps", type=int, default=10)
+    ap.add_argument("--squirm-steps", type=int, default=16)
     ap.add_argument("--no-shear", action="store_true", help="skip the Lees-Edwards sheared suspension")
     ap.add_argument("--shear-steps", type=int, default=20)
     ap.add_argument("--shear-only", action="store_true",

@@ -171,6 +171,12 @@ int kh_step(kh_run *run, int nsteps);
  * Both give the reference's step to round-off. */
 enum { KH_PATH_FUSED = 0, KH_PATH_REFERENCE_ORDER = 1 };
 int kh_set_step_path(kh_run *run, int path);
+
+/* Squirmer runs: the slip iterations (Time_evolution_hydro's loop,
+ * sp_3d_ns.cxx:121-161) run so far in total and the number of slip loops
+ * (steps) they were spread over, since the run was created. A diagnostic
+ * with no counterpart in the reference. */
+int kh_get_slip_iterations(kh_run *run, long long *iterations, long long *loops);
 /* Lees-Edwards shear (SW_EQ == Shear_Navier_Stokes_Lees_Edwards): from the
  * next step on kh_step runs Time_evolution_hydro_OBL (sp_3d_ns.cxx:328-433),
  * the sheared step on an oblique grid that follows the mean flow

@@ -28,6 +28,7 @@ EXPORTS = [
     "kh_server_run_times", "kh_server_get_stats", "kh_server_destroy", "kh_set_shear", "kh_get_shear_state",
     "kh_set_shear_state", "kh_get_shear_record", "kh_set_shear_history", "kh_get_shear_history",
     "kh_server_submit_output", "kh_set_rigid", "kh_get_rigid", "kh_A2a_k_noalloc",
+    "kh_get_slip_iterations",
 ]
 
 # fields of the sheared step's stress record (Mean_shear_stress, sp_3d_ns.h:247-330)
@@ -90,6 +91,7 @@ def lib():
         L.kh_A2a_k.argtypes = [vp, vp]
         L.kh_A_k2a.argtypes = [vp, vp]
         L.kh_A2a_k_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.kh_get_slip_iterations.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.kh_set_rigid.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.kh_get_rigid.argtypes = [vp, vp, C.c_int]
         L.kh_timing_enable.argtypes = [vp, C.c_int]
@@ -356,6 +358,12 @@ class Run:
         out = np.zeros(self.nb, dtype=RIGID_DTYPE)
         _check(lib().kh_get_rigid(self.h, out.ctypes.data, self.nb))
         return out
+
+    def slip_iterations(self):
+        """(total slip iterations, slip loops) run so far (squirmer runs)."""
+        it, lo = C.c_longlong(0), C.c_longlong(0)
+        _check(lib().kh_get_slip_iterations(self.h, C.byref(it), C.byref(lo)))
+        return it.value, lo.value
 
     def A2a_k(self, a):
         a = np.ascontiguousarray(a, dtype=np.float64).copy()

@@ -66,7 +66,8 @@ struct ZTile<float> {
 // out_mask: such rows are not written.
 template <typename T, int L, bool FWD, bool XMODE>
 __global__ __launch_bounds__(PassThreads<T>::value) void k_cols(Comps<T> comps, Grid g, int ncols, int in_mask,
-                                                                int out_mask) {
+                                                                int out_mask, const int *__restrict__ skip) {
+    if (skip && *skip) return;
     cpx<T> *__restrict__ data = comps.p[blockIdx.y];  // one component per grid row
     constexpr int NT  = PassThreads<T>::value;
     constexpr int C   = ColTile<T>::C;
@@ -551,6 +552,124 @@ __global__ __launch_bounds__(XTile<T>::NT) void k_x_f_zeta(Grid g, cpx<T> *__res
     }
 }
 
+// The slip iteration's projection (Solenoidal_u, operate_omega.h:105-112,
+// which truncates to the 2/3 modes) less its z and y passes: forward x of
+// the slip force field's three components on the kept (ky,kz) columns; per
+// kept mode truncation + Solenoidal_uk (operate_omega.cxx:171-195) and the
+// curl of the projected field (U_k2zeta_k, fft_wrapper.cxx:407-476) into
+// (a0, a1), its k = 0 mode into dc; modes beyond the 2/3 rule set to zero;
+// then the inverse x pass of each projected component, in place.
+template <typename T, int L>
+__global__ __launch_bounds__(XTile<T>::NT) void k_x_slip(Grid g, cpx<T> *__restrict__ f0, cpx<T> *__restrict__ f1,
+                                                        cpx<T> *__restrict__ f2, cpx<T> *__restrict__ a0,
+                                                        cpx<T> *__restrict__ a1, double *__restrict__ dc,
+                                                        const int *__restrict__ skip) {
+    constexpr int NT  = XTile<T>::NT;
+    constexpr int C   = XTile<T>::C;
+    constexpr int BS  = ColStride<T, C>::value;
+    constexpr int NPT = (C * L + NT - 1) / NT;
+    if (skip && *skip) return;
+    __shared__ cpx<T> s[L * BS], tw[L];
+    fill_twiddles<T, L, NT>(tw);
+    const int ntiles = (g.trn_z + C - 1) / C;
+    const int o = blockIdx.x / ntiles, kz0 = (blockIdx.x - o * ntiles) * C;
+    const int j = kept_index(o, g.ny, g.trn_y), iky = (j > g.hny) ? j - g.ny : j;
+    const long long lstride = (long long)g.ny * g.nzp;
+    const long long base    = (long long)j * g.nzp + kz0;
+    cpx<T>         *F[3]    = {f0, f1, f2};
+    cpx<T>          Fh[3][NPT];
+    for (int d = 0; d < 3; d++) {
+        cpx<T> v[NPT];
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            const int n = e / C, c = e - n * C;
+            v[i]        = (e < C * L && kz0 + c < g.trn_z) ? F[d][base + n * lstride + c] : cmk((T)0, (T)0);
+        }
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            if (e < C * L) s[(e / C) * BS + e % C] = v[i];
+        }
+        __syncthreads();
+        fft_cols_ip<T, L, C, BS, NT, true>(s, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int p = threadIdx.x + i * NT;
+            if (p < C * L) Fh[d][i] = s[(p / C) * BS + p % C];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int p = threadIdx.x + i * NT;
+        if (p >= C * L) continue;
+        const int n = p / C, c = p - n * C, kz = kz0 + c;
+        const int ikx = (n > g.hnx) ? n - g.nx : n;
+        if (kz >= g.trn_z) continue;
+        if (!kept1(ikx, g.trn_x)) {
+            Fh[0][i] = Fh[1][i] = Fh[2][i] = cmk((T)0, (T)0);
+            continue;
+        }
+        const long long idx = base + n * lstride + c;
+        const double    kx = ikx * g.wave_x, ky = iky * g.wave_y, kzz = kz * g.wave_z;
+        const double    K2  = (g.wave_x * ikx) * (g.wave_x * ikx) + (g.wave_y * iky) * (g.wave_y * iky) +
+                          (g.wave_z * kz) * (g.wave_z * kz);
+        const double IK2 = (K2 > 0.0) ? 1. / K2 : 0.0;
+        // Solenoidal_uk, stored in T as the reference's field is
+        const cpx<T> a = Fh[0][i], b = Fh[1][i], cc = Fh[2][i];
+        const double dr = IK2 * ((double)a.x * kx + (double)b.x * ky + (double)cc.x * kzz);
+        const double di = IK2 * ((double)a.y * kx + (double)b.y * ky + (double)cc.y * kzz);
+        const cpx<T> pa = cmk((T)(a.x - dr * kx), (T)(a.y - di * kx));
+        const cpx<T> pb = cmk((T)(b.x - dr * ky), (T)(b.y - di * ky));
+        const cpx<T> pc = cmk((T)(cc.x - dr * kzz), (T)(cc.y - di * kzz));
+        Fh[0][i] = pa;
+        Fh[1][i] = pb;
+        Fh[2][i] = pc;
+        if (ikx == 0 && iky == 0 && kz == 0) {
+            dc[0] = pa.x;
+            dc[1] = pb.x;
+            dc[2] = pc.x;
+        }
+        double ore[3], oim[3];
+        ore[0] = -(ky * (double)pc.y - kzz * (double)pb.y);
+        oim[0] = (ky * (double)pc.x - kzz * (double)pb.x);
+        ore[1] = -(kzz * (double)pa.y - kx * (double)pc.y);
+        oim[1] = (kzz * (double)pa.x - kx * (double)pc.x);
+        ore[2] = -(kx * (double)pb.y - ky * (double)pa.y);
+        oim[2] = (kx * (double)pb.x - ky * (double)pa.x);
+        int c0, c1;
+        if (ikx != 0) {
+            c0 = 1;
+            c1 = 2;
+        } else if (iky != 0) {
+            c0 = 2;
+            c1 = 0;
+        } else {
+            c0 = 0;
+            c1 = 1;
+        }
+        a0[idx] = cmk((T)ore[c0], (T)oim[c0]);
+        a1[idx] = cmk((T)ore[c1], (T)oim[c1]);
+    }
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int p = threadIdx.x + i * NT;
+            if (p < C * L) s[(p / C) * BS + p % C] = Fh[d][i];
+        }
+        __syncthreads();
+        fft_cols_ip<T, L, C, BS, NT, false>(s, tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            const int e = threadIdx.x + i * NT;
+            const int n = e / C, c = e - n * C;
+            if (e < C * L && kz0 + c < g.trn_z) F[d][base + n * lstride + c] = s[n * BS + c];
+        }
+        __syncthreads();
+    }
+}
+
 // zero zeta outside the kept modes (the reference's state there is
 // round-off after U2zeta_k; here it is exactly zero)
 template <typename T>
@@ -568,13 +687,22 @@ __global__ void k_zeta_clean(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict
 
 // zeta += (a0, a1) and uk_dc += dc: the projected slip field's vorticity
 // and mean velocity, added after the slip iteration.
+// With kept_only, (a0, a1) are read at the 2/3-kept modes only (they hold
+// nothing else).
 template <typename T>
 __global__ void k_zeta_add(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1, const cpx<T> *__restrict__ a0,
-                           const cpx<T> *__restrict__ a1, double *__restrict__ ukdc, const double *__restrict__ dc) {
+                           const cpx<T> *__restrict__ a1, double *__restrict__ ukdc, const double *__restrict__ dc,
+                           int kept_only) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx == 0)
         for (int d = 0; d < 3; d++) ukdc[d] += dc[d];
     if (idx >= g.nspec || (int)(idx % g.nzp) >= g.nzh) return;
+    if (kept_only) {
+        const int       kz   = (int)(idx % g.nzp);
+        const long long rest = idx / g.nzp;
+        const int       j = (int)(rest % g.ny), i = (int)(rest / g.ny);
+        if (!kept_mode(g, kx_of(g, i), ky_of(g, j), kz)) return;
+    }
     z0[idx] = cadd(z0[idx], a0[idx]);
     z1[idx] = cadd(z1[idx], a1[idx]);
 }
@@ -585,14 +713,15 @@ __global__ void k_zeta_add(Grid g, cpx<T> *__restrict__ z0, cpx<T> *__restrict__
 static inline int nky(const Grid &g) { return 2 * g.trn_y - 1; }
 
 template <typename T, bool FWD>
-hipError_t cols_y(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, int out_mask, hipStream_t s) {
+hipError_t cols_y(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, int out_mask, hipStream_t s,
+                  const int *skip) {
     constexpr int C  = ColTile<T>::C;
     const int     nt = (g.trn_z + C - 1) / C;
     switch (g.ny) {
 #define KH_CASE(n)                                                                                                 \
     case n:                                                                                                        \
         hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt, ncomp), dim3(PassThreads<T>::value), 0, s, \
-                           data, g, g.trn_z, in_mask, out_mask);                                                   \
+                           data, g, g.trn_z, in_mask, out_mask, skip);                                             \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
 #undef KH_CASE
@@ -608,7 +737,7 @@ hipError_t cols_x(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, i
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
         hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt, ncomp), dim3(PassThreads<T>::value), 0, s, \
-                           data, g, g.trn_z, in_mask, out_mask);                                                    \
+                           data, g, g.trn_z, in_mask, out_mask, nullptr);                                           \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
 #undef KH_CASE
@@ -656,10 +785,10 @@ bool z_force_clears(const Grid &g) { return zw_supported(g.nz); }
 
 template <typename T>
 hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                   int *tick, hipStream_t s) {
+                   int *tick, hipStream_t s, T *const us2[3], const int *skip) {
     const long long nrows = (long long)g.nx * g.ny;
-    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, tick, s);
-    if (clear || tick) return hipErrorInvalidValue;
+    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, tick, s, us2, skip);
+    if (clear || tick || us2 || skip || !us) return hipErrorInvalidValue;
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                   \
     case n: {                                                                                                        \
@@ -708,10 +837,27 @@ hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, d
 }
 
 template <typename T>
+hipError_t x_slip(const Grid &g, cpx<T> *const f[3], cpx<T> *a0, cpx<T> *a1, double *dc, const int *skip,
+                  hipStream_t s) {
+    constexpr int C  = XTile<T>::C;
+    const int     nt = (g.trn_z + C - 1) / C;
+    switch (g.nx) {
+#define KH_CASE(n)                                                                                                  \
+    case n:                                                                                                         \
+        hipLaunchKernelGGL((k_x_slip<T, n>), dim3(nky(g) * nt), dim3(XTile<T>::NT), 0, s, g, f[0], f[1], f[2], a0, a1, \
+                           dc, skip);                                                                               \
+        return hipGetLastError();
+        KH_FOR_EACH_L(KH_CASE)
+#undef KH_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
 hipError_t zeta_add(const Grid &g, cpx<T> *z0, cpx<T> *z1, const cpx<T> *a0, const cpx<T> *a1, double *ukdc,
-                    const double *dc, hipStream_t s) {
+                    const double *dc, hipStream_t s, int kept_only) {
     hipLaunchKernelGGL(k_zeta_add<T>, dim3((unsigned)((g.nspec + 255) / 256)), dim3(256), 0, s, g, z0, z1, a0, a1, ukdc,
-                       dc);
+                       dc, kept_only);
     return hipGetLastError();
 }
 
@@ -722,20 +868,22 @@ hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s) {
 }
 
 #define KH_INST(T)                                                                                                  \
-    template hipError_t cols_y<T, true>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);                \
-    template hipError_t cols_y<T, false>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);               \
+    template hipError_t cols_y<T, true>(const Grid &, const Comps<T> &, int, int, int, hipStream_t, const int *);   \
+    template hipError_t cols_y<T, false>(const Grid &, const Comps<T> &, int, int, int, hipStream_t, const int *);  \
     template hipError_t cols_x<T, true>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);                \
     template hipError_t cols_x<T, false>(const Grid &, const Comps<T> &, int, int, int, hipStream_t);               \
     template hipError_t x_zeta2u_inv<T>(const Grid &, const cpx<T> *, const cpx<T> *, cpx<T> *const[3], double *,   \
                                         int, hipStream_t);                                                          \
     template hipError_t z_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], hipStream_t);                   \
     template hipError_t z_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,          \
-                                   hipStream_t);                                                                   \
+                                   hipStream_t, T *const[3], const int *);                                         \
+    template hipError_t x_slip<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, const int *,          \
+                                  hipStream_t);                                                                    \
     template hipError_t x_f_zeta<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, hipStream_t);    \
     template hipError_t x_adv_ns<T>(const Grid &, const Comps<T> &, cpx<T> *, cpx<T> *, double, double, hipStream_t); \
     template hipError_t zeta_clean<T>(const Grid &, cpx<T> *, cpx<T> *, hipStream_t);                               \
     template hipError_t zeta_add<T>(const Grid &, cpx<T> *, cpx<T> *, const cpx<T> *, const cpx<T> *, double *,      \
-                                    const double *, hipStream_t);
+                                    const double *, hipStream_t, int);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST

@@ -94,16 +94,38 @@ template <typename T>
 hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
                          double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream, const int *d_ts,
                          double niv, double nio, T *const up[3], hipStream_t s);
-// squirmer slip iteration (sp_3d_ns.cxx:121-161)
+// squirmer slip iteration (sp_3d_ns.cxx:121-161). A non-null `skip` makes
+// a launch do nothing once *skip is nonzero: the loop decided on the device
+// (slip_conv with a SlipLoop) launches every iteration up to the maximum,
+// and those after convergence return at once.
 template <typename T>
 hipError_t slip_moments(const PParams &pp, DevParticle *P, T *const u[3], hipStream_t s);
 template <typename T>
-hipError_t slip_scatter(const PParams &pp, DevParticle *P, T *const u[3], T *const up[3], hipStream_t s);
+hipError_t slip_scatter(const PParams &pp, DevParticle *P, T *const u[3], T *const up[3], hipStream_t s,
+                        const int *skip = nullptr);
 template <typename T>
-hipError_t f_slip(const PParams &pp, DevParticle *P, T *const sf[3], double coef, hipStream_t s);
-hipError_t velocity_slip(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s);
+hipError_t f_slip(const PParams &pp, DevParticle *P, T *const sf[3], double coef, hipStream_t s,
+                  const int *skip = nullptr);
+hipError_t velocity_slip(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s,
+                         const int *skip = nullptr);
 hipError_t reset_forces(const PParams &pp, DevParticle *P, hipStream_t s);
-hipError_t slip_update(const PParams &pp, DevParticle *P, int iter, int restore, hipStream_t s);
-hipError_t slip_conv(const PParams &pp, const DevParticle *P, double *eps, hipStream_t s);
+hipError_t slip_update(const PParams &pp, DevParticle *P, int iter, int restore, hipStream_t s,
+                       const int *skip = nullptr);
+// Device state of a slip loop decided on the device: flag[0] is set once the
+// loop has converged or reached its maximum, flag[1] counts this step's
+// iterations; stat[0] totals iterations and stat[1] completed loops.
+struct SlipLoop {
+    int                *flag;
+    unsigned long long *stat;
+    double              tol;
+    int                 max_iter;
+};
+// eps = the convergence measure; with a non-null loop (its flag clear) the
+// iteration is counted and the flag set when eps < tol or the count reaches
+// the maximum
+hipError_t slip_conv(const PParams &pp, const DevParticle *P, double *eps, hipStream_t s,
+                     const SlipLoop *loop = nullptr);
+// flag[0] = flag[1] = 0: a new loop
+hipError_t slip_loop_begin(const SlipLoop &loop, hipStream_t s);
 
 }  // namespace kh

@@ -938,7 +938,9 @@ template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_slip_scatter(PParams pp, DevParticle *__restrict__ P,
                                                         const T *__restrict__ u0, const T *__restrict__ u1,
                                                         const T *__restrict__ u2, T *__restrict__ up0,
-                                                        T *__restrict__ up1, T *__restrict__ up2) {
+                                                        T *__restrict__ up1, T *__restrict__ up2,
+                                                        const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int         n = blockIdx.x;
     DevParticle      &p = P[n];
     const DevSpecies &s = pp.species[p.spec];
@@ -986,7 +988,9 @@ __global__ __launch_bounds__(PT_NT) void k_slip_scatter(PParams pp, DevParticle 
 // projected slip field carries over each particle's profile.
 template <typename T>
 __global__ __launch_bounds__(PT_NT) void k_f_slip(PParams pp, DevParticle *__restrict__ P, const T *__restrict__ s0,
-                                                  const T *__restrict__ s1, const T *__restrict__ s2, double coef) {
+                                                  const T *__restrict__ s1, const T *__restrict__ s2, double coef,
+                                                  const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int     n     = blockIdx.x;
     const double  xp[3] = {P[n].x[0], P[n].x[1], P[n].x[2]};
     const PCell   c     = particle_cell(pp, xp);
@@ -1015,9 +1019,10 @@ __global__ __launch_bounds__(PT_NT) void k_f_slip(PParams pp, DevParticle *__res
 
 // MD_solver_velocity_slip_Euler / _AB2 (particle_solver.cxx:144-207):
 // the velocity update with the slip force, forces kept for the iteration.
-__global__ void k_velocity_slip(PParams pp, DevParticle *__restrict__ P, int ab2, double h) {
+__global__ void k_velocity_slip(PParams pp, DevParticle *__restrict__ P, int ab2, double h,
+                                const int *__restrict__ skip) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= pp.np) return;
+    if (n >= pp.np || (skip && *skip)) return;
     DevParticle      &p = P[n];
     const DevSpecies &s = pp.species[p.spec];
     const double      dmy = h * s.imass, dmy_rot = h * s.imoi;
@@ -1071,9 +1076,10 @@ __global__ void k_reset_forces(PParams pp, DevParticle *__restrict__ P) {
 
 // Update_slip_particle_velocity (operate_surface.h:101-121); with `restore`
 // the velocities go back to the values before the iteration (reset_iter).
-__global__ void k_slip_update(PParams pp, DevParticle *__restrict__ P, int iter, int restore) {
+__global__ void k_slip_update(PParams pp, DevParticle *__restrict__ P, int iter, int restore,
+                              const int *__restrict__ skip) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= pp.np) return;
+    if (n >= pp.np || (skip && *skip)) return;
     DevParticle &p = P[n];
     for (int d = 0; d < 3; d++) {
         if (iter == 0) {
@@ -1092,8 +1098,14 @@ __global__ void k_slip_update(PParams pp, DevParticle *__restrict__ P, int iter,
 
 // Slip_particle_convergence (operate_surface.h:72-100): max over slip
 // particles of |v_slip - v|^2/|v|^2 + |w_slip - w|^2/|w|^2, square-rooted.
+// With a loop decided on the device (flag non-null): nothing once flag[0] is
+// set; otherwise the iteration is counted and flag[0] set when eps < tol or
+// the count reaches max_iter (the exit test of sp_3d_ns.cxx:153-158).
 __global__ __launch_bounds__(PT_NT) void k_slip_conv(PParams pp, const DevParticle *__restrict__ P,
-                                                     double *__restrict__ eps_out) {
+                                                     double *__restrict__ eps_out, int *__restrict__ flag,
+                                                     unsigned long long *__restrict__ stat, double tol,
+                                                     int max_iter) {
+    if (flag && flag[0]) return;
     __shared__ double red[PT_NT / 64];
     double            e = 0.0;
     for (int n = threadIdx.x; n < pp.np; n += PT_NT) {
@@ -1117,7 +1129,24 @@ __global__ __launch_bounds__(PT_NT) void k_slip_conv(PParams pp, const DevPartic
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < PT_NT / 64; w++) e = fmax(e, red[w]);
-        eps_out[0] = sqrt(e);
+        const double eps = sqrt(e);
+        eps_out[0]       = eps;
+        if (flag) {
+            const int it = flag[1] + 1;
+            flag[1]      = it;
+            stat[0] += 1;
+            if (eps < tol || it == max_iter) {
+                flag[0] = 1;
+                stat[1] += 1;
+            }
+        }
+    }
+}
+
+__global__ void k_slip_loop_begin(int *__restrict__ flag) {
+    if (threadIdx.x == 0) {
+        flag[0] = 0;
+        flag[1] = 0;
     }
 }
 
@@ -1425,31 +1454,39 @@ hipError_t slip_moments(const PParams &pp, DevParticle *P, T *const u[3], hipStr
     return hipGetLastError();
 }
 template <typename T>
-hipError_t slip_scatter(const PParams &pp, DevParticle *P, T *const u[3], T *const up[3], hipStream_t s) {
+hipError_t slip_scatter(const PParams &pp, DevParticle *P, T *const u[3], T *const up[3], hipStream_t s,
+                        const int *skip) {
     if (pp.np > 0)
         hipLaunchKernelGGL(k_slip_scatter<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, u[0], u[1], u[2], up[0], up[1],
-                           up[2]);
+                           up[2], skip);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t f_slip(const PParams &pp, DevParticle *P, T *const sf[3], double coef, hipStream_t s) {
-    if (pp.np > 0) hipLaunchKernelGGL(k_f_slip<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, sf[0], sf[1], sf[2], coef);
+hipError_t f_slip(const PParams &pp, DevParticle *P, T *const sf[3], double coef, hipStream_t s, const int *skip) {
+    if (pp.np > 0)
+        hipLaunchKernelGGL(k_f_slip<T>, dim3(pp.np), dim3(PT_NT), 0, s, pp, P, sf[0], sf[1], sf[2], coef, skip);
     return hipGetLastError();
 }
-hipError_t velocity_slip(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s) {
-    if (pp.np > 0) hipLaunchKernelGGL(k_velocity_slip, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, ab2, h);
+hipError_t velocity_slip(const PParams &pp, DevParticle *P, int ab2, double h, hipStream_t s, const int *skip) {
+    if (pp.np > 0) hipLaunchKernelGGL(k_velocity_slip, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, ab2, h, skip);
     return hipGetLastError();
 }
 hipError_t reset_forces(const PParams &pp, DevParticle *P, hipStream_t s) {
     if (pp.np > 0) hipLaunchKernelGGL(k_reset_forces, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P);
     return hipGetLastError();
 }
-hipError_t slip_update(const PParams &pp, DevParticle *P, int iter, int restore, hipStream_t s) {
-    if (pp.np > 0) hipLaunchKernelGGL(k_slip_update, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, iter, restore);
+hipError_t slip_update(const PParams &pp, DevParticle *P, int iter, int restore, hipStream_t s, const int *skip) {
+    if (pp.np > 0)
+        hipLaunchKernelGGL(k_slip_update, dim3(pblocks(pp.np)), dim3(64), 0, s, pp, P, iter, restore, skip);
     return hipGetLastError();
 }
-hipError_t slip_conv(const PParams &pp, const DevParticle *P, double *eps, hipStream_t s) {
-    hipLaunchKernelGGL(k_slip_conv, dim3(1), dim3(PT_NT), 0, s, pp, P, eps);
+hipError_t slip_conv(const PParams &pp, const DevParticle *P, double *eps, hipStream_t s, const SlipLoop *loop) {
+    hipLaunchKernelGGL(k_slip_conv, dim3(1), dim3(PT_NT), 0, s, pp, P, eps, loop ? loop->flag : nullptr,
+                       loop ? loop->stat : nullptr, loop ? loop->tol : 0.0, loop ? loop->max_iter : 0);
+    return hipGetLastError();
+}
+hipError_t slip_loop_begin(const SlipLoop &loop, hipStream_t s) {
+    hipLaunchKernelGGL(k_slip_loop_begin, dim3(1), dim3(64), 0, s, loop.flag);
     return hipGetLastError();
 }
 
@@ -1517,8 +1554,9 @@ hipError_t frame(const PParams &pp, const DevParticle *P, const int *d_ts, int t
     template hipError_t particle_step<T>(const PParams &, DevParticle *, T *const[3], const T *, const PCache<T> &, \
                                          double, int, double, int, uint64_t, uint32_t, const int *, double, double, \
                                          T *const[3], hipStream_t);                                                \
-    template hipError_t slip_scatter<T>(const PParams &, DevParticle *, T *const[3], T *const[3], hipStream_t);     \
-    template hipError_t f_slip<T>(const PParams &, DevParticle *, T *const[3], double, hipStream_t);
+    template hipError_t slip_scatter<T>(const PParams &, DevParticle *, T *const[3], T *const[3], hipStream_t,      \
+                                        const int *);                                                          \
+    template hipError_t f_slip<T>(const PParams &, DevParticle *, T *const[3], double, hipStream_t, const int *);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST

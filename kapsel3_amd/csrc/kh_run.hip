@@ -105,7 +105,13 @@ struct kh_run {
     bool         zclear       = false;    // the fused step's z_force leaves u_p and phi_sum zero
     bool         pclean       = false;    // u_p and phi_sum are zero now
     double      *slipdc       = nullptr;  // k = 0 mode of the projected slip field (3 doubles)
-    int          slip_iters   = 0;        // iterations of the last slip loop
+    int          slip_iters   = 0;        // iterations of the last slip loop decided on the host
+    long long    slip_iters_host = 0, slip_loops_host = 0;  // totals of the loops decided on the host
+    // the fused step's slip loop decided on the device (power-of-two NZ >= 32):
+    // int flag[2] (converged, iterations this step), then unsigned long long
+    // stat[2] (total iterations, loops) at byte 16
+    bool         slip_dev     = false;
+    int         *slipflag     = nullptr;
     double      *traj         = nullptr;  // per-step particle record (x, v) for traj_steps steps
     int          traj_steps   = 0;
     int          traj_base    = 0;        // ts when recording started
@@ -388,10 +394,61 @@ struct Step {
             KH_LAUNCH("slip_update", slip_update(r->pp, r->dp, iter, 1, r->stream));
         }
         r->slip_iters = iter;
+        r->slip_iters_host += iter;
+        r->slip_loops_host++;
         for (int d = 0; d < 3; d++) std::swap(r->ureal[d], r->up[d]);  // Swap_mem(u, up)
         if (fused)
             KH_LAUNCH("zeta_add", (zeta_add<T>(r->g, Z(r, 0), Z(r, 1), S(r, 3), S(r, 4), r->ukdc, r->slipdc,
                                                r->stream)));
+        return KH_OK;
+    }
+
+    // The same slip iteration in the fused step with its length decided on
+    // the device: every iteration up to slip_iter_max is launched and those
+    // after convergence return at once (slip_conv sets the flag the others
+    // read), so squirmer steps are captured as graphs like the plain step.
+    // The projection of the slip force field runs on the pruned transforms
+    // (Solenoidal_u truncates to the 2/3 modes): forward z of the scattered
+    // field (read at every point: the shell slip force is nonzero also where
+    // phi has underflowed to 0), which leaves up zero as it reads it,
+    // forward y, x_slip (projection, its curl and k = 0 mode, inverse x),
+    // inverse y, inverse z into W only where phi_sum != 0. The projected
+    // field is read only there: the slip force gather, and z_force, which
+    // forms u = u* + W (u + up of the reference) where it needs it.
+    static int slip_iteration_dev(kh_run *r, int ab2) {
+        const Grid &g    = r->g;
+        hipStream_t s    = r->stream;
+        T          *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *up[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+        T          *w[3] = {W(r, 0), W(r, 1), W(r, 2)};
+        cpx<T>     *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
+        Comps<T>    c3{{S(r, 0), S(r, 1), S(r, 2), nullptr, nullptr}};
+        ZIO<T>      zio{{S(r, 0), S(r, 1), S(r, 2)}, {w[0], w[1], w[2]}};
+        const SlipLoop loop{r->slipflag, (unsigned long long *)((char *)r->slipflag + 16), r->P.slip_tol,
+                            r->P.slip_iter_max};
+        const int     *skip  = r->slipflag;
+        const double   coef  = (r->dx3 * r->rho) / r->jikan.dt_fluid;
+        const double   h     = ab2 ? r->jikan.hdt_md : r->jikan.dt_md;
+        const T        scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
+        KH_LAUNCH("slip_moments", slip_moments<T>(r->pp, r->dp, u, s));
+        KH_LAUNCH("slip_update", slip_update(r->pp, r->dp, 0, 0, s));
+        KH_LAUNCH("slip_begin", slip_loop_begin(loop, s));
+        for (int iter = 0; iter < r->P.slip_iter_max; iter++) {
+            KH_LAUNCH("slip_scatter", slip_scatter<T>(r->pp, r->dp, u, up, s, skip));
+            KH_LAUNCH("slip_z_fwd", (z_force<T>(g, up, nullptr, nullptr, s3, 1, nullptr, s, nullptr, skip)));
+            KH_LAUNCH("slip_y_fwd", (cols_y<T, true>(g, c3, 3, 0, 1, s, skip)));
+            KH_LAUNCH("slip_x", (x_slip<T>(g, s3, S(r, 3), S(r, 4), r->slipdc, skip, s)));
+            KH_LAUNCH("slip_y_inv", (cols_y<T, false>(g, c3, 3, 1, 0, s, skip)));
+            KH_LAUNCH("slip_z_inv", (zw_c2r<T>(g.nz, zio, 3, (long long)g.nx * g.ny, scale, g.trn_z, g.nzp, s, PS(r),
+                                               skip)));
+            KH_LAUNCH("f_slip", f_slip<T>(r->pp, r->dp, w, coef, s, skip));
+            if (iter == 0) KH_LAUNCH("forces", forces(r->pp, r->dp, s));
+            KH_LAUNCH("velocity_slip", velocity_slip(r->pp, r->dp, ab2, h, s, skip));
+            KH_LAUNCH("slip_conv", slip_conv(r->pp, r->dp, r->slipeps, s, &loop));
+            if (iter + 1 < r->P.slip_iter_max)
+                KH_LAUNCH("slip_update", slip_update(r->pp, r->dp, iter + 1, 1, s, skip));
+        }
+        KH_LAUNCH("reset_forces", reset_forces(r->pp, r->dp, s));
+        KH_LAUNCH("zeta_add", (zeta_add<T>(g, Z(r, 0), Z(r, 1), S(r, 3), S(r, 4), r->ukdc, r->slipdc, s, 1)));
         return KH_OK;
     }
 
@@ -456,7 +513,7 @@ struct Step {
         // part): phi_sum is then known when u* is formed, and u* is written
         // only where it is read. Squirmer runs keep the reference's order
         // and the whole u* field (the slip iteration reads it on the shell).
-        const bool early = !r->slip;
+        const bool early = !r->slip || r->slip_dev;
         if (early) KH_TRY(particles_begin(r, ab2));
         cpx<T>  *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
         Comps<T> c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
@@ -480,9 +537,16 @@ struct Step {
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
         ZIO<T> zio{{S(r, 0), S(r, 1), S(r, 2)}, {U(r, 0), U(r, 1), U(r, 2)}};
-        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z, early ? PS(r) : nullptr)));
+        // squirmers read u* on the whole slip shell (|r - a| < xi/2), which
+        // reaches points where phi has underflowed to 0: u* is written in full
+        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z, early && !r->slip ? PS(r) : nullptr)));
         if (!early) KH_TRY(particles_begin(r, ab2));
-        if (r->slip) {
+        if (r->slip_dev) {  // the slip loop leaves up zero
+            KH_TRY(calc_f_hydro(r));
+            KH_TRY(slip_iteration_dev(r, ab2));
+            if (r->kbt > 0.) KH_TRY(thermo(r));
+            KH_TRY(scatter_up(r));
+        } else if (r->slip) {
             KH_TRY(calc_f_hydro(r));
             KH_TRY(slip_iteration(r, ab2, true));
             for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
@@ -503,7 +567,9 @@ struct Step {
         }
         // f = P(up - phi u*), u = u* + f, zeta = curl u
         T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
-        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear, r->zclear ? r->d_ts : nullptr, s)));
+        T *w3[3] = {W(r, 0), W(r, 1), W(r, 2)};  // squirmers: u = u* + the projected slip field
+        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear ? 3 : 0, r->zclear ? r->d_ts : nullptr, s,
+                                         r->slip_dev ? w3 : nullptr)));
         if (!r->zclear) KH_LAUNCH("tick", tick(r->d_ts, s));
         r->pclean = r->zclear;
         KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
@@ -860,7 +926,7 @@ int free_run(kh_run *r) {
     void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
                     r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
                     r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek, r->cphi, r->cidx,
-                    r->ccode, r->ccnt, r->d_ts, r->slipeps, r->slipdc, r->xs};
+                    r->ccode, r->ccnt, r->d_ts, r->slipeps, r->slipdc, r->slipflag, r->xs};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, r->stream);
     if (r->traj) (void)hipFreeAsync(r->traj, r->stream);
@@ -1271,6 +1337,7 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     if (st == KH_OK) st = alloc((void **)&r->slipeps, sizeof(double));
     if (st == KH_OK) st = alloc((void **)&r->xs, 4 * sizeof(double) * (size_t)(np > 0 ? np : 1));
     if (st == KH_OK) st = alloc((void **)&r->slipdc, 3 * sizeof(double));
+    if (st == KH_OK) st = alloc((void **)&r->slipflag, 32);
     if (st == KH_OK) st = alloc((void **)&r->dp, sizeof(DevParticle) * (size_t)(np > 0 ? np : 1));
     if (st == KH_OK) st = alloc((void **)&r->ds, sizeof(DevSpecies) * hs.size());
     if (st == KH_OK) st = alloc((void **)&r->sek, sizeof(int) * sek.size());
@@ -1285,7 +1352,8 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     if (hipMemcpyAsync(r->ds, hs.data(), sizeof(DevSpecies) * hs.size(), hipMemcpyHostToDevice, cs) != hipSuccess ||
         hipMemcpyAsync(r->sek, sek.data(), sizeof(int) * sek.size(), hipMemcpyHostToDevice, cs) != hipSuccess ||
         hipMemsetAsync(r->ukdc, 0, 3 * sizeof(double), cs) != hipSuccess ||
-        hipMemsetAsync(r->d_ts, 0, sizeof(int), cs) != hipSuccess || hipMemsetAsync(r->dp, 0, psz, cs) != hipSuccess)
+        hipMemsetAsync(r->d_ts, 0, sizeof(int), cs) != hipSuccess || hipMemsetAsync(r->dp, 0, psz, cs) != hipSuccess ||
+        hipMemsetAsync(r->slipflag, 0, 32, cs) != hipSuccess)
         return bail(fail(KH_ERR_HIP, "initial upload failed"));
     for (int i = 0; i < 2; i++)
         if (hipMemsetAsync(r->zeta[i], 0, sb, cs) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
@@ -1329,7 +1397,10 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
         const int nh    = (int)P->a + (int)P->a_xi + 2, nmin = std::min(P->nx, std::min(P->ny, P->nz));
         pp.direct_dist = 2 * (nh + 2) <= nmin;
     }
-    r->zclear          = !r->slip && z_force_clears(g);
+    // squirmer steps decide the slip loop on the device where the wavefront
+    // z engine and its masked inverse pass take NZ
+    r->slip_dev        = r->slip && zw_supported(g.nz) && g.nz >= 32;
+    r->zclear          = (!r->slip || r->slip_dev) && z_force_clears(g);
 
     CTime t0  = {0, 0.0, P->dt, P->dt * 0.5, P->dt, P->dt * 0.5};  // sp_3d_ns.cxx:782
     r->jikan  = t0;
@@ -1565,9 +1636,10 @@ int kh_step(kh_run *r, int nsteps) {
     if (nsteps < 0) return fail(KH_ERR_ARG, "negative step count");
     if (!r->shear) KH_TRY(settle_zeta(r));
     for (int s = 0; s < nsteps;) {
-        // (the slip loop decides its length on the host: not captured)
+        // (a slip loop decided on the host is not captured)
         const bool graph_ok = r->use_graph && !r->timing && r->step_path == KH_PATH_FUSED && r->jikan.ts > 0 &&
-                              !r->zeta_dirty && !r->slip && !r->shear && (!r->zclear || r->pclean);
+                              !r->zeta_dirty && (!r->slip || r->slip_dev) && !r->shear &&
+                              (!r->zclear || r->pclean);
         int done = 1;
         if (graph_ok && nsteps - s >= GRAPH_STEPS) {
             if (!r->graph_exec_k) KH_TRY(capture_steps(r, GRAPH_STEPS, &r->graph_k, &r->graph_exec_k));
@@ -1999,6 +2071,16 @@ int kh_A_k2a(kh_run *r, double *a) {
     return download_real(r, a, r->work[4]);
 }
 
+int kh_get_slip_iterations(kh_run *r, long long *iterations, long long *loops) {
+    if (!r || !iterations || !loops) return fail(KH_ERR_ARG, "NULL argument");
+    unsigned long long st[2] = {0, 0};
+    KH_HIP(hipMemcpyAsync(st, (char *)r->slipflag + 16, sizeof st, hipMemcpyDeviceToHost, r->stream));
+    KH_HIP(hipStreamSynchronize(r->stream));
+    *iterations = (long long)st[0] + r->slip_iters_host;
+    *loops      = (long long)st[1] + r->slip_loops_host;
+    return KH_OK;
+}
+
 int kh_set_step_path(kh_run *r, int path) {
     if (!r) return fail(KH_ERR_ARG, "NULL run");
     if (path != KH_PATH_FUSED && path != KH_PATH_REFERENCE_ORDER) return fail(KH_ERR_ARG, "unknown step path %d", path);
@@ -2038,6 +2120,7 @@ const char *kh_timing_name(kh_run *r, int i) {
     if (!r || i < 0 || i >= (int)r->names.size()) return nullptr;
     return r->names[i].c_str();
 }
+
 
 // ------------------------------------------------- allocation-free transform
 // kh_A2a_k_noalloc: the forward transform of kh_A2a_k without a run and

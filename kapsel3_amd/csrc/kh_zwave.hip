@@ -171,12 +171,14 @@ __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T sc
 // is left as it was. The mask is read once for all components.
 template <typename T, int L, int E>
 __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, const T *__restrict__ mask,
-                                                       long long nrows, T scale, int kzv, int pitch) {
+                                                       long long nrows, T scale, int kzv, int pitch,
+                                                       const int *__restrict__ skip) {
     using Geo = WaveGeom<T, L, E>;
     using F   = WaveFft<T, L, E, false, false>;
     using RS  = RowStage<T, L>;
     static_assert(L >= 4 * Geo::G, "four points per lane per row");
     constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
+    if (skip && *skip) return;
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
@@ -381,19 +383,22 @@ __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict
 }
 
 // Rigidity force f = up - min(phi_sum, 1) u* (Make_f_particle_dt_nonsole,
-// f_particle.h:35-54; make_phi.cxx:248-259), forward z per component.
+// f_particle.h:35-54; make_phi.cxx:248-259), forward z per component. With
+// us2, u* = us + us2 (the squirmer step's u after the slip iteration, formed
+// here as the reference forms it, u + up, sp_3d_ns.cxx:146-150); with no us
+// and no phis, f = up read at every point (the slip force field of the slip
+// iteration, nonzero on the whole shell |r - a| < xi/2, also where phi has
+// underflowed to 0). clear: bit 0 leaves up zero, bit 1 phi_sum (with
+// phis). A nonzero *skip makes the launch do nothing.
 template <typename T, int L, int E>
-__global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T *__restrict__ up1,
-                                                  T *__restrict__ up2, T *__restrict__ phis,
-                                                  const T *__restrict__ us0, const T *__restrict__ us1,
-                                                  const T *__restrict__ us2, cpx<T> *__restrict__ f0,
-                                                  cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2, int clear,
-                                                  int *__restrict__ tick) {
+__global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int clear, int *__restrict__ tick,
+                                                  const int *__restrict__ skip) {
     using Geo = WaveGeom<T, L, E>;
     using Ff  = WaveFft<T, L, E, true, true>;
     using RS  = RowStage<T, L>;
     static_assert(L >= 4 * Geo::G, "four points per lane per row");
     constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
+    if (skip && *skip) return;
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
@@ -403,6 +408,7 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
     const long long         oa = (2 * wl.line) * L, ob = oa + L;
     using vec                  = typename RS::vec;
     const vec zero             = {(T)0, (T)0, (T)0, (T)0};
+    T        *phis             = io.phis;
     // Outside every particle phi_sum = 0, and so is u_p (it is scattered on
     // the same points with the same phi): f = 0 there whatever u* is, so
     // u_p and u* are read only where some phi_sum of the vector is nonzero.
@@ -410,30 +416,41 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
     bool na[NT4], nb[NT4];
 #pragma unroll
     for (int t = 0; t < NT4; t++) {
+        if (!phis) {  // no mask: up is read everywhere (f = up)
+            na[t] = wl.okA;
+            nb[t] = wl.okB;
+            pa[t] = pb[t] = zero;
+            continue;
+        }
         const int z = 4 * (wl.g + Geo::G * t);
         vec       a = wl.okA ? RS::ld(phis + oa + z) : zero, b = wl.okB ? RS::ld(phis + ob + z) : zero;
         na[t]       = nonzero(a);
         nb[t]       = nonzero(b);
-        if (clear) {  // leave phi_sum zero for the next step's scatter
+        if (clear & 2) {  // leave phi_sum zero for the next step's scatter
             if (na[t]) RS::st(phis + oa + z, zero);
             if (nb[t]) RS::st(phis + ob + z, zero);
         }
         pa[t]       = {fmin(a.x, (T)1), fmin(a.y, (T)1), fmin(a.z, (T)1), fmin(a.w, (T)1)};
         pb[t]       = {fmin(b.x, (T)1), fmin(b.y, (T)1), fmin(b.z, (T)1), fmin(b.w, (T)1)};
     }
-    T       *UP[3] = {up0, up1, up2};
-    const T *US[3] = {us0, us1, us2};
-    cpx<T>  *dst[3] = {f0, f1, f2};
+    auto ld_u = [&](int d, long long o) -> vec {
+        if (!io.us[d]) return zero;
+        const vec a = RS::ld(io.us[d] + o);
+        if (!io.us2[d]) return a;
+        const vec b = RS::ld(io.us2[d] + o);
+        return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
+    };
 #pragma unroll 1
     for (int d = 0; d < 3; d++) {
+        T *UP = io.up[d];
 #pragma unroll
         for (int t = 0; t < NT4; t++) {
             const int z  = 4 * (wl.g + Geo::G * t);
-            const vec ua = na[t] ? RS::ld(UP[d] + oa + z) : zero, ub = nb[t] ? RS::ld(UP[d] + ob + z) : zero;
-            const vec sa = na[t] ? RS::ld(US[d] + oa + z) : zero, sb = nb[t] ? RS::ld(US[d] + ob + z) : zero;
-            if (clear) {  // and u_p zero
-                if (na[t] && nonzero(ua)) RS::st(UP[d] + oa + z, zero);
-                if (nb[t] && nonzero(ub)) RS::st(UP[d] + ob + z, zero);
+            const vec ua = na[t] ? RS::ld(UP + oa + z) : zero, ub = nb[t] ? RS::ld(UP + ob + z) : zero;
+            const vec sa = na[t] ? ld_u(d, oa + z) : zero, sb = nb[t] ? ld_u(d, ob + z) : zero;
+            if (clear & 1) {  // and u_p zero
+                if (na[t] && nonzero(ua)) RS::st(UP + oa + z, zero);
+                if (nb[t] && nonzero(ub)) RS::st(UP + ob + z, zero);
             }
             wl.buf[WavePad<T>::at(z)]     = cmk(ua.x - pa[t].x * sa.x, ub.x - pb[t].x * sb.x);
             wl.buf[WavePad<T>::at(z + 1)] = cmk(ua.y - pa[t].y * sa.y, ub.y - pb[t].y * sb.y);
@@ -446,7 +463,7 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, T *__restrict__ up0, T
         for (int i = 0; i < E; i++) v[i] = wl.buf[WavePad<T>::at(Ff::in_point(wl.g, i))];
         wave_sync();
         Ff::run(v, wl.buf, tw, wl.g);
-        store_pair<T, L, Ff>(v, wl, g.trn_z, dst[d]);
+        store_pair<T, L, Ff>(v, wl, g.trn_z, io.f[d]);
     }
 }
 
@@ -503,13 +520,14 @@ hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, h
 
 template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
-                  hipStream_t s, const T *mask) {
+                  hipStream_t s, const T *mask, const int *skip) {
+    if (skip && !(mask && L >= 32)) return hipErrorInvalidValue;  // only the masked pass takes a skip flag
     if (mask && L >= 32) {  // every length with 4 points per lane per row
         switch (L) {
 #define KH_CASE(n)                                                                                              \
     case n:                                                                                                     \
         if constexpr (n >= 32) {                                                                                \
-            hipLaunchKernelGGL((k_zw_c2r_masked<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, io, ncomp, mask, nrows, scale, kzv, pitch); \
+            hipLaunchKernelGGL((k_zw_c2r_masked<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, io, ncomp, mask, nrows, scale, kzv, pitch, skip); \
             return hipGetLastError();                                                                           \
         }                                                                                                       \
         break;
@@ -566,13 +584,21 @@ hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T sca
 
 template <typename T>
 hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                    int *tick, hipStream_t s) {
+                    int *tick, hipStream_t s, T *const us2[3], const int *skip) {
     const long long nrows = (long long)g.nx * g.ny;
+    ZForceIO<T>     io{};
+    for (int d = 0; d < 3; d++) {
+        io.up[d]  = up[d];
+        io.us[d]  = us ? us[d] : nullptr;
+        io.us2[d] = us2 ? us2[d] : nullptr;
+        io.f[d]   = f[d];
+    }
+    io.phis = phis;
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::force>), dim3(zw_blocks<T, n, ZwE<T, n>::force>(nrows)), dim3(256), 0, s, g, up[0], up[1], up[2], \
-                           phis, us[0], us[1], us[2], f[0], f[1], f[2], clear, tick);                               \
+        hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::force>), dim3(zw_blocks<T, n, ZwE<T, n>::force>(nrows)), dim3(256), 0, s, g, io, \
+                           clear, tick, skip);                                                                      \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)
 #undef KH_CASE
@@ -582,11 +608,12 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
 
 #define KH_INST(T)                                                                                           \
     template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t, int);               \
-    template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t, const T *); \
+    template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t, const T *, \
+                                  const int *);                                                            \
     template hipError_t zw_c2r_cross<T>(int, const ZIO<T> &, const T *const[3], long long, T, int, int, hipStream_t); \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
     template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,        \
-                                    hipStream_t);
+                                    hipStream_t, T *const[3], const int *);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST

@@ -261,3 +261,49 @@ def test_sheared_functions_fp64(gpu):
     zg, zo = gr.get_fluid(), orc.get_fluid()
     assert rel(zg[0], zo[0]) < 1e-12 and rel(zg[1], zo[1]) < 1e-12
     gr.close()
+
+
+def test_deferred_modes_after_remap_and_transitions(gpu):
+    """zeta's modes beyond the 2/3 rule, which sheared steps leave to be
+    formed when the state is read, against the oracle (1e-10, uk_dc
+    included): right after a step that remaps the grid, and after each entry
+    point that completes them before it acts -- kh_set_shear_state,
+    kh_apply of the oblique zeta -> u, kh_set_step_path -- taken right after
+    sheared steps."""
+    n, rate = 32, 0.4
+    P, p, g0 = sheared_system(n)  # strain 0.995 + 0.02 per step: the first step remaps (to 0.015)
+    z = initial_fluid(P, n, g0, rate)
+    gr, orc = _pair(P, p, z, rate, g0)
+    print("deferred zeta modes after a remap step and entry points", flush=True)
+
+    def same(tag):
+        zg, zo = gr.get_fluid(), orc.get_fluid()
+        assert rel(zg[0], zo[0]) < 1e-10 and rel(zg[1], zo[1]) < 1e-10, tag
+        assert np.abs(zg[2] - zo[2]).max() <= 1e-10 * max(np.abs(zo[2]).max(), 1e-30) + 1e-18, tag
+
+    def both_step(k):
+        gr.step(k)
+        orc.step(k)
+
+    both_step(1)
+    assert 0 <= gr.shear_state()[0] < g0 and abs(gr.shear_state()[0] - orc.shear_state()[0]) < 1e-14
+    same("right after the remap step")
+    both_step(2)
+    for r in (gr, orc):
+        r.set_shear_state(0.3, rate)
+    same("set_shear_state after sheared steps")
+    both_step(2)
+    same("steps after set_shear_state")
+    gr.apply("obl_zeta_k2u")  # reads every mode of zeta the steps left
+    orc.le_apply("zeta_k2u")
+    for d in range(3):
+        assert rel(gr.get_field(f"u{d}")[..., :n], orc.get_field(f"u{d}")[..., :n]) < 1e-10, d
+    same("zeta -> u after sheared steps")
+    both_step(2)
+    same("steps after zeta -> u")
+    gr.set_step_path("reference")
+    same("set_step_path after sheared steps")
+    both_step(2)
+    same("sheared steps after the path switch")
+    cmp_particles(gr.get_particles(), orc.get_particles(), 1e-10, keys=KEYS, p0=p, a=4.0)
+    gr.close()

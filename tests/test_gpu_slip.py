@@ -14,14 +14,14 @@ from test_gpu_step import cmp_particles, rel
 pytestmark = pytest.mark.gpu
 
 
-def squirmer_system(n=32, precision=FP64, iters=4, tol=0.005):
+def squirmer_system(n=32, precision=FP64, iters=4, tol=0.005, scale=1.0):
     sp = [make_species(mass_ratio=1.0, propulsion=JP_SLIP, janus_axis=JAX_Z, slip_vel=0.01, slip_mode=1.3,
                        rotlet_c1=2e-3, rotlet_c2=-1e-3),
           make_species(mass_ratio=1.0, propulsion=JP_SLIP, janus_axis=JAX_Y, slip_vel=0.01, slip_mode=-0.8),
           make_species(mass_ratio=1.5, propulsion=JP_NONE)]
     P = Params(n, a=4.0, a_xi=2.0, dt=0.05, species=sp, rotation=1, g=0.0, precision=precision,
                slip_iter_max=iters, slip_tol=tol)
-    pos = np.array([[6.3, 7.6, 8.2], [20.1, 22.4, 9.7], [12.5, 24.0, 23.3], [26.0, 9.1, 24.8]])
+    pos = np.array([[6.3, 7.6, 8.2], [20.1, 22.4, 9.7], [12.5, 24.0, 23.3], [26.0, 9.1, 24.8]]) * scale
     parts = workloads.particles_at(pos, spec=np.array([0, 1, 2, 0]), q=workloads.random_quaternions(4, 3))
     return P, parts
 
@@ -70,16 +70,21 @@ def test_slip_functions_fp64(gpu):
     g.close()
 
 
-@pytest.mark.parametrize("path", ["fused", "reference"])
-def test_squirmer_steps_fp64(gpu, path):
-    print(f"squirmers 32^3 x 4, 40 steps, {path}", flush=True)
-    P, parts = squirmer_system()
+# 32^3: the fused step's slip loop decided on the device (captured as
+# graphs); 48^3: the same step with the loop decided on the host (NZ not a
+# power of two)
+@pytest.mark.parametrize("path,n", [("fused", 32), ("reference", 32), ("fused", 48)])
+def test_squirmer_steps_fp64(gpu, path, n):
+    print(f"squirmers {n}^3 x 4, 40 steps, {path}", flush=True)
+    P, parts = squirmer_system(n=n, scale=n / 32)
     o, g = both(P, parts)
     g.set_step_path(path)
     o.init_fluid()
     g.init_fluid()
     o.step(40)
     g.step(40)
+    it, loops = g.slip_iterations()
+    assert loops == 40 and 40 <= it <= 4 * 40, (it, loops)
     po, pg = o.get_particles(), g.get_particles()
     cmp_particles(pg, po, 1e-10, keys=["x", "x_nopbc", "v", "omega", "q", "v_slip", "omega_slip"], p0=parts, a=4.0)
     zo, zg = o.get_fluid(), g.get_fluid()
