@@ -315,6 +315,12 @@ const char *kh_last_error(void);
  * device until its nsteps are done, then its trajectory (as
  * kh_get_trajectory) is copied to traj_out (nsteps x np x 6 doubles, may be
  * NULL) and its memory is released. Latency = submission to that copy.
+ * Runs of plain spheres on the same grid (size class: grid, precision, dx,
+ * rho, eta, dt, FIX_CELL, a, a_xi) with at most KH_BATCH_NP_CAP particles
+ * share a batch of batch_slots device arenas: after a run's first step, one
+ * launch per fluid pass and one captured graph step every run of the batch
+ * together (its particle launches stay its own); the batch's arenas count
+ * against the budget while any of them is in use.
  * Submit/poll/wait/stats may be called from any thread. */
 typedef struct kh_server kh_server;
 typedef struct {
@@ -322,7 +328,9 @@ typedef struct {
     int       chunk_steps;   /* steps enqueued per run per scheduler visit (>= 1)  */
     double    hbm_fraction;  /* share of free device memory for runs, (0, 1]      */
     long long hbm_budget;    /* bytes; > 0 overrides hbm_fraction                  */
+    int       batch_slots;   /* runs per batch of one size class; < 2: every run steps alone */
 } kh_server_config;
+#define KH_BATCH_NP_CAP 64
 typedef struct {
     long long submitted, completed, failed, queued, active;
     int       peak_active;

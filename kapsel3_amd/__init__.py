@@ -39,7 +39,7 @@ SHEAR_RECORD = ["time", "shear_rate", "degree_oblique", "strain", "lj_dev_stress
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_active", C.c_int), ("chunk_steps", C.c_int), ("hbm_fraction", C.c_double),
-                ("hbm_budget", C.c_longlong)]
+                ("hbm_budget", C.c_longlong), ("batch_slots", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -405,10 +405,12 @@ class Server:
     submit() returns an id at once; the run is admitted when a slot and HBM
     budget are free, stays on the device until its steps are done, and its
     per-step trajectory (nsteps, np, 6) lands in the array submit() returned.
+    Runs of one size class share batches of `batch_slots` (one launch per
+    fluid pass for the whole batch; < 2: every run steps alone).
     """
 
-    def __init__(self, max_active=16, chunk_steps=8, hbm_fraction=0.8, hbm_budget=0):
-        cfg = ServerConfig(max_active, chunk_steps, hbm_fraction, hbm_budget)
+    def __init__(self, max_active=16, chunk_steps=8, hbm_fraction=0.8, hbm_budget=0, batch_slots=16):
+        cfg = ServerConfig(max_active, chunk_steps, hbm_fraction, hbm_budget, batch_slots)
         h = C.c_void_p()
         _check(lib().kh_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -60,7 +60,25 @@ struct Grid {
     double wave_x, wave_y, wave_z;
     long long nreal;         // nx*ny*nz
     long long nspec;         // nx*ny*nzp (allocated spectral elements, row padding included)
+    // A batch of runs of this grid stepped by the same launches (kh_batch):
+    // grid z of a launch is the run, run b's buffers sit b * bstride bytes
+    // after run 0's (equal device arenas), and a run with bactive[b] == 0 is
+    // skipped. nbatch <= 1: one run.
+    int        nbatch;
+    long long  bstride;
+    const int *bactive;
 };
+
+// Launch grid z of a pass over g's batch.
+inline unsigned batch_z(const Grid &g) { return g.nbatch > 1 ? (unsigned)g.nbatch : 1u; }
+// This block's run of the batch is not stepped.
+__device__ __forceinline__ bool batch_skip(const Grid &g) { return g.bactive && !g.bactive[blockIdx.z]; }
+// p, a buffer of run 0, moved to this block's run of the batch.
+template <typename P>
+__device__ __forceinline__ P *brun(P *p, const Grid &g) {
+    if (!p || !g.bstride) return p;
+    return reinterpret_cast<P *>(reinterpret_cast<uintptr_t>(p) + (uintptr_t)((long long)blockIdx.z * g.bstride));
+}
 
 __device__ __forceinline__ int kx_of(const Grid &g, int i) { return (i > g.hnx) ? i - g.nx : i; }
 __device__ __forceinline__ int ky_of(const Grid &g, int j) { return (j > g.hny) ? j - g.ny : j; }

@@ -93,9 +93,10 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_r2c(const T *__
 
 template <typename T, int L>
 __global__ __launch_bounds__(PassThreads<T>::value) void k_fft_z_c2r(ZIO<T> io, long long nrows, T scale,
-                                                                     int kzvalid, int pitch) {
-    const cpx<T> *__restrict__ in  = io.in[blockIdx.y];  // one component per grid row
-    T *__restrict__            out = io.out[blockIdx.y];
+                                                                     int kzvalid, int pitch, Grid bg) {
+    if (batch_skip(bg)) return;
+    const cpx<T> *__restrict__ in  = brun(io.in[blockIdx.y], bg);  // one component per grid row
+    T *__restrict__            out = brun(io.out[blockIdx.y], bg);
     using Z = ZPass<T, L>;
     constexpr int NT = Z::NT, P = Z::P, NZH = Z::NZH, BS = Z::BS;
     __shared__ cpx<T> s[Z::SZ], tw[L];
@@ -199,13 +200,13 @@ static hipError_t launch_z_r2c(int L, const T *in, cpx<T> *out, long long nrows,
 
 template <typename T>
 static hipError_t launch_z_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzvalid,
-                               int pitch, hipStream_t s) {
+                               int pitch, hipStream_t s, const Grid &bg) {
     switch (L) {
 #define KH_CASE(n)                                                                       \
     case n: {                                                                            \
         constexpr int P  = ZPass<T, n>::P;                                               \
         long long     nb = (nrows + 2 * P - 1) / (2 * P);                                \
-        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb, ncomp), dim3(ZPass<T, n>::NT), 0, s, io, nrows, scale, kzvalid, pitch); \
+        hipLaunchKernelGGL((k_fft_z_c2r<T, n>), dim3((unsigned)nb, ncomp, batch_z(bg)), dim3(ZPass<T, n>::NT), 0, s, io, nrows, scale, kzvalid, pitch, bg); \
         return hipGetLastError();                                                        \
     }
         KH_FOR_EACH_L(KH_CASE)
@@ -248,8 +249,8 @@ template <typename T>
 hipError_t fft_z_c2r_multi(const Grid &g, const ZIO<T> &io, int ncomp, hipStream_t s, int kzvalid, const T *mask) {
     const T   scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
     const int kzv   = kzvalid < 0 ? g.nzh : kzvalid;
-    if (zw_supported(g.nz)) return zw_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s, mask);
-    return launch_z_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s);
+    if (zw_supported(g.nz)) return zw_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s, mask, nullptr, &g);
+    return launch_z_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s, g);
 }
 // y: lines along y at stride nzp, columns kz < nzh, one outer block per x
 template <typename T, bool FWD>

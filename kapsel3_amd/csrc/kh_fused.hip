@@ -67,8 +67,8 @@ struct ZTile<float> {
 template <typename T, int L, bool FWD, bool XMODE>
 __global__ __launch_bounds__(PassThreads<T>::value) void k_cols(Comps<T> comps, Grid g, int ncols, int in_mask,
                                                                 int out_mask, const int *__restrict__ skip) {
-    if (skip && *skip) return;
-    cpx<T> *__restrict__ data = comps.p[blockIdx.y];  // one component per grid row
+    if ((skip && *skip) || batch_skip(g)) return;
+    cpx<T> *__restrict__ data = brun(comps.p[blockIdx.y], g);  // one component per grid row
     constexpr int NT  = PassThreads<T>::value;
     constexpr int C   = ColTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
@@ -168,6 +168,9 @@ template <typename T, int L>
 __global__ __launch_bounds__(XZTile<T>::NT) void k_x_zeta2u_inv(
     Grid g, const cpx<T> *__restrict__ z0, const cpx<T> *__restrict__ z1, cpx<T> *__restrict__ u0,
     cpx<T> *__restrict__ u1, cpx<T> *__restrict__ u2, double *__restrict__ ukdc, int fixmask) {
+    if (batch_skip(g)) return;
+    z0 = brun(z0, g), z1 = brun(z1, g), u0 = brun(u0, g), u1 = brun(u1, g), u2 = brun(u2, g);
+    ukdc = brun(ukdc, g);
     constexpr int NT  = XZTile<T>::NT;
     constexpr int C   = XZTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
@@ -242,6 +245,9 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_adv(
     Grid g, const cpx<T> *__restrict__ s0g, const cpx<T> *__restrict__ s1g, const cpx<T> *__restrict__ s2g,
     cpx<T> *__restrict__ p0, cpx<T> *__restrict__ p1, cpx<T> *__restrict__ p2, cpx<T> *__restrict__ p3,
     cpx<T> *__restrict__ p4, T scale) {
+    if (batch_skip(g)) return;
+    s0g = brun(s0g, g), s1g = brun(s1g, g), s2g = brun(s2g, g);
+    p0 = brun(p0, g), p1 = brun(p1, g), p2 = brun(p2, g), p3 = brun(p3, g), p4 = brun(p4, g);
     constexpr int NT  = PassThreads<T>::value;
     constexpr int P   = (ZTile<T>::ELEM / L) > 0 ? ZTile<T>::ELEM / L : 1;
     constexpr int NZH = L / 2 + 1;
@@ -334,6 +340,10 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_force(
     Grid g, const T *__restrict__ up0, const T *__restrict__ up1, const T *__restrict__ up2,
     const T *__restrict__ phis, const T *__restrict__ us0, const T *__restrict__ us1, const T *__restrict__ us2,
     cpx<T> *__restrict__ f0, cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2) {
+    if (batch_skip(g)) return;
+    up0 = brun(up0, g), up1 = brun(up1, g), up2 = brun(up2, g), phis = brun(phis, g);
+    us0 = brun(us0, g), us1 = brun(us1, g), us2 = brun(us2, g);
+    f0 = brun(f0, g), f1 = brun(f1, g), f2 = brun(f2, g);
     constexpr int NT  = PassThreads<T>::value;
     constexpr int P   = (ZTile<T>::ELEM / L) > 0 ? ZTile<T>::ELEM / L : 1;
     constexpr int NZH = L / 2 + 1;
@@ -400,6 +410,9 @@ template <typename T, int L>
 __global__ __launch_bounds__(XTile<T>::NT) void k_x_adv_ns(Grid g, Comps<T> prod, cpx<T> *__restrict__ z0,
                                                                     cpx<T> *__restrict__ z1, double dmy0,
                                                                     double dmy1) {
+    if (batch_skip(g)) return;
+    for (int q = 0; q < 5; q++) prod.p[q] = brun(prod.p[q], g);
+    z0 = brun(z0, g), z1 = brun(z1, g);
     constexpr int NT  = XTile<T>::NT;
     constexpr int C   = XTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
@@ -463,6 +476,8 @@ __global__ __launch_bounds__(XTile<T>::NT) void k_x_f_zeta(Grid g, cpx<T> *__res
                                                                     cpx<T> *__restrict__ f1, cpx<T> *__restrict__ f2,
                                                                     cpx<T> *__restrict__ z0, cpx<T> *__restrict__ z1,
                                                                     double *__restrict__ ukdc) {
+    if (batch_skip(g)) return;
+    f0 = brun(f0, g), f1 = brun(f1, g), f2 = brun(f2, g), z0 = brun(z0, g), z1 = brun(z1, g), ukdc = brun(ukdc, g);
     constexpr int NT  = XTile<T>::NT;
     constexpr int C   = XTile<T>::C;
     constexpr int BS  = ColStride<T, C>::value;
@@ -720,7 +735,7 @@ hipError_t cols_y(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, i
     switch (g.ny) {
 #define KH_CASE(n)                                                                                                 \
     case n:                                                                                                        \
-        hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt, ncomp), dim3(PassThreads<T>::value), 0, s, \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, false>), dim3(g.nx * nt, ncomp, batch_z(g)), dim3(PassThreads<T>::value), 0, s, \
                            data, g, g.trn_z, in_mask, out_mask, skip);                                             \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -736,7 +751,7 @@ hipError_t cols_x(const Grid &g, const Comps<T> &data, int ncomp, int in_mask, i
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt, ncomp), dim3(PassThreads<T>::value), 0, s, \
+        hipLaunchKernelGGL((k_cols<T, n, FWD, true>), dim3(nky(g) * nt, ncomp, batch_z(g)), dim3(PassThreads<T>::value), 0, s, \
                            data, g, g.trn_z, in_mask, out_mask, nullptr);                                           \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -753,7 +768,7 @@ hipError_t x_zeta2u_inv(const Grid &g, const cpx<T> *z0, const cpx<T> *z1, cpx<T
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                   \
     case n:                                                                                                          \
-        hipLaunchKernelGGL((k_x_zeta2u_inv<T, n>), dim3(nky(g) * nt), dim3(XZTile<T>::NT), 0, s, g, z0, z1, u[0], u[1],    \
+        hipLaunchKernelGGL((k_x_zeta2u_inv<T, n>), dim3(nky(g) * nt, 1, batch_z(g)), dim3(XZTile<T>::NT), 0, s, g, z0, z1, u[0], u[1],    \
                            u[2], ukdc, fixmask);                                                                     \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -771,7 +786,7 @@ hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStr
 #define KH_CASE(n)                                                                                                 \
     case n: {                                                                                                      \
         constexpr int P = (ZTile<T>::ELEM / n) > 0 ? ZTile<T>::ELEM / n : 1;                                       \
-        hipLaunchKernelGGL((k_z_adv<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(PassThreads<T>::value), 0, s, g, \
+        hipLaunchKernelGGL((k_z_adv<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P)), 1, batch_z(g)), dim3(PassThreads<T>::value), 0, s, g, \
                            s3[0], s3[1], s3[2], p5[0], p5[1], p5[2], p5[3], p5[4], scale);                         \
         return hipGetLastError();                                                                                  \
     }
@@ -793,7 +808,7 @@ hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T
 #define KH_CASE(n)                                                                                                   \
     case n: {                                                                                                        \
         constexpr int P = (ZTile<T>::ELEM / n) > 0 ? ZTile<T>::ELEM / n : 1;                                         \
-        hipLaunchKernelGGL((k_z_force<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(PassThreads<T>::value), 0, s, g, \
+        hipLaunchKernelGGL((k_z_force<T, n>), dim3((unsigned)((nrows + 2 * P - 1) / (2 * P)), 1, batch_z(g)), dim3(PassThreads<T>::value), 0, s, g, \
                            up[0], up[1], up[2], phis, us[0], us[1], us[2], f[0], f[1], f[2]);                        \
         return hipGetLastError();                                                                                    \
     }
@@ -811,7 +826,7 @@ hipError_t x_adv_ns(const Grid &g, const Comps<T> &prod, cpx<T> *z0, cpx<T> *z1,
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                   \
     case n:                                                                                                          \
-        hipLaunchKernelGGL((k_x_adv_ns<T, n>), dim3(nky(g) * nt), dim3(XTile<T>::NT), 0, s, g, prod, z0, z1, \
+        hipLaunchKernelGGL((k_x_adv_ns<T, n>), dim3(nky(g) * nt, 1, batch_z(g)), dim3(XTile<T>::NT), 0, s, g, prod, z0, z1, \
                            dmy0, dmy1);                                                                              \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)
@@ -827,7 +842,7 @@ hipError_t x_f_zeta(const Grid &g, cpx<T> *const f[3], cpx<T> *z0, cpx<T> *z1, d
     switch (g.nx) {
 #define KH_CASE(n)                                                                                                 \
     case n:                                                                                                        \
-        hipLaunchKernelGGL((k_x_f_zeta<T, n>), dim3(nky(g) * nt), dim3(XTile<T>::NT), 0, s, g, f[0], f[1], f[2], z0, z1, \
+        hipLaunchKernelGGL((k_x_f_zeta<T, n>), dim3(nky(g) * nt, 1, batch_z(g)), dim3(XTile<T>::NT), 0, s, g, f[0], f[1], f[2], z0, z1, \
                            ukdc);                                                                                  \
         return hipGetLastError();
         KH_FOR_EACH_L(KH_CASE)

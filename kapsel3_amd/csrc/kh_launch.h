@@ -26,7 +26,7 @@ hipError_t f_nonsole(long long n, T *const up[3], const T *phisum, T *const u[3]
 template <typename T>
 hipError_t add3(long long n, T *const u[3], T *const f[3], hipStream_t s);
 template <typename T>
-hipError_t fill(long long n, T *p, T v, hipStream_t s);
+hipError_t fill(long long n, T *p, T v, hipStream_t s, const Grid *batch = nullptr);
 
 // kh_shear.hip (the Lees-Edwards step; gd = strain of the oblique grid)
 template <typename T>
@@ -79,7 +79,7 @@ hipError_t velocity(const PParams &pp, DevParticle *P, int ab2, double h, hipStr
                     double nio = 0.0);
 hipError_t thermostat(const PParams &pp, DevParticle *P, uint64_t seed, uint32_t stream, const int *d_ts, double niv,
                       double nio, hipStream_t s);
-hipError_t tick(int *d_ts, hipStream_t s);
+hipError_t tick(int *d_ts, hipStream_t s, const Grid *batch = nullptr);
 // the sheared step's stress record from the per-particle partials and the
 // nacc Reynolds partials of k_obl_to_cart
 hipError_t obl_record(const PParams &pp, const double *acc, long long nacc, const ShearRec &h, double *last,
@@ -94,6 +94,17 @@ template <typename T>
 hipError_t particle_step(const PParams &pp, DevParticle *P, T *const u[3], const T *phisum, const PCache<T> &pc,
                          double coef, int ab2, double h, int noise, uint64_t seed, uint32_t stream, const int *d_ts,
                          double niv, double nio, T *const up[3], hipStream_t s);
+// the phi scatter (with the position update), the fused particle step and
+// the records of a batch of runs, grid z = slot (kh_batch; at most 256
+// particles per run)
+template <typename T>
+hipError_t phi_scatter_batch(const SlotPart<T> *sp, const int *active, int nslots, int np_cap, double dt, double hdt,
+                             hipStream_t s);
+template <typename T>
+hipError_t particle_step_batch(const SlotPart<T> *sp, const int *active, int nslots, int np_cap, double coef, double h,
+                               hipStream_t s);
+template <typename T>
+hipError_t records_batch(const SlotPart<T> *sp, const int *active, int nslots, hipStream_t s);
 // squirmer slip iteration (sp_3d_ns.cxx:121-161). A non-null `skip` makes
 // a launch do nothing once *skip is nonzero: the loop decided on the device
 // (slip_conv with a SlipLoop) launches every iteration up to the maximum,

@@ -99,6 +99,32 @@ struct PCache {
     int *cnt;  // np entries
 };
 
+// Add_random_force_thermostat arguments (fluct.cxx:10-100)
+struct Noise {
+    int        on;
+    uint64_t   seed;
+    uint32_t   stream;
+    const int *d_ts;
+    double     niv, nio;
+};
+
+// One run's arguments of the particle launches of a batch step (kh_batch):
+// the run's own values of what the per-run launches take; nz.d_ts is the
+// run's step counter.
+template <typename T>
+struct SlotPart {
+    PParams      pp;
+    DevParticle *P;
+    T           *phisum;
+    PCache<T>    pc;
+    T           *u[3], *up[3];
+    Noise        nz;
+    double      *traj;
+    int          traj_base, traj_steps;
+    double      *frames;
+    int          frame_ts0, frame_every, frame_n;
+};
+
 __host__ __device__ inline int sek_pack(int i, int j, int k) { return (i + 128) | ((j + 128) << 8) | ((k + 128) << 16); }
 __device__ __forceinline__ void sek_unpack(int c, int *o) {
     o[0] = (c & 255) - 128;

@@ -91,11 +91,10 @@ __device__ __forceinline__ void angular2v(int rotation, const double *om, const 
 // particle (the position integrator of the same step, one lane).
 __device__ void position_one(const PParams &pp, DevParticle *P, int n, int ab2, double dt, double hdt);
 
-template <typename T, bool SCATTER, bool MOVE, int NT, bool OBL = false>
-__global__ __launch_bounds__(NT) void k_phi_scatter(PParams pp, DevParticle *__restrict__ P,
-                                                       T *__restrict__ phisum, PCache<T> pc, int ab2, double dt,
-                                                       double hdt) {
-    const int n = blockIdx.x;
+template <typename T, bool SCATTER, bool MOVE, int NT, bool OBL>
+__device__ __forceinline__ void phi_scatter_block(const PParams &pp, DevParticle *__restrict__ P,
+                                                  T *__restrict__ phisum, const PCache<T> &pc, int ab2, double dt,
+                                                  double hdt, int n) {
     // the first Sekibun offset of this lane, loaded before the position
     // update so its latency hides behind it; each round loads the next
     int next = threadIdx.x < pp.np_domain ? pp.sek[threadIdx.x] : 0;
@@ -159,6 +158,12 @@ __global__ __launch_bounds__(NT) void k_phi_scatter(PParams pp, DevParticle *__r
         off += tot;
     }
     if (threadIdx.x == 0) pc.cnt[n] = off;
+}
+template <typename T, bool SCATTER, bool MOVE, int NT, bool OBL = false>
+__global__ __launch_bounds__(NT) void k_phi_scatter(PParams pp, DevParticle *__restrict__ P,
+                                                       T *__restrict__ phisum, PCache<T> pc, int ab2, double dt,
+                                                       double hdt) {
+    phi_scatter_block<T, SCATTER, MOVE, NT, OBL>(pp, P, phisum, pc, ab2, dt, hdt, blockIdx.x);
 }
 
 // r = cell*dx - residue (Relative_coord, make_phi.h:154-170); on a sheared
@@ -590,13 +595,6 @@ hipError_t obl_record(const PParams &pp, const double *acc, long long nacc, cons
 
 // MD_solver_velocity_Euler (particle_solver.cxx:103-142) / _AB2_hydro
 // (239-281), self_propulsion (88-101) and reset_Forces (11-37).
-struct Noise {  // Add_random_force_thermostat arguments (fluct.cxx:10-100)
-    int        on;
-    uint64_t   seed;
-    uint32_t   stream;
-    const int *d_ts;
-    double     niv, nio;
-};
 __device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise &nz);
 
 __device__ void velocity_one(const PParams &pp, DevParticle *__restrict__ P, int n, int ab2, double h,
@@ -718,13 +716,12 @@ __device__ void add_noise(const PParams &pp, DevParticle *P, int n, const Noise 
 // chip idle and the step's time is one workgroup's chain of dependent
 // loads: NT = 1024 splits each particle's points over four times the lanes.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT, NT == PT_NT ? 8 : 1) void k_particle_step(PParams pp, DevParticle *__restrict__ P,
-                                                         const T *__restrict__ u0, const T *__restrict__ u1,
-                                                         const T *__restrict__ u2, const T *__restrict__ phisum,
-                                                         PCache<T> pc, double coef, int ab2, double h, Noise nz,
-                                                         T *__restrict__ up0, T *__restrict__ up1,
-                                                         T *__restrict__ up2) {
-    const int               n = blockIdx.x;
+__device__ __forceinline__ void particle_step_block(const PParams &pp, DevParticle *__restrict__ P,
+                                                    const T *__restrict__ u0, const T *__restrict__ u1,
+                                                    const T *__restrict__ u2, const T *__restrict__ phisum,
+                                                    const PCache<T> &pc, double coef, int ab2, double h,
+                                                    const Noise &nz, T *__restrict__ up0, T *__restrict__ up1,
+                                                    T *__restrict__ up2, int n) {
     __shared__ PStash<T>    stash;
     f_hydro_block<T, NT>(pp, P, n, u0, u1, u2, phisum, pc, coef, &stash);
     forces_block<NT>(pp, P, n);
@@ -740,6 +737,37 @@ __global__ __launch_bounds__(NT, NT == PT_NT ? 8 : 1) void k_particle_step(PPara
     __syncthreads();
     // each lane reads back only the stash entries it wrote
     up_scatter_block<T, NT>(pp, n, st, st + 3, st + 6, phisum, pc, up0, up1, up2, &stash);
+}
+template <typename T, int NT>
+__global__ __launch_bounds__(NT, NT == PT_NT ? 8 : 1) void k_particle_step(PParams pp, DevParticle *__restrict__ P,
+                                                         const T *__restrict__ u0, const T *__restrict__ u1,
+                                                         const T *__restrict__ u2, const T *__restrict__ phisum,
+                                                         PCache<T> pc, double coef, int ab2, double h, Noise nz,
+                                                         T *__restrict__ up0, T *__restrict__ up1,
+                                                         T *__restrict__ up2) {
+    particle_step_block<T, NT>(pp, P, u0, u1, u2, phisum, pc, coef, ab2, h, nz, up0, up1, up2, blockIdx.x);
+}
+
+// The same two particle launches of a steady-state step for a batch of runs
+// (kh_batch): grid z is the slot, each slot's arguments come from its
+// SlotPart record, slots with active[z] == 0 and blocks past the slot's
+// particle count return.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_phi_scatter_batch(const SlotPart<T> *__restrict__ sp,
+                                                          const int *__restrict__ active, double dt, double hdt) {
+    if (!active[blockIdx.z]) return;
+    const SlotPart<T> &a = sp[blockIdx.z];
+    if ((int)blockIdx.x >= a.pp.np) return;
+    phi_scatter_block<T, true, true, NT, false>(a.pp, a.P, a.phisum, a.pc, 1, dt, hdt, blockIdx.x);
+}
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_particle_step_batch(const SlotPart<T> *__restrict__ sp,
+                                                            const int *__restrict__ active, double coef, double h) {
+    if (!active[blockIdx.z]) return;
+    const SlotPart<T> &a = sp[blockIdx.z];
+    if ((int)blockIdx.x >= a.pp.np) return;
+    particle_step_block<T, NT>(a.pp, a.P, a.u[0], a.u[1], a.u[2], a.phisum, a.pc, coef, 1, h, a.nz, a.up[0], a.up[1],
+                               a.up[2], blockIdx.x);
 }
 
 // ------------------------------------------------------------ squirmers
@@ -1427,11 +1455,13 @@ hipError_t rigid_velocity(const PParams &pp, DevParticle *P, const RigidArgs &ra
 }
 
 // jikan.ts += 1 on the device (the step counter the noise stream is keyed by)
-__global__ void k_tick(int *__restrict__ d_ts) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *d_ts += 1;
+__global__ void k_tick(int *__restrict__ d_ts, Grid bg) {
+    if (batch_skip(bg)) return;
+    if (threadIdx.x == 0 && blockIdx.x == 0) *brun(d_ts, bg) += 1;
 }
-hipError_t tick(int *d_ts, hipStream_t s) {
-    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, d_ts);
+hipError_t tick(int *d_ts, hipStream_t s, const Grid *batch) {
+    const Grid bg = batch ? *batch : Grid{};
+    hipLaunchKernelGGL(k_tick, dim3(1, 1, batch_z(bg)), dim3(64), 0, s, d_ts, bg);
     return hipGetLastError();
 }
 
@@ -1493,9 +1523,9 @@ hipError_t slip_loop_begin(const SlipLoop &loop, hipStream_t s) {
 // Per-step particle record: after the step that made ts = *d_ts, slot
 // ts - base - 1 receives (x, v) of every particle; slots past nslots are
 // not written.
-__global__ void k_record(PParams pp, const DevParticle *__restrict__ P, const int *__restrict__ d_ts, int base,
-                         int nslots, double *__restrict__ out) {
-    const int n    = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void record_one(const PParams &pp, const DevParticle *__restrict__ P,
+                                           const int *__restrict__ d_ts, int base, int nslots,
+                                           double *__restrict__ out, int n) {
     const int slot = *d_ts - base - 1;
     if (n >= pp.np || slot < 0 || slot >= nslots) return;
     double *o = out + ((size_t)slot * pp.np + n) * 6;
@@ -1503,6 +1533,10 @@ __global__ void k_record(PParams pp, const DevParticle *__restrict__ P, const in
         o[d]     = P[n].x[d];
         o[3 + d] = P[n].v[d];
     }
+}
+__global__ void k_record(PParams pp, const DevParticle *__restrict__ P, const int *__restrict__ d_ts, int base,
+                         int nslots, double *__restrict__ out) {
+    record_one(pp, P, d_ts, base, nslots, out, blockIdx.x * blockDim.x + threadIdx.x);
 }
 hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int base, int nslots, double *out,
                   hipStream_t s) {
@@ -1515,9 +1549,9 @@ hipError_t record(const PParams &pp, const DevParticle *P, const int *d_ts, int 
 // R_raw, v, q (q0..q3), omega, f_hydro, torque_hydro, f_r, torque_r (0),
 // f_slip, torque_slip}, the forces and torques being the previous step's, as
 // the reference writes them.
-__global__ void k_frame(PParams pp, const DevParticle *__restrict__ P, const int *__restrict__ d_ts, int ts0,
-                        int every, int nframes, double *__restrict__ out) {
-    const int n  = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void frame_one(const PParams &pp, const DevParticle *__restrict__ P,
+                                          const int *__restrict__ d_ts, int ts0, int every, int nframes,
+                                          double *__restrict__ out, int n) {
     const int ts = *d_ts, dt = ts - ts0;
     if (dt < 0 || dt % every != 0 || dt / every >= nframes) return;
     double *f = out + (size_t)(dt / every) * (1 + (size_t)FRAME_WORDS * pp.np);
@@ -1536,6 +1570,20 @@ __global__ void k_frame(PParams pp, const DevParticle *__restrict__ P, const int
         }
     }
 }
+__global__ void k_frame(PParams pp, const DevParticle *__restrict__ P, const int *__restrict__ d_ts, int ts0,
+                        int every, int nframes, double *__restrict__ out) {
+    frame_one(pp, P, d_ts, ts0, every, nframes, out, blockIdx.x * blockDim.x + threadIdx.x);
+}
+// the per-step record and frames of every stepped run of a batch (grid z = slot)
+template <typename T>
+__global__ void k_records_batch(const SlotPart<T> *__restrict__ sp, const int *__restrict__ active) {
+    if (!active[blockIdx.z]) return;
+    const SlotPart<T> &a = sp[blockIdx.z];
+    for (int n = threadIdx.x; n < (a.pp.np > 0 ? a.pp.np : 1); n += blockDim.x) {
+        if (a.traj) record_one(a.pp, a.P, a.nz.d_ts, a.traj_base, a.traj_steps, a.traj, n);
+        if (a.frames) frame_one(a.pp, a.P, a.nz.d_ts, a.frame_ts0, a.frame_every, a.frame_n, a.frames, n);
+    }
+}
 hipError_t frame(const PParams &pp, const DevParticle *P, const int *d_ts, int ts0, int every, int nframes, double *out,
                  hipStream_t s) {
     hipLaunchKernelGGL(k_frame, dim3(pblocks(pp.np > 0 ? pp.np : 1)), dim3(64), 0, s, pp, P, d_ts, ts0, every, nframes,
@@ -1543,7 +1591,34 @@ hipError_t frame(const PParams &pp, const DevParticle *P, const int *d_ts, int t
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t phi_scatter_batch(const SlotPart<T> *sp, const int *active, int nslots, int np_cap, double dt, double hdt,
+                             hipStream_t s) {
+    if (np_cap < 1 || np_cap > 256) return hipErrorInvalidValue;  // the 1024-lane form of few particles
+    hipLaunchKernelGGL((k_phi_scatter_batch<T, PT_WIDE>), dim3(np_cap, 1, nslots), dim3(PT_WIDE), 0, s, sp, active, dt,
+                       hdt);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t particle_step_batch(const SlotPart<T> *sp, const int *active, int nslots, int np_cap, double coef, double h,
+                               hipStream_t s) {
+    if (np_cap < 1 || np_cap > 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_particle_step_batch<T, PT_WIDE>), dim3(np_cap, 1, nslots), dim3(PT_WIDE), 0, s, sp, active,
+                       coef, h);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t records_batch(const SlotPart<T> *sp, const int *active, int nslots, hipStream_t s) {
+    hipLaunchKernelGGL(k_records_batch<T>, dim3(1, 1, nslots), dim3(64), 0, s, sp, active);
+    return hipGetLastError();
+}
+
 #define KH_INST(T)                                                                                               \
+    template hipError_t phi_scatter_batch<T>(const SlotPart<T> *, const int *, int, int, double, double,          \
+                                             hipStream_t);                                                       \
+    template hipError_t particle_step_batch<T>(const SlotPart<T> *, const int *, int, int, double, double,        \
+                                               hipStream_t);                                                     \
+    template hipError_t records_batch<T>(const SlotPart<T> *, const int *, int, hipStream_t);                     \
     template hipError_t phi_scatter<T>(const PParams &, DevParticle *, T *, const PCache<T> &, int, hipStream_t, int, \
                                        int, double, double);                                                       \
     template hipError_t f_hydro<T>(const PParams &, DevParticle *, T *const[3], const T *, const PCache<T> &,       \

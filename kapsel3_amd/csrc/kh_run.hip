@@ -8,13 +8,16 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
 #include "../../include/kapsel_hip.h"
+#include "kh_batch.h"
 #include "kh_fused.h"
 #include "kh_launch.h"
 #include "kh_zwave.h"
@@ -74,6 +77,14 @@ struct kh_run {
     std::vector<kh_species> species;
     int                     device = 0;
     hipStream_t             stream = nullptr;
+    hipStream_t             own_stream = nullptr;  // stream created with the run (stream differs while batched)
+    // a run inside a batch arena (kh_batch): its device buffers are
+    // sub-allocated from arena in a fixed order, slot * stride after the
+    // batch's first arena
+    char                   *arena      = nullptr;
+    size_t                  arena_used = 0;
+    int                     slot       = -1;
+    bool                    batched    = false;
     Grid                    g;
     PParams                 pp;
     int                     mixed = 0;
@@ -152,6 +163,30 @@ struct kh_run {
     std::vector<PendingEvent>          pending;
     std::vector<hipEvent_t>            event_pool;
     std::vector<std::string>           names;
+};
+
+// A batch of runs of one grid (the serving loop's size classes): `capacity`
+// equal arenas of `stride` bytes in one allocation, and one stream on which
+// the joined runs step together. The batched launches skip slots whose
+// device flag active[slot] is 0 (free slots, and runs still stepping alone).
+// A batch step is captured once as a graph over every slot: which slots
+// step is the device flags' business, and each joined run's particle
+// arguments sit in a device table (SlotPart per slot), so runs join and
+// leave without a new capture.
+struct kh_batch {
+    kh_params             P{};  // the class: grid, precision, dx, rho, eta, dt, FIX_CELL, a, a_xi
+    int                   capacity = 0, np_cap = 0;
+    size_t                stride   = 0;
+    char                 *base     = nullptr;
+    int                  *active   = nullptr;  // device flags, one per slot
+    std::vector<int>      hactive;
+    void                 *dslots = nullptr;    // device SlotPart<T> per slot
+    std::vector<char>     hslots;              // its host copy
+    hipStream_t           stream = nullptr;
+    std::vector<kh_run *> slot_run;  // the run created in each slot (nullptr: free)
+    bool                  dirty  = true;
+    hipGraph_t            graph1 = nullptr, graphk = nullptr;
+    hipGraphExec_t        exec1 = nullptr, execk = nullptr;
 };
 
 namespace {
@@ -476,6 +511,10 @@ struct Step {
             double   *hist = (r->shear_hist && slot >= 0 && slot < r->shear_hist_n) ? r->shear_hist + 12 * slot : nullptr;
             KH_LAUNCH("obl_record", obl_record(r->pp, r->shear_acc, r->shear_nacc, h, r->shear_last, hist, r->stream));
         }
+        return records(r);
+    }
+    // the per-step trajectory record and Output_udf frames, when asked for
+    static int records(kh_run *r) {
         if (r->traj)
             KH_LAUNCH("record", record(r->pp, r->dp, r->d_ts, r->traj_base, r->traj_steps, r->traj, r->stream));
         if (r->frames)
@@ -499,24 +538,14 @@ struct Step {
         return move_scatter_phi(r, ab2);
     }
 
-    static int step_body(kh_run *r) {
-        if (r->step_path == KH_PATH_REFERENCE_ORDER) return step_reference_order(r);
-        const int   ab2 = r->jikan.ts != 0;
-        const Grid &g   = r->g;
-        hipStream_t s   = r->stream;
-        if (r->zeta_dirty) {
-            KH_LAUNCH("zeta_clean", (zeta_clean<T>(g, Z(r, 0), Z(r, 1), s)));
-            r->zeta_dirty = 0;
-        }
-        // The particles move and scatter phi first (the position update
-        // reads only the particle state, so it commutes with the fluid
-        // part): phi_sum is then known when u* is formed, and u* is written
-        // only where it is read. Squirmer runs keep the reference's order
-        // and the whole u* field (the slip iteration reads it on the shell).
-        const bool early = !r->slip || r->slip_dev;
-        if (early) KH_TRY(particles_begin(r, ab2));
-        cpx<T>  *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
-        Comps<T> c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
+    // NS_solver_slavedEuler, FIX_CELL, truncation and Zeta_k2u of the fused
+    // step, up to u* in real space (written only where mask != 0 when a mask
+    // is given), for the run r or, when g carries a batch, for every run of
+    // the batch from r on: one launch per pass covers them all.
+    static int fused_fluid(kh_run *r, const Grid &g, const T *mask) {
+        hipStream_t s     = r->stream;
+        cpx<T>     *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
+        Comps<T>    c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
         // buffer of each advection product (u2u3, u3u1, u1u2, u2^2-u3^2,
         // u3^2-u1^2): z_adv reads u1, u2, u3 from S0, S1, S2 and writes
         // each product only into a buffer it has finished reading
@@ -537,9 +566,55 @@ struct Step {
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
         ZIO<T> zio{{S(r, 0), S(r, 1), S(r, 2)}, {U(r, 0), U(r, 1), U(r, 2)}};
+        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z, mask)));
+        return KH_OK;
+    }
+    // f = P(up - phi u*), u = u* + f, zeta = curl u (u* = u + us2 when us2
+    // is given), and the device step counter; for r or r's batch as above
+    static int fused_force(kh_run *r, const Grid &g, T *const us2[3]) {
+        hipStream_t s      = r->stream;
+        cpx<T>     *s3[3]  = {S(r, 0), S(r, 1), S(r, 2)};
+        Comps<T>    c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
+        T          *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
+        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear ? 3 : 0, r->zclear ? r->d_ts : nullptr, s,
+                                         us2)));
+        if (!r->zclear) KH_LAUNCH("tick", tick(r->d_ts, s, &g));
+        KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
+        KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));
+        return KH_OK;
+    }
+    // F_H, LJ + gravity, velocity (+ noise), u_p scatter of a run of plain
+    // spheres: one launch
+    static int plain_particles(kh_run *r, int ab2) {
+        T     *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *up[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
+        double niv = 0.0, nio = 0.0;
+        if (r->kbt > 0.) noise_amp(r, niv, nio);
+        KH_LAUNCH("particle_step",
+                  (particle_step<T>(r->pp, r->dp, u, PS(r), CP(r), (-r->dx3 * r->rho) / r->jikan.dt_fluid, ab2,
+                                    ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->kbt > 0. ? 1 : 0, r->P.seed,
+                                    r->P.stream_id, r->d_ts, niv, nio, up, r->stream)));
+        return KH_OK;
+    }
+
+    static int step_body(kh_run *r) {
+        if (r->step_path == KH_PATH_REFERENCE_ORDER) return step_reference_order(r);
+        const int   ab2 = r->jikan.ts != 0;
+        const Grid &g   = r->g;
+        hipStream_t s   = r->stream;
+        if (r->zeta_dirty) {
+            KH_LAUNCH("zeta_clean", (zeta_clean<T>(g, Z(r, 0), Z(r, 1), s)));
+            r->zeta_dirty = 0;
+        }
+        // The particles move and scatter phi first (the position update
+        // reads only the particle state, so it commutes with the fluid
+        // part): phi_sum is then known when u* is formed, and u* is written
+        // only where it is read. Squirmer runs with the slip loop decided on
+        // the host keep the reference's order.
+        const bool early = !r->slip || r->slip_dev;
+        if (early) KH_TRY(particles_begin(r, ab2));
         // squirmers read u* on the whole slip shell (|r - a| < xi/2), which
         // reaches points where phi has underflowed to 0: u* is written in full
-        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z, early && !r->slip ? PS(r) : nullptr)));
+        KH_TRY(fused_fluid(r, g, early && !r->slip ? PS(r) : nullptr));
         if (!early) KH_TRY(particles_begin(r, ab2));
         if (r->slip_dev) {  // the slip loop leaves up zero
             KH_TRY(calc_f_hydro(r));
@@ -556,25 +631,65 @@ struct Step {
             KH_TRY(calc_f_hydro(r));
             KH_TRY(velocity_update(r, ab2));
             KH_TRY(scatter_up(r));
-        } else {  // F_H, LJ + gravity, velocity (+ noise), u_p scatter: one launch
-            T     *u[3] = {U(r, 0), U(r, 1), U(r, 2)}, *up[3] = {UP(r, 0), UP(r, 1), UP(r, 2)};
-            double niv = 0.0, nio = 0.0;
-            if (r->kbt > 0.) noise_amp(r, niv, nio);
-            KH_LAUNCH("particle_step",
-                      (particle_step<T>(r->pp, r->dp, u, PS(r), CP(r), (-r->dx3 * r->rho) / r->jikan.dt_fluid,
-                                        ab2, ab2 ? r->jikan.hdt_md : r->jikan.dt_md, r->kbt > 0. ? 1 : 0, r->P.seed,
-                                        r->P.stream_id, r->d_ts, niv, nio, up, r->stream)));
+        } else {
+            KH_TRY(plain_particles(r, ab2));
         }
-        // f = P(up - phi u*), u = u* + f, zeta = curl u
-        T *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
         T *w3[3] = {W(r, 0), W(r, 1), W(r, 2)};  // squirmers: u = u* + the projected slip field
-        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear ? 3 : 0, r->zclear ? r->d_ts : nullptr, s,
-                                         r->slip_dev ? w3 : nullptr)));
-        if (!r->zclear) KH_LAUNCH("tick", tick(r->d_ts, s));
+        KH_TRY(fused_force(r, g, r->slip_dev ? w3 : nullptr));
         r->pclean = r->zclear;
-        KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
-        KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));
         return KH_OK;
+    }
+
+    // One steady-state step of every joined run of a batch (plain spheres
+    // on one grid, ts >= 1): one launch per pass over all slots. v is a view
+    // of slot 0 (a run's state with its device pointers moved to slot 0's
+    // arena) that supplies the fluid passes' run-0 buffers and the class's
+    // constants; the particle launches take each slot's own arguments from
+    // the batch's SlotPart table.
+    static int step_batch(kh_batch *b, kh_run *v) {
+        kh_run     *r = v;
+        Grid        g = r->g;
+        g.nbatch      = b->capacity;
+        g.bstride     = (long long)b->stride;
+        g.bactive     = b->active;
+        hipStream_t s = r->stream;
+        const SlotPart<T> *sp = (const SlotPart<T> *)b->dslots;
+        if (!r->zclear) {  // Reset_phi / Reset_u
+            for (int d = 0; d < 3; d++) KH_LAUNCH("fill", (fill<T>(g.nreal, UP(r, d), (T)0, s, &g)));
+            KH_LAUNCH("fill", (fill<T>(g.nreal, PS(r), (T)0, s, &g)));
+        }
+        KH_LAUNCH("phi_scatter", (phi_scatter_batch<T>(sp, b->active, b->capacity, b->np_cap, r->jikan.dt_md,
+                                                       r->jikan.hdt_md, s)));
+        KH_TRY(fused_fluid(r, g, PS(r)));
+        KH_LAUNCH("particle_step", (particle_step_batch<T>(sp, b->active, b->capacity, b->np_cap,
+                                                           (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->jikan.hdt_md,
+                                                           s)));
+        KH_TRY(fused_force(r, g, nullptr));
+        KH_LAUNCH("records", (records_batch<T>(sp, b->active, b->capacity, s)));
+        return KH_OK;
+    }
+    // a run's arguments of the batched particle launches
+    static SlotPart<T> slot_part(kh_run *r) {
+        SlotPart<T> a{};
+        a.pp     = r->pp;
+        a.P      = r->dp;
+        a.phisum = PS(r);
+        a.pc     = CP(r);
+        for (int d = 0; d < 3; d++) {
+            a.u[d]  = U(r, d);
+            a.up[d] = UP(r, d);
+        }
+        double niv = 0.0, nio = 0.0;
+        if (r->kbt > 0.) noise_amp(r, niv, nio);
+        a.nz          = Noise{r->kbt > 0. ? 1 : 0, r->P.seed, r->P.stream_id, r->d_ts, niv, nio};
+        a.traj        = r->traj;
+        a.traj_base   = r->traj_base;
+        a.traj_steps  = r->traj_steps;
+        a.frames      = r->frames;
+        a.frame_ts0   = r->frame_ts0;
+        a.frame_every = r->frame_every;
+        a.frame_n     = r->frame_n;
+        return a;
     }
 
     // Time_evolution_hydro in the reference's own operation order, one
@@ -922,13 +1037,15 @@ void drop_graph(kh_run *r);
 int free_run(kh_run *r) {
     if (!r) return KH_OK;
     if (r->stream) (void)hipStreamSynchronize(r->stream);
+    r->stream = r->own_stream;  // a batched run goes back to its own stream
     drop_graph(r);
     void *bufs[] = {r->zeta[0], r->zeta[1], r->spec[0], r->spec[1], r->spec[2], r->spec[3], r->spec[4],
                     r->ureal[0], r->ureal[1], r->ureal[2], r->up[0], r->up[1], r->up[2], r->work[0], r->work[1],
                     r->work[2], r->work[3], r->work[4], r->phisum, r->ukdc, r->dp, r->ds, r->sek, r->cphi, r->cidx,
                     r->ccode, r->ccnt, r->d_ts, r->slipeps, r->slipdc, r->slipflag, r->xs};
-    for (void *b : bufs)
-        if (b) (void)hipFreeAsync(b, r->stream);
+    if (!r->arena)  // buffers carved from a batch arena stay with the arena
+        for (void *b : bufs)
+            if (b) (void)hipFreeAsync(b, r->stream);
     if (r->traj) (void)hipFreeAsync(r->traj, r->stream);
     if (r->frames) (void)hipFreeAsync(r->frames, r->stream);
     for (double *b : {r->shear_pstress, r->shear_acc, r->shear_last, r->shear_hist})
@@ -1218,11 +1335,18 @@ const char *kh_strerror(int s) {
     }
 }
 
-int kh_run_create(const kh_params *P, int np, kh_run **out) {
+}  // extern "C"
+
+// kh_run_create; with an arena, the run's device buffers are carved from it
+// (in the same order and sizes for every run of one grid and precision, so
+// the field buffers sit at the same offsets in every arena) and not freed
+// with the run.
+int run_create(const kh_params *P, int np, char *arena, size_t arena_bytes, kh_run **out) {
     if (!out) return fail(KH_ERR_ARG, "out is NULL");
     *out = nullptr;
     KH_TRY(check_params(P, np));
     kh_run *r = new kh_run();
+    r->arena  = arena;
     r->P      = *P;
     r->species.assign(P->species, P->species + P->nspecies);
     r->P.species = r->species.data();
@@ -1237,6 +1361,7 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     if (hipGetDevice(&r->device) != hipSuccess) return bail(fail(KH_ERR_HIP, "no HIP device"));
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(KH_ERR_HIP, "hipStreamCreate failed"));
+    r->own_stream = r->stream;
 
     // Set_global_parameters (input.cxx:424-600)
     Grid &g = r->g;
@@ -1320,6 +1445,16 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     // stream-ordered allocations: creating or destroying a run does not
     // synchronise the device, so runs on other streams keep going
     auto alloc = [&](void **p, size_t bytes) -> int {
+        if (arena) {  // 256-byte aligned, in allocation order
+            const size_t off = (r->arena_used + 255) / 256 * 256;
+            if (off + bytes > arena_bytes) {
+                *p = nullptr;
+                return fail(KH_ERR_NOMEM, "run needs more than its %zu-byte arena", arena_bytes);
+            }
+            *p            = arena + off;
+            r->arena_used = off + bytes;
+            return KH_OK;
+        }
         if (hipMallocAsync(p, bytes, r->stream) != hipSuccess) {
             *p = nullptr;
             return fail(KH_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
@@ -1407,6 +1542,10 @@ int kh_run_create(const kh_params *P, int np, kh_run **out) {
     *out      = r;
     return KH_OK;
 }
+
+extern "C" {
+
+int kh_run_create(const kh_params *P, int np, kh_run **out) { return run_create(P, np, nullptr, 0, out); }
 
 int kh_run_destroy(kh_run *r) { return free_run(r); }
 
@@ -2203,3 +2342,241 @@ int kh_A2a_k_noalloc(double *a, int nx, int ny, int nz) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ batches
+namespace kh {
+
+size_t batch_stride(const kh_params &P, int np_cap) {
+    // the footprint plus the 256-byte alignment of every buffer
+    return (run_footprint(P, np_cap) + 65536 + 65535) / 65536 * 65536;
+}
+
+bool batch_accepts(const kh_batch *b, const kh_params &P, int np) {
+    const kh_params &Q = b->P;
+    if (np > b->np_cap) return false;
+    if (P.nx != Q.nx || P.ny != Q.ny || P.nz != Q.nz || P.precision != Q.precision || P.dx != Q.dx ||
+        P.rho != Q.rho || P.eta != Q.eta || P.dt != Q.dt || P.a != Q.a || P.a_xi != Q.a_xi)
+        return false;
+    for (int d = 0; d < 3; d++)
+        if ((P.fix_cell[d] != 0) != (Q.fix_cell[d] != 0)) return false;
+    for (int i = 0; i < P.nspecies; i++)
+        if (P.species[i].propulsion == KH_JP_SLIP) return false;
+    return run_footprint(P, np) <= b->stride;
+}
+
+int batch_create(const kh_params &P, int np_cap, int capacity, kh_batch **out) {
+    if (!out || capacity < 1 || np_cap < 1 || np_cap > 256) return fail(KH_ERR_ARG, "bad batch arguments");
+    *out        = nullptr;
+    kh_batch *b = new kh_batch();
+    b->P        = P;
+    b->P.species  = nullptr;
+    b->P.nspecies = 0;
+    b->capacity = capacity;
+    b->np_cap   = np_cap;
+    b->stride   = batch_stride(P, np_cap);
+    b->hactive.assign(capacity, 0);
+    b->hslots.assign(sizeof(SlotPart<double>) * (size_t)capacity, 0);
+    b->slot_run.assign(capacity, nullptr);
+    auto bail = [&](int st) {
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        if (b->base) (void)hipFree(b->base);
+        if (b->active) (void)hipFree(b->active);
+        if (b->dslots) (void)hipFree(b->dslots);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return st;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(KH_ERR_HIP, "hipStreamCreate failed"));
+    if (hipMallocAsync((void **)&b->base, b->stride * (size_t)capacity, b->stream) != hipSuccess) {
+        b->base = nullptr;
+        return bail(fail(KH_ERR_NOMEM, "batch arena of %zu bytes", b->stride * (size_t)capacity));
+    }
+    if (hipMallocAsync((void **)&b->active, sizeof(int) * (size_t)capacity, b->stream) != hipSuccess) {
+        b->active = nullptr;
+        return bail(fail(KH_ERR_NOMEM, "batch flags"));
+    }
+    if (hipMallocAsync(&b->dslots, b->hslots.size(), b->stream) != hipSuccess) {
+        b->dslots = nullptr;
+        return bail(fail(KH_ERR_NOMEM, "batch slot table"));
+    }
+    if (hipMemcpyAsync(b->active, b->hactive.data(), sizeof(int) * (size_t)capacity, hipMemcpyHostToDevice,
+                       b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return bail(fail(KH_ERR_HIP, "batch set-up failed"));
+    *out = b;
+    return KH_OK;
+}
+
+int batch_free_slot(const kh_batch *b) {
+    for (int k = 0; k < b->capacity; k++)
+        if (!b->slot_run[k]) return k;
+    return -1;
+}
+
+int batch_run_create(kh_batch *b, int slot, const kh_params *P, int np, kh_run **out) {
+    if (!b || !P || !out || slot < 0 || slot >= b->capacity || b->slot_run[slot])
+        return fail(KH_ERR_ARG, "slot %d is not free", slot);
+    if (!batch_accepts(b, *P, np)) return fail(KH_ERR_ARG, "run is not of the batch's class");
+    KH_TRY(run_create(P, np, b->base + (size_t)slot * b->stride, b->stride, out));
+    (*out)->slot     = slot;
+    b->slot_run[slot] = *out;
+    return KH_OK;
+}
+
+static void batch_drop_graphs(kh_batch *b) {
+    if (b->exec1) (void)hipGraphExecDestroy(b->exec1);
+    if (b->graph1) (void)hipGraphDestroy(b->graph1);
+    if (b->execk) (void)hipGraphExecDestroy(b->execk);
+    if (b->graphk) (void)hipGraphDestroy(b->graphk);
+    b->exec1 = b->execk = nullptr;
+    b->graph1 = b->graphk = nullptr;
+}
+
+int batch_join(kh_batch *b, kh_run *r) {
+    if (!b || !r || r->slot < 0 || r->slot >= b->capacity || b->slot_run[r->slot] != r)
+        return fail(KH_ERR_ARG, "run is not in this batch");
+    if (r->slip || r->rigid || r->shear || r->step_path != KH_PATH_FUSED || r->jikan.ts < 1 || r->zeta_dirty ||
+        r->timing)
+        return fail(KH_ERR_STATE, "only plain runs past their first fused step join a batch");
+    // the batched launches address every run's buffers from the first one's
+    if (r->arena != b->base + (size_t)r->slot * b->stride || (char *)r->zeta[0] != r->arena)
+        return fail(KH_ERR_STATE, "run's buffers are not at the batch's offsets");
+    KH_HIP(hipStreamSynchronize(r->own_stream));
+    const size_t esz = r->mixed ? sizeof(SlotPart<float>) : sizeof(SlotPart<double>);
+    char        *h   = b->hslots.data() + (size_t)r->slot * esz;
+    if (r->mixed) {
+        const SlotPart<float> a = Step<float>::slot_part(r);
+        memcpy(h, &a, esz);
+    } else {
+        const SlotPart<double> a = Step<double>::slot_part(r);
+        memcpy(h, &a, esz);
+    }
+    KH_HIP(hipMemcpyAsync((char *)b->dslots + (size_t)r->slot * esz, h, esz, hipMemcpyHostToDevice, b->stream));
+    r->stream  = b->stream;
+    r->batched = true;
+    b->hactive[r->slot] = 1;
+    b->dirty            = true;
+    return KH_OK;
+}
+
+int batch_leave(kh_batch *b, kh_run *r) {
+    if (!b || !r || r->slot < 0 || r->slot >= b->capacity || b->slot_run[r->slot] != r)
+        return fail(KH_ERR_ARG, "run is not in this batch");
+    KH_HIP(hipStreamSynchronize(b->stream));
+    r->stream  = r->own_stream;
+    r->batched = false;
+    b->hactive[r->slot]  = 0;
+    b->slot_run[r->slot] = nullptr;
+    b->dirty             = true;
+    return KH_OK;
+}
+
+int batch_members(const kh_batch *b) {
+    int n = 0;
+    for (int k = 0; k < b->capacity; k++) n += b->hactive[k];
+    return n;
+}
+
+// Captures `count` batch steps over every slot, through a view of slot 0
+// made from the joined run m.
+template <typename T>
+static int batch_capture(kh_batch *b, kh_run *m, int count, hipGraph_t *graph, hipGraphExec_t *exec) {
+    std::unique_ptr<kh_run> v(new kh_run(*m));
+    const long long         off = -(long long)m->slot * (long long)b->stride;
+    auto                    mv  = [&](auto *&p) {
+        using P = std::remove_reference_t<decltype(p)>;
+        p       = reinterpret_cast<P>(reinterpret_cast<char *>(p) + off);
+    };
+    for (auto &p : v->zeta) mv(p);
+    for (auto &p : v->spec) mv(p);
+    for (auto &p : v->ureal) mv(p);
+    for (auto &p : v->up) mv(p);
+    for (auto &p : v->work) mv(p);
+    mv(v->phisum);
+    mv(v->ukdc);
+    mv(v->d_ts);
+    v->slot      = 0;
+    v->stream    = b->stream;
+    v->timing    = false;
+    v->capturing = true;
+    v->pending.clear();
+    v->event_pool.clear();
+    KH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int st = KH_OK;
+    for (int k = 0; k < count && st == KH_OK; k++) st = Step<T>::step_batch(b, v.get());
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st != KH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(KH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    *graph = g;
+    KH_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return KH_OK;
+}
+
+int batch_step(kh_batch *b, int nsteps) {
+    if (!b || nsteps < 0) return fail(KH_ERR_ARG, "bad batch step");
+    std::vector<kh_run *> members;
+    for (int k = 0; k < b->capacity; k++)
+        if (b->hactive[k]) members.push_back(b->slot_run[k]);
+    if (b->dirty) {  // the slots that step, from the next launch on
+        KH_HIP(hipMemcpyAsync(b->active, b->hactive.data(), sizeof(int) * (size_t)b->capacity, hipMemcpyHostToDevice,
+                              b->stream));
+        KH_HIP(hipStreamSynchronize(b->stream));
+        b->dirty = false;
+    }
+    if (members.empty() || nsteps == 0) return KH_OK;
+    const bool mixed = b->P.precision == KH_MIXED;
+    for (int s = 0; s < nsteps;) {
+        int done = 1;
+        if (nsteps - s >= GRAPH_STEPS) {
+            if (!b->execk)
+                KH_TRY(mixed ? batch_capture<float>(b, members[0], GRAPH_STEPS, &b->graphk, &b->execk)
+                             : batch_capture<double>(b, members[0], GRAPH_STEPS, &b->graphk, &b->execk));
+            KH_HIP(hipGraphLaunch(b->execk, b->stream));
+            done = GRAPH_STEPS;
+        } else {
+            if (!b->exec1)
+                KH_TRY(mixed ? batch_capture<float>(b, members[0], 1, &b->graph1, &b->exec1)
+                             : batch_capture<double>(b, members[0], 1, &b->graph1, &b->exec1));
+            KH_HIP(hipGraphLaunch(b->exec1, b->stream));
+        }
+        for (kh_run *q : members) {
+            q->pcache_valid = 1;
+            q->pclean       = q->zclear;
+            for (int k = 0; k < done; k++) {
+                q->jikan.time += q->jikan.dt_fluid;  // sp_3d_ns.cxx:955
+                q->jikan.ts++;
+            }
+        }
+        s += done;
+    }
+    return KH_OK;
+}
+
+int batch_query(kh_batch *b) {
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(KH_ERR_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
+int batch_destroy(kh_batch *b) {
+    if (!b) return KH_OK;
+    for (kh_run *q : b->slot_run)
+        if (q) return fail(KH_ERR_STATE, "batch still holds a run");
+    (void)hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
+    (void)hipFreeAsync(b->base, b->stream);
+    (void)hipFreeAsync(b->active, b->stream);
+    (void)hipFreeAsync(b->dslots, b->stream);
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return KH_OK;
+}
+
+}  // namespace kh

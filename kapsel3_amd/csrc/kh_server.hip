@@ -3,10 +3,15 @@
 // Clients submit runs (parameters, particles, step count, a host buffer for
 // the per-step trajectory) from any thread; one scheduler thread owns the
 // device side. It admits queued runs while a slot is free and their device
-// footprint fits the HBM budget, keeps every admitted run resident on its
-// own non-blocking stream until it completes, and feeds each run a few steps
-// at a time (after the first step each step is one captured-graph launch),
-// so the work of runs of different sizes overlaps on the GPU. A run's
+// footprint fits the HBM budget and keeps every admitted run resident until
+// it completes. Runs of plain spheres of one size class share a batch
+// (kh_batch.h): a run is created in a free slot of a batch of its class (a
+// new batch of up to batch_slots arenas when none has room), takes its first
+// step alone, then joins, and the batch steps all its joined runs a few
+// steps at a time, one launch per fluid pass for all of them, as one
+// captured graph per membership. Other runs step alone on their own streams
+// (after the first step each step is one captured-graph launch). Batches of
+// different sizes and lone runs overlap on the GPU. A run's
 // trajectory is recorded on the device every step and copied to the
 // client's buffer once, when the run completes; so are its Output_udf frames
 // when the client asked for them. Latency is measured from submission to
@@ -28,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/kapsel_hip.h"
+#include "kh_batch.h"
 
 int kh_set_error(int status, const char *msg);
 
@@ -53,8 +59,11 @@ struct Request {
     int                        status = 1;  // 1 queued, 2 active, 0 done, < 0 failed (kh_status)
     std::string                error;
     // device side (scheduler thread only)
-    kh_run *run      = nullptr;
-    int     enqueued = 0;
+    kh_run   *run      = nullptr;
+    int       enqueued = 0;
+    kh_batch *batch    = nullptr;  // the batch it was created in (nullptr: steps alone)
+    int       slot     = -1;
+    bool      joined   = false;
 };
 
 }  // namespace
@@ -68,6 +77,7 @@ struct kh_server {
     std::deque<Request *>                  queue;
     std::map<long long, std::unique_ptr<Request>> all;
     std::vector<Request *>                 active;  // scheduler thread only
+    std::vector<std::pair<kh_batch *, long long>> batches;  // with their arena bytes (scheduler thread only)
     long long                              used = 0, peak_bytes = 0, next_id = 1;
     int                                    peak_active = 0, nactive = 0;
     bool                                   stop = false, started = false;
@@ -110,14 +120,37 @@ void finish(kh_server *S, Request *q, int status, const char *err) {
     S->cv.notify_all();
 }
 
-// Device side of admission: create the run, upload, initial constraint block.
+// Device side of admission: create the run (in its batch slot, if any),
+// upload, initial constraint block; a batched run takes its first step alone.
 int start(Request *q) {
-    int st = kh_run_create(&q->P, (int)q->parts.size(), &q->run);
-    if (st == KH_OK) st = kh_set_particles(q->run, q->parts.data(), (int)q->parts.size());
+    const int np = (int)q->parts.size();
+    int       st = q->batch ? kh::batch_run_create(q->batch, q->slot, &q->P, np, &q->run)
+                            : kh_run_create(&q->P, np, &q->run);
+    if (st == KH_OK) st = kh_set_particles(q->run, q->parts.data(), np);
     if (st == KH_OK) st = kh_init_fluid(q->run);
     if (st == KH_OK) st = kh_set_trajectory(q->run, q->nsteps);
     if (st == KH_OK && q->frames_out) st = kh_set_output(q->run, q->nframes, q->frame_every);
+    if (st == KH_OK && q->batch) {
+        st = kh_step(q->run, 1);
+        if (st == KH_OK) q->enqueued = 1;
+    }
     return st;
+}
+
+// A run leaves its batch (if it was created in one) before its results are
+// copied and it is destroyed; the slot is free afterwards.
+int leave(Request *q) {
+    if (!q->batch || !q->run) return KH_OK;
+    const int st = kh::batch_leave(q->batch, q->run);
+    q->batch     = nullptr;
+    return st;
+}
+
+bool batchable(const kh_server *S, const Request *q) {
+    if (S->cfg.batch_slots < 2 || (int)q->parts.size() > KH_BATCH_NP_CAP) return false;
+    for (const kh_species &sp : q->species)
+        if (sp.propulsion == KH_JP_SLIP) return false;
+    return true;
 }
 
 void loop(kh_server *S) {
@@ -126,14 +159,43 @@ void loop(kh_server *S) {
         bool worked = false;
         // admission: a free slot and room in the HBM budget, in arrival order
         for (;;) {
-            Request *q = nullptr;
+            Request  *q = nullptr;
+            kh_batch *b = nullptr;
+            int       slot = -1, new_slots = 0;
+            long long arena = 0;
             {
                 std::lock_guard<std::mutex> lk(S->mu);
                 if (S->queue.empty() || (int)S->active.size() >= S->cfg.max_active) break;
-                Request *h = S->queue.front();
-                if (S->used + h->bytes > S->budget && !S->active.empty()) break;
+                Request  *h     = S->queue.front();
+                const int np    = (int)h->parts.size();
+                long long need  = h->bytes;
+                // device bytes beyond the run's own buffers (trajectory, frames)
+                const long long extra = h->bytes - kh_run_bytes(&h->P, np);
+                if (batchable(S, h)) {
+                    for (auto &e : S->batches)
+                        if (kh::batch_accepts(e.first, h->P, np) && (slot = kh::batch_free_slot(e.first)) >= 0) {
+                            b = e.first;
+                            break;
+                        }
+                    if (b) {
+                        need = extra;
+                    } else {  // a new batch, as many slots as the budget leaves (two at least)
+                        const long long stride = (long long)kh::batch_stride(h->P, KH_BATCH_NP_CAP);
+                        const long long room   = S->budget - S->used - extra;
+                        new_slots              = (int)std::min<long long>(S->cfg.batch_slots, room / stride);
+                        if (new_slots >= 2) {
+                            arena = stride * new_slots;
+                            need  = arena + extra;
+                            slot  = 0;
+                        } else {
+                            new_slots = 0;
+                        }
+                    }
+                }
+                if (S->used + need > S->budget && !S->active.empty()) break;
                 S->queue.pop_front();
-                S->used += h->bytes;
+                S->used += need;
+                h->bytes = (b || new_slots) ? extra : h->bytes;  // what finish() releases
                 S->nactive++;
                 S->peak_bytes  = std::max(S->peak_bytes, S->used);
                 S->peak_active = std::max(S->peak_active, S->nactive);
@@ -141,18 +203,39 @@ void loop(kh_server *S) {
                 h->t_admit     = clk::now();
                 q              = h;
             }
+            if (new_slots) {
+                const int bst = kh::batch_create(q->P, KH_BATCH_NP_CAP, new_slots, &b);
+                if (bst != KH_OK) {
+                    const std::string e = kh_last_error();
+                    {
+                        std::lock_guard<std::mutex> lk(S->mu);
+                        S->used -= arena;
+                    }
+                    finish(S, q, bst, e.c_str());
+                    worked = true;
+                    continue;
+                }
+                S->batches.push_back({b, arena});
+            }
+            q->batch  = b;
+            q->slot   = slot;
             const int st = start(q);
             if (st != KH_OK) {
                 const std::string e = kh_last_error();
+                leave(q);
                 finish(S, q, st, e.c_str());
             } else {
                 S->active.push_back(q);
             }
             worked = true;
         }
-        // progress: every run whose last chunk has drained gets the next one
+        // progress: every lone run whose last chunk has drained gets the next one
         for (size_t i = 0; i < S->active.size();) {
-            Request  *q    = S->active[i];
+            Request *q = S->active[i];
+            if (q->batch) {
+                i++;
+                continue;
+            }
             const int idle = q->enqueued ? kh_query(q->run) : 1;
             if (idle == 0) {
                 i++;
@@ -177,6 +260,80 @@ void loop(kh_server *S) {
             q->enqueued += k;
             worked = true;
             i++;
+        }
+        // batches: once a batch's last chunk has drained, finished runs leave,
+        // runs past their first step join, and the members get the next chunk
+        for (size_t bi = 0; bi < S->batches.size();) {
+            kh_batch *b    = S->batches[bi].first;
+            const int idle = kh::batch_query(b);
+            if (idle == 0) {
+                bi++;
+                continue;
+            }
+            std::vector<Request *> members;
+            int                    nruns = 0;
+            for (size_t i = 0; i < S->active.size();) {
+                Request *q = S->active[i];
+                if (q->batch != b) {
+                    i++;
+                    continue;
+                }
+                nruns++;
+                int st = idle < 0 ? idle : KH_OK;
+                if (st == KH_OK && !q->joined) {
+                    const int own = kh_query(q->run);  // its first step, alone
+                    if (own == 0) {
+                        i++;
+                        continue;
+                    }
+                    if (own < 0) st = own;
+                    if (st == KH_OK && q->enqueued < q->nsteps) {
+                        st = kh::batch_join(b, q->run);
+                        if (st == KH_OK) q->joined = true;
+                    }
+                }
+                if (st != KH_OK || q->enqueued == q->nsteps) {
+                    std::string msg = st != KH_OK ? kh_last_error() : "";
+                    const int   ls  = leave(q);
+                    if (st == KH_OK && ls != KH_OK) {
+                        st  = ls;
+                        msg = kh_last_error();
+                    }
+                    finish(S, q, st, st != KH_OK ? msg.c_str() : nullptr);
+                    S->active.erase(S->active.begin() + i);
+                    nruns--;
+                    worked = true;
+                    continue;
+                }
+                if (q->joined) members.push_back(q);
+                i++;
+            }
+            if (!members.empty()) {
+                int k = S->cfg.chunk_steps;
+                for (Request *q : members) k = std::min(k, q->nsteps - q->enqueued);
+                const int st = kh::batch_step(b, k);
+                if (st != KH_OK) {
+                    const std::string msg = kh_last_error();
+                    for (Request *q : members) {
+                        leave(q);
+                        finish(S, q, st, msg.c_str());
+                        S->active.erase(std::find(S->active.begin(), S->active.end(), q));
+                        nruns--;
+                    }
+                } else {
+                    for (Request *q : members) q->enqueued += k;
+                }
+                worked = true;
+            }
+            if (nruns == 0) {  // every slot free: the arenas go back to the budget
+                const long long arena = S->batches[bi].second;
+                kh::batch_destroy(b);
+                S->batches.erase(S->batches.begin() + bi);
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->used -= arena;
+                continue;
+            }
+            bi++;
         }
         if (!worked) {
             std::unique_lock<std::mutex> lk(S->mu);

@@ -108,6 +108,9 @@ __global__ __launch_bounds__(PW_NT) void k_adv_ns(Grid g, const cpx<T> *__restri
                                                   const cpx<T> *__restrict__ P2, const cpx<T> *__restrict__ P3,
                                                   const cpx<T> *__restrict__ P4, cpx<T> *__restrict__ z0,
                                                   cpx<T> *__restrict__ z1, double dmy0, double dmy1) {
+    if (batch_skip(g)) return;
+    P0 = brun(P0, g), P1 = brun(P1, g), P2 = brun(P2, g), P3 = brun(P3, g), P4 = brun(P4, g);
+    z0 = brun(z0, g), z1 = brun(z1, g);
     // one thread per kept mode: (kx, ky) kept rows, kz < TRN_Z
     const long long t  = (long long)blockIdx.x * PW_NT + threadIdx.x;
     const int       ky = nkept(g.trn_y);
@@ -218,7 +221,9 @@ __global__ __launch_bounds__(PW_NT) void k_add3(long long n, T *__restrict__ u0,
 
 // 16-byte stores; fields are hipMalloc'd (256-byte aligned)
 template <typename T>
-__global__ __launch_bounds__(PW_NT) void k_fill(long long n, T *__restrict__ p, T v) {
+__global__ __launch_bounds__(PW_NT) void k_fill(long long n, T *__restrict__ p, T v, Grid bg) {
+    if (batch_skip(bg)) return;
+    p                  = brun(p, bg);
     constexpr int   W  = 16 / sizeof(T);
     const long long i  = ((long long)blockIdx.x * PW_NT + threadIdx.x) * W;
     if (i + W <= n) {
@@ -249,8 +254,8 @@ template <typename T>
 hipError_t adv_ns(const Grid &g, cpx<T> *const P[5], cpx<T> *z0, cpx<T> *z1, double dmy0, double dmy1,
                   hipStream_t s) {
     const long long nk = (long long)(2 * g.trn_x - 1) * (2 * g.trn_y - 1) * g.trn_z;
-    hipLaunchKernelGGL(k_adv_ns<T>, dim3(nblocks(nk)), dim3(PW_NT), 0, s, g, P[0], P[1], P[2], P[3], P[4], z0, z1,
-                       dmy0, dmy1);
+    hipLaunchKernelGGL(k_adv_ns<T>, dim3(nblocks(nk), 1, batch_z(g)), dim3(PW_NT), 0, s, g, P[0], P[1], P[2], P[3], P[4],
+                       z0, z1, dmy0, dmy1);
     return hipGetLastError();
 }
 template <typename T>
@@ -276,9 +281,10 @@ hipError_t add3(long long n, T *const u[3], T *const f[3], hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
-hipError_t fill(long long n, T *p, T v, hipStream_t s) {
-    constexpr int W = 16 / sizeof(T);
-    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks((n + W - 1) / W)), dim3(PW_NT), 0, s, n, p, v);
+hipError_t fill(long long n, T *p, T v, hipStream_t s, const Grid *batch) {
+    constexpr int W  = 16 / sizeof(T);
+    const Grid    bg = batch ? *batch : Grid{};
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks((n + W - 1) / W), 1, batch_z(bg)), dim3(PW_NT), 0, s, n, p, v, bg);
     return hipGetLastError();
 }
 
@@ -292,7 +298,7 @@ hipError_t fill(long long n, T *p, T v, hipStream_t s) {
                                      cpx<T> *, double *, hipStream_t);                                             \
     template hipError_t f_nonsole<T>(long long, T *const[3], const T *, T *const[3], T *const[3], hipStream_t);    \
     template hipError_t add3<T>(long long, T *const[3], T *const[3], hipStream_t);                                 \
-    template hipError_t fill<T>(long long, T *, T, hipStream_t);
+    template hipError_t fill<T>(long long, T *, T, hipStream_t, const Grid *);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST

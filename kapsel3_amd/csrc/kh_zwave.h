@@ -13,10 +13,12 @@ template <typename T>
 hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, hipStream_t s, int kzv = -1);
 // half spectra (modes kz < kzv, rest zero) -> rows of L reals * scale; grid y = component.
 // mask (a real field, L >= 32): only 4-point vectors where it is nonzero are written
-// (a non-null skip, masked passes only: nothing is done once *skip is nonzero)
+// (a non-null skip, masked passes only: nothing is done once *skip is nonzero;
+// a non-null batch, masked passes only: its batch fields make one launch
+// cover the batch's runs)
 template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
-                  hipStream_t s, const T *mask = nullptr, const int *skip = nullptr);
+                  hipStream_t s, const T *mask = nullptr, const int *skip = nullptr, const Grid *batch = nullptr);
 // inverse z (3) -> advection products -> forward z (5), modes kz < TRN_Z
 // inverse z pass of omega fused with u x omega (lengths >= 32)
 bool zw_cross_supported(int L);

@@ -172,13 +172,18 @@ __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T sc
 template <typename T, int L, int E>
 __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, const T *__restrict__ mask,
                                                        long long nrows, T scale, int kzv, int pitch,
-                                                       const int *__restrict__ skip) {
+                                                       const int *__restrict__ skip, Grid bg) {
     using Geo = WaveGeom<T, L, E>;
     using F   = WaveFft<T, L, E, false, false>;
     using RS  = RowStage<T, L>;
     static_assert(L >= 4 * Geo::G, "four points per lane per row");
     constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
-    if (skip && *skip) return;
+    if ((skip && *skip) || batch_skip(bg)) return;
+    for (int c = 0; c < ncomp; c++) {
+        io.in[c]  = brun(io.in[c], bg);
+        io.out[c] = brun(io.out[c], bg);
+    }
+    mask = brun(mask, bg);
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
@@ -297,6 +302,9 @@ __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict
     using Fi  = WaveFft<T, L, E, false, false>;
     using Ff  = WaveFft<T, L, E, true, true>;
     static_assert(Fi::RN == Ff::R0, "inverse output layout must be the forward input layout");
+    if (batch_skip(g)) return;
+    s0 = brun(s0, g), s1 = brun(s1, g), s2 = brun(s2, g);
+    p0 = brun(p0, g), p1 = brun(p1, g), p2 = brun(p2, g), p3 = brun(p3, g), p4 = brun(p4, g);
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
@@ -398,7 +406,15 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
     using RS  = RowStage<T, L>;
     static_assert(L >= 4 * Geo::G, "four points per lane per row");
     constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
-    if (skip && *skip) return;
+    if ((skip && *skip) || batch_skip(g)) return;
+    for (int d = 0; d < 3; d++) {
+        io.up[d]  = brun(io.up[d], g);
+        io.us[d]  = brun(io.us[d], g);
+        io.us2[d] = brun(io.us2[d], g);
+        io.f[d]   = brun(io.f[d], g);
+    }
+    io.phis = brun(io.phis, g);
+    tick    = brun(tick, g);
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
@@ -520,14 +536,16 @@ hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, h
 
 template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
-                  hipStream_t s, const T *mask, const int *skip) {
+                  hipStream_t s, const T *mask, const int *skip, const Grid *batch) {
+    const Grid bg = batch ? *batch : Grid{};
+    if (batch_z(bg) > 1 && !(mask && L >= 32)) return hipErrorInvalidValue;  // only the masked pass is batched
     if (skip && !(mask && L >= 32)) return hipErrorInvalidValue;  // only the masked pass takes a skip flag
     if (mask && L >= 32) {  // every length with 4 points per lane per row
         switch (L) {
 #define KH_CASE(n)                                                                                              \
     case n:                                                                                                     \
         if constexpr (n >= 32) {                                                                                \
-            hipLaunchKernelGGL((k_zw_c2r_masked<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows)), dim3(256), 0, s, io, ncomp, mask, nrows, scale, kzv, pitch, skip); \
+            hipLaunchKernelGGL((k_zw_c2r_masked<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows), 1, batch_z(bg)), dim3(256), 0, s, io, ncomp, mask, nrows, scale, kzv, pitch, skip, bg); \
             return hipGetLastError();                                                                           \
         }                                                                                                       \
         break;
@@ -573,7 +591,7 @@ hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T sca
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                \
     case n:                                                                                                       \
-        hipLaunchKernelGGL((k_zw_adv<T, n, ZwE<T, n>::adv, ZwE<T, n>::live3>), dim3(zw_blocks<T, n, ZwE<T, n>::adv>(nrows)), dim3(256), 0, s, g, s3[0], s3[1], s3[2], \
+        hipLaunchKernelGGL((k_zw_adv<T, n, ZwE<T, n>::adv, ZwE<T, n>::live3>), dim3(zw_blocks<T, n, ZwE<T, n>::adv>(nrows), 1, batch_z(g)), dim3(256), 0, s, g, s3[0], s3[1], s3[2], \
                            p5[0], p5[1], p5[2], p5[3], p5[4], scale);                                             \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)
@@ -597,7 +615,7 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                  \
     case n:                                                                                                         \
-        hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::force>), dim3(zw_blocks<T, n, ZwE<T, n>::force>(nrows)), dim3(256), 0, s, g, io, \
+        hipLaunchKernelGGL((k_zw_force<T, n, ZwE<T, n>::force>), dim3(zw_blocks<T, n, ZwE<T, n>::force>(nrows), 1, batch_z(g)), dim3(256), 0, s, g, io, \
                            clear, tick, skip);                                                                      \
         return hipGetLastError();
         KH_FOR_EACH_WL(KH_CASE)
@@ -609,7 +627,7 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
 #define KH_INST(T)                                                                                           \
     template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t, int);               \
     template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t, const T *, \
-                                  const int *);                                                            \
+                                  const int *, const Grid *);                                              \
     template hipError_t zw_c2r_cross<T>(int, const ZIO<T> &, const T *const[3], long long, T, int, int, hipStream_t); \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
     template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,        \

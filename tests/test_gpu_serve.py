@@ -31,8 +31,9 @@ def test_trajectory_record_matches_steps(gpu):
 
 def test_served_runs_match_alone(gpu):
     """The serving workload's sizes (32, 48, 64 and 96^3, a = 4, up to 64
-    colloids, gravity) for its 200 steps: 24 runs in one mix, three of them
-    (one per size class up to 96^3) again alone."""
+    colloids, gravity) for its 200 steps: 24 runs in one mix (stepped in
+    batches per size class), three of them (one per size class up to 96^3)
+    again alone, outside any batch."""
     print("served mix of 24 runs (32-96^3) x 200 steps vs alone", flush=True)
     reqs = serving.requests(24, seed=11, sizes=(32, 48, 64, 96), pmax=64, precision=MIXED)
     with gpu.Server(max_active=8, chunk_steps=8) as srv:
@@ -46,7 +47,7 @@ def test_served_runs_match_alone(gpu):
     for k in picks:  # the same runs, each alone on a fresh server
         P, parts = reqs[k]
         print(f"  alone: run {k}, {P.c.nx}^3 x {len(parts)}", flush=True)
-        with gpu.Server(max_active=1) as srv:
+        with gpu.Server(max_active=1, batch_slots=0) as srv:
             rid, alone = srv.submit(P, parts, 200)
             srv.wait(rid)
         got = mix[k][1]
