@@ -369,7 +369,7 @@ __global__ __launch_bounds__(PassThreads<T>::value) void k_z_force(
             const int  e  = threadIdx.x + i * NT;
             const bool ok = e < 2 * P * L && row0 + e / L < nrows;
             a[i]          = ok ? UP[d][row0 * L + e] : (T)0;
-            b[i]          = ok ? US[d][row0 * L + e] : (T)0;
+            b[i]          = ok && US[d] ? US[d][row0 * L + e] : (T)0;
         }
 #pragma unroll
         for (int i = 0; i < NR; i++) {
@@ -803,7 +803,9 @@ hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T
                    int *tick, hipStream_t s, T *const us2[3], const int *skip) {
     const long long nrows = (long long)g.nx * g.ny;
     if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, tick, s, us2, skip);
-    if (clear || tick || us2 || skip || !us) return hipErrorInvalidValue;
+    if (clear || tick || us2 || skip) return hipErrorInvalidValue;
+    T *const none[3] = {nullptr, nullptr, nullptr};
+    if (!us) us = none;
     switch (g.nz) {
 #define KH_CASE(n)                                                                                                   \
     case n: {                                                                                                        \

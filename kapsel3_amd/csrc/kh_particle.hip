@@ -281,12 +281,21 @@ __global__ __launch_bounds__(PT_NT) void k_f_hydro(PParams pp, DevParticle *__re
     f_hydro_block<T, PT_NT, OBL>(pp, P, blockIdx.x, u0, u1, u2, phisum, pc, coef);
 }
 
-// Make_u_particle_sum scatter (make_phi.cxx:313-354) over the cache
+// Make_u_particle_sum scatter (make_phi.cxx:313-354) over the cache. With
+// u* given (us0..2) each particle also subtracts its share w u* of the
+// rigidity term, w = phi/max(phi_sum,1): the shares sum to min(phi_sum,1)
+// u*, so the field receives f = u_p - min(phi_sum,1) u*
+// (Make_f_particle_dt_nonsole, f_particle.h:35-54) directly; where one
+// particle alone covers a point (w = phi_sum = phi) the value is the
+// reference's to the last bit.
 template <typename T, int NT = PT_NT, bool OBL = false>
 __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const double *xp, const double *vp,
                                                  const double *om, const T *__restrict__ phisum,
                                                  const PCache<T> &pc, T *__restrict__ up0, T *__restrict__ up1,
-                                                 T *__restrict__ up2, const PStash<T> *stash = nullptr) {
+                                                 T *__restrict__ up2, const PStash<T> *stash = nullptr,
+                                                 const T *__restrict__ us0 = nullptr,
+                                                 const T *__restrict__ us1 = nullptr,
+                                                 const T *__restrict__ us2 = nullptr) {
     const PCell     c     = particle_cell(pp, xp);
     const long long cb    = (long long)n * pp.np_domain;
     const T vt[3] = {(T)vp[0], (T)vp[1], (T)vp[2]}, wt[3] = {(T)om[0], (T)om[1], (T)om[2]};
@@ -316,7 +325,12 @@ __device__ __forceinline__ void up_scatter_block(const PParams &pp, int n, const
         angular2v_t<T>(pp.rotation, wt, r, vr);
         T vx = vt[0];
         if constexpr (OBL) vx = sgn ? (T)(vp[0] - sgn * pp.obl_jump) : vt[0];  // make_phi.cxx:392
-        const T a = (vx + vr[0]) * dmy_phi, b = (vt[1] + vr[1]) * dmy_phi, e = (vt[2] + vr[2]) * dmy_phi;
+        T a = (vx + vr[0]) * dmy_phi, b = (vt[1] + vr[1]) * dmy_phi, e = (vt[2] + vr[2]) * dmy_phi;
+        if (us0) {
+            a = a - dmy_phi * us0[idx];
+            b = b - dmy_phi * us1[idx];
+            e = e - dmy_phi * us2[idx];
+        }
         if (alone) {
             // phi_sum is this particle's phi alone: no other particle adds
             // here (a contribution below half an ulp of phi would be the only
@@ -735,8 +749,9 @@ __device__ __forceinline__ void particle_step_block(const PParams &pp, DevPartic
         }
     }
     __syncthreads();
-    // each lane reads back only the stash entries it wrote
-    up_scatter_block<T, NT>(pp, n, st, st + 3, st + 6, phisum, pc, up0, up1, up2, &stash);
+    // each lane reads back only the stash entries it wrote; the field
+    // receives f = u_p - min(phi_sum,1) u* (z_force then reads it as it is)
+    up_scatter_block<T, NT>(pp, n, st, st + 3, st + 6, phisum, pc, up0, up1, up2, &stash, u0, u1, u2);
 }
 template <typename T, int NT>
 __global__ __launch_bounds__(NT, NT == PT_NT ? 8 : 1) void k_particle_step(PParams pp, DevParticle *__restrict__ P,

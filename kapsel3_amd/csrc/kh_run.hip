@@ -570,14 +570,15 @@ struct Step {
         return KH_OK;
     }
     // f = P(up - phi u*), u = u* + f, zeta = curl u (u* = u + us2 when us2
-    // is given), and the device step counter; for r or r's batch as above
-    static int fused_force(kh_run *r, const Grid &g, T *const us2[3]) {
+    // is given; up_is_f: the particle step left f itself in up), and the
+    // device step counter; for r or r's batch as above
+    static int fused_force(kh_run *r, const Grid &g, T *const us2[3], bool up_is_f) {
         hipStream_t s      = r->stream;
         cpx<T>     *s3[3]  = {S(r, 0), S(r, 1), S(r, 2)};
         Comps<T>    c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
         T          *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
-        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), u3, s3, r->zclear ? 3 : 0, r->zclear ? r->d_ts : nullptr, s,
-                                         us2)));
+        KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), up_is_f ? nullptr : u3, s3, r->zclear ? 3 : 0,
+                                         r->zclear ? r->d_ts : nullptr, s, us2)));
         if (!r->zclear) KH_LAUNCH("tick", tick(r->d_ts, s, &g));
         KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
         KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));
@@ -635,7 +636,7 @@ struct Step {
             KH_TRY(plain_particles(r, ab2));
         }
         T *w3[3] = {W(r, 0), W(r, 1), W(r, 2)};  // squirmers: u = u* + the projected slip field
-        KH_TRY(fused_force(r, g, r->slip_dev ? w3 : nullptr));
+        KH_TRY(fused_force(r, g, r->slip_dev ? w3 : nullptr, !r->slip && !r->rigid));
         r->pclean = r->zclear;
         return KH_OK;
     }
@@ -664,7 +665,7 @@ struct Step {
         KH_LAUNCH("particle_step", (particle_step_batch<T>(sp, b->active, b->capacity, b->np_cap,
                                                            (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->jikan.hdt_md,
                                                            s)));
-        KH_TRY(fused_force(r, g, nullptr));
+        KH_TRY(fused_force(r, g, nullptr, true));
         KH_LAUNCH("records", (records_batch<T>(sp, b->active, b->capacity, s)));
         return KH_OK;
     }
