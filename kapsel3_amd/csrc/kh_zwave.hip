@@ -179,10 +179,8 @@ __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, con
     static_assert(L >= 4 * Geo::G, "four points per lane per row");
     constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
     if ((skip && *skip) || batch_skip(bg)) return;
-    for (int c = 0; c < ncomp; c++) {
-        io.in[c]  = brun(io.in[c], bg);
-        io.out[c] = brun(io.out[c], bg);
-    }
+    // io stays as passed (indexing a modified copy by the loop's c would
+    // put it in scratch); each pointer is moved to this block's run at use
     mask = brun(mask, bg);
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
@@ -199,12 +197,13 @@ __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, con
 #pragma unroll 1
     for (int c = 0; c < ncomp; c++) {
         cpx<T> v[F::E];
-        load_pair<T, L, F>(io.in[c], wl, kzv, v);
+        load_pair<T, L, F>(brun(io.in[c], bg), wl, kzv, v);
         F::run(v, wl.buf, tw, wl.g);
 #pragma unroll
         for (int i = 0; i < F::E; i++) wl.buf[WavePad<T>::at(F::out_point(wl.g, i))] = v[i];
         wave_sync();
-        T *ra = io.out[c] + oa, *rb = io.out[c] + ob;
+        T *const out = brun(io.out[c], bg);
+        T       *ra = out + oa, *rb = out + ob;
 #pragma unroll
         for (int t = 0; t < NT4; t++) {
             const int    z = 4 * (wl.g + Geo::G * t);
@@ -407,14 +406,9 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
     static_assert(L >= 4 * Geo::G, "four points per lane per row");
     constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
     if ((skip && *skip) || batch_skip(g)) return;
-    for (int d = 0; d < 3; d++) {
-        io.up[d]  = brun(io.up[d], g);
-        io.us[d]  = brun(io.us[d], g);
-        io.us2[d] = brun(io.us2[d], g);
-        io.f[d]   = brun(io.f[d], g);
-    }
-    io.phis = brun(io.phis, g);
-    tick    = brun(tick, g);
+    // io stays as passed (a modified copy indexed by the loop's d would sit
+    // in scratch); each pointer is moved to this block's run at use
+    tick = brun(tick, g);
     __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     fill_twiddles<T, L, Geo::NT>(tw);
     __syncthreads();
@@ -424,7 +418,7 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
     const long long         oa = (2 * wl.line) * L, ob = oa + L;
     using vec                  = typename RS::vec;
     const vec zero             = {(T)0, (T)0, (T)0, (T)0};
-    T        *phis             = io.phis;
+    T        *phis             = brun(io.phis, g);
     // Outside every particle phi_sum = 0, and so is u_p (it is scattered on
     // the same points with the same phi): f = 0 there whatever u* is, so
     // u_p and u* are read only where some phi_sum of the vector is nonzero.
@@ -449,21 +443,22 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
         pa[t]       = {fmin(a.x, (T)1), fmin(a.y, (T)1), fmin(a.z, (T)1), fmin(a.w, (T)1)};
         pb[t]       = {fmin(b.x, (T)1), fmin(b.y, (T)1), fmin(b.z, (T)1), fmin(b.w, (T)1)};
     }
-    auto ld_u = [&](int d, long long o) -> vec {
-        if (!io.us[d]) return zero;
-        const vec a = RS::ld(io.us[d] + o);
-        if (!io.us2[d]) return a;
-        const vec b = RS::ld(io.us2[d] + o);
-        return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
-    };
 #pragma unroll 1
     for (int d = 0; d < 3; d++) {
-        T *UP = io.up[d];
+        T *const       UP  = brun(io.up[d], g);
+        const T *const US  = brun(io.us[d], g), *const US2 = brun(io.us2[d], g);
+        auto     ld_u = [&](long long o) -> vec {
+            if (!US) return zero;
+            const vec a = RS::ld(US + o);
+            if (!US2) return a;
+            const vec b = RS::ld(US2 + o);
+            return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
+        };
 #pragma unroll
         for (int t = 0; t < NT4; t++) {
             const int z  = 4 * (wl.g + Geo::G * t);
             const vec ua = na[t] ? RS::ld(UP + oa + z) : zero, ub = nb[t] ? RS::ld(UP + ob + z) : zero;
-            const vec sa = na[t] ? ld_u(d, oa + z) : zero, sb = nb[t] ? ld_u(d, ob + z) : zero;
+            const vec sa = na[t] ? ld_u(oa + z) : zero, sb = nb[t] ? ld_u(ob + z) : zero;
             if (clear & 1) {  // and u_p zero
                 if (na[t] && nonzero(ua)) RS::st(UP + oa + z, zero);
                 if (nb[t] && nonzero(ub)) RS::st(UP + ob + z, zero);
@@ -479,7 +474,7 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
         for (int i = 0; i < E; i++) v[i] = wl.buf[WavePad<T>::at(Ff::in_point(wl.g, i))];
         wave_sync();
         Ff::run(v, wl.buf, tw, wl.g);
-        store_pair<T, L, Ff>(v, wl, g.trn_z, io.f[d]);
+        store_pair<T, L, Ff>(v, wl, g.trn_z, brun(io.f[d], g));
     }
 }
 
