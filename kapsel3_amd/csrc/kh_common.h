@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace kh {
 
 // Complex value in registers/LDS/HBM: (re, im), 8- or 16-byte aligned pair.
@@ -73,11 +75,15 @@ struct Grid {
 inline unsigned batch_z(const Grid &g) { return g.nbatch > 1 ? (unsigned)g.nbatch : 1u; }
 // This block's run of the batch is not stepped.
 __device__ __forceinline__ bool batch_skip(const Grid &g) { return g.bactive && !g.bactive[blockIdx.z]; }
-// p, a buffer of run 0, moved to this block's run of the batch.
+// p, a buffer of run 0, moved to this block's run of the batch. The move
+// is byte arithmetic on the pointer itself (never through an integer), so
+// the compiler still sees a global-memory pointer and emits global_load /
+// global_store rather than flat accesses, which would also count against
+// the LDS wait counter.
 template <typename P>
 __device__ __forceinline__ P *brun(P *p, const Grid &g) {
-    if (!p || !g.bstride) return p;
-    return reinterpret_cast<P *>(reinterpret_cast<uintptr_t>(p) + (uintptr_t)((long long)blockIdx.z * g.bstride));
+    using C = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+    return p ? reinterpret_cast<P *>(reinterpret_cast<C *>(p) + (long long)blockIdx.z * g.bstride) : p;
 }
 
 __device__ __forceinline__ int kx_of(const Grid &g, int i) { return (i > g.hnx) ? i - g.nx : i; }
