@@ -107,6 +107,18 @@ __host__ __device__ constexpr WavePlan make_wave_plan(int L, int E, bool rev) {
     return p;
 }
 
+// Twiddle table of a plan, one block per stage S >= 1: entry
+// off(S) + (r-1) Ns + k holds W_L^(r k L/(Ns R)), r = 1..R-1, k = 0..Ns-1.
+// The lanes of a stage read consecutive k, so the reads are free of LDS
+// bank conflicts (a table indexed by the exponent r k L/(Ns R) is read at
+// a stride of L/(Ns R) entries, up to 8-way conflicting).
+__host__ __device__ constexpr int wave_tw_off(const WavePlan &p, int s) {
+    int o = 0;
+    for (int i = 1; i < s; i++) o += (p.r[i] - 1) * p.ns[i];
+    return o;
+}
+__host__ __device__ constexpr int wave_tw_size(const WavePlan &p) { return wave_tw_off(p, p.n); }
+
 // Points per lane: `pref` (16 for FP32, 8 for FP64 by default), at most L
 // and at least L/64 (a line fits one wavefront).
 template <typename T, int L, int PREF = (sizeof(T) == 4 ? 16 : 8)>
@@ -116,19 +128,31 @@ struct WaveE {
 
 __host__ __device__ constexpr bool wave_fft_ok(int L) { return L >= 8 && L <= 512 && (L & (L - 1)) == 0; }
 
-// LDS strip of one line: 128-byte groups padded by one element.
+// LDS strip of one line in 128-byte groups (8 FP64 / 16 FP32 complex).
+// KH_WAVE_SWIZZLE 0: each group padded by one element; 1: no padding, the
+// element's slot within its group XORed with the group number, so that
+// both consecutive points and points a group apart fall in distinct banks.
+#ifndef KH_WAVE_SWIZZLE
+#define KH_WAVE_SWIZZLE 0
+#endif
 template <typename T>
 struct WavePad {
     static constexpr int shift = sizeof(T) == 4 ? 4 : 3;  // 16 / 8 complex per 128 B
-    __device__ __forceinline__ static int at(int p) { return p + (p >> shift); }
+    __device__ __forceinline__ static int at(int p) {
+        if constexpr (KH_WAVE_SWIZZLE) return p ^ ((p >> shift) & ((1 << shift) - 1));
+        return p + (p >> shift);
+    }
 };
+#ifndef KH_WAVE_STAGGER
+#define KH_WAVE_STAGGER 1
+#endif
 template <typename T, int L>
 struct WaveStrip {
     // padded length, rounded so that neighbouring strips start 64 bytes apart
     // modulo 256 bytes
-    static constexpr int raw   = L + (L >> WavePad<T>::shift);
+    static constexpr int raw   = L + (KH_WAVE_SWIZZLE ? 0 : (L >> WavePad<T>::shift));
     static constexpr int unit  = 64 / (int)sizeof(cpx<T>);
-    static constexpr int value = ((raw + 4 * unit - 1) / (4 * unit)) * 4 * unit + unit;
+    static constexpr int value = ((raw + 4 * unit - 1) / (4 * unit)) * 4 * unit + KH_WAVE_STAGGER * unit;
 };
 
 template <typename T, int L, int E_, bool FWD, bool REV>
@@ -137,11 +161,32 @@ struct WaveFft {
     static constexpr int      G = L / E;
     static constexpr WavePlan P = make_wave_plan(L, E, REV);
     static constexpr int      R0 = P.r[0], RN = P.r[P.n - 1];
+    static constexpr int      TW = wave_tw_size(P) > 0 ? wave_tw_size(P) : 1;  // twiddle entries (LDS)
     static_assert(G >= 1 && G <= 64, "line must fit one wavefront");
+
+    // the plan's twiddle table (forward W_L^m = exp(-2 pi i m / L); the
+    // inverse multiplies by the conjugate), by the workgroup's NT threads
+    template <int NT>
+    __device__ __forceinline__ static void fill_tw(cpx<T> *tw) {
+        for (int e = threadIdx.x; e < TW; e += NT) {
+            int m = 0;
+#pragma unroll
+            for (int s = 1; s < P.n; s++) {
+                const int o = wave_tw_off(P, s), n = (P.r[s] - 1) * P.ns[s];
+                if (e >= o && e < o + n) {
+                    const int r = 1 + (e - o) / P.ns[s], k = (e - o) % P.ns[s];
+                    m           = r * k * (L / (P.ns[s] * P.r[s]));
+                }
+            }
+            double sn, cs;
+            sincospi(2.0 * (double)m / (double)L, &sn, &cs);
+            tw[e] = cmk((T)cs, (T)(-sn));
+        }
+    }
 
     template <int S>
     __device__ __forceinline__ static void stages(cpx<T> *v, cpx<T> *buf, const cpx<T> *tw, int g) {
-        constexpr int R = P.r[S], Ns = P.ns[S], NB = E / R, tstride = L / (Ns * R);
+        constexpr int R = P.r[S], Ns = P.ns[S], NB = E / R, to = wave_tw_off(P, S);
         static_assert(E % R == 0, "radix must divide the points per lane");
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -150,7 +195,7 @@ struct WaveFft {
             if constexpr (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    const cpx<T> w = tw[r * k * tstride];
+                    const cpx<T> w = tw[to + (r - 1) * Ns + k];
                     v[b * R + r]   = FWD ? cmul(v[b * R + r], w) : cmulc(v[b * R + r], w);
                 }
             }

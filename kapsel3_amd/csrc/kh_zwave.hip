@@ -95,8 +95,8 @@ __global__ __launch_bounds__(256) void k_zw_r2c(const T *__restrict__ in, cpx<T>
                                                 int pitch, int kzv) {
     using Geo = WaveGeom<T, L, E>;
     using F   = WaveFft<T, L, E, true, true>;
-    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
-    fill_twiddles<T, L, Geo::NT>(tw);
+    __shared__ cpx<T> tw[F::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    F::template fill_tw<Geo::NT>(tw);
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, nrows, pitch);
     const T             *ra = in + (2 * wl.line) * L;
@@ -132,8 +132,8 @@ __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T sc
     using Geo = WaveGeom<T, L, E>;
     using F   = WaveFft<T, L, E, false, false>;
     using RS  = RowStage<T, L>;
-    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
-    fill_twiddles<T, L, Geo::NT>(tw);
+    __shared__ cpx<T> tw[F::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    F::template fill_tw<Geo::NT>(tw);
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, nrows, pitch);
     cpx<T>                  v[F::E];
@@ -182,8 +182,8 @@ __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, con
     // io stays as passed (indexing a modified copy by the loop's c would
     // put it in scratch); each pointer is moved to this block's run at use
     mask = brun(mask, bg);
-    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
-    fill_twiddles<T, L, Geo::NT>(tw);
+    __shared__ cpx<T> tw[F::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    F::template fill_tw<Geo::NT>(tw);
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, nrows, pitch);
     const long long         oa = (2 * wl.line) * L, ob = oa + L;
@@ -239,8 +239,8 @@ __global__ __launch_bounds__(256) void k_zw_c2r_cross(ZIO<T> io, const T *__rest
     using V   = typename RS::vec;
     static_assert(L >= 4 * Geo::G, "four points per lane per row");
     constexpr int NT4 = RS::NV / Geo::G;  // vectors per lane per row
-    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
-    fill_twiddles<T, L, Geo::NT>(tw);
+    __shared__ cpx<T> tw[F::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    F::template fill_tw<Geo::NT>(tw);
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, nrows, pitch);
     const long long         oa = (2 * wl.line) * L, ob = oa + L;
@@ -304,8 +304,9 @@ __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict
     if (batch_skip(g)) return;
     s0 = brun(s0, g), s1 = brun(s1, g), s2 = brun(s2, g);
     p0 = brun(p0, g), p1 = brun(p1, g), p2 = brun(p2, g), p3 = brun(p3, g), p4 = brun(p4, g);
-    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
-    fill_twiddles<T, L, Geo::NT>(tw);
+    __shared__ cpx<T> twi[Fi::TW], twf[Ff::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    Fi::template fill_tw<Geo::NT>(twi);
+    Ff::template fill_tw<Geo::NT>(twf);
     __syncthreads();
     const WaveLane<T, L, E> wl(strips, (long long)g.nx * g.ny, g.nzp);
     const int            kzv = g.trn_z;
@@ -314,9 +315,9 @@ __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict
         load_pair<T, L, Fi>(s0, wl, kzv, a);
         load_pair<T, L, Fi>(s1, wl, kzv, b);
         load_pair<T, L, Fi>(s2, wl, kzv, c);
-        Fi::run(a, wl.buf, tw, wl.g);
-        Fi::run(b, wl.buf, tw, wl.g);
-        Fi::run(c, wl.buf, tw, wl.g);
+        Fi::run(a, wl.buf, twi, wl.g);
+        Fi::run(b, wl.buf, twi, wl.g);
+        Fi::run(c, wl.buf, twi, wl.g);
 #pragma unroll
         for (int i = 0; i < E; i++) {
             a[i] = cscale(a[i], scale);
@@ -324,68 +325,68 @@ __global__ __launch_bounds__(256) void k_zw_adv(Grid g, const cpx<T> *__restrict
             c[i] = cscale(c[i], scale);
             v[i] = cmk(a[i].x * b[i].x, a[i].y * b[i].y);  // u1 u2
         }
-        Ff::run(v, wl.buf, tw, wl.g);
+        Ff::run(v, wl.buf, twf, wl.g);
         store_pair<T, L, Ff>(v, wl, kzv, p2);
 #pragma unroll
         for (int i = 0; i < E; i++) v[i] = cmk(c[i].x * a[i].x, c[i].y * a[i].y);  // u3 u1
-        Ff::run(v, wl.buf, tw, wl.g);
+        Ff::run(v, wl.buf, twf, wl.g);
         store_pair<T, L, Ff>(v, wl, kzv, p1);
 #pragma unroll
         for (int i = 0; i < E; i++)  // u3^2 - u1^2
             v[i] = cmk((c[i].x - a[i].x) * (c[i].x + a[i].x), (c[i].y - a[i].y) * (c[i].y + a[i].y));
-        Ff::run(v, wl.buf, tw, wl.g);
+        Ff::run(v, wl.buf, twf, wl.g);
         store_pair<T, L, Ff>(v, wl, kzv, p4);
 #pragma unroll
         for (int i = 0; i < E; i++) v[i] = cmk(b[i].x * c[i].x, b[i].y * c[i].y);  // u2 u3
-        Ff::run(v, wl.buf, tw, wl.g);
+        Ff::run(v, wl.buf, twf, wl.g);
         store_pair<T, L, Ff>(v, wl, kzv, p0);
 #pragma unroll
         for (int i = 0; i < E; i++)  // u2^2 - u3^2
             v[i] = cmk((b[i].x - c[i].x) * (b[i].x + c[i].x), (b[i].y - c[i].y) * (b[i].y + c[i].y));
-        Ff::run(v, wl.buf, tw, wl.g);
+        Ff::run(v, wl.buf, twf, wl.g);
         store_pair<T, L, Ff>(v, wl, kzv, p3);
         return;
     }
     cpx<T> ua[E], ub[E], v[E];
     load_pair<T, L, Fi>(s0, wl, kzv, ua);
-    Fi::run(ua, wl.buf, tw, wl.g);
+    Fi::run(ua, wl.buf, twi, wl.g);
     load_pair<T, L, Fi>(s1, wl, kzv, ub);
-    Fi::run(ub, wl.buf, tw, wl.g);
+    Fi::run(ub, wl.buf, twi, wl.g);
 #pragma unroll
     for (int i = 0; i < E; i++) {
         ua[i] = cscale(ua[i], scale);
         ub[i] = cscale(ub[i], scale);
         v[i]  = cmk(ua[i].x * ub[i].x, ua[i].y * ub[i].y);  // u1 u2
     }
-    Ff::run(v, wl.buf, tw, wl.g);
+    Ff::run(v, wl.buf, twf, wl.g);
     store_pair<T, L, Ff>(v, wl, kzv, p2);
     load_pair<T, L, Fi>(s2, wl, kzv, ub);  // ub := u3
-    Fi::run(ub, wl.buf, tw, wl.g);
+    Fi::run(ub, wl.buf, twi, wl.g);
 #pragma unroll
     for (int i = 0; i < E; i++) {
         ub[i] = cscale(ub[i], scale);
         v[i]  = cmk(ub[i].x * ua[i].x, ub[i].y * ua[i].y);  // u3 u1
     }
-    Ff::run(v, wl.buf, tw, wl.g);
+    Ff::run(v, wl.buf, twf, wl.g);
     store_pair<T, L, Ff>(v, wl, kzv, p1);
 #pragma unroll
     for (int i = 0; i < E; i++)  // u3^2 - u1^2
         v[i] = cmk((ub[i].x - ua[i].x) * (ub[i].x + ua[i].x), (ub[i].y - ua[i].y) * (ub[i].y + ua[i].y));
-    Ff::run(v, wl.buf, tw, wl.g);
+    Ff::run(v, wl.buf, twf, wl.g);
     store_pair<T, L, Ff>(v, wl, kzv, p4);
     load_pair<T, L, Fi>(s1, wl, kzv, ua);  // ua := u2 again
-    Fi::run(ua, wl.buf, tw, wl.g);
+    Fi::run(ua, wl.buf, twi, wl.g);
 #pragma unroll
     for (int i = 0; i < E; i++) {
         ua[i] = cscale(ua[i], scale);
         v[i]  = cmk(ua[i].x * ub[i].x, ua[i].y * ub[i].y);  // u2 u3
     }
-    Ff::run(v, wl.buf, tw, wl.g);
+    Ff::run(v, wl.buf, twf, wl.g);
     store_pair<T, L, Ff>(v, wl, kzv, p0);
 #pragma unroll
     for (int i = 0; i < E; i++)  // u2^2 - u3^2
         v[i] = cmk((ua[i].x - ub[i].x) * (ua[i].x + ub[i].x), (ua[i].y - ub[i].y) * (ua[i].y + ub[i].y));
-    Ff::run(v, wl.buf, tw, wl.g);
+    Ff::run(v, wl.buf, twf, wl.g);
     store_pair<T, L, Ff>(v, wl, kzv, p3);
 }
 
@@ -409,8 +410,8 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
     // io stays as passed (a modified copy indexed by the loop's d would sit
     // in scratch); each pointer is moved to this block's run at use
     tick = brun(tick, g);
-    __shared__ cpx<T> tw[L], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
-    fill_twiddles<T, L, Geo::NT>(tw);
+    __shared__ cpx<T> tw[Ff::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
+    Ff::template fill_tw<Geo::NT>(tw);
     __syncthreads();
     if (tick && blockIdx.x == 0 && threadIdx.x == 0) *tick += 1;  // the step counter, after every reader
     const long long         nrows = (long long)g.nx * g.ny;
@@ -484,8 +485,8 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
 bool zw_supported(int L) { return wave_fft_ok(L); }
 
 // points per lane of each pass (the advection pass holds three components
-// in registers, so it may take fewer points per lane; the FP64 force pass,
-// which streams seven fields per point, runs faster at four)
+// in registers, so it may take fewer points per lane; the force pass, which
+// streams several fields per point, runs faster at four in both precisions)
 #ifndef KH_ADV_E32
 #define KH_ADV_E32 8
 #endif
@@ -495,6 +496,15 @@ bool zw_supported(int L) { return wave_fft_ok(L); }
 #ifndef KH_ADV_E64
 #define KH_ADV_E64 8
 #endif
+#ifndef KH_PASS_E64
+#define KH_PASS_E64 8
+#endif
+#ifndef KH_FORCE_E64
+#define KH_FORCE_E64 4
+#endif
+#ifndef KH_FORCE_E32
+#define KH_FORCE_E32 4
+#endif
 // the advection pass keeps all three components live (each read once) at
 // this many points per lane or fewer: at 8 (FP32 and FP64) the extra
 // registers cost less than the second read and transform of u2
@@ -503,10 +513,10 @@ bool zw_supported(int L) { return wave_fft_ok(L); }
 #endif
 template <typename T, int L>
 struct ZwE {
-    static constexpr int  pass  = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 8>::value;
+    static constexpr int  pass  = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : KH_PASS_E64>::value;
     static constexpr int  adv   = WaveE<T, L, sizeof(T) == 4 ? KH_ADV_E32 : KH_ADV_E64>::value;
     static constexpr bool live3 = adv <= KH_ADV_LIVE3_E;
-    static constexpr int  force = WaveE<T, L, sizeof(T) == 4 ? KH_PASS_E32 : 4>::value;
+    static constexpr int  force = WaveE<T, L, sizeof(T) == 4 ? KH_FORCE_E32 : KH_FORCE_E64>::value;
 };
 
 template <typename T, int L, int E>
