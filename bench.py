@@ -66,8 +66,10 @@ def kernel_bytes(name, n, p, npts, sf, nphi=None):
         # u* is read only where phi_sum > 0 (F_H gather, rigidity force):
         # three spectra in, phi_sum everywhere, u* written at those points
         "z_c2r": cs * 3 * ycol + sf * real + sf * 3 * nphi,
-        # phi_sum everywhere; u_p and u* only where phi_sum > 0: elsewhere f = 0
-        "z_force": sf * real + 6 * sf * nphi + cs * 3 * ycol,
+        # phi_sum everywhere and f only where phi_sum > 0 (elsewhere f = 0):
+        # the particle step leaves f = u_p - min(phi_sum,1) u* in the u_p
+        # field, reading u* at the points its force gather already read
+        "z_force": sf * real + 3 * sf * nphi + cs * 3 * ycol,
         "y_fwd_force": cs * (3 * ycol + 3 * xcol),
         "x_f_zeta": cs * (3 * xcol + 2 * kept + 2 * kept),
         "fill": sf * real,
