@@ -86,6 +86,19 @@ __device__ __forceinline__ P *brun(P *p, const Grid &g) {
     return p ? reinterpret_cast<P *>(reinterpret_cast<C *>(p) + (long long)blockIdx.z * g.bstride) : p;
 }
 
+// Occupancy of a run's 4-point vectors along z: bit (v & 63) of word
+// row * words + (v >> 6) is set when some particle's phi may be nonzero in
+// vector v (points 4v..4v+3) of real-space row `row` (the line along z at
+// (x, y), row = x * ny + y). The phi scatter sets the bits; the last pass
+// that reads them clears them again.
+struct Occ {
+    unsigned long long *w     = nullptr;
+    int                 words = 0;
+};
+__device__ __forceinline__ bool occ_bit(const unsigned long long *w, int words, long long row, int v) {
+    return (w[row * words + (v >> 6)] >> (v & 63)) & 1ull;
+}
+
 __device__ __forceinline__ int kx_of(const Grid &g, int i) { return (i > g.hnx) ? i - g.nx : i; }
 __device__ __forceinline__ int ky_of(const Grid &g, int j) { return (j > g.hny) ? j - g.ny : j; }
 __device__ __forceinline__ bool kept1(int k, int trn) { return (k < 0 ? -k : k) < trn; }

@@ -24,7 +24,7 @@ struct ZIO {
 template <typename T>
 // mask: see zw_c2r (ignored by the LDS engine, which writes every point)
 hipError_t fft_z_c2r_multi(const Grid &g, const ZIO<T> &io, int ncomp, hipStream_t s, int kzvalid = -1,
-                           const T *mask = nullptr);
+                           const T *mask = nullptr, Occ occ = {});
 template <typename T, bool FWD>
 hipError_t fft_y(const Grid &g, cpx<T> *data, hipStream_t s);
 template <typename T, bool FWD>

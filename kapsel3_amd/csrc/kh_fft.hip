@@ -246,10 +246,13 @@ hipError_t fft_z_c2r(const Grid &g, const cpx<T> *in, T *out, hipStream_t s, int
     return fft_z_c2r_multi<T>(g, io, 1, s, kzvalid);
 }
 template <typename T>
-hipError_t fft_z_c2r_multi(const Grid &g, const ZIO<T> &io, int ncomp, hipStream_t s, int kzvalid, const T *mask) {
+hipError_t fft_z_c2r_multi(const Grid &g, const ZIO<T> &io, int ncomp, hipStream_t s, int kzvalid, const T *mask,
+                           Occ occ) {
     const T   scale = (T)(1.0 / ((double)g.nx * (double)g.ny * (double)g.nz));
     const int kzv   = kzvalid < 0 ? g.nzh : kzvalid;
-    if (zw_supported(g.nz)) return zw_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s, mask, nullptr, &g);
+    if (zw_supported(g.nz))
+        return zw_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s, mask, nullptr, &g, occ);
+    if (occ.w) return hipErrorInvalidValue;
     return launch_z_c2r<T>(g.nz, io, ncomp, (long long)g.nx * g.ny, scale, kzv, g.nzp, s, g);
 }
 // y: lines along y at stride nzp, columns kz < nzh, one outer block per x
@@ -284,7 +287,7 @@ hipError_t fft3d_c2r(const Grid &g, cpx<T> *spec_inout, T *real_out, hipStream_t
 #define KH_INST(T)                                                                   \
     template hipError_t fft_z_r2c<T>(const Grid &, const T *, cpx<T> *, hipStream_t, int); \
     template hipError_t fft_z_c2r<T>(const Grid &, const cpx<T> *, T *, hipStream_t, int); \
-    template hipError_t fft_z_c2r_multi<T>(const Grid &, const ZIO<T> &, int, hipStream_t, int, const T *); \
+    template hipError_t fft_z_c2r_multi<T>(const Grid &, const ZIO<T> &, int, hipStream_t, int, const T *, Occ); \
     template hipError_t fft_y<T, true>(const Grid &, cpx<T> *, hipStream_t);          \
     template hipError_t fft_y<T, false>(const Grid &, cpx<T> *, hipStream_t);         \
     template hipError_t fft_x<T, true>(const Grid &, cpx<T> *, hipStream_t);          \

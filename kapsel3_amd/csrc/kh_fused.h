@@ -35,7 +35,7 @@ hipError_t z_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], hipStr
 bool z_force_clears(const Grid &g);
 template <typename T>
 hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                   int *tick, hipStream_t s, T *const us2[3] = nullptr, const int *skip = nullptr);
+                   int *tick, hipStream_t s, T *const us2[3] = nullptr, const int *skip = nullptr, Occ occ = {});
 // the slip iteration's projection on the pruned transform: forward x (3) of
 // the slip force field -> truncation + Solenoidal_uk, its curl into (a0, a1)
 // at the kept modes and its k = 0 mode into dc -> inverse x (3), in place

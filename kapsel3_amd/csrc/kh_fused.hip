@@ -800,10 +800,10 @@ bool z_force_clears(const Grid &g) { return zw_supported(g.nz); }
 
 template <typename T>
 hipError_t z_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                   int *tick, hipStream_t s, T *const us2[3], const int *skip) {
+                   int *tick, hipStream_t s, T *const us2[3], const int *skip, Occ occ) {
     const long long nrows = (long long)g.nx * g.ny;
-    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, tick, s, us2, skip);
-    if (clear || tick || us2 || skip) return hipErrorInvalidValue;
+    if (zw_supported(g.nz)) return zw_force<T>(g, up, phis, us, f, clear, tick, s, us2, skip, occ);
+    if (clear || tick || us2 || skip || occ.w) return hipErrorInvalidValue;
     T *const none[3] = {nullptr, nullptr, nullptr};
     if (!us) us = none;
     switch (g.nz) {
@@ -893,7 +893,7 @@ hipError_t zeta_clean(const Grid &g, cpx<T> *z0, cpx<T> *z1, hipStream_t s) {
                                         int, hipStream_t);                                                          \
     template hipError_t z_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], hipStream_t);                   \
     template hipError_t z_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,          \
-                                   hipStream_t, T *const[3], const int *);                                         \
+                                   hipStream_t, T *const[3], const int *, Occ);                                    \
     template hipError_t x_slip<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, const int *,          \
                                   hipStream_t);                                                                    \
     template hipError_t x_f_zeta<T>(const Grid &, cpx<T> *const[3], cpx<T> *, cpx<T> *, double *, hipStream_t);    \

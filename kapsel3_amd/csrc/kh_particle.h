@@ -62,6 +62,9 @@ struct PParams {
     // particles), and per bead the LJ force from partners of higher index
     const int        *bid;
     double           *flo;
+    // the fused plain step: the phi scatter marks the 4-point vectors it
+    // reaches (Occ, w == nullptr: not kept)
+    Occ               occ;
 };
 
 // One rigid body (rigid.h's per-body state, input.cxx:1655-1695).

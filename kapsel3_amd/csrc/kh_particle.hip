@@ -139,6 +139,20 @@ __device__ __forceinline__ void phi_scatter_block(const PParams &pp, DevParticle
             if (SCATTER && ph != (T)0) unsafeAtomicAdd(&phisum[idx], ph);
         }
         const bool               keep = ph != (T)0;
+        if constexpr (SCATTER && !OBL) {
+            if (pp.occ.w) {
+                // mark this point's 4-point vector; Sekibun order keeps a
+                // row's points on consecutive lanes, so only the first lane
+                // of a run of points in one vector sets the bit
+                const long long          row = idx / pp.nz;
+                const int                v   = (int)(idx - row * pp.nz) >> 2;
+                const unsigned long long kv  = m < pp.np_domain ? ((unsigned long long)row << 16) | (unsigned)v : ~0ull;
+                const unsigned long long pkv = __shfl_up(kv, 1);
+                const int                pk  = __shfl_up((int)keep, 1);
+                if (keep && (lane == 0 || !pk || pkv != kv))
+                    atomicOr(pp.occ.w + row * pp.occ.words + (v >> 6), 1ull << (v & 63));
+            }
+        }
         const unsigned long long b    = __ballot(keep);
         if (lane == 0) wcnt[par][wave] = __popcll(b);
         __syncthreads();

@@ -101,6 +101,8 @@ struct kh_run {
     void       *up[3]    = {};
     void       *work[5]  = {};
     void       *phisum   = nullptr;
+    void       *occ      = nullptr;  // occupancy bits of the 4-point z vectors (Occ)
+    int         occw     = 0;        // words per row
     double     *ukdc     = nullptr;
     DevParticle *dp      = nullptr;
     DevSpecies  *ds      = nullptr;
@@ -247,6 +249,9 @@ struct Timed {
     } while (0)
 
 // ------------------------------------------------------------ the step
+// the run's occupancy bits of the 4-point z vectors
+static inline Occ occ_of(const kh_run *r) { return Occ{(unsigned long long *)r->occ, r->occw}; }
+
 template <typename T>
 struct Step {
     static cpx<T> *S(kh_run *r, int i) { return (cpx<T> *)r->spec[i]; }
@@ -267,6 +272,13 @@ struct Step {
         KH_LAUNCH("fft_y_inv", (fft_y<T, false>(r->g, in, r->stream)));
         KH_LAUNCH("fft_z_c2r", (fft_z_c2r<T>(r->g, in, out, r->stream)));
         return KH_OK;
+    }
+    // The occupancy bits replace phi_sum as the mask of z_c2r and z_force
+    // for plain spheres in FP64. In FP32 the bits cost the scatter more than
+    // the mask reads they save (a 4-point vector is 16 bytes there, and
+    // z_force's clearing stores into lines it has not read are slower).
+    static bool occ_on(const kh_run *r) {
+        return sizeof(T) == 8 && r->zclear && !r->slip && !r->rigid && !r->pp.obl;
     }
     static int zero(kh_run *r, T *p) {
         KH_LAUNCH("fill", (fill<T>(r->g.nreal, p, (T)0, r->stream)));
@@ -529,6 +541,7 @@ struct Step {
         if (!(r->zclear && r->pclean)) {
             for (int d = 0; d < 3; d++) KH_TRY(zero(r, UP(r, d)));
             KH_TRY(zero(r, PS(r)));
+            KH_LAUNCH("fill", (fill<double>((long long)r->occw * r->g.nx * r->g.ny, (double *)r->occ, 0.0, r->stream)));
         }
         r->pclean = false;
         if (r->rigid) {  // the bodies move, then their beads scatter phi
@@ -542,7 +555,7 @@ struct Step {
     // step, up to u* in real space (written only where mask != 0 when a mask
     // is given), for the run r or, when g carries a batch, for every run of
     // the batch from r on: one launch per pass covers them all.
-    static int fused_fluid(kh_run *r, const Grid &g, const T *mask) {
+    static int fused_fluid(kh_run *r, const Grid &g, const T *mask, Occ occ = {}) {
         hipStream_t s     = r->stream;
         cpx<T>     *s3[3] = {S(r, 0), S(r, 1), S(r, 2)};
         Comps<T>    c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
@@ -566,19 +579,19 @@ struct Step {
         KH_LAUNCH("x_zeta2u_inv", (x_zeta2u_inv<T>(g, Z(r, 0), Z(r, 1), s3, r->ukdc, r->fixmask, s)));
         KH_LAUNCH("y_inv", (cols_y<T, false>(g, c5, 3, 1, 0, s)));
         ZIO<T> zio{{S(r, 0), S(r, 1), S(r, 2)}, {U(r, 0), U(r, 1), U(r, 2)}};
-        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z, mask)));
+        KH_LAUNCH("z_c2r", (fft_z_c2r_multi<T>(g, zio, 3, s, g.trn_z, mask, occ)));
         return KH_OK;
     }
     // f = P(up - phi u*), u = u* + f, zeta = curl u (u* = u + us2 when us2
     // is given; up_is_f: the particle step left f itself in up), and the
     // device step counter; for r or r's batch as above
-    static int fused_force(kh_run *r, const Grid &g, T *const us2[3], bool up_is_f) {
+    static int fused_force(kh_run *r, const Grid &g, T *const us2[3], bool up_is_f, Occ occ = {}) {
         hipStream_t s      = r->stream;
         cpx<T>     *s3[3]  = {S(r, 0), S(r, 1), S(r, 2)};
         Comps<T>    c5{{S(r, 0), S(r, 1), S(r, 2), S(r, 3), S(r, 4)}};
         T          *up3[3] = {UP(r, 0), UP(r, 1), UP(r, 2)}, *u3[3] = {U(r, 0), U(r, 1), U(r, 2)};
         KH_LAUNCH("z_force", (z_force<T>(g, up3, PS(r), up_is_f ? nullptr : u3, s3, r->zclear ? 3 : 0,
-                                         r->zclear ? r->d_ts : nullptr, s, us2)));
+                                         r->zclear ? r->d_ts : nullptr, s, us2, nullptr, occ)));
         if (!r->zclear) KH_LAUNCH("tick", tick(r->d_ts, s, &g));
         KH_LAUNCH("y_fwd_force", (cols_y<T, true>(g, c5, 3, 0, 1, s)));
         KH_LAUNCH("x_f_zeta", (x_f_zeta<T>(g, s3, Z(r, 0), Z(r, 1), r->ukdc, s)));
@@ -612,10 +625,14 @@ struct Step {
         // only where it is read. Squirmer runs with the slip loop decided on
         // the host keep the reference's order.
         const bool early = !r->slip || r->slip_dev;
+        // plain spheres in FP64: the phi scatter marks the 4-point vectors it
+        // reaches, and z_c2r and z_force read those bits instead of phi_sum
+        const Occ occ = occ_on(r) ? occ_of(r) : Occ{};
+        r->pp.occ     = occ;
         if (early) KH_TRY(particles_begin(r, ab2));
         // squirmers read u* on the whole slip shell (|r - a| < xi/2), which
         // reaches points where phi has underflowed to 0: u* is written in full
-        KH_TRY(fused_fluid(r, g, early && !r->slip ? PS(r) : nullptr));
+        KH_TRY(fused_fluid(r, g, early && !r->slip ? PS(r) : nullptr, occ));
         if (!early) KH_TRY(particles_begin(r, ab2));
         if (r->slip_dev) {  // the slip loop leaves up zero
             KH_TRY(calc_f_hydro(r));
@@ -636,7 +653,7 @@ struct Step {
             KH_TRY(plain_particles(r, ab2));
         }
         T *w3[3] = {W(r, 0), W(r, 1), W(r, 2)};  // squirmers: u = u* + the projected slip field
-        KH_TRY(fused_force(r, g, r->slip_dev ? w3 : nullptr, !r->slip && !r->rigid));
+        KH_TRY(fused_force(r, g, r->slip_dev ? w3 : nullptr, !r->slip && !r->rigid, occ));
         r->pclean = r->zclear;
         return KH_OK;
     }
@@ -661,11 +678,12 @@ struct Step {
         }
         KH_LAUNCH("phi_scatter", (phi_scatter_batch<T>(sp, b->active, b->capacity, b->np_cap, r->jikan.dt_md,
                                                        r->jikan.hdt_md, s)));
-        KH_TRY(fused_fluid(r, g, PS(r)));
+        const Occ occ = occ_on(r) ? occ_of(r) : Occ{};
+        KH_TRY(fused_fluid(r, g, PS(r), occ));
         KH_LAUNCH("particle_step", (particle_step_batch<T>(sp, b->active, b->capacity, b->np_cap,
                                                            (-r->dx3 * r->rho) / r->jikan.dt_fluid, r->jikan.hdt_md,
                                                            s)));
-        KH_TRY(fused_force(r, g, nullptr, true));
+        KH_TRY(fused_force(r, g, nullptr, true, occ));
         KH_LAUNCH("records", (records_batch<T>(sp, b->active, b->capacity, s)));
         return KH_OK;
     }
@@ -673,6 +691,7 @@ struct Step {
     static SlotPart<T> slot_part(kh_run *r) {
         SlotPart<T> a{};
         a.pp     = r->pp;
+        a.pp.occ = occ_on(r) ? occ_of(r) : Occ{};
         a.P      = r->dp;
         a.phisum = PS(r);
         a.pc     = CP(r);
@@ -1468,6 +1487,8 @@ int run_create(const kh_params *P, int np, char *arena, size_t arena_bytes, kh_r
     for (int i = 0; i < 3 && st == KH_OK; i++) st = alloc(&r->up[i], rb);
     for (int i = 0; i < 5 && st == KH_OK; i++) st = alloc(&r->work[i], rb);
     if (st == KH_OK) st = alloc(&r->phisum, rb);
+    r->occw = ((g.nz + 3) / 4 + 63) / 64;
+    if (st == KH_OK) st = alloc(&r->occ, sizeof(unsigned long long) * (size_t)r->occw * (size_t)g.nx * g.ny);
     if (st == KH_OK) st = alloc((void **)&r->ukdc, 3 * sizeof(double));
     if (st == KH_OK) st = alloc((void **)&r->d_ts, sizeof(int));
     if (st == KH_OK) st = alloc((void **)&r->slipeps, sizeof(double));
@@ -1499,6 +1520,8 @@ int run_create(const kh_params *P, int np, char *arena, size_t arena_bytes, kh_r
                   r->work[0], r->work[1], r->work[2], r->work[3], r->work[4], r->phisum};
     for (void *b : zr)
         if (hipMemsetAsync(b, 0, rb, cs) != hipSuccess) return bail(fail(KH_ERR_HIP, "hipMemset failed"));
+    if (hipMemsetAsync(r->occ, 0, sizeof(unsigned long long) * (size_t)r->occw * (size_t)g.nx * g.ny, cs) != hipSuccess)
+        return bail(fail(KH_ERR_HIP, "hipMemset failed"));
     if (hipStreamSynchronize(cs) != hipSuccess) return bail(fail(KH_ERR_HIP, "initial upload failed"));
 
     PParams &pp        = r->pp;
@@ -2495,6 +2518,7 @@ static int batch_capture(kh_batch *b, kh_run *m, int count, hipGraph_t *graph, h
     for (auto &p : v->up) mv(p);
     for (auto &p : v->work) mv(p);
     mv(v->phisum);
+    mv(v->occ);
     mv(v->ukdc);
     mv(v->d_ts);
     v->slot      = 0;

@@ -18,7 +18,8 @@ hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, h
 // cover the batch's runs)
 template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
-                  hipStream_t s, const T *mask = nullptr, const int *skip = nullptr, const Grid *batch = nullptr);
+                  hipStream_t s, const T *mask = nullptr, const int *skip = nullptr, const Grid *batch = nullptr,
+                  Occ occ = {});
 // inverse z (3) -> advection products -> forward z (5), modes kz < TRN_Z
 // inverse z pass of omega fused with u x omega (lengths >= 32)
 bool zw_cross_supported(int L);
@@ -39,9 +40,10 @@ struct ZForceIO {
     const T *us[3];
     const T *us2[3];
     cpx<T>  *f[3];
+    Occ      occ;  // with u* absent: where f may be nonzero (cleared with phi_sum)
 };
 template <typename T>
 hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                    int *tick, hipStream_t s, T *const us2[3] = nullptr, const int *skip = nullptr);
+                    int *tick, hipStream_t s, T *const us2[3] = nullptr, const int *skip = nullptr, Occ occ = {});
 
 }  // namespace kh

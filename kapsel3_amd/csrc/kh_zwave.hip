@@ -168,11 +168,12 @@ __global__ __launch_bounds__(256) void k_zw_c2r(ZIO<T> io, long long nrows, T sc
 // real-space vector of 4 points only where the mask (phi_sum) has a nonzero
 // point: u* is read only there afterwards (the F_H gather at points with
 // phi > 0, the rigidity force where phi_sum != 0), so the rest of the field
-// is left as it was. The mask is read once for all components.
+// is left as it was. The mask is read once for all components; with an
+// occupancy bitmask (occ.w) its bits replace the mask's values.
 template <typename T, int L, int E>
 __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, const T *__restrict__ mask,
                                                        long long nrows, T scale, int kzv, int pitch,
-                                                       const int *__restrict__ skip, Grid bg) {
+                                                       const int *__restrict__ skip, Grid bg, Occ occ) {
     using Geo = WaveGeom<T, L, E>;
     using F   = WaveFft<T, L, E, false, false>;
     using RS  = RowStage<T, L>;
@@ -181,7 +182,8 @@ __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, con
     if ((skip && *skip) || batch_skip(bg)) return;
     // io stays as passed (indexing a modified copy by the loop's c would
     // put it in scratch); each pointer is moved to this block's run at use
-    mask = brun(mask, bg);
+    mask                                  = brun(mask, bg);
+    const unsigned long long *__restrict__ ow = brun(occ.w, bg);
     __shared__ cpx<T> tw[F::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     F::template fill_tw<Geo::NT>(tw);
     __syncthreads();
@@ -191,8 +193,13 @@ __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, con
 #pragma unroll
     for (int t = 0; t < NT4; t++) {
         const int z = 4 * (wl.g + Geo::G * t);
-        wa[t]       = wl.okA && nonzero(RS::ld(mask + oa + z));
-        wb[t]       = wl.okB && nonzero(RS::ld(mask + ob + z));
+        if (ow) {
+            wa[t] = wl.okA && occ_bit(ow, occ.words, 2 * wl.line, z >> 2);
+            wb[t] = wl.okB && occ_bit(ow, occ.words, 2 * wl.line + 1, z >> 2);
+        } else {
+            wa[t] = wl.okA && nonzero(RS::ld(mask + oa + z));
+            wb[t] = wl.okB && nonzero(RS::ld(mask + ob + z));
+        }
     }
 #pragma unroll 1
     for (int c = 0; c < ncomp; c++) {
@@ -420,6 +427,7 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
     using vec                  = typename RS::vec;
     const vec zero             = {(T)0, (T)0, (T)0, (T)0};
     T        *phis             = brun(io.phis, g);
+    unsigned long long *const ow = brun(io.occ.w, g);
     // Outside every particle phi_sum = 0, and so is u_p (it is scattered on
     // the same points with the same phi): f = 0 there whatever u* is, so
     // u_p and u* are read only where some phi_sum of the vector is nonzero.
@@ -427,6 +435,17 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
     bool na[NT4], nb[NT4];
 #pragma unroll
     for (int t = 0; t < NT4; t++) {
+        if (ow) {  // the occupancy bits: f = up (u* absent), phi_sum only cleared
+            const int z = 4 * (wl.g + Geo::G * t);
+            na[t]       = wl.okA && occ_bit(ow, io.occ.words, 2 * wl.line, z >> 2);
+            nb[t]       = wl.okB && occ_bit(ow, io.occ.words, 2 * wl.line + 1, z >> 2);
+            pa[t] = pb[t] = zero;
+            if ((clear & 2) && phis) {
+                if (na[t]) RS::st(phis + oa + z, zero);
+                if (nb[t]) RS::st(phis + ob + z, zero);
+            }
+            continue;
+        }
         if (!phis) {  // no mask: up is read everywhere (f = up)
             na[t] = wl.okA;
             nb[t] = wl.okB;
@@ -476,6 +495,16 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
         wave_sync();
         Ff::run(v, wl.buf, tw, wl.g);
         store_pair<T, L, Ff>(v, wl, g.trn_z, brun(io.f[d], g));
+    }
+    if (ow && (clear & 2)) {  // leave the bits zero for the next step's scatter
+#pragma unroll
+        for (int t = 0; t < NT4; t++) {
+            const int v = wl.g + Geo::G * t;
+            if ((v & 63) == 0) {
+                if (wl.okA) ow[(2 * wl.line) * io.occ.words + (v >> 6)] = 0ull;
+                if (wl.okB) ow[(2 * wl.line + 1) * io.occ.words + (v >> 6)] = 0ull;
+            }
+        }
     }
 }
 
@@ -541,16 +570,18 @@ hipError_t zw_r2c(int L, const T *in, cpx<T> *out, long long nrows, int pitch, h
 
 template <typename T>
 hipError_t zw_c2r(int L, const ZIO<T> &io, int ncomp, long long nrows, T scale, int kzv, int pitch,
-                  hipStream_t s, const T *mask, const int *skip, const Grid *batch) {
-    const Grid bg = batch ? *batch : Grid{};
-    if (batch_z(bg) > 1 && !(mask && L >= 32)) return hipErrorInvalidValue;  // only the masked pass is batched
-    if (skip && !(mask && L >= 32)) return hipErrorInvalidValue;  // only the masked pass takes a skip flag
-    if (mask && L >= 32) {  // every length with 4 points per lane per row
+                  hipStream_t s, const T *mask, const int *skip, const Grid *batch, Occ occ) {
+    const Grid bg     = batch ? *batch : Grid{};
+    const bool masked = (mask || occ.w) && L >= 32;
+    if (occ.w && !masked) return hipErrorInvalidValue;
+    if (batch_z(bg) > 1 && !masked) return hipErrorInvalidValue;  // only the masked pass is batched
+    if (skip && !masked) return hipErrorInvalidValue;  // only the masked pass takes a skip flag
+    if (masked) {  // every length with 4 points per lane per row
         switch (L) {
 #define KH_CASE(n)                                                                                              \
     case n:                                                                                                     \
         if constexpr (n >= 32) {                                                                                \
-            hipLaunchKernelGGL((k_zw_c2r_masked<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows), 1, batch_z(bg)), dim3(256), 0, s, io, ncomp, mask, nrows, scale, kzv, pitch, skip, bg); \
+            hipLaunchKernelGGL((k_zw_c2r_masked<T, n, ZwE<T, n>::pass>), dim3(zw_blocks<T, n, ZwE<T, n>::pass>(nrows), 1, batch_z(bg)), dim3(256), 0, s, io, ncomp, mask, nrows, scale, kzv, pitch, skip, bg, occ); \
             return hipGetLastError();                                                                           \
         }                                                                                                       \
         break;
@@ -607,9 +638,11 @@ hipError_t zw_adv(const Grid &g, cpx<T> *const s3[3], cpx<T> *const p5[5], T sca
 
 template <typename T>
 hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<T> *const f[3], int clear,
-                    int *tick, hipStream_t s, T *const us2[3], const int *skip) {
+                    int *tick, hipStream_t s, T *const us2[3], const int *skip, Occ occ) {
     const long long nrows = (long long)g.nx * g.ny;
+    if (occ.w && (us || us2)) return hipErrorInvalidValue;  // the bits replace phi_sum only where f = up
     ZForceIO<T>     io{};
+    io.occ = occ;
     for (int d = 0; d < 3; d++) {
         io.up[d]  = up[d];
         io.us[d]  = us ? us[d] : nullptr;
@@ -632,11 +665,11 @@ hipError_t zw_force(const Grid &g, T *const up[3], T *phis, T *const us[3], cpx<
 #define KH_INST(T)                                                                                           \
     template hipError_t zw_r2c<T>(int, const T *, cpx<T> *, long long, int, hipStream_t, int);               \
     template hipError_t zw_c2r<T>(int, const ZIO<T> &, int, long long, T, int, int, hipStream_t, const T *, \
-                                  const int *, const Grid *);                                              \
+                                  const int *, const Grid *, Occ);                                         \
     template hipError_t zw_c2r_cross<T>(int, const ZIO<T> &, const T *const[3], long long, T, int, int, hipStream_t); \
     template hipError_t zw_adv<T>(const Grid &, cpx<T> *const[3], cpx<T> *const[5], T, hipStream_t);         \
     template hipError_t zw_force<T>(const Grid &, T *const[3], T *, T *const[3], cpx<T> *const[3], int, int *,        \
-                                    hipStream_t, T *const[3], const int *);
+                                    hipStream_t, T *const[3], const int *, Occ);
 KH_INST(double)
 KH_INST(float)
 #undef KH_INST
