@@ -38,13 +38,16 @@ def step_bytes(n, p, npts, sf, sp):
     return sf * field_words(n) * 113 + sp * p * npts * 13
 
 
-def kernel_bytes(name, n, p, npts, sf, nphi=None):
+def kernel_bytes(name, n, p, npts, sf, nphi=None, occ=None):
     """Algorithmic HBM bytes of one launch: every distinct input read once
     and every output written once, over the modes and points the kernel's
     function needs (2/3-kept modes |kx|,|ky| < T, kz < T with T = (N+2)/3;
     Sekibun points of every particle). sf = bytes of one field scalar; nphi
     = grid points where phi_sum > 0 (the only points where u* is read and
-    f = u_p - phi u* is nonzero; all points when not given)."""
+    f = u_p - phi u* is nonzero; all points when not given). occ: the step
+    keeps the per-row occupancy bits (the FP64 plain step; default sf == 8),
+    so z_c2r and z_force read 8 bytes per row instead of phi_sum, and
+    z_force's clearing of u_p and phi_sum is counted as its writes."""
     t = (n + 2) // 3
     kx, ky, kz = 2 * t - 1, 2 * t - 1, t
     cs = 2 * sf                      # one complex
@@ -54,6 +57,8 @@ def kernel_bytes(name, n, p, npts, sf, nphi=None):
     real = n ** 3
     pt = p * npts
     nphi = real if nphi is None else min(nphi, real)
+    occ = (sf == 8) if occ is None else occ
+    bits = 8 * n * n * ((n // 4 + 63) // 64)
     table = {
         # fused pipeline
         "x_zeta2u_inv": cs * (2 * kept + 3 * xcol),
@@ -65,11 +70,11 @@ def kernel_bytes(name, n, p, npts, sf, nphi=None):
         "x_adv_ns": cs * (5 * xcol + 2 * kept + 2 * kept),
         # u* is read only where phi_sum > 0 (F_H gather, rigidity force):
         # three spectra in, phi_sum everywhere, u* written at those points
-        "z_c2r": cs * 3 * ycol + sf * real + sf * 3 * nphi,
+        "z_c2r": cs * 3 * ycol + (bits if occ else sf * real) + sf * 3 * nphi,
         # phi_sum everywhere and f only where phi_sum > 0 (elsewhere f = 0):
         # the particle step leaves f = u_p - min(phi_sum,1) u* in the u_p
         # field, reading u* at the points its force gather already read
-        "z_force": sf * real + 3 * sf * nphi + cs * 3 * ycol,
+        "z_force": (bits + 4 * sf * nphi if occ else sf * real) + 3 * sf * nphi + cs * 3 * ycol,
         "y_fwd_force": cs * (3 * ycol + 3 * xcol),
         "x_f_zeta": cs * (3 * xcol + 2 * kept + 2 * kept),
         "fill": sf * real,
