@@ -95,8 +95,23 @@ struct Occ {
     unsigned long long *w     = nullptr;
     int                 words = 0;
 };
+// the bits are kept for FP64 fields only unless this is set (in FP32 they
+// cost the scatter as much as they save the masked passes)
+#ifndef KH_OCC_FP32
+#define KH_OCC_FP32 0
+#endif
+template <typename T>
+constexpr bool occ_kept() { return sizeof(T) == 8 || KH_OCC_FP32; }
 __device__ __forceinline__ bool occ_bit(const unsigned long long *w, int words, long long row, int v) {
     return (w[row * words + (v >> 6)] >> (v & 63)) & 1ull;
+}
+// set the bits of vectors a..b (a <= b) of one row
+__device__ __forceinline__ void occ_mark(const Occ &o, int row, int a, int b) {
+    for (int w = a >> 6; w <= (b >> 6); w++) {
+        const int                lo = (a > 64 * w ? a : 64 * w) - 64 * w, hi = (b < 64 * w + 63 ? b : 64 * w + 63) - 64 * w;
+        const unsigned long long m  = (hi == 63 ? ~0ull : (1ull << (hi + 1)) - 1ull) & ~((1ull << lo) - 1ull);
+        atomicOr(o.w + (long long)row * o.words + w, m);
+    }
 }
 
 __device__ __forceinline__ int kx_of(const Grid &g, int i) { return (i > g.hnx) ? i - g.nx : i; }

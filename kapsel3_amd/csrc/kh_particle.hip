@@ -139,18 +139,32 @@ __device__ __forceinline__ void phi_scatter_block(const PParams &pp, DevParticle
             if (SCATTER && ph != (T)0) unsafeAtomicAdd(&phisum[idx], ph);
         }
         const bool               keep = ph != (T)0;
-        if constexpr (SCATTER && !OBL) {
+        if constexpr (SCATTER && !OBL && occ_kept<T>()) {
             if (pp.occ.w) {
-                // mark this point's 4-point vector; Sekibun order keeps a
-                // row's points on consecutive lanes, so only the first lane
-                // of a run of points in one vector sets the bit
-                const long long          row = idx / pp.nz;
-                const int                v   = (int)(idx - row * pp.nz) >> 2;
-                const unsigned long long kv  = m < pp.np_domain ? ((unsigned long long)row << 16) | (unsigned)v : ~0ull;
-                const unsigned long long pkv = __shfl_up(kv, 1);
-                const int                pk  = __shfl_up((int)keep, 1);
-                if (keep && (lane == 0 || !pk || pkv != kv))
-                    atomicOr(pp.occ.w + row * pp.occ.words + (v >> 6), 1ull << (v & 63));
+                // mark the 4-point vectors this round reaches. Sekibun order
+                // keeps a row's points on consecutive lanes and its nonzero
+                // points form one chord, so the first lane of each run of
+                // nonzero points in one row marks the run's whole range of
+                // vectors (wrapping at the row's end) at once.
+                const int  row  = (int)(idx / pp.nz);
+                const int  v    = (int)(idx - (long long)row * pp.nz) >> 2;
+                const int  prow = __shfl_up(row, 1), nrow = __shfl_down(row, 1);
+                const int  pk = __shfl_up((int)keep, 1), nk = __shfl_down((int)keep, 1);
+                const bool head = keep && (lane == 0 || !pk || prow != row);
+                const bool tail = keep && (lane == 63 || !nk || nrow != row);
+                const unsigned long long tails = __ballot(tail);
+                const unsigned long long after = tails & (~0ull << lane);
+                const int e  = after ? __ffsll((long long)after) - 1 : lane;  // the run's last lane
+                const int ve = __shfl(v, e);
+                if (head) {
+                    const int nv = (pp.nz + 3) / 4;
+                    if (v <= ve) {
+                        occ_mark(pp.occ, row, v, ve);
+                    } else {
+                        occ_mark(pp.occ, row, v, nv - 1);
+                        occ_mark(pp.occ, row, 0, ve);
+                    }
+                }
             }
         }
         const unsigned long long b    = __ballot(keep);

@@ -278,7 +278,7 @@ struct Step {
     // the mask reads they save (a 4-point vector is 16 bytes there, and
     // z_force's clearing stores into lines it has not read are slower).
     static bool occ_on(const kh_run *r) {
-        return sizeof(T) == 8 && r->zclear && !r->slip && !r->rigid && !r->pp.obl;
+        return occ_kept<T>() && r->zclear && !r->slip && !r->rigid && !r->pp.obl;
     }
     static int zero(kh_run *r, T *p) {
         KH_LAUNCH("fill", (fill<T>(r->g.nreal, p, (T)0, r->stream)));

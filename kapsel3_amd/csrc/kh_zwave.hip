@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_zw_c2r_masked(ZIO<T> io, int ncomp, con
     // io stays as passed (indexing a modified copy by the loop's c would
     // put it in scratch); each pointer is moved to this block's run at use
     mask                                  = brun(mask, bg);
-    const unsigned long long *__restrict__ ow = brun(occ.w, bg);
+    const unsigned long long *__restrict__ ow = occ_kept<T>() ? brun(occ.w, bg) : nullptr;
     __shared__ cpx<T> tw[F::TW], strips[(Geo::NT / 64) * Geo::W * Geo::LP];
     F::template fill_tw<Geo::NT>(tw);
     __syncthreads();
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256) void k_zw_force(Grid g, ZForceIO<T> io, int cl
     using vec                  = typename RS::vec;
     const vec zero             = {(T)0, (T)0, (T)0, (T)0};
     T        *phis             = brun(io.phis, g);
-    unsigned long long *const ow = brun(io.occ.w, g);
+    unsigned long long *const ow = occ_kept<T>() ? brun(io.occ.w, g) : nullptr;
     // Outside every particle phi_sum = 0, and so is u_p (it is scattered on
     // the same points with the same phi): f = 0 there whatever u* is, so
     // u_p and u* are read only where some phi_sum of the vector is nonzero.
